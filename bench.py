@@ -1,0 +1,150 @@
+"""Throughput of the channel timestep on one MI355X.
+
+Workload: turbulent channel at Re_tau=180, 128 x 129 x 128, domain
+4*pi x 2 x 4*pi/3, tanh y-grid (CS=1.75), no-slip walls, constant mean
+pressure gradient, FP64, CFL=0.5 variable dt; seeded laminar-plus-noise
+initial condition built on the device.  One step = the three RK_CHAN_1
+substeps of diablo.f:70-84 plus TIME+=DELTA_T.
+
+Prints one JSON line: Mpts*RK3-steps/s = NX*NY*NZ*steps/wall/1e6, the HBM
+roofline of the dominant kernel (algorithmic bytes / mean HIP-event
+duration), the roofline of the whole step against the algorithmic bytes
+per step, and the CPU baseline (the oracle, OpenMP on the host cores).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mpts·RK3-steps/s, Re_tau=180 channel; learned-LES speedup at matched U+ stats"
+UNIT = "Mpts·RK3-steps/s"
+HBM_PEAK_GBS = 8000.0
+
+
+def plane_bytes(p):
+    nkx, nkz = p["nx"] // 3, p["nz"] // 3
+    F = 16 * (nkx + 1) * (2 * nkz + 1)  # one de-aliased complex plane
+    P = 8 * p["nx"] * p["nz"]            # one physical plane
+    return F, P
+
+
+def step_bytes(p):
+    """Algorithmic bytes per RK3 step (FP64, LES off): (66+11 N_TH) F +
+    (36+12 N_TH) P with F, P whole fields over NY+2 planes."""
+    Fp, Pp = plane_bytes(p)
+    nyy, nth = p["ny"] + 2, p["n_th"]
+    return (66 + 11 * nth) * Fp * nyy + (36 + 12 * nth) * Pp * nyy
+
+
+def phase_bytes(p, rk):
+    """Algorithmic (unique) bytes each kernel of substep rk must move, for
+    the velocity fields (N_TH=0 workload): planes actually read/written."""
+    Fp, Pp = plane_bytes(p)
+    NY = p["ny"]
+    nyy = NY + 2
+    nf = 3 + p["n_th"]
+    b = {}
+    b["to_physical"] = nf * nyy * (Fp + Pp)
+    cf_old = 3 * (NY - 1) * Fp if rk > 1 else 0
+    b["nonlinear"] = 3 * (NY + 1) * Pp + 3 * (NY - 1) * Fp + NY * Fp + cf_old + 3 * (NY - 1) * Fp + 3 * (NY - 1) * Pp
+    b["cn_solve"] = 3 * (NY - 1) * Pp + 3 * nyy * Pp + 3 * (NY + 1) * Pp
+    b["courant"] = 0
+    b["to_fourier"] = nf * nyy * (Fp + Pp)
+    b["rem_div"] = 3 * NY * Fp + 3 * NY * Fp + 2 * (NY - 1) * Fp + NY * Fp
+    return b
+
+
+def cpu_baseline(p, gy, ic, budget_s=12.0, max_steps=6):
+    from oracle import dor
+    from tests import helpers
+    o, *_ = helpers.make_oracle(dor, "channel_re180")
+    o.time_step(1)  # first step outside the sample (page-in, thread start)
+    n, t0 = 0, time.perf_counter()
+    while n < max_steps:
+        o.time_step(1)
+        n += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    pts = p["nx"] * p["ny"] * p["nz"]
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": pts * n / dt / 1e6, "unit": UNIT, "cores": cores, "kind": "port",
+            "sample": f"{n} RK3 steps of the same 128x129x128 channel from the same seeded IC "
+                      f"(after 1 untimed step), FP64 C restatement with OpenMP over planes/columns, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--profile-reps", type=int, default=10)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the multi-GPU y-slab design is documented, not built", file=sys.stderr)
+        sys.exit(2)
+
+    from diablo_amd import channel, configs
+
+    p, gy, ic = configs.channel_re180()
+    ch = channel.Channel(p, gy)
+    ch.initial_condition(ic)
+    ch.set_clock(0.0, p["delta_t"], 0)
+    ch.time_step(args.warmup)
+    ch.synchronize()
+    t0 = time.perf_counter()
+    ch.time_step(args.steps)
+    ch.synchronize()
+    wall = time.perf_counter() - t0
+    t_sim, dt_now, _, ts = ch.clock()
+    pts = p["nx"] * p["ny"] * p["nz"]
+    value = pts * args.steps / wall / 1e6
+    ms_step = wall / args.steps * 1e3
+
+    # per-kernel HIP-event timings on the context stream (state keeps evolving)
+    phases = {}
+    for rk in (1, 2, 3):
+        for k, v in ch.profile_substep(rk, args.profile_reps).items():
+            phases[f"rk{rk}:{k}"] = v
+    per_kernel = {}
+    for k, v in phases.items():
+        name = k.split(":", 1)[1]
+        per_kernel.setdefault(name, []).append((int(k[2]), v))
+    dominant = max(per_kernel, key=lambda n: sum(v for _, v in per_kernel[n]))
+    launches = per_kernel[dominant]
+    byts = sum(phase_bytes(p, rk)[dominant] for rk, _ in launches) / len(launches)
+    mean_ms = sum(v for _, v in launches) / len(launches)
+    achieved = byts / (mean_ms * 1e-3) / 1e9
+    B = step_bytes(p)
+    step_gbs = B / (ms_step * 1e-3) / 1e9
+
+    out = {
+        "metric": METRIC, "value": round(value, 3), "unit": UNIT, "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded laminar Poiseuille + Gaussian noise, projected divergence-free on the device)",
+        "config": {"workload": "channel_re180: turbulent channel Re_tau=180, 128x129x128, 4pi x 2 x 4pi/3, "
+                               "tanh grid CS=1.75, FP64, CFL=0.5 variable dt",
+                   "nx": p["nx"], "ny": p["ny"], "nz": p["nz"], "n_th": p["n_th"]},
+        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_per_launch": int(byts), "mean_ms": round(mean_ms, 5)},
+        "step_roofline": {"bytes_per_step": B, "achieved": round(step_gbs, 1), "peak": HBM_PEAK_GBS,
+                          "unit": "GB/s", "frac": round(step_gbs / HBM_PEAK_GBS, 4)},
+        "phases_ms": {k: round(v, 5) for k, v in phases.items()},
+        "sim": {"time": t_sim, "dt": dt_now, "time_step": ts},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(p, gy, ic)
+    ch.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

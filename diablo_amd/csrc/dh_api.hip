@@ -1,0 +1,654 @@
+// C ABI of libdiablo_hip.so (declarations and reference citations in
+// include/diablo_hip.h): context set-up, host<->device layout conversion,
+// substep/step dispatch and the captured step graph.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+#include "dh_internal.h"
+
+using namespace dh;
+
+struct dh_ctx {
+  Ctx c;
+};
+
+namespace {
+
+const double kPi = 3.14159265358979323846;
+
+int fail(Ctx &c, int code, const std::string &msg) {
+  c.err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                          \
+  do {                                                                                        \
+    hipError_t e_ = (expr);                                                                   \
+    if (e_ != hipSuccess) return fail(c, DH_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+bool make_plan(int n, FftPlan &p) {
+  p.n = n;
+  p.nst = 0;
+  int m = n;
+  while (m % 8 == 0) { p.rad[p.nst++] = 8; m /= 8; }
+  while (m % 4 == 0) { p.rad[p.nst++] = 4; m /= 4; }
+  while (m % 2 == 0) { p.rad[p.nst++] = 2; m /= 2; }
+  while (m % 3 == 0) { p.rad[p.nst++] = 3; m /= 3; }
+  return m == 1 && p.nst <= kMaxStages;
+}
+
+// Slot of each output index after the in-place DIF transform of plan p:
+// slot s holds index digrev(s), digrev(s) = s/S + R*digrev'(s%S).
+void digrev(int n, const int *rad, int nst, std::vector<int> &idx_of_slot) {
+  idx_of_slot.assign(n, 0);
+  if (n == 1 || nst == 0) return;
+  const int R = rad[0], S = n / R;
+  std::vector<int> sub;
+  digrev(S, rad + 1, nst - 1, sub);
+  for (int s = 0; s < n; ++s) idx_of_slot[s] = s / S + R * sub[s % S];
+}
+
+std::vector<int> slot_of_index(const FftPlan &p) {
+  std::vector<int> ios, pos(p.n);
+  digrev(p.n, p.rad, p.nst, ios);
+  for (int s = 0; s < p.n; ++s) pos[ios[s]] = s;
+  return pos;
+}
+
+std::vector<double2> twiddles(int n, int count) {
+  std::vector<double2> t(count);
+  for (int m = 0; m < count; ++m) {
+    long double a = -2.0L * 3.141592653589793238462643383279502884L * (long double)m / (long double)n;
+    t[m] = make_double2((double)cosl(a), (double)sinl(a));
+  }
+  return t;
+}
+
+template <class T>
+hipError_t dalloc(T **p, size_t count) {
+  hipError_t e = hipMalloc((void **)p, sizeof(T) * (count ? count : 1));
+  if (e == hipSuccess) e = hipMemset(*p, 0, sizeof(T) * (count ? count : 1));
+  return e;
+}
+
+void free_all(Ctx &c) {
+  void *ptrs[] = {c.tw_row, c.tw_col, c.tw_real, c.posr, c.posz, c.kx, c.kz, c.kx2, c.kz2, c.d_dy, c.d_dyf, c.d_gyf, c.cu[0], c.cu[1],
+                  c.cu[2], c.cp, c.cf[0], c.cf[1], c.cf[2], c.cth, c.cfth, c.phi, c.uphys, c.rphys, c.rd_bp,
+                  c.rd_g, c.clock, c.fac.ap, c.fac.bp, c.fac.cc, c.fac.isbc, c.fac.vconst, c.rd_l, c.rd_u};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  if (c.step_graph) (void)hipGraphExecDestroy(c.step_graph);
+  if (c.stream) (void)hipStreamDestroy(c.stream);
+}
+
+// courant.f:15-31: the part of the time-step bound independent of U.
+double dt_cap_of(const Ctx &c) {
+  const dh_params &p = c.p;
+  double dt = 999.0;
+  const double dx1 = p.lx / p.nx, dy1 = c.dy[1];
+  const double m = dx1 < dy1 ? dx1 : dy1;
+  const double v = 0.5 * (m * m) / p.nu;
+  if (v < dt) dt = v;
+  for (int n = 0; n < p.n_th; ++n) {
+    const double w = dt * p.nu / (p.nu / p.pr[n]);
+    if (w < dt) dt = w;
+  }
+  if (p.i_ro != 0.0) {
+    const double w = 2.0 * kPi / std::fabs(p.i_ro) / 20.0;
+    if (w < dt) dt = w;
+  }
+  for (int n = 0; n < p.n_th; ++n) {
+    if (p.ri[n] != 0.0) {
+      const double Nmax = std::sqrt(std::fabs(p.ri[n] * p.th_bc_ymin_c1[n]));
+      const double w = (double)0.1f * 2.0 * kPi / Nmax;  // 0.1 is a default-real literal
+      if (w < dt) dt = w;
+    }
+  }
+  return dt;
+}
+
+int upload_factors(Ctx &c, double dt) {
+  std::vector<double> ap, bp, cc, vc;
+  std::vector<int> isbc;
+  build_real_factors_host(c, dt, ap, bp, cc, isbc, vc);
+  HIPCHK(hipMemcpyAsync(c.fac.ap, ap.data(), ap.size() * 8, hipMemcpyHostToDevice, c.stream));
+  HIPCHK(hipMemcpyAsync(c.fac.bp, bp.data(), bp.size() * 8, hipMemcpyHostToDevice, c.stream));
+  HIPCHK(hipMemcpyAsync(c.fac.cc, cc.data(), cc.size() * 8, hipMemcpyHostToDevice, c.stream));
+  HIPCHK(hipMemcpyAsync(c.fac.isbc, isbc.data(), isbc.size() * sizeof(int), hipMemcpyHostToDevice, c.stream));
+  HIPCHK(hipMemcpyAsync(c.fac.vconst, vc.data(), vc.size() * 8, hipMemcpyHostToDevice, c.stream));
+  HIPCHK(hipStreamSynchronize(c.stream));
+  return DH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dh_create(const dh_params *p, dh_ctx **out) {
+  if (!p || !out) return DH_EARG;
+  *out = nullptr;
+  dh_ctx *h = new (std::nothrow) dh_ctx();
+  if (!h) return DH_EARG;
+  Ctx &c = h->c;
+  c.p = *p;
+  auto reject = [&](const char *msg) {
+    c.err = msg;
+    std::fprintf(stderr, "dh_create: %s\n", msg);
+    delete h;
+    return DH_EARG;
+  };
+  if (p->nx < 8 || p->nx % 4 != 0) return reject("NX must be a multiple of 4, >= 8");
+  if (p->nz < 1 || p->ny < 4) return reject("NZ >= 1 and NY >= 4 required");
+  if (p->n_th < 0 || p->n_th > DH_MAX_TH) return reject("N_TH out of range");
+  const int bcs[] = {p->u_bc_ymin, p->v_bc_ymin, p->w_bc_ymin, p->u_bc_ymax, p->v_bc_ymax, p->w_bc_ymax};
+  for (int b : bcs)
+    if (b != 0 && b != 1) return reject("velocity BC types must be 0 (Dirichlet) or 1 (Neumann)");
+  for (int n = 0; n < p->n_th; ++n)
+    if ((p->th_bc_ymin[n] != 0 && p->th_bc_ymin[n] != 1) || (p->th_bc_ymax[n] != 0 && p->th_bc_ymax[n] != 1))
+      return reject("scalar BC types must be 0 or 1");
+  if (p->variable_dt && p->update_dt < 1) return reject("UPDATE_DT must be >= 1");
+  Geo &g = c.g;
+  g.nx = p->nx;
+  g.ny = p->ny;
+  g.nz = p->nz;
+  g.nkx = p->nx / 3;
+  g.nkz = p->nz / 3;
+  g.tnkz = 2 * g.nkz;
+  g.nkx1 = g.nkx + 1;
+  g.nkz1 = g.tnkz + 1;
+  g.pf = g.nkx1 * g.nkz1;
+  g.lw = p->nx / 2;
+  g.pp = (long long)p->nx * p->nz;
+  if (!make_plan(g.lw, g.prow) || !make_plan(g.nz, g.pcol))
+    return reject("NX/2 and NZ must factor into 2 and 3");
+  c.nfields = 3 + p->n_th;
+  c.lds_bytes = g.nz * g.lw * (int)sizeof(double2);
+  if (c.lds_bytes > 160 * 1024) return reject("x-z plane too large for the LDS-resident plane kernels");
+  int thr = (g.nz * g.lw + 7) / 8;
+  thr = ((thr + 63) / 64) * 64;
+  if (thr < 64) thr = 64;
+  if (thr > 512) thr = 512;
+  while (thr * 8 < g.pf && thr < 512) thr += 64;  // k_nonlinear keeps <= 8 modes per thread
+  if (thr * 8 < g.pf) return reject("plane has too many Fourier modes");
+  c.plane_threads = thr;
+  // mpi.f:1022-1046 (RANKY=0 branch on one rank)
+  c.jstart = 2;
+  c.jend = g.ny;
+  for (int n = 0; n < DH_MAX_TH; ++n) {
+    c.jend_th[n] = g.ny;
+    c.jstart_th[n] = (n < p->n_th && p->th_bc_ymin[n] == 1) ? 1 : 2;
+  }
+  // write-back rows (channel.f:826-830, 895-899, 959-963, 1023-1027)
+  c.wlo[0] = c.jstart - 1; c.whi[0] = c.jend + 1;
+  c.wlo[1] = 1; c.whi[1] = g.ny + 1;
+  c.wlo[2] = c.jstart - 1; c.whi[2] = c.jend + 1;
+  for (int n = 0; n < p->n_th; ++n) {
+    c.wlo[3 + n] = c.jstart_th[n] - 1;
+    c.whi[3 + n] = c.jend_th[n] + 1;
+  }
+  c.gy.assign(g.ny + 2, 0.0);
+  c.gyf.assign(g.ny + 2, 0.0);
+  c.dy.assign(g.ny + 2, 0.0);
+  c.dyf.assign(g.ny + 2, 0.0);
+
+  hipError_t e;
+#define ACHK(expr)                          \
+  if ((e = (expr)) != hipSuccess) {         \
+    c.err = hipGetErrorString(e);           \
+    std::fprintf(stderr, "dh_create: %s: %s\n", #expr, c.err.c_str()); \
+    free_all(c);                            \
+    delete h;                               \
+    return DH_EHIP;                         \
+  }
+  ACHK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+  const size_t nyy = (size_t)g.ny + 2;
+  const size_t cf_elems = nyy * g.pf;
+  const size_t ph_elems = nyy * (size_t)g.pp;
+  ACHK(dalloc(&c.tw_row, g.lw));
+  ACHK(dalloc(&c.tw_col, g.nz));
+  ACHK(dalloc(&c.tw_real, g.lw + 1));
+  ACHK(dalloc(&c.posr, g.lw));
+  ACHK(dalloc(&c.posz, g.nz));
+  ACHK(dalloc(&c.kx, g.nkx1));
+  ACHK(dalloc(&c.kx2, g.nkx1));
+  ACHK(dalloc(&c.kz, g.nkz1));
+  ACHK(dalloc(&c.kz2, g.nkz1));
+  ACHK(dalloc(&c.d_dy, nyy));
+  ACHK(dalloc(&c.d_dyf, nyy));
+  ACHK(dalloc(&c.d_gyf, nyy));
+  for (int i = 0; i < 3; ++i) {
+    ACHK(dalloc(&c.cu[i], cf_elems));
+    ACHK(dalloc(&c.cf[i], cf_elems));
+  }
+  ACHK(dalloc(&c.cp, cf_elems));
+  ACHK(dalloc(&c.phi, cf_elems));
+  ACHK(dalloc(&c.cth, cf_elems * (p->n_th > 0 ? p->n_th : 1)));
+  ACHK(dalloc(&c.cfth, cf_elems * (p->n_th > 0 ? p->n_th : 1)));
+  ACHK(dalloc(&c.uphys, ph_elems * c.nfields));
+  ACHK(dalloc(&c.rphys, ph_elems * c.nfields));
+  ACHK(dalloc(&c.rd_bp, cf_elems));
+  ACHK(dalloc(&c.rd_g, cf_elems));
+  ACHK(dalloc(&c.rd_l, nyy));
+  ACHK(dalloc(&c.rd_u, nyy));
+  ACHK(dalloc(&c.clock, 1));
+  ACHK(dalloc(&c.fac.ap, 3 * (size_t)c.nfields * nyy));
+  ACHK(dalloc(&c.fac.bp, 3 * (size_t)c.nfields * nyy));
+  ACHK(dalloc(&c.fac.cc, 3 * (size_t)c.nfields * nyy));
+  ACHK(dalloc(&c.fac.isbc, (size_t)c.nfields * nyy));
+  ACHK(dalloc(&c.fac.vconst, (size_t)c.nfields * nyy));
+  {
+    auto tr = twiddles(g.lw, g.lw), tc = twiddles(g.nz, g.nz), tx = twiddles(g.nx, g.lw + 1);
+    ACHK(hipMemcpy(c.tw_row, tr.data(), tr.size() * sizeof(double2), hipMemcpyHostToDevice));
+    ACHK(hipMemcpy(c.tw_col, tc.data(), tc.size() * sizeof(double2), hipMemcpyHostToDevice));
+    ACHK(hipMemcpy(c.tw_real, tx.data(), tx.size() * sizeof(double2), hipMemcpyHostToDevice));
+    auto pr = slot_of_index(g.prow), pz = slot_of_index(g.pcol);
+    ACHK(hipMemcpy(c.posr, pr.data(), pr.size() * sizeof(int), hipMemcpyHostToDevice));
+    ACHK(hipMemcpy(c.posz, pz.data(), pz.size() * sizeof(int), hipMemcpyHostToDevice));
+    // fft.f:77-100
+    const double PI = 4.0 * std::atan(1.0);
+    std::vector<double> kx(g.nkx1), kx2(g.nkx1), kz(g.nkz1), kz2(g.nkz1);
+    for (int i = 0; i < g.nkx1; ++i) {
+      kx[i] = (double)i * (2. * PI) / p->lx;
+      kx2[i] = kx[i] * kx[i];
+    }
+    for (int k = 0; k <= g.nkz; ++k) kz[k] = (double)k * (2. * PI) / p->lz;
+    for (int k = 1; k <= g.nkz; ++k) kz[g.tnkz + 1 - k] = -(double)k * (2. * PI) / p->lz;
+    for (int k = 0; k < g.nkz1; ++k) kz2[k] = kz[k] * kz[k];
+    ACHK(hipMemcpy(c.kx, kx.data(), g.nkx1 * 8, hipMemcpyHostToDevice));
+    ACHK(hipMemcpy(c.kx2, kx2.data(), g.nkx1 * 8, hipMemcpyHostToDevice));
+    ACHK(hipMemcpy(c.kz, kz.data(), g.nkz1 * 8, hipMemcpyHostToDevice));
+    ACHK(hipMemcpy(c.kz2, kz2.data(), g.nkz1 * 8, hipMemcpyHostToDevice));
+  }
+  g.tw_row = c.tw_row;
+  g.tw_col = c.tw_col;
+  g.tw_real = c.tw_real;
+  g.posr = c.posr;
+  g.posz = c.posz;
+  g.kx = c.kx;
+  g.kz = c.kz;
+  g.kx2 = c.kx2;
+  g.kz2 = c.kz2;
+  g.dy = c.d_dy;
+  g.dyf = c.d_dyf;
+  {
+    Clock ck{};
+    ck.time = 0.0;
+    ck.dt = p->delta_t;
+    // diablo_io.f:116-124
+    ck.h_bar[0] = p->delta_t * (8.0 / 15.0);
+    ck.h_bar[1] = p->delta_t * (2.0 / 15.0);
+    ck.h_bar[2] = p->delta_t * (5.0 / 15.0);
+    ck.beta_bar[0] = 1.0;
+    ck.beta_bar[1] = 25.0 / 8.0;
+    ck.beta_bar[2] = 9.0 / 4.0;
+    ck.zeta_bar[0] = 0.0;
+    ck.zeta_bar[1] = -17.0 / 8.0;
+    ck.zeta_bar[2] = -5.0 / 4.0;
+    ck.cfl_min_bits = 0x7FF0000000000000ull;
+    ck.time_step = 0;
+    ACHK(hipMemcpy(c.clock, &ck, sizeof(Clock), hipMemcpyHostToDevice));
+  }
+  ACHK(prepare_fft_kernels(c));
+  ACHK(prepare_step_kernels(c));
+#undef ACHK
+  *out = h;
+  return DH_OK;
+}
+
+int dh_destroy(dh_ctx *h) {
+  if (!h) return DH_EARG;
+  (void)hipStreamSynchronize(h->c.stream);
+  free_all(h->c);
+  delete h;
+  return DH_OK;
+}
+
+const char *dh_last_error(dh_ctx *h) { return h ? h->c.err.c_str() : "null context"; }
+
+int dh_set_grid(dh_ctx *h, const double *gy, const double *gyf, const double *dy, const double *dyf) {
+  if (!h || !gy || !gyf || !dy || !dyf) return DH_EARG;
+  Ctx &c = h->c;
+  const int nyy = c.g.ny + 2;
+  c.gy.assign(gy, gy + nyy);
+  c.gyf.assign(gyf, gyf + nyy);
+  c.dy.assign(dy, dy + nyy);
+  c.dyf.assign(dyf, dyf + nyy);
+  for (int j = 1; j <= c.g.ny; ++j)
+    if (!(c.dy[j] > 0.0) || !(c.dyf[j] > 0.0)) return fail(c, DH_EARG, "DY/DYF must be positive on rows 1..NY");
+  HIPCHK(hipMemcpy(c.d_dy, dy, nyy * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c.d_dyf, dyf, nyy * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c.d_gyf, gyf, nyy * 8, hipMemcpyHostToDevice));
+  // channel.f:1126-1134 coefficient tables
+  std::vector<double> rl(nyy, 0.0), ru(nyy, 0.0);
+  for (int j = 1; j <= c.g.ny; ++j) {
+    rl[j] = 1. / (c.dy[j] * c.dyf[j]);
+    ru[j] = 1. / (c.dy[j + 1] * c.dyf[j]);
+  }
+  HIPCHK(hipMemcpy(c.rd_l, rl.data(), nyy * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c.rd_u, ru.data(), nyy * 8, hipMemcpyHostToDevice));
+  c.grid_set = true;
+  c.dt_cap = dt_cap_of(c);
+  Clock ck;
+  HIPCHK(hipMemcpy(&ck, c.clock, sizeof(Clock), hipMemcpyDeviceToHost));
+  return upload_factors(c, ck.dt);
+}
+
+int dh_set_clock(dh_ctx *h, double time, double delta_t, int32_t time_step) {
+  if (!h) return DH_EARG;
+  Ctx &c = h->c;
+  if (!c.grid_set) return fail(c, DH_ESTATE, "dh_set_grid first");
+  HIPCHK(hipStreamSynchronize(c.stream));
+  Clock ck;
+  HIPCHK(hipMemcpy(&ck, c.clock, sizeof(Clock), hipMemcpyDeviceToHost));
+  ck.time = time;
+  ck.time_step = time_step;
+  const bool newdt = (delta_t != ck.dt);
+  if (newdt) {
+    ck.dt = delta_t;
+    ck.h_bar[0] = delta_t * (8.0 / 15.0);
+    ck.h_bar[1] = delta_t * (2.0 / 15.0);
+    ck.h_bar[2] = delta_t * (5.0 / 15.0);
+  }
+  HIPCHK(hipMemcpy(c.clock, &ck, sizeof(Clock), hipMemcpyHostToDevice));
+  if (newdt) return upload_factors(c, delta_t);
+  return DH_OK;
+}
+
+int dh_get_clock(dh_ctx *h, double *time, double *delta_t, double *h_bar3, int32_t *time_step) {
+  if (!h) return DH_EARG;
+  Ctx &c = h->c;
+  HIPCHK(hipStreamSynchronize(c.stream));
+  Clock ck;
+  HIPCHK(hipMemcpy(&ck, c.clock, sizeof(Clock), hipMemcpyDeviceToHost));
+  if (time) *time = ck.time;
+  if (delta_t) *delta_t = ck.dt;
+  if (h_bar3) std::memcpy(h_bar3, ck.h_bar, 3 * sizeof(double));
+  if (time_step) *time_step = ck.time_step;
+  return DH_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Fortran complex (0:NXP,0:NZ+1,0:NY+1) <-> de-aliased [J][K][I].
+int pack_fourier(Ctx &c, const double *src, double2 *dst_dev, int nfield, const char *name) {
+  const Geo &g = c.g;
+  const int nxp = g.nx / 2, nzz = g.nz + 2, nyy = g.ny + 2;
+  std::vector<double2> buf((size_t)nyy * g.pf);
+  for (int f = 0; f < nfield; ++f) {
+    const double *s = src + (size_t)f * 2 * (nxp + 1) * nzz * nyy;
+    for (int j = 0; j < nyy; ++j)
+      for (int k = 0; k < g.nkz1; ++k)
+        for (int i = 0; i <= nxp; ++i) {
+          const size_t si = 2 * ((size_t)i + (size_t)(nxp + 1) * ((size_t)k + (size_t)nzz * j));
+          if (i < g.nkx1) {
+            buf[(size_t)j * g.pf + (size_t)k * g.nkx1 + i] = make_double2(s[si], s[si + 1]);
+          } else if (s[si] != 0.0 || s[si + 1] != 0.0) {
+            char msg[160];
+            std::snprintf(msg, sizeof msg, "%s(%d,%d,%d) is non-zero beyond the de-aliased modes (I>NKX=%d)", name, i,
+                          k, j, g.nkx);
+            return fail(c, DH_ELAYOUT, msg);
+          }
+        }
+    HIPCHK(hipMemcpy(dst_dev + (size_t)f * nyy * g.pf, buf.data(), buf.size() * sizeof(double2),
+                     hipMemcpyHostToDevice));
+  }
+  return DH_OK;
+}
+
+int unpack_fourier(Ctx &c, const double2 *src_dev, double *dst, int nfield) {
+  const Geo &g = c.g;
+  const int nxp = g.nx / 2, nzz = g.nz + 2, nyy = g.ny + 2;
+  std::vector<double2> buf((size_t)nyy * g.pf);
+  for (int f = 0; f < nfield; ++f) {
+    HIPCHK(hipMemcpy(buf.data(), src_dev + (size_t)f * nyy * g.pf, buf.size() * sizeof(double2),
+                     hipMemcpyDeviceToHost));
+    double *d = dst + (size_t)f * 2 * (nxp + 1) * nzz * nyy;
+    std::memset(d, 0, sizeof(double) * 2 * (size_t)(nxp + 1) * nzz * nyy);
+    for (int j = 0; j < nyy; ++j)
+      for (int k = 0; k < g.nkz1; ++k)
+        for (int i = 0; i < g.nkx1; ++i) {
+          const size_t di = 2 * ((size_t)i + (size_t)(nxp + 1) * ((size_t)k + (size_t)nzz * j));
+          const double2 v = buf[(size_t)j * g.pf + (size_t)k * g.nkx1 + i];
+          d[di] = v.x;
+          d[di + 1] = v.y;
+        }
+  }
+  return DH_OK;
+}
+
+// Fortran real (0:NX+1,0:NZ+1,0:NY+1) <-> [J][z][x]
+int pack_phys(Ctx &c, const double *src, double *dst_dev, int nfield) {
+  const Geo &g = c.g;
+  const int nxx = g.nx + 2, nzz = g.nz + 2, nyy = g.ny + 2;
+  std::vector<double> buf((size_t)nyy * g.pp);
+  for (int f = 0; f < nfield; ++f) {
+    const double *s = src + (size_t)f * nxx * nzz * nyy;
+    for (int j = 0; j < nyy; ++j)
+      for (int k = 0; k < g.nz; ++k)
+        for (int i = 0; i < g.nx; ++i)
+          buf[(size_t)j * g.pp + (size_t)k * g.nx + i] = s[(size_t)i + (size_t)nxx * ((size_t)k + (size_t)nzz * j)];
+    HIPCHK(hipMemcpy(dst_dev + (size_t)f * nyy * g.pp, buf.data(), buf.size() * 8, hipMemcpyHostToDevice));
+  }
+  return DH_OK;
+}
+
+int unpack_phys(Ctx &c, const double *src_dev, double *dst, int nfield) {
+  const Geo &g = c.g;
+  const int nxx = g.nx + 2, nzz = g.nz + 2, nyy = g.ny + 2;
+  std::vector<double> buf((size_t)nyy * g.pp);
+  for (int f = 0; f < nfield; ++f) {
+    HIPCHK(hipMemcpy(buf.data(), src_dev + (size_t)f * nyy * g.pp, buf.size() * 8, hipMemcpyDeviceToHost));
+    double *d = dst + (size_t)f * nxx * nzz * nyy;
+    std::memset(d, 0, sizeof(double) * (size_t)nxx * nzz * nyy);
+    for (int j = 0; j < nyy; ++j)
+      for (int k = 0; k < g.nz; ++k)
+        for (int i = 0; i < g.nx; ++i)
+          d[(size_t)i + (size_t)nxx * ((size_t)k + (size_t)nzz * j)] = buf[(size_t)j * g.pp + (size_t)k * g.nx + i];
+  }
+  return DH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dh_upload(dh_ctx *h, const dh_host_state *st) {
+  if (!h || !st) return DH_EARG;
+  Ctx &c = h->c;
+  HIPCHK(hipStreamSynchronize(c.stream));
+  const long long fs = (long long)(c.g.ny + 2) * c.g.pp;
+  int r;
+  const double *fl[7] = {st->cu1, st->cu2, st->cu3, st->cp, st->cf1, st->cf2, st->cf3};
+  double2 *dv[7] = {c.cu[0], c.cu[1], c.cu[2], c.cp, c.cf[0], c.cf[1], c.cf[2]};
+  const char *nm[7] = {"CU1", "CU2", "CU3", "CP", "CF1", "CF2", "CF3"};
+  for (int i = 0; i < 7; ++i)
+    if (fl[i] && (r = pack_fourier(c, fl[i], dv[i], 1, nm[i])) != DH_OK) return r;
+  const int nth = c.p.n_th;
+  if (nth > 0) {
+    if (st->cth && (r = pack_fourier(c, st->cth, c.cth, nth, "CTH")) != DH_OK) return r;
+    if (st->cfth && (r = pack_fourier(c, st->cfth, c.cfth, nth, "CFTH")) != DH_OK) return r;
+    if (st->th && (r = pack_phys(c, st->th, c.uphys + 3 * fs, nth)) != DH_OK) return r;
+  }
+  return DH_OK;
+}
+
+int dh_download(dh_ctx *h, dh_host_state *st) {
+  if (!h || !st) return DH_EARG;
+  Ctx &c = h->c;
+  HIPCHK(hipStreamSynchronize(c.stream));
+  const long long fs = (long long)(c.g.ny + 2) * c.g.pp;
+  int r;
+  double *fl[7] = {st->cu1, st->cu2, st->cu3, st->cp, st->cf1, st->cf2, st->cf3};
+  double2 *dv[7] = {c.cu[0], c.cu[1], c.cu[2], c.cp, c.cf[0], c.cf[1], c.cf[2]};
+  for (int i = 0; i < 7; ++i)
+    if (fl[i] && (r = unpack_fourier(c, dv[i], fl[i], 1)) != DH_OK) return r;
+  const int nth = c.p.n_th;
+  if (nth > 0) {
+    if (st->cth && (r = unpack_fourier(c, c.cth, st->cth, nth)) != DH_OK) return r;
+    if (st->cfth && (r = unpack_fourier(c, c.cfth, st->cfth, nth)) != DH_OK) return r;
+    if (st->th && (r = unpack_phys(c, c.uphys + 3 * fs, st->th, nth)) != DH_OK) return r;
+  }
+  return DH_OK;
+}
+
+int dh_rk_substep(dh_ctx *h, int32_t rk_step) {
+  if (!h) return DH_EARG;
+  Ctx &c = h->c;
+  if (rk_step < 1 || rk_step > 3) return fail(c, DH_EARG, "rk_step must be 1..3");
+  if (!c.grid_set) return fail(c, DH_ESTATE, "dh_set_grid first");
+  HIPCHK(launch_substep(c, rk_step - 1));
+  return DH_OK;
+}
+
+static int enqueue_step(Ctx &c) {
+  HIPCHK(launch_step_begin(c));
+  for (int rk = 0; rk < 3; ++rk) HIPCHK(launch_substep(c, rk));
+  HIPCHK(launch_end_step(c));
+  return DH_OK;
+}
+
+int dh_time_step(dh_ctx *h, int32_t nsteps) {
+  if (!h || nsteps < 0) return DH_EARG;
+  Ctx &c = h->c;
+  if (!c.grid_set) return fail(c, DH_ESTATE, "dh_set_grid first");
+  if (nsteps == 0) return DH_OK;
+  if (!c.graph_ok) {
+    // capture one step; fall back to direct launches if capture fails
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal);
+    int r = DH_OK;
+    if (e == hipSuccess) {
+      r = enqueue_step(c);
+      e = hipStreamEndCapture(c.stream, &graph);
+    }
+    if (e == hipSuccess && r == DH_OK && graph) {
+      e = hipGraphInstantiate(&c.step_graph, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (e == hipSuccess) c.graph_ok = true;
+    } else {
+      (void)hipGetLastError();
+      if (graph) (void)hipGraphDestroy(graph);
+    }
+  }
+  for (int s = 0; s < nsteps; ++s) {
+    if (c.graph_ok) {
+      HIPCHK(hipGraphLaunch(c.step_graph, c.stream));
+    } else {
+      int r = enqueue_step(c);
+      if (r != DH_OK) return r;
+    }
+  }
+  return DH_OK;
+}
+
+int dh_profile_substep(dh_ctx *h, int32_t rk_step, int32_t nrep, double *ms, int32_t nms) {
+  if (!h || !ms || nrep < 1 || nms < kNumPhases) return DH_EARG;
+  Ctx &c = h->c;
+  if (rk_step < 1 || rk_step > 3) return fail(c, DH_EARG, "rk_step must be 1..3");
+  if (!c.grid_set) return fail(c, DH_ESTATE, "dh_set_grid first");
+  hipEvent_t ev[kNumPhases + 1];
+  for (auto &e : ev) HIPCHK(hipEventCreate(&e));
+  for (int i = 0; i < nms; ++i) ms[i] = 0.0;
+  for (int r = 0; r < nrep; ++r) {
+    HIPCHK(hipEventRecord(ev[0], c.stream));
+    for (int ph = 0; ph < kNumPhases; ++ph) {
+      HIPCHK(launch_phase(c, rk_step - 1, ph));
+      HIPCHK(hipEventRecord(ev[ph + 1], c.stream));
+    }
+    HIPCHK(hipEventSynchronize(ev[kNumPhases]));
+    for (int ph = 0; ph < kNumPhases; ++ph) {
+      float t = 0.f;
+      HIPCHK(hipEventElapsedTime(&t, ev[ph], ev[ph + 1]));
+      ms[ph] += t / nrep;
+    }
+  }
+  for (auto &e : ev) (void)hipEventDestroy(e);
+  return DH_OK;
+}
+
+const char *dh_phase_name(int32_t phase) {
+  return (phase >= 0 && phase < kNumPhases) ? kPhaseNames[phase] : "";
+}
+
+int dh_synchronize(dh_ctx *h) {
+  if (!h) return DH_EARG;
+  Ctx &c = h->c;
+  HIPCHK(hipStreamSynchronize(c.stream));
+  return DH_OK;
+}
+
+int dh_create_flow_chan(dh_ctx *h, int32_t ic_type, double ubulk0, uint64_t seed, double amp, double ymask) {
+  if (!h) return DH_EARG;
+  Ctx &c = h->c;
+  if (!c.grid_set) return fail(c, DH_ESTATE, "dh_set_grid first");
+  if (ic_type < 0 || ic_type > 3) return fail(c, DH_EARG, "IC_TYPE must be 0..3");
+  HIPCHK(launch_create_flow(c, ic_type, ubulk0, seed, amp, ymask));
+  HIPCHK(hipStreamSynchronize(c.stream));
+  return DH_OK;
+}
+
+int dh_create_th_chan(dh_ctx *h, int32_t ic_type) {
+  if (!h) return DH_EARG;
+  Ctx &c = h->c;
+  if (!c.grid_set) return fail(c, DH_ESTATE, "dh_set_grid first");
+  HIPCHK(launch_create_th(c, ic_type));
+  HIPCHK(hipStreamSynchronize(c.stream));
+  return DH_OK;
+}
+
+int dh_apply_bc_vel(dh_ctx *h) {
+  if (!h) return DH_EARG;
+  Ctx &c = h->c;
+  if (!c.grid_set) return fail(c, DH_ESTATE, "dh_set_grid first");
+  HIPCHK(launch_apply_bc_vel(c));
+  return DH_OK;
+}
+
+int dh_rem_div(dh_ctx *h) {
+  if (!h) return DH_EARG;
+  Ctx &c = h->c;
+  if (!c.grid_set) return fail(c, DH_ESTATE, "dh_set_grid first");
+  HIPCHK(launch_rem_div(c, -1));
+  return DH_OK;
+}
+
+int dh_fft_xz_to_fourier_dev(dh_ctx *h, const double *phys, double *four, int32_t nplanes) {
+  if (!h || !phys || !four || nplanes < 0) return DH_EARG;
+  Ctx &c = h->c;
+  if (nplanes == 0) return DH_OK;
+  HIPCHK(launch_fft_to_four(c, phys, reinterpret_cast<double2 *>(four), nplanes));
+  return DH_OK;
+}
+
+int dh_fft_xz_to_physical_dev(dh_ctx *h, const double *four, double *phys, int32_t nplanes) {
+  if (!h || !phys || !four || nplanes < 0) return DH_EARG;
+  Ctx &c = h->c;
+  if (nplanes == 0) return DH_OK;
+  HIPCHK(launch_fft_to_phys(c, reinterpret_cast<const double2 *>(four), phys, nplanes));
+  return DH_OK;
+}
+
+double *dh_field_dev(dh_ctx *h, int32_t which) {
+  if (!h) return nullptr;
+  Ctx &c = h->c;
+  const int nth = c.p.n_th;
+  const long long cfs = (long long)(c.g.ny + 2) * c.g.pf;
+  const long long fs = (long long)(c.g.ny + 2) * c.g.pp;
+  if (which >= 0 && which < 3) return reinterpret_cast<double *>(c.cu[which]);
+  if (which == 3) return reinterpret_cast<double *>(c.cp);
+  if (which >= 4 && which < 7) return reinterpret_cast<double *>(c.cf[which - 4]);
+  if (which == 7) return reinterpret_cast<double *>(c.phi);
+  if (which >= 8 && which < 8 + nth) return reinterpret_cast<double *>(c.cth + (which - 8) * cfs);
+  if (which >= 8 + nth && which < 8 + 2 * nth) return reinterpret_cast<double *>(c.cfth + (which - 8 - nth) * cfs);
+  if (which >= 100 && which < 100 + c.nfields) return c.uphys + (which - 100) * fs;
+  return nullptr;
+}
+
+void *dh_stream(dh_ctx *h) { return h ? (void *)h->c.stream : nullptr; }
+
+}  // extern "C"

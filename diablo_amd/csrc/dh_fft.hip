@@ -1,0 +1,118 @@
+// Plane transforms between the de-aliased Fourier layout and physical
+// planes: FFT_XZ_TO_PHYSICAL (mpi.f:1547-1606) and FFT_XZ_TO_FOURIER
+// (mpi.f:1477-1542) for a batch of y-planes and fields.  One workgroup owns
+// one x-z plane of one field; the whole plane lives in LDS between the z
+// and x passes, so each plane is read from and written to HBM once.
+#include "dh_fft_dev.h"
+
+namespace dh {
+
+struct PlaneBatch {
+  const double2 *four[kMaxFields];  // Fourier source/destination per field
+  double *phys[kMaxFields];         // physical destination/source per field
+  int j0;                           // first plane
+};
+
+// Fourier -> physical for planes j0 + blockIdx.x of field blockIdx.y.
+__global__ void __launch_bounds__(512) k_plane_inv(Geo g, PlaneBatch b) {
+  extern __shared__ double2 A[];
+  const int f = blockIdx.y;
+  const long long j = b.j0 + blockIdx.x;
+  plane_unpack(A, g, b.four[f] + j * g.pf);
+  plane_cols_inv(A, g);
+  plane_rows_inv(A, g);
+  double2 *dst = reinterpret_cast<double2 *>(b.phys[f] + j * g.pp);
+  const int n = g.nz * g.lw;
+  for (int e = threadIdx.x; e < n; e += blockDim.x) {
+    const int z = e / g.lw, m = e - z * g.lw;
+    dst[e] = A[inv_slot(g, z, m)];
+  }
+}
+
+// physical -> Fourier (x r2c with 1/NX and truncation, z c2c with 1/NZ and
+// packing of kz = -NKZ..-1 after 0..NKZ).
+__global__ void __launch_bounds__(512) k_plane_fwd(Geo g, PlaneBatch b) {
+  extern __shared__ double2 A[];
+  const int f = blockIdx.y;
+  const long long j = b.j0 + blockIdx.x;
+  const double2 *src = reinterpret_cast<const double2 *>(b.phys[f] + j * g.pp);
+  const int n = g.nz * g.lw;
+  for (int e = threadIdx.x; e < n; e += blockDim.x) A[e] = src[e];
+  __syncthreads();
+  plane_rows_fwd(A, g);
+  plane_cols_fwd(A, g);
+  double2 *dst = const_cast<double2 *>(b.four[f]) + j * g.pf;
+  for (int m = threadIdx.x; m < g.pf; m += blockDim.x) {
+    const int kzp = m / g.nkx1, kx = m - kzp * g.nkx1;
+    const double2 v = A[fwd_slot(g, kx, kzp)];
+    dst[m] = make_double2(v.x / g.nz, v.y / g.nz);
+  }
+}
+
+static hipError_t set_lds(const void *fn, int bytes) {
+  return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+}
+
+hipError_t launch_fft_to_phys(Ctx &c, const double2 *four, double *phys, int nplanes) {
+  PlaneBatch b{};
+  b.four[0] = four;
+  b.phys[0] = phys;
+  b.j0 = 0;
+  hipError_t e = set_lds((const void *)k_plane_inv, c.lds_bytes);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_plane_inv, dim3(nplanes, 1), dim3(c.plane_threads), c.lds_bytes, c.stream, c.g, b);
+  return hipGetLastError();
+}
+
+hipError_t launch_fft_to_four(Ctx &c, const double *phys, double2 *four, int nplanes) {
+  PlaneBatch b{};
+  b.four[0] = four;
+  b.phys[0] = const_cast<double *>(phys);
+  b.j0 = 0;
+  hipError_t e = set_lds((const void *)k_plane_fwd, c.lds_bytes);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_plane_fwd, dim3(nplanes, 1), dim3(c.plane_threads), c.lds_bytes, c.stream, c.g, b);
+  return hipGetLastError();
+}
+
+// All state fields (U1..3, TH) to physical space, planes 0..NY+1
+// (channel.f:386-399).
+hipError_t launch_state_to_phys(Ctx &c) {
+  PlaneBatch b{};
+  const long long fs = (long long)(c.g.ny + 2) * c.g.pp;
+  const long long cfs = (long long)(c.g.ny + 2) * c.g.pf;
+  for (int f = 0; f < c.nfields; ++f) {
+    b.four[f] = (f < 3) ? c.cu[f] : c.cth + (f - 3) * cfs;
+    b.phys[f] = c.uphys + f * fs;
+  }
+  b.j0 = 0;
+  hipLaunchKernelGGL(k_plane_inv, dim3(c.g.ny + 2, c.nfields), dim3(c.plane_threads), c.lds_bytes, c.stream, c.g,
+                     b);
+  return hipGetLastError();
+}
+
+// Fields f0..f1-1 (U1,U2,U3,TH...) back to Fourier space, planes 0..NY+1
+// (channel.f:1043-1050).
+hipError_t launch_fields_to_four(Ctx &c, int f0, int f1) {
+  PlaneBatch b{};
+  const long long fs = (long long)(c.g.ny + 2) * c.g.pp;
+  const long long cfs = (long long)(c.g.ny + 2) * c.g.pf;
+  for (int f = f0; f < f1; ++f) {
+    b.four[f - f0] = (f < 3) ? c.cu[f] : c.cth + (f - 3) * cfs;
+    b.phys[f - f0] = c.uphys + f * fs;
+  }
+  b.j0 = 0;
+  if (f1 <= f0) return hipSuccess;
+  hipLaunchKernelGGL(k_plane_fwd, dim3(c.g.ny + 2, f1 - f0), dim3(c.plane_threads), c.lds_bytes, c.stream, c.g, b);
+  return hipGetLastError();
+}
+
+hipError_t launch_state_to_four(Ctx &c) { return launch_fields_to_four(c, 0, c.nfields); }
+
+hipError_t prepare_fft_kernels(Ctx &c) {
+  hipError_t e = set_lds((const void *)k_plane_inv, c.lds_bytes);
+  if (e != hipSuccess) return e;
+  return set_lds((const void *)k_plane_fwd, c.lds_bytes);
+}
+
+}  // namespace dh

@@ -1,0 +1,264 @@
+// Initial-condition pipeline on the device: CREATE_FLOW_CHAN
+// (set_ics.f:2-200), CREATE_TH_CHAN (set_ics.f:204-273) and
+// APPLY_BC_VEL_LOWER/UPPER (channel.f:2285-2492).  The compiler's
+// RANDOM_NUMBER is replaced by a counter-based generator (SplitMix64 +
+// Box-Muller), so every seed gives the same field on every device.
+#include "dh_fft_dev.h"
+
+namespace dh {
+
+__device__ __forceinline__ unsigned long long splitmix64(unsigned long long x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+__device__ double gauss(unsigned long long seed, unsigned long long counter) {
+  const unsigned long long a = splitmix64(seed * 0x100000001B3ull ^ splitmix64(2 * counter));
+  const unsigned long long b = splitmix64(seed * 0x100000001B3ull ^ splitmix64(2 * counter + 1));
+  const double u1 = ((double)(a >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+  const double u2 = ((double)(b >> 11)) * (1.0 / 9007199254740992.0);
+  return sqrt(-2.0 * log(u1)) * cos(2.0 * 3.141592653589793 * u2);
+}
+
+struct IcArgs {
+  Geo g;
+  double *uphys;
+  long long fs;
+  const double *gyf;
+  int ic_type;
+  double ubulk0, amp, ymask, lx;
+  unsigned long long seed;
+  int n_th;
+  double th_min_c1[DH_MAX_TH], th_max_c1[DH_MAX_TH];
+  int th_min[DH_MAX_TH], th_max[DH_MAX_TH];
+};
+
+// Physical U1..U3 of set_ics.f:26-108 on planes 0..NY+1 (profile on
+// 0..NY, noise on 0..NY+1 where |GYF| < ymask).
+__global__ void k_ic_velocity(IcArgs a) {
+  const Geo &g = a.g;
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long n = (long long)(g.ny + 2) * g.pp;
+  if (e >= n) return;
+  const int J = (int)(e / g.pp);
+  const long long r = e - (long long)J * g.pp;
+  const int K = (int)(r / g.nx), i = (int)(r - (long long)K * g.nx);
+  const double y = a.gyf[J];
+  double u1 = 0.0;
+  if (J <= g.ny) {
+    if (a.ic_type == 0)
+      u1 = (3. / 2.) * a.ubulk0 * (1.0 - y * y);
+    else if (a.ic_type == 2)
+      u1 = y;
+    else if (a.ic_type == 3)
+      u1 = tanh(y);
+  }
+  double u2 = 0.0, u3 = 0.0;
+  if (a.ymask <= 0.0 || fabs(y) < a.ymask) {
+    const unsigned long long c =
+        3ull * ((unsigned long long)i + (unsigned long long)g.nx * ((unsigned long long)K + (unsigned long long)g.nz * J));
+    u1 += a.amp * gauss(a.seed, c);
+    u2 += a.amp * gauss(a.seed, c + 1);
+    u3 += a.amp * gauss(a.seed, c + 2);
+  }
+  a.uphys[e] = u1;
+  a.uphys[a.fs + e] = u2;
+  a.uphys[2 * a.fs + e] = u3;
+}
+
+// Physical TH of set_ics.f:214-262 on planes 1..NY (0 and NY+1 zero).
+__global__ void k_ic_scalar(IcArgs a) {
+  const Geo &g = a.g;
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long n = (long long)(g.ny + 2) * g.pp;
+  if (e >= n) return;
+  const int J = (int)(e / g.pp);
+  const long long r = e - (long long)J * g.pp;
+  const int K = (int)(r / g.nx), i = (int)(r - (long long)K * g.nx);
+  (void)K;
+  const double y = a.gyf[J];
+  for (int nn = 0; nn < a.n_th; ++nn) {
+    double v = 0.0;
+    if (J >= 1 && J <= g.ny) {
+      if (a.ic_type == 0) {
+        const double PI = 3.14159265358979323846;
+        const double gx = (i * a.lx) / g.nx;
+        v = sin(2.0 * PI * gx / a.lx) / (4.0 * PI * PI);
+      } else if (a.ic_type == 1 || a.ic_type == 2) {
+        if (a.th_min[nn] == 0 && a.th_max[nn] == 0)
+          v = (y <= 2.0) ? (a.th_max_c1[nn] - a.th_min_c1[nn]) * (y + 1.) / 2.0 + a.th_min_c1[nn] : a.th_max_c1[nn];
+        else if (a.th_min[nn] == 1 && a.th_max[nn] == 1)
+          v = a.th_max_c1[nn] * y;
+      } else {
+        v = tanh(y);
+      }
+    }
+    a.uphys[(3 + nn) * a.fs + e] = v;
+  }
+}
+
+struct BcArgs {
+  Geo g;
+  double2 *cu1, *cu2, *cu3;
+  const double *dy, *dyf;
+  int ubmin, vbmin, wbmin, ubmax, vbmax, wbmax;
+  double uc1min, vc1min, wc1min, uc1max, vc1max, wc1max;
+};
+
+// APPLY_BC_VEL_LOWER/UPPER in Fourier space, one thread per mode (RANKZ=0).
+__global__ void k_apply_bc_vel(BcArgs a) {
+  const Geo &g = a.g;
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= g.pf) return;
+  const long long pf = g.pf;
+  const int NY = g.ny;
+  const bool mean = (m == 0);
+  const double2 zero = make_double2(0.0, 0.0);
+  double2 *U1 = a.cu1 + m, *U2 = a.cu2 + m, *U3 = a.cu3 + m;
+  auto at = [&](double2 *F, int j) -> double2 & { return F[pf * j]; };
+  // lower wall
+  if (a.ubmin == 0) {
+    at(U1, 0) = zero;
+    at(U1, 1) = zero;
+    if (mean) at(U1, 1) = at(U1, 0) = make_double2(a.uc1min, 0.0);
+  } else {
+    at(U1, 1) = at(U1, 2);
+    at(U1, 0) = at(U1, 1);
+    if (mean) {
+      at(U1, 1) = make_double2(at(U1, 2).x - a.dy[2] * a.uc1min, at(U1, 2).y);
+      at(U1, 0) = make_double2(at(U1, 1).x - a.dy[1] * a.uc1min, at(U1, 1).y);
+    }
+  }
+  if (a.wbmin == 0) {
+    at(U3, 0) = zero;
+    at(U3, 1) = zero;
+    if (mean) at(U3, 1) = at(U3, 0) = make_double2(a.wc1min, 0.0);
+  } else {
+    at(U3, 1) = at(U3, 2);
+    at(U3, 0) = at(U3, 1);
+    if (mean) {
+      at(U3, 1) = make_double2(at(U3, 2).x - a.dy[2] * a.wc1min, at(U3, 2).y);
+      at(U3, 0) = make_double2(at(U3, 1).x - a.dy[1] * a.wc1min, at(U3, 1).y);
+    }
+  }
+  if (a.vbmin == 0) {
+    const double2 v2 = at(U2, 2);
+    at(U2, 1) = make_double2(2.0 * a.vc1min - v2.x, -v2.y);
+    at(U2, 0) = at(U2, 1);
+  } else {
+    at(U2, 1) = at(U2, 2);
+    at(U2, 0) = at(U2, 1);
+    if (mean) {
+      at(U2, 1) = make_double2(at(U2, 2).x - a.dyf[1] * a.vc1min, at(U2, 2).y);
+      at(U2, 0) = make_double2(at(U2, 1).x - a.dyf[1] * a.vc1min, at(U2, 1).y);
+    }
+  }
+  // upper wall
+  if (a.ubmax == 0) {
+    at(U1, NY) = zero;
+    at(U1, NY + 1) = zero;
+    if (mean) at(U1, NY) = at(U1, NY + 1) = make_double2(a.uc1max, 0.0);
+  } else {
+    at(U1, NY) = at(U1, NY - 1);
+    at(U1, NY + 1) = at(U1, NY);
+    if (mean) {
+      at(U1, NY) = make_double2(at(U1, NY - 1).x + a.dy[NY] * a.uc1max, at(U1, NY - 1).y);
+      at(U1, NY + 1) = make_double2(at(U1, NY).x + a.dy[NY] * a.uc1max, at(U1, NY).y);
+    }
+  }
+  if (a.wbmax == 0) {
+    at(U3, NY) = zero;
+    at(U3, NY + 1) = zero;
+    if (mean) {
+      at(U3, NY) = make_double2(a.wc1max, 0.0);
+      at(U3, NY + 1) = zero;  // channel.f:2445 overrides the ghost value
+    }
+  } else {
+    at(U3, NY) = at(U3, NY - 1);
+    at(U3, NY + 1) = at(U3, NY);
+    if (mean) {
+      at(U3, NY) = make_double2(at(U3, NY - 1).x + a.dy[NY] * a.wc1max, at(U3, NY - 1).y);
+      at(U3, NY + 1) = make_double2(at(U3, NY).x + a.dy[NY] * a.wc1max, at(U3, NY).y);
+    }
+  }
+  if (a.vbmax == 0) {
+    if (mean) at(U2, NY + 1) = make_double2(2.0 * a.vc1max - at(U2, NY).x, -at(U2, NY).y);
+  } else {
+    at(U2, NY) = at(U2, NY - 1);
+    at(U2, NY + 1) = at(U2, NY);
+    if (mean) at(U2, NY + 1) = make_double2(at(U2, NY).x + a.dy[NY] * a.vc1max, at(U2, NY).y);
+  }
+}
+
+__global__ void k_fill_zero(double2 *p, long long n) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) p[e] = make_double2(0.0, 0.0);
+}
+
+static IcArgs ic_args(Ctx &c) {
+  IcArgs a{};
+  a.g = c.g;
+  a.uphys = c.uphys;
+  a.fs = (long long)(c.g.ny + 2) * c.g.pp;
+  a.gyf = c.d_gyf;
+  a.lx = c.p.lx;
+  a.n_th = c.p.n_th;
+  for (int n = 0; n < c.p.n_th; ++n) {
+    a.th_min[n] = c.p.th_bc_ymin[n];
+    a.th_max[n] = c.p.th_bc_ymax[n];
+    a.th_min_c1[n] = c.p.th_bc_ymin_c1[n];
+    a.th_max_c1[n] = c.p.th_bc_ymax_c1[n];
+  }
+  return a;
+}
+
+hipError_t launch_apply_bc_vel(Ctx &c) {
+  BcArgs b{};
+  b.g = c.g;
+  b.cu1 = c.cu[0];
+  b.cu2 = c.cu[1];
+  b.cu3 = c.cu[2];
+  b.dy = c.d_dy;
+  b.dyf = c.d_dyf;
+  const dh_params &p = c.p;
+  b.ubmin = p.u_bc_ymin; b.vbmin = p.v_bc_ymin; b.wbmin = p.w_bc_ymin;
+  b.ubmax = p.u_bc_ymax; b.vbmax = p.v_bc_ymax; b.wbmax = p.w_bc_ymax;
+  b.uc1min = p.u_bc_ymin_c1; b.vc1min = p.v_bc_ymin_c1; b.wc1min = p.w_bc_ymin_c1;
+  b.uc1max = p.u_bc_ymax_c1; b.vc1max = p.v_bc_ymax_c1; b.wc1max = p.w_bc_ymax_c1;
+  hipLaunchKernelGGL(k_apply_bc_vel, dim3((c.g.pf + 63) / 64), dim3(64), 0, c.stream, b);
+  return hipGetLastError();
+}
+
+hipError_t launch_create_flow(Ctx &c, int ic_type, double ubulk0, unsigned long long seed, double amp, double ymask) {
+  IcArgs a = ic_args(c);
+  a.ic_type = ic_type;
+  a.ubulk0 = ubulk0;
+  a.seed = seed;
+  a.amp = amp;
+  a.ymask = ymask;
+  const long long n = (long long)(c.g.ny + 2) * c.g.pp;
+  hipLaunchKernelGGL(k_ic_velocity, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if ((e = launch_fields_to_four(c, 0, 3)) != hipSuccess) return e;
+  const long long cf = (long long)(c.g.ny + 2) * c.g.pf;
+  hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)((cf + 255) / 256)), dim3(256), 0, c.stream, c.cp, cf);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if ((e = launch_apply_bc_vel(c)) != hipSuccess) return e;
+  return launch_rem_div(c, -1);
+}
+
+hipError_t launch_create_th(Ctx &c, int ic_type) {
+  if (c.p.n_th == 0) return hipSuccess;
+  IcArgs a = ic_args(c);
+  a.ic_type = ic_type;
+  const long long n = (long long)(c.g.ny + 2) * c.g.pp;
+  hipLaunchKernelGGL(k_ic_scalar, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return launch_fields_to_four(c, 3, c.nfields);
+}
+
+}  // namespace dh

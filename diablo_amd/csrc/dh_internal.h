@@ -1,0 +1,121 @@
+// Internal definitions shared by the HIP sources of libdiablo_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/diablo_hip.h"
+
+namespace dh {
+
+constexpr int kMaxStages = 8;
+constexpr int kMaxFields = 3 + DH_MAX_TH;  // U1,U2,U3,TH(1..N_TH)
+
+// Radix plan of one 1-D transform length (in-place DIF, radices 2,3,4,8).
+struct FftPlan {
+  int n;
+  int nst;
+  int rad[kMaxStages];
+};
+
+// Geometry and transform tables of one grid, passed by value to kernels.
+struct Geo {
+  int nx, ny, nz;       // grid_def
+  int nkx, nkz, tnkz;   // header:21-28
+  int nkx1, nkz1;       // de-aliased extents NKX+1, TNKZ+1
+  int pf;               // Fourier plane elements nkx1*nkz1
+  int lw;               // LDS row width in complex elements = NX/2
+  long long pp;         // physical plane elements NX*NZ
+  FftPlan prow, pcol;   // row c2c (length NX/2), column c2c (length NZ)
+  const double2 *tw_row;   // exp(-2 pi i m/(NX/2)), m < NX/2
+  const double2 *tw_col;   // exp(-2 pi i m/NZ),     m < NZ
+  const double2 *tw_real;  // exp(-2 pi i k/NX),      k <= NX/2
+  const double *kx, *kz, *kx2, *kz2;  // wavenumbers (fft.f:77-100), de-aliased
+  const double *dy, *dyf;             // grid spacing (0:NY+1)
+  const int *posr;  // slot of index k after a row transform (length NX/2)
+  const int *posz;  // slot of index k after a column transform (length NZ)
+};
+
+// Device-resident clock (/RUNTIME_VARS/ + /RKW3_PARAMS/).
+struct Clock {
+  double time;
+  double dt;
+  double h_bar[3];
+  double beta_bar[3];
+  double zeta_bar[3];
+  double h_cp;                      // H_BAR(3) of the substep in which COURANT ran (channel.f:80)
+  unsigned long long cfl_min_bits;  // running CFL minimum (non-negative double bits)
+  int time_step;
+  int pad;
+};
+
+// LU factors of the y-implicit (Crank-Nicolson) matrices, one set per RK
+// substep and field; the matrices depend only on j when LES is off.
+struct RealFactors {
+  // [rk 0..2][field][j] : multiplier A', pivot B', upper C, per row
+  double *ap, *bp, *cc;
+  // [field][j] : 1 if row j takes a constant right-hand side (boundary row)
+  int *isbc;
+  double *vconst;  // [field][j]
+};
+
+struct Ctx {
+  dh_params p;
+  Geo g;
+  int nfields;  // 3 + n_th
+  int jstart, jend, jstart_th[DH_MAX_TH], jend_th[DH_MAX_TH];
+  int wlo[kMaxFields], whi[kMaxFields];  // solution write-back rows per field
+  int lds_bytes;
+  int plane_threads;
+  hipStream_t stream = nullptr;
+  std::string err;
+  bool grid_set = false;
+  // host copies
+  std::vector<double> gy, gyf, dy, dyf;
+  double dt_cap;  // courant.f:15-31 limits that do not depend on U
+  // device buffers
+  double2 *tw_row = nullptr, *tw_col = nullptr, *tw_real = nullptr;
+  int *posr = nullptr, *posz = nullptr;
+  double *kx = nullptr, *kz = nullptr, *kx2 = nullptr, *kz2 = nullptr;
+  double *d_dy = nullptr, *d_dyf = nullptr, *d_gyf = nullptr;
+  double2 *cu[3] = {}, *cp = nullptr, *cf[3] = {}, *cth = nullptr, *cfth = nullptr;
+  double2 *phi = nullptr;       // CR1 after REM_DIV
+  double *uphys = nullptr;      // [nfields][NY+2][NZ][NX]: U1,U2,U3,TH
+  double *rphys = nullptr;      // [nfields][NY+2][NZ][NX]: R1,R2,R3,RTH
+  double *rd_bp = nullptr;      // REM_DIV scratch pivots [NY+2][pf]
+  double2 *rd_g = nullptr;      // REM_DIV scratch rhs    [NY+2][pf]
+  Clock *clock = nullptr;
+  RealFactors fac{};
+  double *rd_l = nullptr, *rd_u = nullptr;  // 1/(DY DYF) tables for REM_DIV
+  // graph of one full time step
+  hipGraphExec_t step_graph = nullptr;
+  bool graph_ok = false;
+};
+
+// ---- launchers (dh_fft.hip) ----
+hipError_t prepare_fft_kernels(Ctx &c);
+hipError_t launch_fft_to_phys(Ctx &c, const double2 *four, double *phys, int nplanes);
+hipError_t launch_fft_to_four(Ctx &c, const double *phys, double2 *four, int nplanes);
+hipError_t launch_state_to_phys(Ctx &c);
+hipError_t launch_state_to_four(Ctx &c);
+hipError_t launch_fields_to_four(Ctx &c, int f0, int f1);
+// ---- launchers (dh_ics.hip) ----
+hipError_t launch_apply_bc_vel(Ctx &c);
+hipError_t launch_create_flow(Ctx &c, int ic_type, double ubulk0, unsigned long long seed, double amp, double ymask);
+hipError_t launch_create_th(Ctx &c, int ic_type);
+// ---- launchers (dh_step.hip) ----
+hipError_t prepare_step_kernels(Ctx &c);
+hipError_t launch_substep(Ctx &c, int rk);            // rk 0..2
+constexpr int kNumPhases = 6;
+extern const char *const kPhaseNames[kNumPhases];
+hipError_t launch_phase(Ctx &c, int rk, int phase);
+hipError_t launch_rem_div(Ctx &c, int rk_for_cp);     // rk_for_cp<0: no CP update
+hipError_t launch_step_begin(Ctx &c);
+hipError_t launch_end_step(Ctx &c);
+void build_real_factors_host(const Ctx &c, double dt, std::vector<double> &ap, std::vector<double> &bp,
+                             std::vector<double> &cc, std::vector<int> &isbc, std::vector<double> &vconst);
+
+}  // namespace dh

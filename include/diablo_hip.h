@@ -1,0 +1,162 @@
+/*
+ * diablo_hip.h -- C ABI of libdiablo_hip.so, the MI355X (gfx950) implementation
+ * of DIABLO's channel timestep RK_CHAN_1 and its callees.
+ *
+ * Each entry point names the reference interface it replaces
+ * (paths relative to diablo_2.0/for/ of vvslaroche/DIABLO).
+ *
+ * Conventions
+ *   - All reals are IEEE double, all integers int32.
+ *   - Host arrays passed to dh_upload/dh_download use the reference's Fortran
+ *     layout (first index fastest, lower bound 0):
+ *       Fourier fields  CA(0:NXP, 0:NZ+1, 0:NY+1) COMPLEX*16, NXP = NX/2
+ *       physical fields  A(0:NX+1, 0:NZ+1, 0:NY+1) REAL*8
+ *       scalars add a trailing (1:N_TH) index       (header:212-258).
+ *     The library copies; it never keeps a host pointer.
+ *   - Entry points whose names end in _dev take device pointers in the
+ *     library's own device layout (below) and run on the context's stream.
+ *   - Return value: 0 on success, a negative DH_E* code otherwise; the text
+ *     of the last error is returned by dh_last_error().
+ *   - One context per process and device; not thread-safe.
+ *
+ * Device layout (one GPU, replaces the NPROCY x NPROCZ decomposition of
+ * mpi.f:868-1081):
+ *   Fourier  [NY+2][TNKZ+1][NKX+1] complex double (kx fastest): only the
+ *            de-aliased modes I<=NKX, K<=TNKZ; the reference keeps the rest
+ *            exactly zero (mpi.f:1501-1506) or never reads them.
+ *   Physical [NY+2][NZ][NX] double.
+ */
+#ifndef DIABLO_HIP_H
+#define DIABLO_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DH_MAX_TH 4
+
+enum {
+  DH_OK = 0,
+  DH_EARG = -1,     /* invalid argument / unsupported configuration */
+  DH_EHIP = -2,     /* HIP runtime error */
+  DH_ESTATE = -3,   /* call out of order (e.g. no grid yet) */
+  DH_ELAYOUT = -4,  /* host array violates the de-aliased layout */
+};
+
+/* Run-time parameters: grid_def (NX,NY,NZ,N_TH), /INPUT/, /SCALAR_VARS/,
+ * /GEOPHYS_VARS/, /OSCILL_VARS/, /VARIABLE_VISC/, /DRHODX_VARS/ of
+ * header:35-190.  BC types: 0 Dirichlet, 1 Neumann (channel.f:1660-2282). */
+typedef struct dh_params {
+  int32_t nx, ny, nz, n_th;
+  double nu, beta, lx, lz, nu_v_scale;
+  double delta_t, cfl;
+  int32_t variable_dt, update_dt;
+  int32_t f_type;
+  double px0, omega0, amp_omega0;
+  double i_ro, grav_x, grav_y, grav_z;
+  int32_t u_bc_ymin, v_bc_ymin, w_bc_ymin, u_bc_ymax, v_bc_ymax, w_bc_ymax;
+  double u_bc_ymin_c1, v_bc_ymin_c1, w_bc_ymin_c1;
+  double u_bc_ymax_c1, v_bc_ymax_c1, w_bc_ymax_c1;
+  int32_t th_bc_ymin[DH_MAX_TH], th_bc_ymax[DH_MAX_TH];
+  double th_bc_ymin_c1[DH_MAX_TH], th_bc_ymax_c1[DH_MAX_TH];
+  double ri[DH_MAX_TH], pr[DH_MAX_TH];
+  double drhodx[DH_MAX_TH], drhodz[DH_MAX_TH];
+  int32_t create_new_flow;
+} dh_params;
+
+/* Host-side state in Fortran layout (/GLOBAL_VARS/, header:260-261).
+ * Any pointer may be NULL to skip that field.  cth/cfth/th hold N_TH
+ * consecutive fields. */
+typedef struct dh_host_state {
+  double *cu1, *cu2, *cu3, *cp; /* COMPLEX*16 (0:NXP,0:NZ+1,0:NY+1) */
+  double *cf1, *cf2, *cf3;      /* RK carry of the explicit terms */
+  double *cth, *cfth;           /* COMPLEX*16 (...,1:N_TH) */
+  double *th;                   /* REAL*8 (0:NX+1,0:NZ+1,0:NY+1,1:N_TH) */
+} dh_host_state;
+
+typedef struct dh_ctx dh_ctx;
+
+/* INIT_FFT (fft.f:55-143: wavenumbers, transform set-up), INIT_CHAN_MPI
+ * index rules on one rank (mpi.f:1013-1081) and the RKW3 constants
+ * (diablo_io.f:116-124).  Allocates all device state on the current device. */
+int dh_create(const dh_params *p, dh_ctx **out);
+/* Frees the device state (END_RUN_MPI, mpi.f:1668-1679, has no device part). */
+int dh_destroy(dh_ctx *ctx);
+/* Text of the last error on this context ("" if none). */
+const char *dh_last_error(dh_ctx *ctx);
+
+/* /GRID_PARAMS/ (header:205-207): GY, GYF, DY, DYF each (0:NY+1), as built by
+ * CREATE_GRID_CHAN (channel.f:1564-1655) + ReadGridHDF5 (hdf5.f:548-710). */
+int dh_set_grid(dh_ctx *ctx, const double *gy, const double *gyf, const double *dy, const double *dyf);
+
+/* Copy the host state in / out (the COMMON arrays RK_CHAN_1 reads and
+ * writes, channel.f:62-72).  Upload fails with DH_ELAYOUT if a mode outside
+ * the de-aliased set is non-zero in CU*, CP, CF*, CTH or CFTH. */
+int dh_upload(dh_ctx *ctx, const dh_host_state *st);
+int dh_download(dh_ctx *ctx, dh_host_state *st);
+
+/* /RUNTIME_VARS/ TIME, TIME_STEP and DELTA_T (header:115-117); setting
+ * DELTA_T recomputes H_BAR(1:3) as diablo_io.f:116-118.  The device keeps
+ * its own copy, which COURANT (courant.f:1-65) and diablo.f:109 update. */
+int dh_set_clock(dh_ctx *ctx, double time, double delta_t, int32_t time_step);
+int dh_get_clock(dh_ctx *ctx, double *time, double *delta_t, double *h_bar3, int32_t *time_step);
+
+/* RK_CHAN_1 (channel.f:62-1073), one RK3/CN substep rk_step in 1..3 on the
+ * device state, LES off, empty USER_RHS hooks.  COURANT runs inside when
+ * VARIABLE_DT, rk_step==3 and MOD(TIME_STEP,UPDATE_DT)==0 (channel.f:1037).
+ * Asynchronous on the context stream. */
+int dh_rk_substep(dh_ctx *ctx, int32_t rk_step);
+/* nsteps iterations of the loop body of diablo.f:66-110 without the I/O
+ * hooks: TIME_STEP+=1, RK_CHAN_1 for RK_STEP=1..3, TIME=TIME+DELTA_T.
+ * Replays a captured hipGraph of one step after the first call. */
+int dh_time_step(dh_ctx *ctx, int32_t nsteps);
+/* Per-phase timing of one substep (the reference's only timer is the
+ * wall-clock "Seconds per Iteration" of diablo.f:123-127): runs substep
+ * rk_step nrep times with HIP events between its kernels on the context
+ * stream and returns the mean milliseconds of each phase in ms[0..5]
+ * (nms >= 6); the phases are named by dh_phase_name().  Advances the state. */
+int dh_profile_substep(dh_ctx *ctx, int32_t rk_step, int32_t nrep, double *ms, int32_t nms);
+const char *dh_phase_name(int32_t phase);
+/* Wait for all work queued on the context. */
+int dh_synchronize(dh_ctx *ctx);
+
+/* REM_DIV_CHAN (channel.f:1103-1214 with APPLY_BC_REM_DIV_MPI
+ * mpi.f:1239-1278 and THOMAS_*_COMPLEX_MPI mpi.f:708-865) on the device
+ * state; leaves phi in the context's phi buffer. */
+int dh_rem_div(dh_ctx *ctx);
+
+/* CREATE_FLOW_CHAN (set_ics.f:2-200) on the device: IC_TYPE 0 (laminar
+ * Poiseuille, 3/2 UBULK0 (1-GYF^2)), 2 (Couette), 3 (tanh shear layer), plus
+ * Gaussian noise of standard deviation amp on U1..U3 wherever |GYF| < ymask
+ * (ymask <= 0: everywhere), drawn from a counter-based generator seeded by
+ * `seed` in place of the compiler's RANDOM_NUMBER; then the transform to
+ * Fourier space, APPLY_BC_VEL_MPI and REM_DIV_CHAN as set_ics.f:115-186.
+ * CP is set to zero. */
+int dh_create_flow_chan(dh_ctx *ctx, int32_t ic_type, double ubulk0, uint64_t seed, double amp, double ymask);
+/* CREATE_TH_CHAN (set_ics.f:204-273) for IC_TYPE 0..4 on the device. */
+int dh_create_th_chan(dh_ctx *ctx, int32_t ic_type);
+/* APPLY_BC_VEL_MPI (mpi.f:1317-1331 -> channel.f:2285-2492): ghost and wall
+ * planes of CU1..3 in Fourier space. */
+int dh_apply_bc_vel(dh_ctx *ctx);
+
+/* FFT_XZ_TO_FOURIER (mpi.f:1477-1542) of nplanes consecutive physical
+ * planes [nplanes][NZ][NX] into de-aliased Fourier planes
+ * [nplanes][TNKZ+1][NKX+1] (device pointers). */
+int dh_fft_xz_to_fourier_dev(dh_ctx *ctx, const double *phys, double *four, int32_t nplanes);
+/* FFT_XZ_TO_PHYSICAL (mpi.f:1547-1606), the inverse direction. */
+int dh_fft_xz_to_physical_dev(dh_ctx *ctx, const double *four, double *phys, int32_t nplanes);
+
+/* Device pointer of one state field in the device layout, for tests and
+ * benchmarks: which = 0..2 CU1..3, 3 CP, 4..6 CF1..3, 7 phi (CR1 after
+ * REM_DIV), 8 + n CTH(n), 8 + N_TH + n CFTH(n); 100+c physical U(c) /
+ * TH (c>=3) as left by the last substep's solve. */
+double *dh_field_dev(dh_ctx *ctx, int32_t which);
+/* HIP stream (hipStream_t) the context runs on. */
+void *dh_stream(dh_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,98 @@
+"""Shared helpers for the tests: oracle set-up from a deck and field
+comparison on the de-aliased modes."""
+import numpy as np
+
+from diablo_amd import configs
+
+STATE_KEYS = ("cu1", "cu2", "cu3", "cp", "cf1", "cf2", "cf3", "cth", "cfth", "th")
+
+
+def make_oracle(dor, name, **kw):
+    p, gy, ic = configs.CONFIGS[name](**kw)
+    o = dor.Oracle(p)
+    o.set_grid_gy(gy)
+    if p["n_th"] > 0 and ic.get("th_ic_type") is not None:
+        o.create_th(ic["th_ic_type"])
+    o.create_flow(ic["ic_type"], ic["seed"], ic["amp"], ic["ymask"], ic["ubulk0"])
+    return o, p, gy, ic
+
+
+def oracle_state(dor, o):
+    """Copy of the oracle's state in the host layout of diablo_amd.channel."""
+    st = {
+        "cu1": o.carr(dor.CU1).copy(), "cu2": o.carr(dor.CU2).copy(), "cu3": o.carr(dor.CU3).copy(),
+        "cp": o.carr(dor.CP).copy(), "cf1": o.carr(dor.CF1).copy(), "cf2": o.carr(dor.CF2).copy(),
+        "cf3": o.carr(dor.CF3).copy(),
+    }
+    nth = o.n_th
+    cshape = st["cu1"].shape
+    if nth > 0:
+        st["cth"] = o.carr(dor.CTH).reshape((nth,) + cshape).copy()
+        st["cfth"] = o.carr(dor.CFTH).reshape((nth,) + cshape).copy()
+        st["th"] = o.rarr(dor.TH).reshape((nth, o.ny + 2, o.nz + 2, o.nx + 2)).copy()
+    else:
+        st["cth"] = np.zeros((1,) + cshape, np.complex128)
+        st["cfth"] = np.zeros((1,) + cshape, np.complex128)
+        st["th"] = np.zeros((1, o.ny + 2, o.nz + 2, o.nx + 2))
+    # only the de-aliased block is state; the rest of the reference arrays is
+    # scratch (K > TNKZ holds unnormalised transform residue, mpi.f:1530-1539)
+    nk1, nz1 = o.nkx + 1, o.tnkz + 1
+    for k in ("cu1", "cu2", "cu3", "cp", "cf1", "cf2", "cf3"):
+        st[k][:, nz1:, :] = 0
+    for k in ("cth", "cfth"):
+        st[k][:, :, nz1:, :] = 0
+    return st
+
+
+def dealiased(a, nkx, tnkz):
+    return a[..., : tnkz + 1, : nkx + 1]
+
+
+def rel_l2(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    den = np.linalg.norm(b.ravel())
+    num = np.linalg.norm((a - b).ravel())
+    if den == 0.0:
+        return num
+    return num / den
+
+
+def perturb(dor, o, eps, seed=5):
+    """Multiply the de-aliased velocity modes by (1 + eps*N(0,1))."""
+    rng = np.random.default_rng(seed)
+    for w in (dor.CU1, dor.CU2, dor.CU3):
+        a = o.carr(w)
+        blk = a[:, : o.tnkz + 1, : o.nkx + 1]
+        blk *= 1 + eps * rng.standard_normal(blk.shape)
+
+
+def field_errors(dor, o, st_test, st_ref=None, keys=("cu1", "cu2", "cu3", "cp")):
+    """Relative L2 of st_test against the oracle's current state (or st_ref)
+    on the de-aliased modes; scalars in Fourier and physical space."""
+    so = st_ref if st_ref is not None else oracle_state(dor, o)
+    errs = {}
+    for k in keys:
+        errs[k] = rel_l2(dealiased(st_test[k], o.nkx, o.tnkz), dealiased(so[k], o.nkx, o.tnkz))
+    for n in range(o.n_th):
+        errs[f"cth{n}"] = rel_l2(dealiased(st_test["cth"][n], o.nkx, o.tnkz), dealiased(so["cth"][n], o.nkx, o.tnkz))
+        errs[f"th{n}"] = rel_l2(st_test["th"][n][:, : o.nz, : o.nx], so["th"][n][:, : o.nz, : o.nx])
+    return errs
+
+
+def roundoff_floor(dor, name, advance, keys=("cu1", "cu2", "cu3", "cp"), eps=1e-16):
+    """Per-field relative L2 change of the oracle's own result when its
+    input velocity is perturbed at the 1e-16 level: the spread any two
+    correct FP64 implementations with differently ordered transforms show.
+    `advance(o)` runs the same work as the test."""
+    o1, *_ = make_oracle(dor, name)
+    o2, *_ = make_oracle(dor, name)
+    perturb(dor, o2, eps)
+    advance(o1)
+    advance(o2)
+    return field_errors(dor, o1, oracle_state(dor, o2), keys=keys)
+
+
+def within(errs, floor, bar, factor=30.0):
+    """Fields whose error exceeds max(bar, factor*floor)."""
+    return {k: (v, floor.get(k, 0.0)) for k, v in errs.items() if not (v <= max(bar, factor * floor.get(k, 0.0)))}

@@ -1,0 +1,135 @@
+"""Parity of the HIP path with the CPU oracle (needs an MI355X).
+
+All work is FP64.  Each field must agree with the oracle to relative L2
+<= max(bar, 30 x floor), where `floor` is the oracle's own spread: the
+relative L2 change of its result when its input velocity is perturbed at
+the 1e-16 level (what two correct FP64 implementations with differently
+ordered FFTs differ by).  bar = 1e-12 for single substeps and plane
+operations, 1e-10 after 10 RK3 steps.  Fields computed from O(1) data but
+themselves small (the pressure of a 1e-4 perturbation) have floors above
+the bar; all others are held to the bar itself."""
+import numpy as np
+import pytest
+
+from diablo_amd import channel
+from tests import helpers
+
+pytestmark = pytest.mark.gpu
+
+ALL = ("cu1", "cu2", "cu3", "cp", "cf1", "cf2", "cf3")
+STATE = ("cu1", "cu2", "cu3", "cp")
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def _fmt(d):
+    return {k: f"{v:.1e}" for k, v in d.items()}
+
+
+@pytest.fixture(scope="module")
+def dor():
+    from oracle import dor as d
+    d.load()
+    return d
+
+
+def _pair(dor, name):
+    """Reference oracle, perturbed oracle (for the floor), GPU context."""
+    o, p, gy, ic = helpers.make_oracle(dor, name)
+    o2, *_ = helpers.make_oracle(dor, name)
+    helpers.perturb(dor, o2, 1e-16)
+    ch = channel.Channel(p, gy)
+    ch.upload(helpers.oracle_state(dor, o))
+    return o, o2, ch, p
+
+
+@pytest.mark.parametrize("name", ["small_case", "kh_shear_layer", "channel_re180"])
+def test_one_substep_each_rk(dor, name):
+    _say(f"[parity] substeps {name}")
+    o, o2, ch, p = _pair(dor, name)
+    for x in (o, o2):
+        x.set_clock(0.0, p["delta_t"], 1)
+    ch.set_clock(0.0, p["delta_t"], 1)
+    for rk in (1, 2, 3):
+        o.rk_chan_1(rk)
+        o2.rk_chan_1(rk)
+        ch.rk_substep(rk)
+        st = ch.download()
+        errs = helpers.field_errors(dor, o, st, keys=ALL)
+        floor = helpers.field_errors(dor, o, helpers.oracle_state(dor, o2), keys=ALL)
+        _say(f"  rk={rk} err", _fmt(errs))
+        _say(f"  rk={rk} floor", _fmt(floor))
+        bad = helpers.within(errs, floor, 1e-12)
+        assert not bad, f"rk={rk}: {bad}"
+    ch.close()
+
+
+def test_rem_div_projection(dor):
+    _say("[parity] rem_div small_case")
+    o, p, gy, ic = helpers.make_oracle(dor, "small_case")
+    rng = np.random.default_rng(3)
+    for w in (dor.CU1, dor.CU2, dor.CU3):
+        blk = o.carr(w)[:, : o.tnkz + 1, : o.nkx + 1]
+        blk += 0.01 * (rng.standard_normal(blk.shape) + 1j * rng.standard_normal(blk.shape))
+    ch = channel.Channel(p, gy)
+    ch.upload(helpers.oracle_state(dor, o))
+    o.rem_div()
+    ch.rem_div()
+    st = ch.download()
+    errs = helpers.field_errors(dor, o, st, keys=STATE)
+    errs.pop("th0", None)
+    _say("  err", _fmt(errs))
+    assert max(errs.values()) <= 1e-12, errs
+    ch.close()
+
+
+@pytest.mark.parametrize("name", ["small_case", "kh_shear_layer", "channel_re180"])
+def test_ten_steps(dor, name):
+    """10 RK3 steps from identical ICs; channel_re180 is the full 128x129x128
+    channel with variable dt (COURANT on the device)."""
+    steps = 10
+    _say(f"[parity] {steps} steps {name}")
+    o, o2, ch, p = _pair(dor, name)
+    ch.set_clock(0.0, p["delta_t"], 0)
+    o.time_step(steps)
+    o2.time_step(steps)
+    ch.time_step(steps)
+    t_o, dt_o, ts_o = o.clock()
+    t_g, dt_g, hb, ts_g = ch.clock()
+    _say(f"  time oracle {t_o!r} gpu {t_g!r}; dt oracle {dt_o!r} gpu {dt_g!r}")
+    assert ts_g == ts_o == steps
+    assert abs(t_g - t_o) <= 1e-12 * max(1.0, abs(t_o))
+    assert abs(dt_g - dt_o) <= 1e-12 * dt_o
+    st = ch.download()
+    errs = helpers.field_errors(dor, o, st, keys=STATE)
+    floor = helpers.field_errors(dor, o, helpers.oracle_state(dor, o2), keys=STATE)
+    _say("  err  ", _fmt(errs))
+    _say("  floor", _fmt(floor))
+    bad = helpers.within(errs, floor, 1e-10)
+    assert not bad, bad
+    ch.close()
+
+
+@pytest.mark.parametrize("name", ["small_case", "kh_shear_layer", "channel_re180"])
+def test_initial_condition_pipeline(dor, name):
+    """CREATE_TH_CHAN + CREATE_FLOW_CHAN on the device (same seeded generator,
+    BC planes, REM_DIV) against the oracle's restatement of set_ics.f."""
+    _say(f"[parity] initial condition {name}")
+    o, p, gy, ic = helpers.make_oracle(dor, name)
+    o2, *_ = helpers.make_oracle(dor, name)
+    helpers.perturb(dor, o2, 1e-16)
+    o2.rem_div()
+    o.rem_div()  # the floor compares one extra projection of identical inputs
+    ch = channel.Channel(p, gy)
+    ch.initial_condition(ic)
+    st = ch.download()
+    o3, *_ = helpers.make_oracle(dor, name)
+    errs = helpers.field_errors(dor, o3, st, keys=STATE)
+    floor = helpers.field_errors(dor, o, helpers.oracle_state(dor, o2), keys=STATE)
+    _say("  err  ", _fmt(errs))
+    _say("  floor", _fmt(floor))
+    bad = helpers.within(errs, floor, 1e-12)
+    assert not bad, bad
+    ch.close()
