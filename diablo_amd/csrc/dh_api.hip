@@ -3,6 +3,7 @@
 // substep/step dispatch and the captured step graph.
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 
@@ -29,18 +30,9 @@ int fail(Ctx &c, int code, const std::string &msg) {
     if (e_ != hipSuccess) return fail(c, DH_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
-bool make_plan(int n, FftPlan &p) {
-  p.n = n;
-  p.nst = 0;
-  int m = n;
-  while (m % 8 == 0) { p.rad[p.nst++] = 8; m /= 8; }
-  while (m % 4 == 0) { p.rad[p.nst++] = 4; m /= 4; }
-  while (m % 2 == 0) { p.rad[p.nst++] = 2; m /= 2; }
-  while (m % 3 == 0) { p.rad[p.nst++] = 3; m /= 3; }
-  return m == 1 && p.nst <= kMaxStages;
-}
+unsigned long long recip(unsigned d) { return ((1ull << 32) + d - 1) / d; }
 
-// Slot of each output index after the in-place DIF transform of plan p:
+// Slot of each output index after the in-place DIF with radices rad[0..nst):
 // slot s holds index digrev(s), digrev(s) = s/S + R*digrev'(s%S).
 void digrev(int n, const int *rad, int nst, std::vector<int> &idx_of_slot) {
   idx_of_slot.assign(n, 0);
@@ -51,11 +43,49 @@ void digrev(int n, const int *rad, int nst, std::vector<int> &idx_of_slot) {
   for (int s = 0; s < n; ++s) idx_of_slot[s] = s / S + R * sub[s % S];
 }
 
-std::vector<int> slot_of_index(const FftPlan &p) {
-  std::vector<int> ios, pos(p.n);
-  digrev(p.n, p.rad, p.nst, ios);
-  for (int s = 0; s < p.n; ++s) pos[ios[s]] = s;
-  return pos;
+bool make_plan(int n, FftPlan &p, std::vector<int> &dit_in_slot, std::vector<int> &dif_out_slot) {
+  int rad[kMaxStages];
+  int nst = 0, m = n;
+  for (int R : {8, 4, 2, 3})
+    while (m % R == 0 && nst < kMaxStages) {
+      rad[nst++] = R;
+      m /= R;
+    }
+  if (m != 1) return false;
+  p.n = n;
+  p.nst = nst;
+  int L = n;
+  for (int s = 0; s < nst; ++s) {  // decimation in frequency
+    FftStage &st = p.dif[s];
+    st.R = rad[s];
+    st.L = L;
+    st.S = L / st.R;
+    st.twm = n / L;
+    st.fS = recip(st.S);
+    st.fnb = recip(n / st.R);
+    L /= st.R;
+  }
+  L = 1;
+  for (int s = 0; s < nst; ++s) {  // decimation in time
+    FftStage &st = p.dit[s];
+    st.R = rad[s];
+    L *= st.R;
+    st.L = L;
+    st.S = L / st.R;
+    st.twm = n / L;
+    st.fS = recip(st.S);
+    st.fnb = recip(n / st.R);
+  }
+  std::vector<int> ios;
+  digrev(n, rad, nst, ios);  // DIF: slot s holds index ios[s]
+  dif_out_slot.assign(n, 0);
+  for (int s = 0; s < n; ++s) dif_out_slot[ios[s]] = s;
+  int rrev[kMaxStages];
+  for (int s = 0; s < nst; ++s) rrev[s] = rad[nst - 1 - s];
+  digrev(n, rrev, nst, ios);  // DIT input: slot s must hold index ios[s]
+  dit_in_slot.assign(n, 0);
+  for (int s = 0; s < n; ++s) dit_in_slot[ios[s]] = s;
+  return true;
 }
 
 std::vector<double2> twiddles(int n, int count) {
@@ -77,7 +107,7 @@ hipError_t dalloc(T **p, size_t count) {
 void free_all(Ctx &c) {
   void *ptrs[] = {c.tw_row, c.tw_col, c.tw_real, c.posr, c.posz, c.kx, c.kz, c.kx2, c.kz2, c.d_dy, c.d_dyf, c.d_gyf, c.cu[0], c.cu[1],
                   c.cu[2], c.cp, c.cf[0], c.cf[1], c.cf[2], c.cth, c.cfth, c.phi, c.uphys, c.rphys, c.rd_bp,
-                  c.rd_g, c.clock, c.fac.ap, c.fac.bp, c.fac.cc, c.fac.isbc, c.fac.vconst, c.rd_l, c.rd_u};
+                  c.rd_g, c.clock, c.fac.k1, c.fac.k2, c.fac.bfin, c.fac.isbc, c.fac.vconst, c.rd_l, c.rd_u};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c.step_graph) (void)hipGraphExecDestroy(c.step_graph);
@@ -110,15 +140,8 @@ double dt_cap_of(const Ctx &c) {
   return dt;
 }
 
-int upload_factors(Ctx &c, double dt) {
-  std::vector<double> ap, bp, cc, vc;
-  std::vector<int> isbc;
-  build_real_factors_host(c, dt, ap, bp, cc, isbc, vc);
-  HIPCHK(hipMemcpyAsync(c.fac.ap, ap.data(), ap.size() * 8, hipMemcpyHostToDevice, c.stream));
-  HIPCHK(hipMemcpyAsync(c.fac.bp, bp.data(), bp.size() * 8, hipMemcpyHostToDevice, c.stream));
-  HIPCHK(hipMemcpyAsync(c.fac.cc, cc.data(), cc.size() * 8, hipMemcpyHostToDevice, c.stream));
-  HIPCHK(hipMemcpyAsync(c.fac.isbc, isbc.data(), isbc.size() * sizeof(int), hipMemcpyHostToDevice, c.stream));
-  HIPCHK(hipMemcpyAsync(c.fac.vconst, vc.data(), vc.size() * 8, hipMemcpyHostToDevice, c.stream));
+int rebuild_tables(Ctx &c) {
+  HIPCHK(launch_pcr_tables(c));
   HIPCHK(hipStreamSynchronize(c.stream));
   return DH_OK;
 }
@@ -162,18 +185,30 @@ int dh_create(const dh_params *p, dh_ctx **out) {
   g.pf = g.nkx1 * g.nkz1;
   g.lw = p->nx / 2;
   g.pp = (long long)p->nx * p->nz;
-  if (!make_plan(g.lw, g.prow) || !make_plan(g.nz, g.pcol))
+  std::vector<int> posr, dif_row_unused, dit_col_unused, posz;
+  if (!make_plan(g.lw, g.prow, posr, dif_row_unused) || !make_plan(g.nz, g.pcol, dit_col_unused, posz))
     return reject("NX/2 and NZ must factor into 2 and 3");
   c.nfields = 3 + p->n_th;
-  c.lds_bytes = g.nz * g.lw * (int)sizeof(double2);
+  c.lds_bytes = g.nz * g.lw * (int)sizeof(double2) + (2 * g.lw + g.nz + 1) * (int)sizeof(double2) +
+                (g.lw + g.nz) * (int)sizeof(int);
   if (c.lds_bytes > 160 * 1024) return reject("x-z plane too large for the LDS-resident plane kernels");
+  if (g.ny + 2 > 288) return reject("NY+2 > 288 rows is beyond the column solver's register tiles");
   int thr = (g.nz * g.lw + 7) / 8;
   thr = ((thr + 63) / 64) * 64;
   if (thr < 64) thr = 64;
-  if (thr > 512) thr = 512;
-  while (thr * 8 < g.pf && thr < 512) thr += 64;  // k_nonlinear keeps <= 8 modes per thread
-  if (thr * 8 < g.pf) return reject("plane has too many Fourier modes");
+  if (thr > 1024) thr = 1024;
+  while (thr * 4 < g.pf && thr < 1024) thr += 64;  // k_nonlinear keeps <= 4 modes per thread
+  if (thr * 4 < g.pf) return reject("plane has too many Fourier modes");
   c.plane_threads = thr;
+  {
+    const char *rd = std::getenv("DIABLO_REMDIV");
+    c.rd_pcr = rd && std::strcmp(rd, "pcr") == 0;
+  }
+  g.f_npair = recip(g.lw / 2 + 1);
+  g.f_nkx1 = recip(g.nkx1);
+  g.f_lw = recip(g.lw);
+  g.rows_pairs_per_thread = (g.nz * (g.lw / 2 + 1) + thr - 1) / thr;
+  if (g.rows_pairs_per_thread > 8) return reject("plane needs more than 8 row pairs per thread");
   // mpi.f:1022-1046 (RANKY=0 branch on one rank)
   c.jstart = 2;
   c.jend = g.ny;
@@ -234,9 +269,11 @@ int dh_create(const dh_params *p, dh_ctx **out) {
   ACHK(dalloc(&c.rd_l, nyy));
   ACHK(dalloc(&c.rd_u, nyy));
   ACHK(dalloc(&c.clock, 1));
-  ACHK(dalloc(&c.fac.ap, 3 * (size_t)c.nfields * nyy));
-  ACHK(dalloc(&c.fac.bp, 3 * (size_t)c.nfields * nyy));
-  ACHK(dalloc(&c.fac.cc, 3 * (size_t)c.nfields * nyy));
+  ACHK(dalloc(&c.fac.k1, 3 * (size_t)c.nfields * kMaxPcr * nyy));
+  ACHK(dalloc(&c.fac.k2, 3 * (size_t)c.nfields * kMaxPcr * nyy));
+  ACHK(dalloc(&c.fac.bfin, 3 * (size_t)c.nfields * nyy));
+  c.fac.nsteps = 0;
+  while ((1 << c.fac.nsteps) < (int)nyy) ++c.fac.nsteps;
   ACHK(dalloc(&c.fac.isbc, (size_t)c.nfields * nyy));
   ACHK(dalloc(&c.fac.vconst, (size_t)c.nfields * nyy));
   {
@@ -244,9 +281,8 @@ int dh_create(const dh_params *p, dh_ctx **out) {
     ACHK(hipMemcpy(c.tw_row, tr.data(), tr.size() * sizeof(double2), hipMemcpyHostToDevice));
     ACHK(hipMemcpy(c.tw_col, tc.data(), tc.size() * sizeof(double2), hipMemcpyHostToDevice));
     ACHK(hipMemcpy(c.tw_real, tx.data(), tx.size() * sizeof(double2), hipMemcpyHostToDevice));
-    auto pr = slot_of_index(g.prow), pz = slot_of_index(g.pcol);
-    ACHK(hipMemcpy(c.posr, pr.data(), pr.size() * sizeof(int), hipMemcpyHostToDevice));
-    ACHK(hipMemcpy(c.posz, pz.data(), pz.size() * sizeof(int), hipMemcpyHostToDevice));
+    ACHK(hipMemcpy(c.posr, posr.data(), posr.size() * sizeof(int), hipMemcpyHostToDevice));
+    ACHK(hipMemcpy(c.posz, posz.data(), posz.size() * sizeof(int), hipMemcpyHostToDevice));
     // fft.f:77-100
     const double PI = 4.0 * std::atan(1.0);
     std::vector<double> kx(g.nkx1), kx2(g.nkx1), kz(g.nkz1), kz2(g.nkz1);
@@ -331,9 +367,7 @@ int dh_set_grid(dh_ctx *h, const double *gy, const double *gyf, const double *dy
   HIPCHK(hipMemcpy(c.rd_u, ru.data(), nyy * 8, hipMemcpyHostToDevice));
   c.grid_set = true;
   c.dt_cap = dt_cap_of(c);
-  Clock ck;
-  HIPCHK(hipMemcpy(&ck, c.clock, sizeof(Clock), hipMemcpyDeviceToHost));
-  return upload_factors(c, ck.dt);
+  return rebuild_tables(c);
 }
 
 int dh_set_clock(dh_ctx *h, double time, double delta_t, int32_t time_step) {
@@ -353,7 +387,7 @@ int dh_set_clock(dh_ctx *h, double time, double delta_t, int32_t time_step) {
     ck.h_bar[2] = delta_t * (5.0 / 15.0);
   }
   HIPCHK(hipMemcpy(c.clock, &ck, sizeof(Clock), hipMemcpyHostToDevice));
-  if (newdt) return upload_factors(c, delta_t);
+  if (newdt) return rebuild_tables(c);
   return DH_OK;
 }
 

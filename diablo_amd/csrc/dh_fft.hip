@@ -14,37 +14,36 @@ struct PlaneBatch {
 };
 
 // Fourier -> physical for planes j0 + blockIdx.x of field blockIdx.y.
-__global__ void __launch_bounds__(512) k_plane_inv(Geo g, PlaneBatch b) {
+__global__ void __launch_bounds__(1024) k_plane_inv(Geo g, PlaneBatch b) {
   extern __shared__ double2 A[];
+  const Tables T = load_tables(A, g);
   const int f = blockIdx.y;
   const long long j = b.j0 + blockIdx.x;
   plane_unpack(A, g, b.four[f] + j * g.pf);
-  plane_cols_inv(A, g);
-  plane_rows_inv(A, g);
+  plane_cols_inv(A, g, T);
+  plane_rows_inv<8>(A, g, T);
   double2 *dst = reinterpret_cast<double2 *>(b.phys[f] + j * g.pp);
   const int n = g.nz * g.lw;
   for (int e = threadIdx.x; e < n; e += blockDim.x) {
-    const int z = e / g.lw, m = e - z * g.lw;
-    dst[e] = A[inv_slot(g, z, m)];
+    const int z = (int)fdiv((unsigned)e, g.f_lw), m = e - z * g.lw;
+    dst[e] = A[inv_slot(g, T, z, m)];
   }
 }
 
 // physical -> Fourier (x r2c with 1/NX and truncation, z c2c with 1/NZ and
 // packing of kz = -NKZ..-1 after 0..NKZ).
-__global__ void __launch_bounds__(512) k_plane_fwd(Geo g, PlaneBatch b) {
+__global__ void __launch_bounds__(1024) k_plane_fwd(Geo g, PlaneBatch b) {
   extern __shared__ double2 A[];
+  const Tables T = load_tables(A, g);
   const int f = blockIdx.y;
   const long long j = b.j0 + blockIdx.x;
-  const double2 *src = reinterpret_cast<const double2 *>(b.phys[f] + j * g.pp);
-  const int n = g.nz * g.lw;
-  for (int e = threadIdx.x; e < n; e += blockDim.x) A[e] = src[e];
-  __syncthreads();
-  plane_rows_fwd(A, g);
-  plane_cols_fwd(A, g);
+  plane_load_phys(A, g, T, reinterpret_cast<const double2 *>(b.phys[f] + j * g.pp));
+  plane_rows_fwd(A, g, T);
+  plane_cols_fwd(A, g, T);
   double2 *dst = const_cast<double2 *>(b.four[f]) + j * g.pf;
   for (int m = threadIdx.x; m < g.pf; m += blockDim.x) {
-    const int kzp = m / g.nkx1, kx = m - kzp * g.nkx1;
-    const double2 v = A[fwd_slot(g, kx, kzp)];
+    const int kzp = (int)fdiv((unsigned)m, g.f_nkx1), kx = m - kzp * g.nkx1;
+    const double2 v = A[fwd_slot(g, T, kx, kzp)];
     dst[m] = make_double2(v.x / g.nz, v.y / g.nz);
   }
 }

@@ -1,15 +1,19 @@
 // Device-side FFT engine: batched in-place radix-2/3/4/8 transforms of one
-// x-z plane held in LDS, shared by the plane-transform kernels and the fused nonlinear
-// kernel.
+// x-z plane held in LDS, shared by the plane-transform kernels and the fused
+// nonlinear kernel.
 //
 // Plane layout in LDS: A[z][0..LW-1] complex, LW = NX/2.
-//   physical view: A[z][m] = (x_{2m}, x_{2m+1}) of row z (a real row of NX
-//                  doubles is exactly LW complex values);
-//   spectral view: A[z][kx] for kx <= NKX after the x transform.
+//   physical view: row z holds (x_{2m}, x_{2m+1}) in the LW complex slots;
+//   spectral view: A[.][kx] for kx <= NKX after the x transform.
+// Rows are transformed by decimation in time (input in digit-reversed slots,
+// output in natural order); columns by decimation in frequency (natural input,
+// output rows in digit-reversed order, undone by the posz table at pack/unpack
+// time).  Both run in place: every butterfly writes back to the slots it
+// read, so a stage needs one barrier and only R complex values per thread.
 //
-// The real-to-complex x transform of length NX is done as a complex
-// transform of length NX/2 plus the usual even/odd split; FFTW's r2c/c2r
-// (fft.f:73-76) compute the same unnormalised DFTs.
+// The real x transform of length NX is a complex transform of length NX/2
+// plus the even/odd split; FFTW's r2c/c2r (fft.f:73-76) compute the same
+// unnormalised DFTs.
 #pragma once
 
 #include "dh_internal.h"
@@ -22,7 +26,6 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 __device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
-__device__ __forceinline__ double2 cscale(double s, double2 a) { return make_double2(s * a.x, s * a.y); }
 // (i*k) * z with the component order of a COMPLEX*16 product (0,k)*(x,y)
 __device__ __forceinline__ double2 cmul_ik(double k, double2 z) { return make_double2(-(k * z.y), k * z.x); }
 // multiply by -i (forward) or +i (inverse)
@@ -31,10 +34,9 @@ __device__ __forceinline__ double2 cmul_mi(double2 a) {
   return INV ? make_double2(-a.y, a.x) : make_double2(a.y, -a.x);
 }
 
-template <bool INV>
-__device__ __forceinline__ double2 tw_get(const double2 *tw, int idx) {
-  double2 w = tw[idx];
-  return INV ? cconj(w) : w;
+// n / d for n, d < 2^16 by multiply-high (m = ceil(2^32 / d)).
+__device__ __forceinline__ unsigned fdiv(unsigned n, unsigned long long m) {
+  return (unsigned)(((unsigned long long)n * m) >> 32);
 }
 
 template <int R, bool INV>
@@ -49,8 +51,7 @@ __device__ __forceinline__ void dft_small(double2 *v) {
     double2 t1 = cadd(v[1], v[2]);
     double2 t2 = csub(v[1], v[2]);
     double2 m = make_double2(v[0].x + c1 * t1.x, v[0].y + c1 * t1.y);
-    // i*s1*t2
-    double2 it = make_double2(-s1 * t2.y, s1 * t2.x);
+    double2 it = make_double2(-s1 * t2.y, s1 * t2.x);  // i*s1*t2
     v[0] = cadd(v[0], t1);
     v[1] = cadd(m, it);
     v[2] = csub(m, it);
@@ -84,37 +85,65 @@ __device__ __forceinline__ void dft_small(double2 *v) {
   }
 }
 
-// One in-place decimation-in-frequency stage (radix R, block length L,
-// span S = L/R) over `ntrans` transforms of length N.  Each butterfly writes
-// its outputs back to the addresses it read, so a stage needs no read/write
-// barrier; after all stages position p holds output index digrev(p) (the
-// pos_* tables of Geo give the position of each index).
-//   rows (COLS=false): transform t is LDS row t, element i at A[t*lw + i]
-//   cols (COLS=true) : transform t is column colpos[t] (or t), element i at
-//                      A[i*lw + col]; consecutive threads take consecutive
-//                      columns so LDS accesses stay on distinct banks.
-template <int R, bool INV, bool COLS>
-__device__ __forceinline__ void dif_stage(double2 *A, int N, int L, int ntrans, int lw, const int *colpos,
-                                          const double2 *tw) {
-  const int S = L / R;
+// Transform tables staged in LDS behind the plane (see load_tables).
+struct Tables {
+  double2 *twr;  // exp(-2 pi i m/LW), m < LW
+  double2 *twc;  // exp(-2 pi i m/NZ), m < NZ
+  double2 *twx;  // exp(-2 pi i k/NX), k <= LW
+  int *posr;     // DIT input slot of row index m
+  int *posz;     // slot of column output index k after the DIF
+};
+
+// Copies the tables into LDS; returns pointers into `smem` after the plane.
+__device__ Tables load_tables(double2 *smem, const Geo &g) {
+  Tables t;
+  double2 *p = smem + g.nz * g.lw;
+  t.twr = p;
+  t.twc = p + g.lw;
+  t.twx = p + g.lw + g.nz;
+  int *q = reinterpret_cast<int *>(p + g.lw + g.nz + g.lw + 1);
+  t.posr = q;
+  t.posz = q + g.lw;
+  for (int i = threadIdx.x; i < g.lw; i += blockDim.x) {
+    t.twr[i] = g.tw_row[i];
+    t.posr[i] = g.posr[i];
+  }
+  for (int i = threadIdx.x; i < g.nz; i += blockDim.x) {
+    t.twc[i] = g.tw_col[i];
+    t.posz[i] = g.posz[i];
+  }
+  for (int i = threadIdx.x; i <= g.lw; i += blockDim.x) t.twx[i] = g.tw_real[i];
+  __syncthreads();
+  return t;
+}
+
+// One in-place stage of radix R over `ntrans` transforms of length N.
+//   DIT: twiddles W_L^{r j} on the inputs, then the butterfly (L grows);
+//   DIF: butterfly, then twiddles W_L^{q j} on the outputs (L shrinks).
+// rows (COLS=false): transform t is LDS row t, element i at A[t*lw + i];
+// cols (COLS=true):  transform t is column t, element i at A[i*lw + t],
+//                    consecutive threads take consecutive columns.
+template <int R, bool INV, bool COLS, bool DIT>
+__device__ __forceinline__ void fft_stage(double2 *A, const FftStage &st, int N, int ntrans,
+                                          unsigned long long fnt, int lw, const double2 *tw) {
+  const int S = st.S;
   const int nb = N / R;
   const int total = ntrans * nb;
-  const int twm = N / L;
   for (int gi = threadIdx.x; gi < total; gi += blockDim.x) {
     int t, b;
     if (COLS) {
-      t = gi % ntrans;
-      b = gi / ntrans;
+      b = (int)fdiv((unsigned)gi, fnt);
+      t = gi - b * ntrans;
     } else {
-      t = gi / nb;
+      t = (int)fdiv((unsigned)gi, st.fnb);
       b = gi - t * nb;
     }
-    const int g = b / S, j = b - g * S;
-    const int i0 = g * L + j;
+    const int g = (int)fdiv((unsigned)b, st.fS), j = b - g * S;
+    const int i0 = g * st.L + j;
     double2 *base;
     int es;
     if (COLS) {
-      base = A + (colpos ? colpos[t] : t) + i0 * lw;
+      base = A + t + i0 * lw;
       es = S * lw;
     } else {
       base = A + t * lw + i0;
@@ -123,10 +152,20 @@ __device__ __forceinline__ void dif_stage(double2 *A, int N, int L, int ntrans, 
     double2 v[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) v[r] = base[r * es];
-    dft_small<R, INV>(v);
-    if (j > 0) {
+    if (DIT && j > 0) {
 #pragma unroll
-      for (int q = 1; q < R; ++q) v[q] = cmul(v[q], tw_get<INV>(tw, q * j * twm));
+      for (int r = 1; r < R; ++r) {
+        double2 w = tw[r * j * st.twm];
+        v[r] = cmul(v[r], INV ? cconj(w) : w);
+      }
+    }
+    dft_small<R, INV>(v);
+    if (!DIT && j > 0) {
+#pragma unroll
+      for (int q = 1; q < R; ++q) {
+        double2 w = tw[q * j * st.twm];
+        v[q] = cmul(v[q], INV ? cconj(w) : w);
+      }
     }
 #pragma unroll
     for (int q = 0; q < R; ++q) base[q * es] = v[q];
@@ -134,126 +173,181 @@ __device__ __forceinline__ void dif_stage(double2 *A, int N, int L, int ntrans, 
   __syncthreads();
 }
 
-template <bool INV, bool COLS>
-__device__ void lds_fft(double2 *A, const FftPlan &pl, int ntrans, int lw, const int *colpos, const double2 *tw) {
-  int L = pl.n;
+template <bool INV, bool COLS, bool DIT>
+__device__ void lds_fft(double2 *A, const FftPlan &pl, int ntrans, unsigned long long fnt, int lw, const double2 *tw) {
   for (int s = 0; s < pl.nst; ++s) {
-    const int R = pl.rad[s];
-    switch (R) {
-      case 8: dif_stage<8, INV, COLS>(A, pl.n, L, ntrans, lw, colpos, tw); break;
-      case 4: dif_stage<4, INV, COLS>(A, pl.n, L, ntrans, lw, colpos, tw); break;
-      case 3: dif_stage<3, INV, COLS>(A, pl.n, L, ntrans, lw, colpos, tw); break;
-      default: dif_stage<2, INV, COLS>(A, pl.n, L, ntrans, lw, colpos, tw); break;
+    const FftStage &st = DIT ? pl.dit[s] : pl.dif[s];
+    switch (st.R) {
+      case 8: fft_stage<8, INV, COLS, DIT>(A, st, pl.n, ntrans, fnt, lw, tw); break;
+      case 4: fft_stage<4, INV, COLS, DIT>(A, st, pl.n, ntrans, fnt, lw, tw); break;
+      case 3: fft_stage<3, INV, COLS, DIT>(A, st, pl.n, ntrans, fnt, lw, tw); break;
+      default: fft_stage<2, INV, COLS, DIT>(A, st, pl.n, ntrans, fnt, lw, tw); break;
     }
-    L /= R;
   }
 }
 
-// Forward x transform of every row (FFT_XZ_TO_FOURIER's r2c, mpi.f:1490-1506):
-// A holds the physical plane (natural order); on exit the LDS slot
-// posr[k] of each row holds X_k / NX for k <= NKX.
-__device__ void plane_rows_fwd(double2 *A, const Geo &g) {
+// LDS slot through which physical pair m of row z enters the forward transform.
+__device__ __forceinline__ int phys_slot(const Tables &T, const Geo &g, int z, int m) { return z * g.lw + T.posr[m]; }
+
+// Forward x transform of every row (FFT_XZ_TO_FOURIER's r2c, mpi.f:1490-1506).
+// On entry row z holds (x_{2m}, x_{2m+1}) in slot posr[m]; on exit slot k
+// holds X_k / NX for k <= NKX (natural order).
+__device__ void plane_rows_fwd(double2 *A, const Geo &g, const Tables &T) {
   const int h = g.lw;  // NX/2
-  lds_fft<false, false>(A, g.prow, g.nz, g.lw, nullptr, g.tw_row);
+  lds_fft<false, false, true>(A, g.prow, g.nz, 0, g.lw, T.twr);
   // even/odd split: X_k = (Z_k + conj(Z_{h-k}))/2 - i w^-k (Z_k - conj(Z_{h-k}))/2
   const int npair = h / 2 + 1;  // k = 0..h/2 paired with h-k
   const int total = g.nz * npair;
-  const int *pos = g.posr;
   for (int e = threadIdx.x; e < total; e += blockDim.x) {
-    const int z = e / npair, k = e - z * npair;
+    const int z = (int)fdiv((unsigned)e, g.f_npair), k = e - z * npair;
     double2 *row = A + z * g.lw;
     const int k2 = (k == 0) ? 0 : h - k;
-    const int pk = pos[k], pk2 = pos[k2];
-    const double2 zk = row[pk];
-    const double2 zm = row[pk2];
+    const double2 zk = row[k];
+    const double2 zm = row[k2];
     {
       double2 a = make_double2(0.5 * (zk.x + zm.x), 0.5 * (zk.y - zm.y));  // (Z_k + conj Z_{h-k})/2
       double2 b = make_double2(0.5 * (zk.x - zm.x), 0.5 * (zk.y + zm.y));  // (Z_k - conj Z_{h-k})/2
-      double2 wb = cmul(g.tw_real[k], b);                                 // w = e^{-2 pi i k/NX}
+      double2 wb = cmul(T.twx[k], b);                                     // w = e^{-2 pi i k/NX}
       double2 X = make_double2(a.x + wb.y, a.y - wb.x);                  // a - i*w*b
-      if (k <= g.nkx) row[pk] = make_double2(X.x / g.nx, X.y / g.nx);
+      if (k <= g.nkx) row[k] = make_double2(X.x / g.nx, X.y / g.nx);
     }
     if (k != 0 && k2 != k && k2 <= g.nkx) {
       double2 a = make_double2(0.5 * (zm.x + zk.x), 0.5 * (zm.y - zk.y));
       double2 b = make_double2(0.5 * (zm.x - zk.x), 0.5 * (zm.y + zk.y));
-      double2 wb = cmul(g.tw_real[k2], b);
+      double2 wb = cmul(T.twx[k2], b);
       double2 X = make_double2(a.x + wb.y, a.y - wb.x);
-      row[pk2] = make_double2(X.x / g.nx, X.y / g.nx);
+      row[k2] = make_double2(X.x / g.nx, X.y / g.nx);
     }
   }
   __syncthreads();
 }
 
-// Inverse x transform of every row (FFT_XZ_TO_PHYSICAL's c2r, mpi.f:1600-1604):
-// on entry A[z][k], k <= NKX in natural order (higher modes and the Nyquist
-// term are zero, the imaginary part of the k=0 term is ignored); on exit
-// z_m = (x_{2m}, x_{2m+1}) sits in slot posr[m] of each row.
-__device__ void plane_rows_inv(double2 *A, const Geo &g) {
+// Inverse x transform of every row (FFT_XZ_TO_PHYSICAL's c2r, mpi.f:1600-1604).
+// On entry slot k of each row holds X_k for k <= NKX (higher modes and the
+// Nyquist term are zero; the imaginary part of the k=0 term is ignored); on
+// exit slot m holds (x_{2m}, x_{2m+1}).  The even/odd combination reads
+// natural slots and writes digit-reversed ones, so it runs in two phases
+// with the values held in registers (MAXP pairs per thread at most).
+template <int MAXP>
+__device__ void plane_rows_inv(double2 *A, const Geo &g, const Tables &T) {
   const int h = g.lw;
   const int npair = h / 2 + 1;
   const int total = g.nz * npair;
-  for (int e = threadIdx.x; e < total; e += blockDim.x) {
-    const int z = e / npair, k = e - z * npair;
-    double2 *row = A + z * g.lw;
-    const int k2 = h - k;  // partner (k=0 pairs with the Nyquist term)
-    double2 xk = (k <= g.nkx) ? row[k] : make_double2(0.0, 0.0);
-    double2 xm = (k2 <= g.nkx && k2 < h) ? row[k2] : make_double2(0.0, 0.0);
-    if (k == 0) xk.y = 0.0;
-    // Z_k = (X_k + conj X_{h-k}) + i w^k (X_k - conj X_{h-k}),  w = e^{+2 pi i/NX}
-    {
-      double2 a = make_double2(xk.x + xm.x, xk.y - xm.y);
-      double2 b = make_double2(xk.x - xm.x, xk.y + xm.y);
-      double2 wb = cmul(cconj(g.tw_real[k]), b);
-      if (k < h) row[k] = make_double2(a.x - wb.y, a.y + wb.x);
-    }
-    if (k != 0 && k2 != k) {
-      double2 a = make_double2(xm.x + xk.x, xm.y - xk.y);
-      double2 b = make_double2(xm.x - xk.x, xm.y + xk.y);
-      double2 wb = cmul(cconj(g.tw_real[k2]), b);
-      row[k2] = make_double2(a.x - wb.y, a.y + wb.x);
+  double2 zk_[MAXP], zm_[MAXP];
+#pragma unroll
+  for (int q = 0; q < MAXP; ++q) {
+    const int e = threadIdx.x + q * blockDim.x;
+    if (e < total) {
+      const int z = (int)fdiv((unsigned)e, g.f_npair), k = e - z * npair;
+      const double2 *row = A + z * g.lw;
+      const int k2 = h - k;  // partner (k=0 pairs with the Nyquist term)
+      double2 xk = (k <= g.nkx) ? row[k] : make_double2(0.0, 0.0);
+      double2 xm = (k2 <= g.nkx && k2 < h) ? row[k2] : make_double2(0.0, 0.0);
+      if (k == 0) xk.y = 0.0;
+      // Z_k = (X_k + conj X_{h-k}) + i w^k (X_k - conj X_{h-k}),  w = e^{+2 pi i/NX}
+      {
+        double2 a = make_double2(xk.x + xm.x, xk.y - xm.y);
+        double2 b = make_double2(xk.x - xm.x, xk.y + xm.y);
+        double2 wb = cmul(cconj(T.twx[k]), b);
+        zk_[q] = make_double2(a.x - wb.y, a.y + wb.x);
+      }
+      {
+        double2 a = make_double2(xm.x + xk.x, xm.y - xk.y);
+        double2 b = make_double2(xm.x - xk.x, xm.y + xk.y);
+        double2 wb = cmul(cconj(T.twx[k2]), b);
+        zm_[q] = make_double2(a.x - wb.y, a.y + wb.x);
+      }
     }
   }
   __syncthreads();
-  lds_fft<true, false>(A, g.prow, g.nz, g.lw, nullptr, g.tw_row);
+#pragma unroll
+  for (int q = 0; q < MAXP; ++q) {
+    const int e = threadIdx.x + q * blockDim.x;
+    if (e < total) {
+      const int z = (int)fdiv((unsigned)e, g.f_npair), k = e - z * npair;
+      double2 *row = A + z * g.lw;
+      const int k2 = h - k;
+      if (k < h) row[T.posr[k]] = zk_[q];
+      if (k != 0 && k2 != k) row[T.posr[k2]] = zm_[q];
+    }
+  }
+  __syncthreads();
+  lds_fft<true, false, true>(A, g.prow, g.nz, 0, g.lw, T.twr);
 }
 
-// z transforms over the de-aliased columns kx = 0..NKX.  Forward: the
-// columns sit at the row-transform output slots posr[kx]; inverse: natural.
-// Output row position p holds z (or kz) index digrev(p); posz maps back.
-__device__ __forceinline__ void plane_cols_fwd(double2 *A, const Geo &g) {
-  lds_fft<false, true>(A, g.pcol, g.nkx1, g.lw, g.posr, g.tw_col);
+// z transforms over the de-aliased columns kx = 0..NKX; the output row slot
+// of z (or kz) index k is posz[k].
+__device__ __forceinline__ void plane_cols_fwd(double2 *A, const Geo &g, const Tables &T) {
+  lds_fft<false, true, false>(A, g.pcol, g.nkx1, g.f_nkx1, g.lw, T.twc);
 }
-__device__ __forceinline__ void plane_cols_inv(double2 *A, const Geo &g) {
-  lds_fft<true, true>(A, g.pcol, g.nkx1, g.lw, nullptr, g.tw_col);
+__device__ __forceinline__ void plane_cols_inv(double2 *A, const Geo &g, const Tables &T) {
+  lds_fft<true, true, false>(A, g.pcol, g.nkx1, g.f_nkx1, g.lw, T.twc);
 }
-
-// LDS slot of the forward-transformed mode (kx, packed kz index kzp).
-__device__ __forceinline__ int fwd_slot(const Geo &g, int kx, int kzp);
 
 // Row of the unpacked z spectrum that holds packed index kzp (mpi.f:1564-1568).
 __device__ __forceinline__ int kz_row(const Geo &g, int kzp) {
   return (kzp <= g.nkz) ? kzp : (g.nz - 1 + (kzp - g.nkz) - g.nkz);
 }
 
-__device__ __forceinline__ int fwd_slot(const Geo &g, int kx, int kzp) {
-  return g.posz[kz_row(g, kzp)] * g.lw + g.posr[kx];
+// LDS slot of the forward-transformed mode (kx, packed kz index kzp).
+__device__ __forceinline__ int fwd_slot(const Geo &g, const Tables &T, int kx, int kzp) {
+  return T.posz[kz_row(g, kzp)] * g.lw + kx;
 }
 
 // LDS slot of physical pair m (x = 2m, 2m+1) of row z after the inverse.
-__device__ __forceinline__ int inv_slot(const Geo &g, int z, int m) { return g.posz[z] * g.lw + g.posr[m]; }
+__device__ __forceinline__ int inv_slot(const Geo &g, const Tables &T, int z, int m) { return T.posz[z] * g.lw + m; }
 
 // Scatter a de-aliased Fourier plane into A (unpack + zero fill of the
-// columns 0..NKX, mpi.f:1562-1574).  `src` may be null (all zero).
+// columns 0..NKX, mpi.f:1562-1574).  `src` may be null (all zero).  Loads
+// are issued in batches of 8 per thread before any LDS store, so their
+// latencies overlap.
 __device__ void plane_unpack(double2 *A, const Geo &g, const double2 *src) {
+  constexpr int B = 8;
   const int total = g.nz * g.nkx1;
-  for (int e = threadIdx.x; e < total; e += blockDim.x) {
-    const int zr = e / g.nkx1, kx = e - zr * g.nkx1;
-    int kzp = -1;
-    if (zr <= g.nkz)
-      kzp = zr;
-    else if (zr >= g.nz - g.nkz)
-      kzp = zr - (g.nz - 1 - g.nkz) + g.nkz;
-    A[zr * g.lw + kx] = (kzp >= 0 && src) ? src[kzp * g.nkx1 + kx] : make_double2(0.0, 0.0);
+  for (int base = threadIdx.x; base < total; base += B * blockDim.x) {
+    double2 v[B];
+    int slot[B];
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+      const int e = base + b * blockDim.x;
+      slot[b] = -1;
+      if (e < total) {
+        const int zr = (int)fdiv((unsigned)e, g.f_nkx1), kx = e - zr * g.nkx1;
+        int kzp = -1;
+        if (zr <= g.nkz)
+          kzp = zr;
+        else if (zr >= g.nz - g.nkz)
+          kzp = zr - (g.nz - 1 - g.nkz) + g.nkz;
+        slot[b] = zr * g.lw + kx;
+        v[b] = (kzp >= 0 && src) ? src[kzp * g.nkx1 + kx] : make_double2(0.0, 0.0);
+      }
+    }
+#pragma unroll
+    for (int b = 0; b < B; ++b)
+      if (slot[b] >= 0) A[slot[b]] = v[b];
+  }
+  __syncthreads();
+}
+
+// Physical plane (natural order in HBM) into the DIT input slots.
+__device__ void plane_load_phys(double2 *A, const Geo &g, const Tables &T, const double2 *src) {
+  constexpr int B = 8;
+  const int n = g.nz * g.lw;
+  for (int base = threadIdx.x; base < n; base += B * blockDim.x) {
+    double2 v[B];
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+      const int e = base + b * blockDim.x;
+      if (e < n) v[b] = src[e];
+    }
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+      const int e = base + b * blockDim.x;
+      if (e < n) {
+        const int z = (int)fdiv((unsigned)e, g.f_lw), m = e - z * g.lw;
+        A[phys_slot(T, g, z, m)] = v[b];
+      }
+    }
   }
   __syncthreads();
 }

@@ -14,11 +14,21 @@ namespace dh {
 constexpr int kMaxStages = 8;
 constexpr int kMaxFields = 3 + DH_MAX_TH;  // U1,U2,U3,TH(1..N_TH)
 
-// Radix plan of one 1-D transform length (in-place DIF, radices 2,3,4,8).
+// One in-place radix-R stage of a 1-D transform of length n: block length
+// L, span S = L/R, twiddle stride n/L; fS/fnb are multiply-high
+// reciprocals of S and n/R.
+struct FftStage {
+  int R, L, S, twm;
+  unsigned long long fS, fnb;
+};
+
+// Radix plan of one 1-D transform length (radices 8,4,2,3), as decimation
+// in time (L grows) and decimation in frequency (L shrinks).
 struct FftPlan {
   int n;
   int nst;
-  int rad[kMaxStages];
+  FftStage dit[kMaxStages];
+  FftStage dif[kMaxStages];
 };
 
 // Geometry and transform tables of one grid, passed by value to kernels.
@@ -35,8 +45,10 @@ struct Geo {
   const double2 *tw_real;  // exp(-2 pi i k/NX),      k <= NX/2
   const double *kx, *kz, *kx2, *kz2;  // wavenumbers (fft.f:77-100), de-aliased
   const double *dy, *dyf;             // grid spacing (0:NY+1)
-  const int *posr;  // slot of index k after a row transform (length NX/2)
-  const int *posz;  // slot of index k after a column transform (length NZ)
+  const int *posr;  // DIT input slot of row index m (length NX/2)
+  const int *posz;  // DIF output slot of column index k (length NZ)
+  unsigned long long f_npair, f_nkx1, f_lw;  // reciprocals of NX/4+1, NKX+1, NX/2
+  int rows_pairs_per_thread;                 // ceil(NZ*(NX/4+1)/threads)
 };
 
 // Device-resident clock (/RUNTIME_VARS/ + /RKW3_PARAMS/).
@@ -52,14 +64,16 @@ struct Clock {
   int pad;
 };
 
-// LU factors of the y-implicit (Crank-Nicolson) matrices, one set per RK
-// substep and field; the matrices depend only on j when LES is off.
+// Parallel-cyclic-reduction tables of the y-implicit (Crank-Nicolson)
+// matrices, one set per RK substep and field; with LES off each matrix
+// depends only on j, so every (x,z) column of a field shares it.
+constexpr int kMaxPcr = 12;  // PCR steps: NY+2 <= 2^12 rows
 struct RealFactors {
-  // [rk 0..2][field][j] : multiplier A', pivot B', upper C, per row
-  double *ap, *bp, *cc;
-  // [field][j] : 1 if row j takes a constant right-hand side (boundary row)
-  int *isbc;
-  double *vconst;  // [field][j]
+  double *k1, *k2;  // [rk 0..2][field][step][row] elimination multipliers
+  double *bfin;     // [rk][field][row] diagonal after the last step
+  int *isbc;        // [field][row] 1: row takes a constant right-hand side
+  double *vconst;   // [field][row]
+  int nsteps;       // ceil(log2(NY+2))
 };
 
 struct Ctx {
@@ -70,6 +84,7 @@ struct Ctx {
   int wlo[kMaxFields], whi[kMaxFields];  // solution write-back rows per field
   int lds_bytes;
   int plane_threads;
+  bool rd_pcr = false;  // REM_DIV by cyclic reduction instead of Thomas sweeps
   hipStream_t stream = nullptr;
   std::string err;
   bool grid_set = false;
@@ -115,7 +130,6 @@ hipError_t launch_phase(Ctx &c, int rk, int phase);
 hipError_t launch_rem_div(Ctx &c, int rk_for_cp);     // rk_for_cp<0: no CP update
 hipError_t launch_step_begin(Ctx &c);
 hipError_t launch_end_step(Ctx &c);
-void build_real_factors_host(const Ctx &c, double dt, std::vector<double> &ap, std::vector<double> &bp,
-                             std::vector<double> &cc, std::vector<int> &isbc, std::vector<double> &vconst);
+hipError_t launch_pcr_tables(Ctx &c);  // for the DELTA_T in the device clock
 
 }  // namespace dh

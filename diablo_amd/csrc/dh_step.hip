@@ -13,7 +13,7 @@
 
 namespace dh {
 
-constexpr int kMaxQ = 8;  // Fourier modes owned per thread in k_nonlinear
+constexpr int kMaxQ = 4;  // Fourier modes owned per thread in k_nonlinear (pf <= kMaxQ*threads)
 
 struct StepArgs {
   Geo g;
@@ -136,9 +136,10 @@ __device__ __forceinline__ double2 cf_init(const StepArgs &a, int c, int J, int 
 }
 
 // Fused explicit pass for plane J (1..NY) and component c (U1,U2,U3,TH...).
-__global__ void __launch_bounds__(512) k_nonlinear(StepArgs a) {
+__global__ void __launch_bounds__(1024) k_nonlinear(StepArgs a) {
   extern __shared__ double2 A[];
   const Geo &g = a.g;
+  const Tables T = load_tables(A, g);
   const int J = blockIdx.x + 1, c = blockIdx.y;
   int jlo = 2, jhi = g.ny;
   if (c >= 3) {
@@ -159,27 +160,41 @@ __global__ void __launch_bounds__(512) k_nonlinear(StepArgs a) {
   for (int q = 0; q < kMaxQ; ++q) {
     const int m = threadIdx.x + q * nthr;
     if (m < g.pf) {
-      const int kzp = m / g.nkx1, kx = m - kzp * g.nkx1;
+      const int kzp = (int)fdiv((unsigned)m, g.f_nkx1), kx = m - kzp * g.nkx1;
       acc[q] = cf_init(a, c, J, kx, kzp, m);
     }
   }
   // three products, forward transformed and accumulated in statement order
   const int npair = g.nz * g.lw;
   for (int p = 0; p < 3; ++p) {
-    for (int e = threadIdx.x; e < npair; e += nthr) {
-      const int z = e / g.lw, xp = e - z * g.lw;
-      A[e] = product_pair(a, c, p, J, (long long)z * g.nx + 2 * xp);
+    constexpr int PB = 2;  // products per thread per batch: loads issued before LDS stores
+    for (int base = threadIdx.x; base < npair; base += PB * nthr) {
+      double2 v[PB];
+      int sl[PB];
+#pragma unroll
+      for (int b = 0; b < PB; ++b) {
+        const int e = base + b * nthr;
+        sl[b] = -1;
+        if (e < npair) {
+          const int z = (int)fdiv((unsigned)e, g.f_lw), xp = e - z * g.lw;
+          sl[b] = phys_slot(T, g, z, xp);
+          v[b] = product_pair(a, c, p, J, (long long)z * g.nx + 2 * xp);
+        }
+      }
+#pragma unroll
+      for (int b = 0; b < PB; ++b)
+        if (sl[b] >= 0) A[sl[b]] = v[b];
     }
     __syncthreads();
-    plane_rows_fwd(A, g);
-    plane_cols_fwd(A, g);
+    plane_rows_fwd(A, g, T);
+    plane_cols_fwd(A, g, T);
     // derivative factor: c==0 takes (ikz, ikx, 1); all others (ikx, ikz, 1)
 #pragma unroll
     for (int q = 0; q < kMaxQ; ++q) {
       const int m = threadIdx.x + q * nthr;
       if (m < g.pf) {
-        const int kzp = m / g.nkx1, kx = m - kzp * g.nkx1;
-        double2 s = A[fwd_slot(g, kx, kzp)];
+        const int kzp = (int)fdiv((unsigned)m, g.f_nkx1), kx = m - kzp * g.nkx1;
+        double2 s = A[fwd_slot(g, T, kx, kzp)];
         s = make_double2(s.x / g.nz, s.y / g.nz);
         if (p == 2) {
           acc[q] = csub(acc[q], s);
@@ -195,7 +210,7 @@ __global__ void __launch_bounds__(512) k_nonlinear(StepArgs a) {
   // RK combination (channel.f:88-167, 581-595; scalars 254-270, 723-729),
   // new CF stored, CR scattered into LDS for the inverse transform.
   for (int e = threadIdx.x; e < g.nz * g.nkx1; e += nthr) {
-    const int zr = e / g.nkx1, kx = e - zr * g.nkx1;
+    const int zr = (int)fdiv((unsigned)e, g.f_nkx1), kx = e - zr * g.nkx1;
     if (zr > g.nkz && zr < g.nz - g.nkz) A[zr * g.lw + kx] = make_double2(0.0, 0.0);
   }
   double2 *CF = (c < 3) ? a.cf[c] : a.cfth + (c - 3) * a.cfs;
@@ -204,7 +219,7 @@ __global__ void __launch_bounds__(512) k_nonlinear(StepArgs a) {
   for (int q = 0; q < kMaxQ; ++q) {
     const int m = threadIdx.x + q * nthr;
     if (m < g.pf) {
-      const int kzp = m / g.nkx1, kx = m - kzp * g.nkx1;
+      const int kzp = (int)fdiv((unsigned)m, g.f_nkx1), kx = m - kzp * g.nkx1;
       const long long o = base + m;
       double2 cr = (c < 3) ? a.cu[c][o] : a.cth[(c - 3) * a.cfs + o];
       if (rk > 0) {
@@ -233,19 +248,168 @@ __global__ void __launch_bounds__(512) k_nonlinear(StepArgs a) {
     }
   }
   __syncthreads();
-  plane_cols_inv(A, g);
-  plane_rows_inv(A, g);
+  plane_cols_inv(A, g, T);
+  plane_rows_inv<8>(A, g, T);
   double2 *dst = reinterpret_cast<double2 *>(a.rphys + (long long)c * a.fs + (long long)J * g.pp);
   for (int e = threadIdx.x; e < npair; e += nthr) {
-    const int z = e / g.lw, m = e - z * g.lw;
-    dst[e] = A[inv_slot(g, z, m)];
+    const int z = (int)fdiv((unsigned)e, g.f_lw), m = e - z * g.lw;
+    dst[e] = A[inv_slot(g, T, z, m)];
+  }
+}
+// ---------------------------------------------------------------------------
+// Real Crank-Nicolson systems (matrix build of channel.f:771-1005, BC rows
+// of APPLY_BC_{1,2,3,TH}_{LOWER,UPPER} channel.f:1660-2282).  With LES off
+// the matrix of each field depends only on j, so all (x,z) columns share it.
+// ---------------------------------------------------------------------------
+struct FactorIn {
+  int ny, nfields;
+  double nu, nu_v_scale, pr[DH_MAX_TH];
+  int jstart, jend, jstart_th[DH_MAX_TH], jend_th[DH_MAX_TH];
+  int bcmin[kMaxFields], bcmax[kMaxFields];
+  double c1min[kMaxFields], c1max[kMaxFields];
+  const double *dy, *dyf;
+};
+
+// Row J (0..NY+1) of the system of field f: lower L, diagonal D, upper U,
+// constant right-hand side V of a boundary row (isbc=1), exactly as the
+// reference fills MATL/MATD/MATU/VEC: initialise (0,1,0,0), interior rows,
+// then the BC rows overwrite.
+__host__ __device__ inline void real_row(const FactorIn &in, int f, double temp1, int J, double &L, double &D,
+                                         double &U, double &V, int &isbc) {
+  const int NY = in.ny;
+  const double *DY = in.dy, *DYF = in.dyf;
+  L = 0.;
+  D = 1.;
+  U = 0.;
+  V = 0.;
+  isbc = 1;
+  if (f == 1) {  // U2, channel.f:856-862; APPLY_BC_2_LOWER/UPPER
+    if (J >= 2 && J <= NY) {
+      L = -temp1 * in.nu_v_scale / (DYF[J - 1] * DY[J]);
+      D = 1. + temp1 * in.nu_v_scale / (DYF[J] * DY[J]) + temp1 * in.nu_v_scale / (DYF[J - 1] * DY[J]);
+      U = -temp1 * in.nu_v_scale / (DYF[J] * DY[J]);
+      isbc = 0;
+    }
+    if (in.bcmin[f] == 0 && (J == 1 || J == 2)) {
+      L = 0.; D = 1.; U = 0.; V = in.c1min[f]; isbc = 1;
+    } else if (in.bcmin[f] == 1 && J == 1) {
+      D = -1.; U = 1.; L = 0.; V = DYF[1] * in.c1min[f]; isbc = 1;
+    }
+    if (J == 0) {
+      L = 0.; D = 1.; U = 0.; V = 0.; isbc = 1;
+    }
+    if (in.bcmax[f] == 0 && (J == NY || J == NY + 1)) {
+      L = 0.; D = 1.; U = 0.; V = in.c1max[f]; isbc = 1;
+    } else if (in.bcmax[f] == 1 && J == NY + 1) {
+      L = -1.; D = 1.; U = 0.; V = DYF[NY] * in.c1max[f]; isbc = 1;
+    }
+    return;
+  }
+  if (f < 3) {  // U1, U3: channel.f:914-920, 978-984
+    if (J >= in.jstart && J <= in.jend) {
+      L = -temp1 * in.nu_v_scale / (DY[J] * DYF[J]);
+      D = 1. - temp1 * in.nu_v_scale * (-1. / (DY[J + 1] * DYF[J]) - 1. / (DY[J] * DYF[J]));
+      U = -temp1 * in.nu_v_scale / (DY[J + 1] * DYF[J]);
+      isbc = 0;
+    }
+  } else {  // TH: channel.f:783-791
+    const int n = f - 3;
+    if (J >= in.jstart_th[n] && J <= in.jend_th[n]) {
+      const double c = temp1 / in.pr[n] * in.nu_v_scale;
+      L = -c / (DY[J] * DYF[J]);
+      D = 1. + c / (DY[J + 1] * DYF[J]) + c / (DY[J] * DYF[J]);
+      U = -c / (DY[J + 1] * DYF[J]);
+      isbc = 0;
+    }
+  }
+  // APPLY_BC_{1,3,TH}_LOWER / _UPPER
+  if (J == 0) {
+    L = 0.; D = 1.; U = 0.; V = 0.; isbc = 1;
+  } else if (J == 1) {
+    if (in.bcmin[f] == 0) {
+      L = 0.; D = 1.; U = 0.; V = in.c1min[f];
+    } else {
+      L = 0.; D = -1.; U = 1.; V = DY[2] * in.c1min[f];
+    }
+    isbc = 1;
+  }
+  if (J == NY + 1) {
+    L = 0.; D = 1.; U = 0.; V = 0.; isbc = 1;
+  } else if (J == NY) {
+    if (in.bcmax[f] == 0) {
+      L = 0.; D = 1.; U = 0.; V = in.c1max[f];
+    } else {
+      L = -1.; D = 1.; U = 0.; V = DY[NY] * in.c1max[f];
+    }
+    isbc = 1;
   }
 }
 
+// Parallel cyclic reduction tables of the shared matrix of (rk, field):
+// step s (distance 2^s) eliminates x_{i-2^s}, x_{i+2^s} from row i with the
+// multipliers k1 = a_i/b_{i-d}, k2 = c_i/b_{i+d}; after all steps x_i =
+// d_i / b_i.  Rows outside 0..NY+1 are the identity.  One block per (rk,
+// field); dynamic LDS 3*(NY+2) doubles.  The constant right-hand sides of
+// the boundary rows (dt independent) are written by the rk=0 blocks.
+__global__ void __launch_bounds__(256) k_pcr_tables(const Clock *clk, FactorIn in, RealFactors fac) {
+  extern __shared__ double sm[];
+  const int nrow = in.ny + 2;
+  double *a = sm, *b = sm + nrow, *c = sm + 2 * nrow;
+  const int rk = blockIdx.x / in.nfields, f = blockIdx.x - rk * in.nfields;
+  const double temp1 = in.nu * clk->h_bar[rk] / 2.0;
+  for (int i = threadIdx.x; i < nrow; i += blockDim.x) {
+    double L, D, U, V;
+    int bc;
+    real_row(in, f, temp1, i, L, D, U, V, bc);
+    a[i] = L;
+    b[i] = D;
+    c[i] = U;
+    if (rk == 0) {
+      fac.isbc[(long long)f * nrow + i] = bc;
+      fac.vconst[(long long)f * nrow + i] = V;
+    }
+  }
+  __syncthreads();
+  const long long tb = ((long long)rk * in.nfields + f) * (long long)kMaxPcr * nrow;
+  int s = 0;
+  for (int d = 1; d < nrow; d *= 2, ++s) {
+    double na[8], nb[8], nc[8];
+    int q = 0;
+    for (int i = threadIdx.x; i < nrow; i += blockDim.x, ++q) {
+      const bool lo = i - d >= 0, hi = i + d < nrow;
+      const double k1 = lo ? a[i] / b[i - d] : 0.0;
+      const double k2 = hi ? c[i] / b[i + d] : 0.0;
+      na[q] = lo ? -a[i - d] * k1 : 0.0;
+      nc[q] = hi ? -c[i + d] * k2 : 0.0;
+      nb[q] = b[i] - (lo ? c[i - d] * k1 : 0.0) - (hi ? a[i + d] * k2 : 0.0);
+      fac.k1[tb + (long long)s * nrow + i] = k1;
+      fac.k2[tb + (long long)s * nrow + i] = k2;
+    }
+    __syncthreads();
+    q = 0;
+    for (int i = threadIdx.x; i < nrow; i += blockDim.x, ++q) {
+      a[i] = na[q];
+      b[i] = nb[q];
+      c[i] = nc[q];
+    }
+    __syncthreads();
+  }
+  for (int i = threadIdx.x; i < nrow; i += blockDim.x)
+    fac.bfin[((long long)rk * in.nfields + f) * nrow + i] = b[i];
+}
+
 // ---------------------------------------------------------------------------
-// Real Crank-Nicolson solves (channel.f:606-626, 739-747 explicit part;
-// 771-1029 implicit part with APPLY_BC_* rows and THOMAS_*_REAL_MPI).
+// Real Crank-Nicolson solves of every (x,z) column (channel.f:606-626 and
+// 739-747 explicit half; 771-1029 implicit half), solved by applying the
+// shared PCR tables to the right-hand sides held in LDS; fused CFL minimum
+// of COURANT (courant.f:34-43).  Block: kSolveCols consecutive x columns of
+// one field; LDS [row][col].
 // ---------------------------------------------------------------------------
+constexpr int kSolveCols = 32;
+constexpr int kSolveThreads = 1024;
+constexpr int kSolveGroups = kSolveThreads / kSolveCols;  // row groups
+constexpr int kSolveRows = 9;  // contiguous rows per group: NY+2 <= 9 * 32
+
 struct SolveArgs {
   Geo g;
   Clock *clk;
@@ -261,235 +425,160 @@ struct SolveArgs {
   double cfl, dx, dz;
 };
 
-__global__ void __launch_bounds__(256) k_solve(SolveArgs a) {
+__global__ void __launch_bounds__(kSolveThreads) k_solve(SolveArgs a) {
+  extern __shared__ double sd[];
+  __shared__ double red[kSolveThreads / 64];
   const Geo &g = a.g;
   const int f = blockIdx.y;
-  const long long col = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long pp = g.pp;
-  const int NY = g.ny;
+  const int NY = g.ny, nrow = NY + 2;
   const int rk = a.rk;
+  const int nst = a.fac.nsteps;
   const double hb = a.clk->h_bar[rk];
   const double T1 = a.nu * hb / 2.0;
   const double coef = (f < 3) ? T1 * a.nu_v_scale : (T1 / a.pr[f - 3]) * a.nu_v_scale;
-  const int nrow = NY + 2;
-  const double *Ap = a.fac.ap + ((long long)rk * a.nfields + f) * nrow;
-  const double *Bp = a.fac.bp + ((long long)rk * a.nfields + f) * nrow;
-  const double *Cc = a.fac.cc + ((long long)rk * a.nfields + f) * nrow;
-  const int *isbc = a.fac.isbc + (long long)f * nrow;
-  const double *vc = a.fac.vconst + (long long)f * nrow;
-  const double *dy = g.dy, *dyf = g.dyf;
-  const int wlo = a.wlo[f];
-  double dtmin = __longlong_as_double(0x7FF0000000000000ll);
-  if (col < pp) {
-    double *U = a.uphys + (long long)f * a.fs + col;
-    const double *R = a.rphys + (long long)f * a.fs + col;
-    double um = 0.0, u0 = U[0], up;
-    double gprev = 0.0;
-    for (int J = 0; J <= NY + 1; ++J) {
-      up = (J <= NY) ? U[(long long)(J + 1) * pp] : 0.0;
-      double rhs;
-      if (isbc[J]) {
-        rhs = vc[J];
+  // LDS: right-hand sides [row][col], then this (rk, field)'s tables
+  double *D = sd;
+  double *K1 = D + nrow * kSolveCols, *K2 = K1 + nst * nrow;
+  double *BF = K2 + nst * nrow, *VC = BF + nrow, *DYs = VC + nrow, *DYFs = DYs + nrow;
+  int *IB = reinterpret_cast<int *>(DYFs + nrow);
+  const long long tb = ((long long)rk * a.nfields + f) * (long long)kMaxPcr * nrow;
+  for (int i = threadIdx.x; i < nst * nrow; i += blockDim.x) {
+    K1[i] = a.fac.k1[tb + i];
+    K2[i] = a.fac.k2[tb + i];
+  }
+  for (int i = threadIdx.x; i < nrow; i += blockDim.x) {
+    BF[i] = a.fac.bfin[((long long)rk * a.nfields + f) * nrow + i];
+    VC[i] = a.fac.vconst[(long long)f * nrow + i];
+    IB[i] = a.fac.isbc[(long long)f * nrow + i];
+    DYs[i] = g.dy[i];
+    DYFs[i] = g.dyf[i];
+  }
+  const int lane = threadIdx.x % kSolveCols, rg = threadIdx.x / kSolveCols;
+  const int rows = (nrow + kSolveGroups - 1) / kSolveGroups;
+  const int i0 = rg * rows;
+  const long long col = (long long)blockIdx.x * kSolveCols + lane;
+  const bool live = col < pp;
+  double *U = a.uphys + (long long)f * a.fs + col;
+  const double *R = a.rphys + (long long)f * a.fs + col;
+  // U_old rows i0-1 .. i0+rows and R rows i0 .. i0+rows-1, all loads in flight
+  double u[kSolveRows + 2], r[kSolveRows];
+#pragma unroll
+  for (int q = 0; q < kSolveRows + 2; ++q) {
+    const int i = i0 - 1 + q;
+    u[q] = (live && q <= rows + 1 && i >= 0 && i < nrow) ? U[(long long)i * pp] : 0.0;
+  }
+#pragma unroll
+  for (int q = 0; q < kSolveRows; ++q) {
+    const int i = i0 + q;
+    r[q] = (live && q < rows && i < nrow) ? R[(long long)i * pp] : 0.0;
+  }
+  __syncthreads();
+  // right-hand sides: boundary constants or R + explicit CN term of U_old
+#pragma unroll
+  for (int q = 0; q < kSolveRows; ++q) {
+    const int i = i0 + q;
+    if (q < rows && i < nrow) {
+      double d = 0.0;
+      if (IB[i]) {
+        d = VC[i];
       } else {
+        const double um = u[q], u0 = u[q + 1], up = u[q + 2];
         double ex;
         if (f == 1)
-          ex = ((up - u0) / dyf[J] - (u0 - um) / dyf[J - 1]) / dy[J];
+          ex = ((up - u0) / DYFs[i] - (u0 - um) / DYFs[i - 1]) / DYs[i];
         else
-          ex = ((up - u0) / dy[J + 1] - (u0 - um) / dy[J]) / dyf[J];
-        rhs = R[(long long)J * pp] + coef * ex;
+          ex = ((up - u0) / DYs[i + 1] - (u0 - um) / DYs[i]) / DYFs[i];
+        d = r[q] + coef * ex;
       }
-      const double gj = (J == 0) ? rhs : rhs + Ap[J] * gprev;
-      if (J >= wlo) U[(long long)J * pp] = gj;
-      gprev = gj;
-      um = u0;
-      u0 = up;
-    }
-    double x = gprev / Bp[NY + 1];
-    U[(long long)(NY + 1) * pp] = x;
-    const bool cour = a.courant && f < 3;
-    const double dxf = (f == 0) ? a.dx : a.dz;
-    for (int J = NY; J >= wlo; --J) {
-      const double gj = U[(long long)J * pp];
-      const double b = Bp[J];
-      x = (b != 0.0) ? (gj - Cc[J] * x) / b : 0.0;
-      U[(long long)J * pp] = x;
-      if (cour && J >= 1) {
-        const double d = (f == 1) ? dy[J] : dxf;
-        const double t = a.cfl * d / fabs(x);
-        dtmin = fmin(dtmin, t);
-      }
+      D[i * kSolveCols + lane] = d;
     }
   }
-  if (a.courant && f < 3) {
-    __shared__ double red[256];
-    red[threadIdx.x] = dtmin;
-    __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-      if (threadIdx.x < s) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + s]);
-      __syncthreads();
+  __syncthreads();
+  for (int s = 0, dd = 1; s < nst; ++s, dd *= 2) {
+    const double *k1 = K1 + s * nrow, *k2 = K2 + s * nrow;
+    double nd[kSolveRows];
+#pragma unroll
+    for (int q = 0; q < kSolveRows; ++q) {
+      const int i = i0 + q;
+      if (q < rows && i < nrow) {
+        double v = D[i * kSolveCols + lane];
+        if (i - dd >= 0) v -= k1[i] * D[(i - dd) * kSolveCols + lane];
+        if (i + dd < nrow) v -= k2[i] * D[(i + dd) * kSolveCols + lane];
+        nd[q] = v;
+      }
     }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kSolveRows; ++q) {
+      const int i = i0 + q;
+      if (q < rows && i < nrow) D[i * kSolveCols + lane] = nd[q];
+    }
+    __syncthreads();
+  }
+  const bool cour = a.courant && f < 3;
+  const double dxf = (f == 0) ? a.dx : a.dz;
+  double dtmin = __longlong_as_double(0x7FF0000000000000ll);
+  const int wlo = a.wlo[f], whi = a.whi[f];
+#pragma unroll
+  for (int q = 0; q < kSolveRows; ++q) {
+    const int i = i0 + q;
+    if (!live || q >= rows || i >= nrow || i < wlo || i > whi) continue;
+    const double x = D[i * kSolveCols + lane] / BF[i];
+    U[(long long)i * pp] = x;
+    if (cour && i >= 1 && i <= NY) {
+      const double dl = (f == 1) ? DYs[i] : dxf;
+      dtmin = fmin(dtmin, a.cfl * dl / fabs(x));
+    }
+  }
+  if (cour) {
+    // wave minimum by shuffles, then one LDS slot per wave
+    for (int o = 32; o > 0; o >>= 1) dtmin = fmin(dtmin, __shfl_down(dtmin, o));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = dtmin;
+    __syncthreads();
     if (threadIdx.x == 0) {
-      const double v = red[0];
+      double v = red[0];
+      for (int w = 1; w < (int)(blockDim.x >> 6); ++w) v = fmin(v, red[w]);
       if (v >= 0.0) atomicMin(&a.clk->cfl_min_bits, (unsigned long long)__double_as_longlong(v));
     }
   }
 }
 
-// ---------------------------------------------------------------------------
-// Factor tables of the real CN matrices (matrix build of channel.f:771-1005,
-// BC rows of channel.f:1660-2282, forward elimination of mpi.f:630-703).
-// ---------------------------------------------------------------------------
-struct FactorIn {
-  int ny, nfields;
-  double nu, nu_v_scale, pr[DH_MAX_TH];
-  int jstart, jend, jstart_th[DH_MAX_TH], jend_th[DH_MAX_TH];
-  int bcmin[kMaxFields], bcmax[kMaxFields];
-  double c1min[kMaxFields], c1max[kMaxFields];
-  const double *dy, *dyf;
-};
-
-// One field, one RK substep.  L,D,Uu,V are caller scratch of NY+2 rows.
-__host__ __device__ inline void build_real_system(const FactorIn &in, int f, double temp1, double *L, double *D,
-                                                  double *Uu, double *V, int *isbc) {
-  const int NY = in.ny;
-  const double *DY = in.dy, *DYF = in.dyf;
-  for (int j = 0; j <= NY + 1; ++j) {
-    L[j] = 0.;
-    D[j] = 1.;
-    Uu[j] = 0.;
-    V[j] = 0.;
-    isbc[j] = 1;
+// COURANT (courant.f:45-62) from the CFL minimum of the solve kernel; the
+// PCR tables for the new DELTA_T are rebuilt right after by k_pcr_tables.
+__global__ void k_courant_dt(Clock *clk, double dt_cap, int variable_dt, int update_dt) {
+  if (threadIdx.x != 0) return;
+  Clock &c = *clk;
+  // TEMP4 of this substep was fixed before COURANT (channel.f:80); the
+  // pressure update after REM_DIV (channel.f:1059-1065) still divides by it
+  c.h_cp = c.h_bar[2];
+  if (variable_dt && (c.time_step % update_dt) == 0) {
+    double dt = dt_cap;
+    const double m = __longlong_as_double((long long)c.cfl_min_bits);
+    if (m < dt) dt = m;
+    double DT;
+    if (dt <= 0)
+      DT = 0.0001;
+    else if (dt >= 999.)
+      DT = 999.0;
+    else
+      DT = dt;
+    c.dt = DT;
+    c.h_bar[0] = DT * (8.0 / 15.0);
+    c.h_bar[1] = DT * (2.0 / 15.0);
+    c.h_bar[2] = DT * (5.0 / 15.0);
   }
-  if (f == 1) {  // U2, channel.f:856-862
-    for (int J = 2; J <= NY; ++J) {
-      L[J] = -temp1 * in.nu_v_scale / (DYF[J - 1] * DY[J]);
-      D[J] = 1. + temp1 * in.nu_v_scale / (DYF[J] * DY[J]) + temp1 * in.nu_v_scale / (DYF[J - 1] * DY[J]);
-      Uu[J] = -temp1 * in.nu_v_scale / (DYF[J] * DY[J]);
-      isbc[J] = 0;
-    }
-    // APPLY_BC_2_LOWER / _UPPER
-    if (in.bcmin[f] == 0) {
-      for (int J = 1; J <= 2; ++J) {
-        L[J] = 0.; D[J] = 1.; Uu[J] = 0.; V[J] = in.c1min[f]; isbc[J] = 1;
-      }
-    } else if (in.bcmin[f] == 1) {
-      D[1] = -1.; Uu[1] = 1.; L[1] = 0.; V[1] = DYF[1] * in.c1min[f]; isbc[1] = 1;
-    }
-    L[0] = 0.; D[0] = 1.; Uu[0] = 0.; V[0] = 0.; isbc[0] = 1;
-    if (in.bcmax[f] == 0) {
-      L[NY + 1] = 0.; D[NY + 1] = 1.; Uu[NY + 1] = 0.; V[NY + 1] = in.c1max[f]; isbc[NY + 1] = 1;
-      L[NY] = 0.; D[NY] = 1.; Uu[NY] = 0.; V[NY] = in.c1max[f]; isbc[NY] = 1;
-    } else if (in.bcmax[f] == 1) {
-      L[NY + 1] = -1.; D[NY + 1] = 1.; Uu[NY + 1] = 0.; V[NY + 1] = DYF[NY] * in.c1max[f]; isbc[NY + 1] = 1;
-    }
-    return;
-  }
-  int js, je;
-  double c;
-  if (f < 3) {  // U1, U3: channel.f:914-920, 978-984
-    js = in.jstart;
-    je = in.jend;
-    for (int J = js; J <= je; ++J) {
-      L[J] = -temp1 * in.nu_v_scale / (DY[J] * DYF[J]);
-      D[J] = 1. - temp1 * in.nu_v_scale * (-1. / (DY[J + 1] * DYF[J]) - 1. / (DY[J] * DYF[J]));
-      Uu[J] = -temp1 * in.nu_v_scale / (DY[J + 1] * DYF[J]);
-      isbc[J] = 0;
-    }
-  } else {  // TH: channel.f:783-791
-    const int n = f - 3;
-    js = in.jstart_th[n];
-    je = in.jend_th[n];
-    c = temp1 / in.pr[n] * in.nu_v_scale;
-    for (int J = js; J <= je; ++J) {
-      L[J] = -c / (DY[J] * DYF[J]);
-      D[J] = 1. + c / (DY[J + 1] * DYF[J]) + c / (DY[J] * DYF[J]);
-      Uu[J] = -c / (DY[J + 1] * DYF[J]);
-      isbc[J] = 0;
-    }
-  }
-  // APPLY_BC_{1,3,TH}_LOWER / _UPPER
-  if (in.bcmin[f] == 0) {
-    L[0] = 0.; D[0] = 1.; Uu[0] = 0.; V[0] = 0.; isbc[0] = 1;
-    L[1] = 0.; D[1] = 1.; Uu[1] = 0.; V[1] = in.c1min[f]; isbc[1] = 1;
-  } else {
-    L[0] = 0.; D[0] = 1.; Uu[0] = 0.; V[0] = 0.; isbc[0] = 1;
-    L[1] = 0.; D[1] = -1.; Uu[1] = 1.; V[1] = DY[2] * in.c1min[f]; isbc[1] = 1;
-  }
-  if (in.bcmax[f] == 0) {
-    L[NY + 1] = 0.; D[NY + 1] = 1.; Uu[NY + 1] = 0.; V[NY + 1] = 0.; isbc[NY + 1] = 1;
-    L[NY] = 0.; D[NY] = 1.; Uu[NY] = 0.; V[NY] = in.c1max[f]; isbc[NY] = 1;
-  } else {
-    L[NY] = -1.; D[NY] = 1.; Uu[NY] = 0.; V[NY] = DY[NY] * in.c1max[f]; isbc[NY] = 1;
-    L[NY + 1] = 0.; D[NY + 1] = 1.; Uu[NY + 1] = 0.; V[NY + 1] = 0.; isbc[NY + 1] = 1;
-  }
+  c.cfl_min_bits = 0x7FF0000000000000ull;
 }
 
-// Forward elimination of the matrix only (mpi.f:668-702 on one rank):
-// A(J) <- -A(J)/B(J-1), B(J) <- B(J) + A(J) C(J-1), J = 1..NY+1.
-__host__ __device__ inline void factor_real(int NY, double *L, double *D, const double *Uu) {
-  for (int J = 1; J <= NY + 1; ++J) {
-    L[J] = -L[J] / D[J - 1];
-    D[J] = D[J] + L[J] * Uu[J - 1];
-  }
-}
-
-// COURANT (courant.f:45-62) after the CFL minimum of the solve kernel, then
-// the factor tables of all three substeps for the new DELTA_T: one thread
-// per (rk, field) chain.  V/isbc rows of the builder go to LDS scratch.
-__global__ void k_courant_fin(Clock *clk, double dt_cap, int variable_dt, int update_dt, FactorIn in,
-                              RealFactors fac) {
-  extern __shared__ double fsm[];
-  __shared__ int changed;
-  if (threadIdx.x == 0) {
-    changed = 0;
-    Clock &c = *clk;
-    // TEMP4 of this substep was fixed before COURANT (channel.f:80); the
-    // pressure update after REM_DIV (channel.f:1059-1065) still divides by it
-    c.h_cp = c.h_bar[2];
-    if (variable_dt && (c.time_step % update_dt) == 0) {
-      double dt = dt_cap;
-      const double m = __longlong_as_double((long long)c.cfl_min_bits);
-      if (m < dt) dt = m;
-      double DT;
-      if (dt <= 0)
-        DT = 0.0001;
-      else if (dt >= 999.)
-        DT = 999.0;
-      else
-        DT = dt;
-      c.dt = DT;
-      c.h_bar[0] = DT * (8.0 / 15.0);
-      c.h_bar[1] = DT * (2.0 / 15.0);
-      c.h_bar[2] = DT * (5.0 / 15.0);
-      changed = 1;
-    }
-    c.cfl_min_bits = 0x7FF0000000000000ull;
-  }
-  __syncthreads();
-  if (!changed) return;
-  const int nrow = in.ny + 2;
-  const int t = threadIdx.x;
-  if (t < 3 * in.nfields) {
-    const int rk = t / in.nfields, f = t - rk * in.nfields;
-    const double temp1 = in.nu * clk->h_bar[rk] / 2.0;
-    double *L = fac.ap + ((long long)rk * in.nfields + f) * nrow;
-    double *D = fac.bp + ((long long)rk * in.nfields + f) * nrow;
-    double *Uu = fac.cc + ((long long)rk * in.nfields + f) * nrow;
-    double *Vs = fsm + (long long)t * nrow;
-    int *Is = reinterpret_cast<int *>(fsm + (long long)3 * in.nfields * nrow) + (long long)t * nrow;
-    build_real_system(in, f, temp1, L, D, Uu, Vs, Is);
-    factor_real(in.ny, L, D, Uu);
-  }
-}
-
-void build_real_factors_host(const Ctx &c, double dt, std::vector<double> &ap, std::vector<double> &bp,
-                             std::vector<double> &cc, std::vector<int> &isbc, std::vector<double> &vconst);
-
 // ---------------------------------------------------------------------------
-// REM_DIV_CHAN (channel.f:1103-1214) + CP update (channel.f:1059-1065).
-// One thread per (kx, kz) mode; the pivots and modified right-hand sides of
-// the forward sweep are parked in [J][mode] scratch for the backward sweep.
+// REM_DIV_CHAN (channel.f:1103-1214 with APPLY_BC_REM_DIV_MPI mpi.f:1239-1278)
+// and the CP update (channel.f:1059-1065).  Each block holds the Poisson
+// systems of kRdModes consecutive (kx,kz) modes, rows 1..NY, in LDS and
+// solves them by parallel cyclic reduction; then projects CU and updates CP.
 // ---------------------------------------------------------------------------
+constexpr int kRdModes = 8;  // rows per thread <= 9: NY <= 9 * 256 / kRdModes
+
 struct RemDivArgs {
   Geo g;
   Clock *clk;
@@ -498,97 +587,253 @@ struct RemDivArgs {
   double2 *cu[3];
   double2 *cp;
   double2 *phi;
-  double *sbp;
-  double2 *sg;
   const double *rl, *ru;  // 1/(DY(J)DYF(J)), 1/(DY(J+1)DYF(J))
 };
 
-__global__ void __launch_bounds__(64) k_rem_div(RemDivArgs a) {
+__global__ void __launch_bounds__(256) k_rem_div(RemDivArgs a) {
+  extern __shared__ double sr[];
   const Geo &g = a.g;
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= g.pf) return;
-  const int kzp = m / g.nkx1, kx = m - kzp * g.nkx1;
   const int NY = g.ny;
+  const int n = NY;  // rows 1..NY stored at 0..NY-1
+  double *A = sr, *B = sr + n * kRdModes, *C = sr + 2 * n * kRdModes;
+  double2 *D = reinterpret_cast<double2 *>(sr + 3 * n * kRdModes);
+  const int lane = threadIdx.x % kRdModes, rg = threadIdx.x / kRdModes;
+  const int ngrp = blockDim.x / kRdModes;
+  const int m = blockIdx.x * kRdModes + lane;
+  const bool live = m < g.pf;
   const long long pf = g.pf;
+  int kx = 0, kzp = 0;
+  if (live) {
+    kzp = (int)fdiv((unsigned)m, g.f_nkx1);
+    kx = m - kzp * g.nkx1;
+  }
   const double kxv = g.kx[kx], kzv = g.kz[kzp], kx2 = g.kx2[kx], kz2 = g.kz2[kzp];
   const bool mean = (kx == 0 && kzp == 0);
-  double2 *CU1 = a.cu[0] + m, *CU2 = a.cu[1] + m, *CU3 = a.cu[2] + m;
-  // row 1 (APPLY_BC_REM_DIV_MPI, mpi.f:1244-1262)
-  double bprev = 1.0;
-  double uprev = mean ? 0.0 : -1.0;
-  double2 gprev = make_double2(0.0, 0.0);
-  double2 cu2_j = CU2[pf * 2];
-  for (int J = 2; J <= NY; ++J) {
-    double L, D, Uu;
-    double2 V;
-    if (J < NY) {
+  // system rows J = 1..NY (channel.f:1126-1143, BC rows mpi.f:1244-1272)
+  for (int r = rg; r < n; r += ngrp) {
+    const int J = r + 1;
+    double L, Dg, Uu;
+    double2 V = make_double2(0.0, 0.0);
+    if (J == 1) {
+      L = 0.; Dg = 1.; Uu = mean ? 0. : -1.;
+    } else if (J == NY) {
+      L = 1.; Dg = -1.; Uu = 0.;
+    } else {
       L = a.rl[J];
       Uu = a.ru[J];
-      D = -kx2 - kz2 - Uu - L;
-      const double2 c1 = CU1[pf * J], c3 = CU3[pf * J];
-      const double2 c2p = CU2[pf * (J + 1)];
-      const double2 t1 = cmul_ik(kxv, c1), t3 = cmul_ik(kzv, c3);
-      const double d = g.dyf[J];
-      V = make_double2(t1.x + (c2p.x - cu2_j.x) / d + t3.x, t1.y + (c2p.y - cu2_j.y) / d + t3.y);
-      cu2_j = c2p;
-    } else {  // row NY (mpi.f:1265-1272)
-      L = 1.;
-      D = -1.;
-      Uu = 0.;
-      V = make_double2(0.0, 0.0);
+      Dg = -kx2 - kz2 - Uu - L;
+      if (live) {
+        const double2 c1 = a.cu[0][pf * J + m], c3 = a.cu[2][pf * J + m];
+        const double2 c2 = a.cu[1][pf * J + m], c2p = a.cu[1][pf * (J + 1) + m];
+        const double2 t1 = cmul_ik(kxv, c1), t3 = cmul_ik(kzv, c3);
+        const double d = g.dyf[J];
+        V = make_double2(t1.x + (c2p.x - c2.x) / d + t3.x, t1.y + (c2p.y - c2.y) / d + t3.y);
+      }
     }
-    const double A = (bprev != 0.0) ? -L / bprev : 0.0;
-    const double B = D + A * uprev;
-    const double2 G = make_double2(V.x + A * gprev.x, V.y + A * gprev.y);
-    a.sbp[pf * J + m] = B;
-    a.sg[pf * J + m] = G;
-    bprev = B;
-    uprev = Uu;
-    gprev = G;
+    A[r * kRdModes + lane] = L;
+    B[r * kRdModes + lane] = Dg;
+    C[r * kRdModes + lane] = Uu;
+    D[r * kRdModes + lane] = V;
   }
-  // backward sweep with projection (channel.f:1196-1211) and CP update
+  __syncthreads();
+  for (int dd = 1; dd < n; dd *= 2) {
+    double na[9], nb[9], nc[9];
+    double2 nd[9];
+    int q = 0;
+    for (int r = rg; r < n; r += ngrp, ++q) {
+      const int o = r * kRdModes + lane;
+      const bool lo = r - dd >= 0, hi = r + dd < n;
+      const int ol = (r - dd) * kRdModes + lane, oh = (r + dd) * kRdModes + lane;
+      const double k1 = lo ? A[o] / B[ol] : 0.0;
+      const double k2 = hi ? C[o] / B[oh] : 0.0;
+      na[q] = lo ? -A[ol] * k1 : 0.0;
+      nc[q] = hi ? -C[oh] * k2 : 0.0;
+      nb[q] = B[o] - (lo ? C[ol] * k1 : 0.0) - (hi ? A[oh] * k2 : 0.0);
+      double2 v = D[o];
+      if (lo) v = make_double2(v.x - k1 * D[ol].x, v.y - k1 * D[ol].y);
+      if (hi) v = make_double2(v.x - k2 * D[oh].x, v.y - k2 * D[oh].y);
+      nd[q] = v;
+    }
+    __syncthreads();
+    q = 0;
+    for (int r = rg; r < n; r += ngrp, ++q) {
+      const int o = r * kRdModes + lane;
+      A[o] = na[q];
+      B[o] = nb[q];
+      C[o] = nc[q];
+      D[o] = nd[q];
+    }
+    __syncthreads();
+  }
+  // phi = d / b (held in D), then the projection and pressure update
+  for (int r = rg; r < n; r += ngrp) {
+    const int o = r * kRdModes + lane;
+    const double b = B[o];
+    const double2 v = D[o];
+    D[o] = (b != 0.0) ? make_double2(v.x / b, v.y / b) : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  if (!live) return;
   const double hb = (a.rk < 0) ? 1.0 : (a.cp_old ? a.clk->h_cp : a.clk->h_bar[a.rk]);
-  double2 phi_next = make_double2(0.0, 0.0);
-  for (int J = NY; J >= 1; --J) {
-    double2 phi;
-    if (J == NY) {
-      phi = (bprev != 0.0) ? make_double2(gprev.x / bprev, gprev.y / bprev) : make_double2(0.0, 0.0);
-    } else {
-      double B, Uu;
+  for (int r = rg; r < n; r += ngrp) {
+    const int J = r + 1;
+    const double2 phi = D[r * kRdModes + lane];
+    a.phi[pf * J + m] = phi;
+    double2 t = cmul_ik(kxv, phi), u = a.cu[0][pf * J + m];
+    a.cu[0][pf * J + m] = make_double2(u.x - t.x, u.y - t.y);
+    t = cmul_ik(kzv, phi);
+    u = a.cu[2][pf * J + m];
+    a.cu[2][pf * J + m] = make_double2(u.x - t.x, u.y - t.y);
+    if (J >= 2) {
+      const double2 pm = D[(r - 1) * kRdModes + lane];
+      const double d = g.dy[J];
+      u = a.cu[1][pf * J + m];
+      a.cu[1][pf * J + m] = make_double2(u.x - (phi.x - pm.x) / d, u.y - (phi.y - pm.y) / d);
+      if (a.rk >= 0) {
+        double2 p = a.cp[pf * J + m];
+        a.cp[pf * J + m] = make_double2(p.x + phi.x / hb, p.y + phi.y / hb);
+      }
+    }
+  }
+}
+
+// REM_DIV_CHAN with the reference's Thomas sweeps (THOMAS_FORWARD/BACKWARD_
+// COMPLEX_MPI, mpi.f:708-865) in their operation order, one thread per
+// (kx,kz) mode: all threads stage the right-hand sides of kRdtModes modes
+// into LDS with coalesced loads, one thread per mode runs the sequential
+// sweeps out of LDS, and all threads apply the projection and the CP update.
+constexpr int kRdtModes = 32;
+
+__global__ void __launch_bounds__(256) k_rem_div_thomas(RemDivArgs a) {
+  extern __shared__ double st[];
+  const Geo &g = a.g;
+  const int NY = g.ny;
+  double2 *Gs = reinterpret_cast<double2 *>(st);                  // [J][mode], J = 0..NY
+  double *Bs = st + 2 * (NY + 1) * kRdtModes;                       // [J][mode]
+  double *rl = Bs + (NY + 1) * kRdtModes, *ru = rl + (NY + 1);       // [J]
+  const int lane = threadIdx.x % kRdtModes, rg = threadIdx.x / kRdtModes;
+  const int ngrp = blockDim.x / kRdtModes;
+  const int m = blockIdx.x * kRdtModes + lane;
+  const bool live = m < g.pf;
+  const long long pf = g.pf;
+  int kx = 0, kzp = 0;
+  if (live) {
+    kzp = (int)fdiv((unsigned)m, g.f_nkx1);
+    kx = m - kzp * g.nkx1;
+  }
+  const double kxv = g.kx[kx], kzv = g.kz[kzp], kx2 = g.kx2[kx], kz2 = g.kz2[kzp];
+  const bool mean = (kx == 0 && kzp == 0);
+  for (int j = threadIdx.x; j <= NY; j += blockDim.x) {
+    rl[j] = a.rl[j];
+    ru[j] = a.ru[j];
+  }
+  // right-hand sides of rows 2..NY-1 (channel.f:1137-1143); rows 1, NY are BC
+  // rows.  Batches of 4 rows per thread keep 16 loads in flight.
+  for (int J0 = 2 + rg; J0 < NY; J0 += 4 * ngrp) {
+    double2 c1[4], c2[4], c2p[4], c3[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int J = J0 + b * ngrp;
+      if (live && J < NY) {
+        c1[b] = a.cu[0][pf * J + m];
+        c3[b] = a.cu[2][pf * J + m];
+        c2[b] = a.cu[1][pf * J + m];
+        c2p[b] = a.cu[1][pf * (J + 1) + m];
+      }
+    }
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int J = J0 + b * ngrp;
+      if (J < NY) {
+        double2 V = make_double2(0.0, 0.0);
+        if (live) {
+          const double2 t1 = cmul_ik(kxv, c1[b]), t3 = cmul_ik(kzv, c3[b]);
+          const double d = g.dyf[J];
+          V = make_double2(t1.x + (c2p[b].x - c2[b].x) / d + t3.x, t1.y + (c2p[b].y - c2[b].y) / d + t3.y);
+        }
+        Gs[J * kRdtModes + lane] = V;
+      }
+    }
+  }
+  __syncthreads();
+  if (rg == 0 && live) {
+    // forward, J = 2..NY (mpi.f:736-747); row 1: D=1, U=0 (mean) or -1, G=0
+    double bprev = 1.0, uprev = mean ? 0.0 : -1.0;
+    double2 gprev = make_double2(0.0, 0.0);
+    for (int J = 2; J <= NY; ++J) {
+      double L, D, U;
+      double2 V;
+      if (J < NY) {
+        L = rl[J];
+        U = ru[J];
+        D = -kx2 - kz2 - U - L;
+        V = Gs[J * kRdtModes + lane];
+      } else {
+        L = 1.;
+        D = -1.;
+        U = 0.;
+        V = make_double2(0.0, 0.0);
+      }
+      const double A = (bprev != 0.0) ? -L / bprev : 0.0;
+      const double B = D + A * uprev;
+      const double2 G = make_double2(V.x + A * gprev.x, V.y + A * gprev.y);
+      Bs[J * kRdtModes + lane] = B;
+      Gs[J * kRdtModes + lane] = G;
+      bprev = B;
+      uprev = U;
+      gprev = G;
+    }
+    // backward, J = NY..1 (mpi.f:838-856)
+    double2 x = (bprev != 0.0) ? make_double2(gprev.x / bprev, gprev.y / bprev) : make_double2(0.0, 0.0);
+    Gs[NY * kRdtModes + lane] = x;
+    for (int J = NY - 1; J >= 1; --J) {
+      double B, U;
       double2 G;
       if (J >= 2) {
-        B = a.sbp[pf * J + m];
-        G = a.sg[pf * J + m];
-        Uu = a.ru[J];
+        B = Bs[J * kRdtModes + lane];
+        G = Gs[J * kRdtModes + lane];
+        U = ru[J];
       } else {
         B = 1.0;
         G = make_double2(0.0, 0.0);
-        Uu = mean ? 0.0 : -1.0;
+        U = mean ? 0.0 : -1.0;
       }
-      phi = (B != 0.0) ? make_double2((G.x - Uu * phi_next.x) / B, (G.y - Uu * phi_next.y) / B)
-                       : make_double2(0.0, 0.0);
+      x = (B != 0.0) ? make_double2((G.x - U * x.x) / B, (G.y - U * x.y) / B) : make_double2(0.0, 0.0);
+      Gs[J * kRdtModes + lane] = x;
     }
-    a.phi[pf * J + m] = phi;
-    // CU1, CU3 at J
-    {
+  }
+  __syncthreads();
+  if (!live) return;
+  const double hb = (a.rk < 0) ? 1.0 : (a.cp_old ? a.clk->h_cp : a.clk->h_bar[a.rk]);
+  for (int J0 = 1 + rg; J0 <= NY; J0 += 4 * ngrp) {
+    double2 u1[4], u2[4], u3[4], pc[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int J = J0 + b * ngrp;
+      if (J <= NY) {
+        u1[b] = a.cu[0][pf * J + m];
+        u3[b] = a.cu[2][pf * J + m];
+        u2[b] = a.cu[1][pf * J + m];
+        if (a.rk >= 0) pc[b] = a.cp[pf * J + m];
+      }
+    }
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int J = J0 + b * ngrp;
+      if (J > NY) continue;
+      const double2 phi = Gs[J * kRdtModes + lane];
+      a.phi[pf * J + m] = phi;
       double2 t = cmul_ik(kxv, phi);
-      double2 u = CU1[pf * J];
-      CU1[pf * J] = make_double2(u.x - t.x, u.y - t.y);
+      a.cu[0][pf * J + m] = make_double2(u1[b].x - t.x, u1[b].y - t.y);
       t = cmul_ik(kzv, phi);
-      u = CU3[pf * J];
-      CU3[pf * J] = make_double2(u.x - t.x, u.y - t.y);
+      a.cu[2][pf * J + m] = make_double2(u3[b].x - t.x, u3[b].y - t.y);
+      if (J >= 2) {
+        const double2 pm = Gs[(J - 1) * kRdtModes + lane];
+        const double d = g.dy[J];
+        a.cu[1][pf * J + m] = make_double2(u2[b].x - (phi.x - pm.x) / d, u2[b].y - (phi.y - pm.y) / d);
+        if (a.rk >= 0) a.cp[pf * J + m] = make_double2(pc[b].x + phi.x / hb, pc[b].y + phi.y / hb);
+      }
     }
-    // CU2 at J+1 (needs phi(J+1), phi(J)), J+1 in 2..NY
-    if (J + 1 <= NY) {
-      const double d = g.dy[J + 1];
-      double2 u = CU2[pf * (J + 1)];
-      CU2[pf * (J + 1)] = make_double2(u.x - (phi_next.x - phi.x) / d, u.y - (phi_next.y - phi.y) / d);
-    }
-    if (a.rk >= 0 && J >= 2) {
-      double2 p = a.cp[pf * J + m];
-      a.cp[pf * J + m] = make_double2(p.x + phi.x / hb, p.y + phi.y / hb);
-    }
-    phi_next = phi;
   }
 }
 
@@ -634,7 +879,7 @@ static StepArgs make_step_args(Ctx &c, int rk) {
   return a;
 }
 
-static FactorIn make_factor_in(const Ctx &c, const double *dy, const double *dyf) {
+static FactorIn make_factor_in(const Ctx &c) {
   FactorIn in{};
   in.ny = c.g.ny;
   in.nfields = c.nfields;
@@ -657,46 +902,27 @@ static FactorIn make_factor_in(const Ctx &c, const double *dy, const double *dyf
     in.c1min[3 + n] = p.th_bc_ymin_c1[n];
     in.c1max[3 + n] = p.th_bc_ymax_c1[n];
   }
-  in.dy = dy;
-  in.dyf = dyf;
+  in.dy = c.d_dy;
+  in.dyf = c.d_dyf;
   return in;
 }
 
-void build_real_factors_host(const Ctx &c, double dt, std::vector<double> &ap, std::vector<double> &bp,
-                             std::vector<double> &cc, std::vector<int> &isbc, std::vector<double> &vconst) {
-  const int nrow = c.g.ny + 2, nf = c.nfields;
-  FactorIn in = make_factor_in(c, c.dy.data(), c.dyf.data());
-  ap.assign((size_t)3 * nf * nrow, 0.0);
-  bp.assign((size_t)3 * nf * nrow, 0.0);
-  cc.assign((size_t)3 * nf * nrow, 0.0);
-  isbc.assign((size_t)nf * nrow, 1);
-  vconst.assign((size_t)nf * nrow, 0.0);
-  const double hbar[3] = {dt * (8.0 / 15.0), dt * (2.0 / 15.0), dt * (5.0 / 15.0)};
-  std::vector<double> V(nrow);
-  std::vector<int> I(nrow);
-  for (int rk = 0; rk < 3; ++rk)
-    for (int f = 0; f < nf; ++f) {
-      const double temp1 = c.p.nu * hbar[rk] / 2.0;
-      double *L = &ap[((size_t)rk * nf + f) * nrow];
-      double *D = &bp[((size_t)rk * nf + f) * nrow];
-      double *U = &cc[((size_t)rk * nf + f) * nrow];
-      build_real_system(in, f, temp1, L, D, U, V.data(), I.data());
-      factor_real(c.g.ny, L, D, U);
-      if (rk == 0)
-        for (int j = 0; j < nrow; ++j) {
-          isbc[(size_t)f * nrow + j] = I[j];
-          vconst[(size_t)f * nrow + j] = V[j];
-        }
-    }
+hipError_t launch_pcr_tables(Ctx &c) {
+  FactorIn in = make_factor_in(c);
+  const size_t shm = (size_t)3 * (c.g.ny + 2) * sizeof(double);
+  hipLaunchKernelGGL(k_pcr_tables, dim3(3 * c.nfields), dim3(256), shm, c.stream, c.clock, in, c.fac);
+  return hipGetLastError();
 }
-
-hipError_t launch_state_to_phys(Ctx &c);
-hipError_t launch_state_to_four(Ctx &c);
 
 static hipError_t launch_nonlinear(Ctx &c, int rk) {
   StepArgs a = make_step_args(c, rk);
   hipLaunchKernelGGL(k_nonlinear, dim3(c.g.ny, c.nfields), dim3(c.plane_threads), c.lds_bytes, c.stream, a);
   return hipGetLastError();
+}
+
+static size_t solve_lds(const Ctx &c) {
+  const size_t nrow = c.g.ny + 2;
+  return (nrow * kSolveCols + 2 * (size_t)c.fac.nsteps * nrow + 4 * nrow) * sizeof(double) + nrow * sizeof(int);
 }
 
 static hipError_t launch_solve(Ctx &c, int rk, int courant) {
@@ -720,9 +946,13 @@ static hipError_t launch_solve(Ctx &c, int rk, int courant) {
   a.cfl = c.p.cfl;
   a.dx = c.p.lx / c.g.nx;
   a.dz = c.p.lz / c.g.nz;
-  const int nb = (int)((c.g.pp + 255) / 256);
-  hipLaunchKernelGGL(k_solve, dim3(nb, c.nfields), dim3(256), 0, c.stream, a);
+  const int nb = (int)((c.g.pp + kSolveCols - 1) / kSolveCols);
+  hipLaunchKernelGGL(k_solve, dim3(nb, c.nfields), dim3(kSolveThreads), solve_lds(c), c.stream, a);
   return hipGetLastError();
+}
+
+static size_t rem_div_thomas_lds(const Ctx &c) {
+  return (size_t)(c.g.ny + 1) * kRdtModes * 3 * sizeof(double) + 2 * (size_t)(c.g.ny + 1) * sizeof(double);
 }
 
 hipError_t launch_rem_div(Ctx &c, int rk) {
@@ -730,26 +960,29 @@ hipError_t launch_rem_div(Ctx &c, int rk) {
   a.g = c.g;
   a.clk = c.clock;
   a.rk = rk;
-  a.cp_old = (c.p.variable_dt && rk == 2) ? 1 : 0;
+  a.cp_old = (rk == 2 && c.p.variable_dt) ? 1 : 0;
   for (int i = 0; i < 3; ++i) a.cu[i] = c.cu[i];
   a.cp = c.cp;
   a.phi = c.phi;
-  a.sbp = c.rd_bp;
-  a.sg = c.rd_g;
   a.rl = c.rd_l;
   a.ru = c.rd_u;
-  const int nb = (c.g.pf + 63) / 64;
-  hipLaunchKernelGGL(k_rem_div, dim3(nb), dim3(64), 0, c.stream, a);
+  if (c.rd_pcr) {
+    const int nb = (c.g.pf + kRdModes - 1) / kRdModes;
+    const size_t shm = (size_t)c.g.ny * kRdModes * 5 * sizeof(double);
+    hipLaunchKernelGGL(k_rem_div, dim3(nb), dim3(256), shm, c.stream, a);
+  } else {
+    const int nb = (c.g.pf + kRdtModes - 1) / kRdtModes;
+    hipLaunchKernelGGL(k_rem_div_thomas, dim3(nb), dim3(256), rem_div_thomas_lds(c), c.stream, a);
+  }
   return hipGetLastError();
 }
 
-static hipError_t launch_courant_fin(Ctx &c) {
-  FactorIn in = make_factor_in(c, c.d_dy, c.d_dyf);
-  const int nrow = c.g.ny + 2;
-  const size_t shm = (size_t)3 * c.nfields * nrow * (sizeof(double) + sizeof(int));
-  hipLaunchKernelGGL(k_courant_fin, dim3(1), dim3(64), shm, c.stream, c.clock, c.dt_cap, c.p.variable_dt,
-                     c.p.update_dt, in, c.fac);
-  return hipGetLastError();
+static hipError_t launch_courant(Ctx &c) {
+  hipLaunchKernelGGL(k_courant_dt, dim3(1), dim3(64), 0, c.stream, c.clock, c.dt_cap, c.p.variable_dt,
+                     c.p.update_dt);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return launch_pcr_tables(c);
 }
 
 // Kernels of one substep in launch order (channel.f:386-1065).
@@ -762,7 +995,7 @@ hipError_t launch_phase(Ctx &c, int rk, int phase) {
     case 0: return launch_state_to_phys(c);
     case 1: return launch_nonlinear(c, rk);
     case 2: return launch_solve(c, rk, courant);
-    case 3: return courant ? launch_courant_fin(c) : hipSuccess;
+    case 3: return courant ? launch_courant(c) : hipSuccess;
     case 4: return launch_state_to_four(c);
     case 5: return launch_rem_div(c, rk);
   }
@@ -794,7 +1027,13 @@ hipError_t launch_end_step(Ctx &c) {
 }
 
 hipError_t prepare_step_kernels(Ctx &c) {
-  return hipFuncSetAttribute((const void *)k_nonlinear, hipFuncAttributeMaxDynamicSharedMemorySize, c.lds_bytes);
+  hipError_t e =
+      hipFuncSetAttribute((const void *)k_nonlinear, hipFuncAttributeMaxDynamicSharedMemorySize, c.lds_bytes);
+  if (e != hipSuccess) return e;
+  e = hipFuncSetAttribute((const void *)k_solve, hipFuncAttributeMaxDynamicSharedMemorySize, (int)solve_lds(c));
+  if (e != hipSuccess) return e;
+  return hipFuncSetAttribute((const void *)k_rem_div_thomas, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)rem_div_thomas_lds(c));
 }
 
 }  // namespace dh
