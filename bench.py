@@ -77,6 +77,39 @@ def cpu_baseline(p, gy, ic, budget_s=12.0, max_steps=6):
                       f"(after 1 untimed step), FP64 C restatement with OpenMP over planes/columns, {dt:.1f} s"}
 
 
+def copy_calibration(mib, reps=5):
+    """Plain device-to-device copies of `mib` MiB (HIP's blit kernel): the
+    reference reading for the FETCH_SIZE / WRITE_SIZE counters, which count
+    L2-to-fabric traffic, and the achievable copy bandwidth of the device."""
+    import ctypes
+    from diablo_amd import channel
+    channel.load_library()
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    hip.hipMemset.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t]
+    hip.hipFree.argtypes = [ctypes.c_void_p]
+    n = mib << 20
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    for b in (src, dst):
+        if hip.hipMalloc(ctypes.byref(b), n) != 0:
+            raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(src, 1, n)
+    hip.hipDeviceSynchronize()
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        if hip.hipMemcpy(dst, src, n, 3) != 0:
+            raise RuntimeError("hipMemcpy failed")
+        hip.hipDeviceSynchronize()
+        times.append(time.perf_counter() - t0)
+    for b in (src, dst):
+        hip.hipFree(b)
+    best = min(times)
+    print(json.dumps({"copy_bytes": n, "read_plus_write_GBs": round(2 * n / best / 1e9, 1),
+                      "best_ms": round(best * 1e3, 4)}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -84,11 +117,16 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-reps", type=int, default=10)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run device-to-device copies of MIB MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "the multi-GPU y-slab design is documented, not built", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
 
     from diablo_amd import channel, configs
 

@@ -105,9 +105,9 @@ hipError_t dalloc(T **p, size_t count) {
 }
 
 void free_all(Ctx &c) {
-  void *ptrs[] = {c.tw_row, c.tw_col, c.tw_real, c.posr, c.posz, c.kx, c.kz, c.kx2, c.kz2, c.d_dy, c.d_dyf, c.d_gyf, c.cu[0], c.cu[1],
-                  c.cu[2], c.cp, c.cf[0], c.cf[1], c.cf[2], c.cth, c.cfth, c.phi, c.uphys, c.rphys, c.rd_bp,
-                  c.rd_g, c.clock, c.fac.k1, c.fac.k2, c.fac.bfin, c.fac.isbc, c.fac.vconst, c.rd_l, c.rd_u};
+  void *ptrs[] = {c.tw_row, c.tw_col, c.tw_real, c.posr, c.posrd, c.posz, c.posc, c.kx, c.kz, c.kx2, c.kz2, c.d_dy, c.d_dyf, c.d_gyf, c.cu[0], c.cu[1],
+                  c.cu[2], c.cp, c.cf[0], c.cf[1], c.cf[2], c.cth, c.cfth, c.phi, c.uphys, c.rphys, c.rd_ta,
+                  c.rd_tb, c.clock, c.fac.k1, c.fac.k2, c.fac.bfin, c.fac.tl, c.fac.tb, c.fac.tc, c.fac.tr, c.fac.isbc, c.rd_tr, c.fac.vconst, c.rd_l, c.rd_u};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c.step_graph) (void)hipGraphExecDestroy(c.step_graph);
@@ -185,12 +185,13 @@ int dh_create(const dh_params *p, dh_ctx **out) {
   g.pf = g.nkx1 * g.nkz1;
   g.lw = p->nx / 2;
   g.pp = (long long)p->nx * p->nz;
-  std::vector<int> posr, dif_row_unused, dit_col_unused, posz;
-  if (!make_plan(g.lw, g.prow, posr, dif_row_unused) || !make_plan(g.nz, g.pcol, dit_col_unused, posz))
+  g.ls = g.lw + 1;
+  std::vector<int> posr, posrd, posc, posz;
+  if (!make_plan(g.lw, g.prow, posr, posrd) || !make_plan(g.nz, g.pcol, posc, posz))
     return reject("NX/2 and NZ must factor into 2 and 3");
   c.nfields = 3 + p->n_th;
-  c.lds_bytes = g.nz * g.lw * (int)sizeof(double2) + (2 * g.lw + g.nz + 1) * (int)sizeof(double2) +
-                (g.lw + g.nz) * (int)sizeof(int);
+  c.lds_bytes = g.nz * g.ls * (int)sizeof(double2) + (2 * g.lw + g.nz + 1) * (int)sizeof(double2) +
+                (2 * g.lw + 2 * g.nz) * (int)sizeof(int);
   if (c.lds_bytes > 160 * 1024) return reject("x-z plane too large for the LDS-resident plane kernels");
   if (g.ny + 2 > 288) return reject("NY+2 > 288 rows is beyond the column solver's register tiles");
   int thr = (g.nz * g.lw + 7) / 8;
@@ -201,14 +202,13 @@ int dh_create(const dh_params *p, dh_ctx **out) {
   if (thr * 4 < g.pf) return reject("plane has too many Fourier modes");
   c.plane_threads = thr;
   {
-    const char *rd = std::getenv("DIABLO_REMDIV");
-    c.rd_pcr = rd && std::strcmp(rd, "pcr") == 0;
+    const char *cn = std::getenv("DIABLO_CNSOLVE");
+    c.cn_pcr = cn && std::strcmp(cn, "pcr") == 0;
   }
   g.f_npair = recip(g.lw / 2 + 1);
   g.f_nkx1 = recip(g.nkx1);
   g.f_lw = recip(g.lw);
-  g.rows_pairs_per_thread = (g.nz * (g.lw / 2 + 1) + thr - 1) / thr;
-  if (g.rows_pairs_per_thread > 8) return reject("plane needs more than 8 row pairs per thread");
+  g.f_nz = recip(g.nz);
   // mpi.f:1022-1046 (RANKY=0 branch on one rank)
   c.jstart = 2;
   c.jend = g.ny;
@@ -246,7 +246,9 @@ int dh_create(const dh_params *p, dh_ctx **out) {
   ACHK(dalloc(&c.tw_col, g.nz));
   ACHK(dalloc(&c.tw_real, g.lw + 1));
   ACHK(dalloc(&c.posr, g.lw));
+  ACHK(dalloc(&c.posrd, g.lw));
   ACHK(dalloc(&c.posz, g.nz));
+  ACHK(dalloc(&c.posc, g.nz));
   ACHK(dalloc(&c.kx, g.nkx1));
   ACHK(dalloc(&c.kx2, g.nkx1));
   ACHK(dalloc(&c.kz, g.nkz1));
@@ -264,14 +266,19 @@ int dh_create(const dh_params *p, dh_ctx **out) {
   ACHK(dalloc(&c.cfth, cf_elems * (p->n_th > 0 ? p->n_th : 1)));
   ACHK(dalloc(&c.uphys, ph_elems * c.nfields));
   ACHK(dalloc(&c.rphys, ph_elems * c.nfields));
-  ACHK(dalloc(&c.rd_bp, cf_elems));
-  ACHK(dalloc(&c.rd_g, cf_elems));
+  ACHK(dalloc(&c.rd_ta, cf_elems));
+  ACHK(dalloc(&c.rd_tb, cf_elems));
+  ACHK(dalloc(&c.rd_tr, cf_elems));
   ACHK(dalloc(&c.rd_l, nyy));
   ACHK(dalloc(&c.rd_u, nyy));
   ACHK(dalloc(&c.clock, 1));
   ACHK(dalloc(&c.fac.k1, 3 * (size_t)c.nfields * kMaxPcr * nyy));
   ACHK(dalloc(&c.fac.k2, 3 * (size_t)c.nfields * kMaxPcr * nyy));
   ACHK(dalloc(&c.fac.bfin, 3 * (size_t)c.nfields * nyy));
+  ACHK(dalloc(&c.fac.tl, 3 * (size_t)c.nfields * nyy));
+  ACHK(dalloc(&c.fac.tb, 3 * (size_t)c.nfields * nyy));
+  ACHK(dalloc(&c.fac.tc, 3 * (size_t)c.nfields * nyy));
+  ACHK(dalloc(&c.fac.tr, 3 * (size_t)c.nfields * nyy));
   c.fac.nsteps = 0;
   while ((1 << c.fac.nsteps) < (int)nyy) ++c.fac.nsteps;
   ACHK(dalloc(&c.fac.isbc, (size_t)c.nfields * nyy));
@@ -282,7 +289,9 @@ int dh_create(const dh_params *p, dh_ctx **out) {
     ACHK(hipMemcpy(c.tw_col, tc.data(), tc.size() * sizeof(double2), hipMemcpyHostToDevice));
     ACHK(hipMemcpy(c.tw_real, tx.data(), tx.size() * sizeof(double2), hipMemcpyHostToDevice));
     ACHK(hipMemcpy(c.posr, posr.data(), posr.size() * sizeof(int), hipMemcpyHostToDevice));
+    ACHK(hipMemcpy(c.posrd, posrd.data(), posrd.size() * sizeof(int), hipMemcpyHostToDevice));
     ACHK(hipMemcpy(c.posz, posz.data(), posz.size() * sizeof(int), hipMemcpyHostToDevice));
+    ACHK(hipMemcpy(c.posc, posc.data(), posc.size() * sizeof(int), hipMemcpyHostToDevice));
     // fft.f:77-100
     const double PI = 4.0 * std::atan(1.0);
     std::vector<double> kx(g.nkx1), kx2(g.nkx1), kz(g.nkz1), kz2(g.nkz1);
@@ -302,7 +311,9 @@ int dh_create(const dh_params *p, dh_ctx **out) {
   g.tw_col = c.tw_col;
   g.tw_real = c.tw_real;
   g.posr = c.posr;
+  g.posrd = c.posrd;
   g.posz = c.posz;
+  g.posc = c.posc;
   g.kx = c.kx;
   g.kz = c.kz;
   g.kx2 = c.kx2;
@@ -367,6 +378,7 @@ int dh_set_grid(dh_ctx *h, const double *gy, const double *gyf, const double *dy
   HIPCHK(hipMemcpy(c.rd_u, ru.data(), nyy * 8, hipMemcpyHostToDevice));
   c.grid_set = true;
   c.dt_cap = dt_cap_of(c);
+  HIPCHK(launch_rem_div_factors(c));
   return rebuild_tables(c);
 }
 

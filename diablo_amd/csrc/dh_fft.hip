@@ -19,15 +19,10 @@ __global__ void __launch_bounds__(1024) k_plane_inv(Geo g, PlaneBatch b) {
   const Tables T = load_tables(A, g);
   const int f = blockIdx.y;
   const long long j = b.j0 + blockIdx.x;
-  plane_unpack(A, g, b.four[f] + j * g.pf);
+  plane_unpack(A, g, T, b.four[f] + j * g.pf);
   plane_cols_inv(A, g, T);
-  plane_rows_inv<8>(A, g, T);
-  double2 *dst = reinterpret_cast<double2 *>(b.phys[f] + j * g.pp);
-  const int n = g.nz * g.lw;
-  for (int e = threadIdx.x; e < n; e += blockDim.x) {
-    const int z = (int)fdiv((unsigned)e, g.f_lw), m = e - z * g.lw;
-    dst[e] = A[inv_slot(g, T, z, m)];
-  }
+  plane_rows_inv(A, g, T);
+  plane_store_phys(A, g, T, reinterpret_cast<double2 *>(b.phys[f] + j * g.pp));
 }
 
 // physical -> Fourier (x r2c with 1/NX and truncation, z c2c with 1/NZ and

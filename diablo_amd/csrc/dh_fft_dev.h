@@ -2,14 +2,27 @@
 // x-z plane held in LDS, shared by the plane-transform kernels and the fused
 // nonlinear kernel.
 //
-// Plane layout in LDS: A[z][0..LW-1] complex, LW = NX/2.
-//   physical view: row z holds (x_{2m}, x_{2m+1}) in the LW complex slots;
+// Plane layout in LDS: A[z][0..LW-1] complex, LW = NX/2, row stride LS =
+// LW + 1 complex (16 B of padding per row, so that the same slot of
+// consecutive rows falls on different LDS banks).
+//   physical view: row z holds (x_{2m}, x_{2m+1}) in natural slot m;
 //   spectral view: A[.][kx] for kx <= NKX after the x transform.
-// Rows are transformed by decimation in time (input in digit-reversed slots,
-// output in natural order); columns by decimation in frequency (natural input,
-// output rows in digit-reversed order, undone by the posz table at pack/unpack
-// time).  Both run in place: every butterfly writes back to the slots it
+// Every transform runs between natural order and a digit-reversed order
+// that is absorbed by a table where the plane meets HBM:
+//   forward: physical pairs are loaded into digit-reversed row slots (posr),
+//            rows by decimation in time (natural output), even/odd split in
+//            place, columns by decimation in frequency (rows come out in
+//            the order posz, read back by the pack);
+//   inverse: the unpack writes rows in the order posc, columns by
+//            decimation in time (natural rows), even/odd split in place,
+//            rows by decimation in frequency (pairs come out in the order
+//            posrd, read back by the store).
+// All stages run in place: every butterfly writes back to the slots it
 // read, so a stage needs one barrier and only R complex values per thread.
+// Work is spread across the batch dimension first (consecutive threads take
+// consecutive rows for a row pass and consecutive columns for a column
+// pass), so the lanes of a wave touch distinct banks and read the same
+// twiddle (an LDS broadcast).
 //
 // The real x transform of length NX is a complex transform of length NX/2
 // plus the even/odd split; FFTW's r2c/c2r (fft.f:73-76) compute the same
@@ -32,6 +45,27 @@ __device__ __forceinline__ double2 cmul_ik(double k, double2 z) { return make_do
 template <bool INV>
 __device__ __forceinline__ double2 cmul_mi(double2 a) {
   return INV ? make_double2(-a.y, a.x) : make_double2(a.y, -a.x);
+}
+
+// Workgroup barrier that orders LDS only: global loads already issued stay
+// in flight across it (a plain __syncthreads also waits for them).  None of
+// these kernels exchanges data between the threads of a block through
+// global memory.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// n / b for a divisor b whose correctly rounded reciprocal r = 1/b is known:
+// q0 = n r, e = n - b q0 (exact by fma), q = q0 + e r.  With r correctly
+// rounded this is the correctly rounded quotient (Markstein's theorem) away
+// from overflow and underflow, i.e. the same double as n / b, in three
+// dependent operations instead of a full division.
+__device__ __forceinline__ double div_by_recip(double n, double b, double r) {
+  const double q0 = n * r;
+  const double e = fma(-b, q0, n);
+  return fma(e, r, q0);
 }
 
 // n / d for n, d < 2^16 by multiply-high (m = ceil(2^32 / d)).
@@ -91,62 +125,63 @@ struct Tables {
   double2 *twc;  // exp(-2 pi i m/NZ), m < NZ
   double2 *twx;  // exp(-2 pi i k/NX), k <= LW
   int *posr;     // DIT input slot of row index m
+  int *posrd;    // slot of row output index k after the DIF
   int *posz;     // slot of column output index k after the DIF
+  int *posc;     // DIT input slot of column index k
 };
 
 // Copies the tables into LDS; returns pointers into `smem` after the plane.
 __device__ Tables load_tables(double2 *smem, const Geo &g) {
   Tables t;
-  double2 *p = smem + g.nz * g.lw;
+  double2 *p = smem + g.nz * g.ls;
   t.twr = p;
   t.twc = p + g.lw;
   t.twx = p + g.lw + g.nz;
   int *q = reinterpret_cast<int *>(p + g.lw + g.nz + g.lw + 1);
   t.posr = q;
-  t.posz = q + g.lw;
+  t.posrd = q + g.lw;
+  t.posz = q + 2 * g.lw;
+  t.posc = q + 2 * g.lw + g.nz;
   for (int i = threadIdx.x; i < g.lw; i += blockDim.x) {
     t.twr[i] = g.tw_row[i];
     t.posr[i] = g.posr[i];
+    t.posrd[i] = g.posrd[i];
   }
   for (int i = threadIdx.x; i < g.nz; i += blockDim.x) {
     t.twc[i] = g.tw_col[i];
     t.posz[i] = g.posz[i];
+    t.posc[i] = g.posc[i];
   }
   for (int i = threadIdx.x; i <= g.lw; i += blockDim.x) t.twx[i] = g.tw_real[i];
-  __syncthreads();
+  lds_barrier();
   return t;
 }
 
 // One in-place stage of radix R over `ntrans` transforms of length N.
 //   DIT: twiddles W_L^{r j} on the inputs, then the butterfly (L grows);
 //   DIF: butterfly, then twiddles W_L^{q j} on the outputs (L shrinks).
-// rows (COLS=false): transform t is LDS row t, element i at A[t*lw + i];
-// cols (COLS=true):  transform t is column t, element i at A[i*lw + t],
-//                    consecutive threads take consecutive columns.
+// rows (COLS=false): transform t is LDS row t, element i at A[t*ls + i];
+// cols (COLS=true):  transform t is column t, element i at A[i*ls + t].
+// Consecutive threads take consecutive transforms t (fnt = reciprocal of
+// ntrans) with the same butterfly.
 template <int R, bool INV, bool COLS, bool DIT>
 __device__ __forceinline__ void fft_stage(double2 *A, const FftStage &st, int N, int ntrans,
-                                          unsigned long long fnt, int lw, const double2 *tw) {
+                                          unsigned long long fnt, int ls, const double2 *tw) {
   const int S = st.S;
   const int nb = N / R;
   const int total = ntrans * nb;
   for (int gi = threadIdx.x; gi < total; gi += blockDim.x) {
-    int t, b;
-    if (COLS) {
-      b = (int)fdiv((unsigned)gi, fnt);
-      t = gi - b * ntrans;
-    } else {
-      t = (int)fdiv((unsigned)gi, st.fnb);
-      b = gi - t * nb;
-    }
+    const int b = (int)fdiv((unsigned)gi, fnt);
+    const int t = gi - b * ntrans;
     const int g = (int)fdiv((unsigned)b, st.fS), j = b - g * S;
     const int i0 = g * st.L + j;
     double2 *base;
     int es;
     if (COLS) {
-      base = A + t + i0 * lw;
-      es = S * lw;
+      base = A + t + i0 * ls;
+      es = S * ls;
     } else {
-      base = A + t * lw + i0;
+      base = A + t * ls + i0;
       es = S;
     }
     double2 v[R];
@@ -170,37 +205,59 @@ __device__ __forceinline__ void fft_stage(double2 *A, const FftStage &st, int N,
 #pragma unroll
     for (int q = 0; q < R; ++q) base[q * es] = v[q];
   }
-  __syncthreads();
+  lds_barrier();
 }
 
 template <bool INV, bool COLS, bool DIT>
-__device__ void lds_fft(double2 *A, const FftPlan &pl, int ntrans, unsigned long long fnt, int lw, const double2 *tw) {
+__device__ void lds_fft(double2 *A, const FftPlan &pl, int ntrans, unsigned long long fnt, int ls, const double2 *tw) {
   for (int s = 0; s < pl.nst; ++s) {
     const FftStage &st = DIT ? pl.dit[s] : pl.dif[s];
     switch (st.R) {
-      case 8: fft_stage<8, INV, COLS, DIT>(A, st, pl.n, ntrans, fnt, lw, tw); break;
-      case 4: fft_stage<4, INV, COLS, DIT>(A, st, pl.n, ntrans, fnt, lw, tw); break;
-      case 3: fft_stage<3, INV, COLS, DIT>(A, st, pl.n, ntrans, fnt, lw, tw); break;
-      default: fft_stage<2, INV, COLS, DIT>(A, st, pl.n, ntrans, fnt, lw, tw); break;
+      case 8: fft_stage<8, INV, COLS, DIT>(A, st, pl.n, ntrans, fnt, ls, tw); break;
+      case 4: fft_stage<4, INV, COLS, DIT>(A, st, pl.n, ntrans, fnt, ls, tw); break;
+      case 3: fft_stage<3, INV, COLS, DIT>(A, st, pl.n, ntrans, fnt, ls, tw); break;
+      default: fft_stage<2, INV, COLS, DIT>(A, st, pl.n, ntrans, fnt, ls, tw); break;
     }
   }
 }
 
 // LDS slot through which physical pair m of row z enters the forward transform.
-__device__ __forceinline__ int phys_slot(const Tables &T, const Geo &g, int z, int m) { return z * g.lw + T.posr[m]; }
+__device__ __forceinline__ int phys_slot(const Tables &T, const Geo &g, int z, int m) { return z * g.ls + T.posr[m]; }
+
+// Work split for filling the digit-reversed input slots from natural-order
+// HBM rows: a wave covers a tile of 8 rows x 8 consecutive pairs, lane =
+// 8 * pair + row.  Each row segment is one 128 B read, and for radix-8 row
+// plans the 16 lanes of an LDS cycle land on 16 distinct banks.
+struct LoadTile {
+  int ntm, ntiles;
+  unsigned long long fntm;
+  __device__ LoadTile(const Geo &g) {
+    ntm = (g.lw + 7) >> 3;
+    ntiles = ((g.nz + 7) >> 3) * ntm;
+    fntm = ((1ull << 32) + ntm - 1) / ntm;
+  }
+  // (z, m) of this lane in tile t; false when outside the plane
+  __device__ __forceinline__ bool at(const Geo &g, int t, int &z, int &m) const {
+    const int lane = threadIdx.x & 63;
+    const int tz = (int)fdiv((unsigned)t, fntm), tm = t - tz * ntm;
+    z = (tz << 3) + (lane & 7);
+    m = (tm << 3) + (lane >> 3);
+    return z < g.nz && m < g.lw;
+  }
+};
 
 // Forward x transform of every row (FFT_XZ_TO_FOURIER's r2c, mpi.f:1490-1506).
 // On entry row z holds (x_{2m}, x_{2m+1}) in slot posr[m]; on exit slot k
 // holds X_k / NX for k <= NKX (natural order).
 __device__ void plane_rows_fwd(double2 *A, const Geo &g, const Tables &T) {
   const int h = g.lw;  // NX/2
-  lds_fft<false, false, true>(A, g.prow, g.nz, 0, g.lw, T.twr);
+  lds_fft<false, false, true>(A, g.prow, g.nz, g.f_nz, g.ls, T.twr);
   // even/odd split: X_k = (Z_k + conj(Z_{h-k}))/2 - i w^-k (Z_k - conj(Z_{h-k}))/2
   const int npair = h / 2 + 1;  // k = 0..h/2 paired with h-k
   const int total = g.nz * npair;
   for (int e = threadIdx.x; e < total; e += blockDim.x) {
-    const int z = (int)fdiv((unsigned)e, g.f_npair), k = e - z * npair;
-    double2 *row = A + z * g.lw;
+    const int k = (int)fdiv((unsigned)e, g.f_nz), z = e - k * g.nz;
+    double2 *row = A + z * g.ls;
     const int k2 = (k == 0) ? 0 : h - k;
     const double2 zk = row[k];
     const double2 zm = row[k2];
@@ -219,69 +276,50 @@ __device__ void plane_rows_fwd(double2 *A, const Geo &g, const Tables &T) {
       row[k2] = make_double2(X.x / g.nx, X.y / g.nx);
     }
   }
-  __syncthreads();
+  lds_barrier();
 }
 
 // Inverse x transform of every row (FFT_XZ_TO_PHYSICAL's c2r, mpi.f:1600-1604).
 // On entry slot k of each row holds X_k for k <= NKX (higher modes and the
 // Nyquist term are zero; the imaginary part of the k=0 term is ignored); on
-// exit slot m holds (x_{2m}, x_{2m+1}).  The even/odd combination reads
-// natural slots and writes digit-reversed ones, so it runs in two phases
-// with the values held in registers (MAXP pairs per thread at most).
-template <int MAXP>
+// exit slot posrd[m] holds (x_{2m}, x_{2m+1}).
 __device__ void plane_rows_inv(double2 *A, const Geo &g, const Tables &T) {
   const int h = g.lw;
   const int npair = h / 2 + 1;
   const int total = g.nz * npair;
-  double2 zk_[MAXP], zm_[MAXP];
-#pragma unroll
-  for (int q = 0; q < MAXP; ++q) {
-    const int e = threadIdx.x + q * blockDim.x;
-    if (e < total) {
-      const int z = (int)fdiv((unsigned)e, g.f_npair), k = e - z * npair;
-      const double2 *row = A + z * g.lw;
-      const int k2 = h - k;  // partner (k=0 pairs with the Nyquist term)
-      double2 xk = (k <= g.nkx) ? row[k] : make_double2(0.0, 0.0);
-      double2 xm = (k2 <= g.nkx && k2 < h) ? row[k2] : make_double2(0.0, 0.0);
-      if (k == 0) xk.y = 0.0;
-      // Z_k = (X_k + conj X_{h-k}) + i w^k (X_k - conj X_{h-k}),  w = e^{+2 pi i/NX}
-      {
-        double2 a = make_double2(xk.x + xm.x, xk.y - xm.y);
-        double2 b = make_double2(xk.x - xm.x, xk.y + xm.y);
-        double2 wb = cmul(cconj(T.twx[k]), b);
-        zk_[q] = make_double2(a.x - wb.y, a.y + wb.x);
-      }
-      {
-        double2 a = make_double2(xm.x + xk.x, xm.y - xk.y);
-        double2 b = make_double2(xm.x - xk.x, xm.y + xk.y);
-        double2 wb = cmul(cconj(T.twx[k2]), b);
-        zm_[q] = make_double2(a.x - wb.y, a.y + wb.x);
-      }
+  for (int e = threadIdx.x; e < total; e += blockDim.x) {
+    const int k = (int)fdiv((unsigned)e, g.f_nz), z = e - k * g.nz;
+    double2 *row = A + z * g.ls;
+    const int k2 = h - k;  // partner (k=0 pairs with the Nyquist term)
+    double2 xk = (k <= g.nkx) ? row[k] : make_double2(0.0, 0.0);
+    double2 xm = (k2 <= g.nkx && k2 < h) ? row[k2] : make_double2(0.0, 0.0);
+    if (k == 0) xk.y = 0.0;
+    // Z_k = (X_k + conj X_{h-k}) + i w^k (X_k - conj X_{h-k}),  w = e^{+2 pi i/NX}
+    {
+      double2 a = make_double2(xk.x + xm.x, xk.y - xm.y);
+      double2 b = make_double2(xk.x - xm.x, xk.y + xm.y);
+      double2 wb = cmul(cconj(T.twx[k]), b);
+      if (k < h) row[k] = make_double2(a.x - wb.y, a.y + wb.x);
+    }
+    if (k != 0 && k2 != k) {
+      double2 a = make_double2(xm.x + xk.x, xm.y - xk.y);
+      double2 b = make_double2(xm.x - xk.x, xm.y + xk.y);
+      double2 wb = cmul(cconj(T.twx[k2]), b);
+      row[k2] = make_double2(a.x - wb.y, a.y + wb.x);
     }
   }
-  __syncthreads();
-#pragma unroll
-  for (int q = 0; q < MAXP; ++q) {
-    const int e = threadIdx.x + q * blockDim.x;
-    if (e < total) {
-      const int z = (int)fdiv((unsigned)e, g.f_npair), k = e - z * npair;
-      double2 *row = A + z * g.lw;
-      const int k2 = h - k;
-      if (k < h) row[T.posr[k]] = zk_[q];
-      if (k != 0 && k2 != k) row[T.posr[k2]] = zm_[q];
-    }
-  }
-  __syncthreads();
-  lds_fft<true, false, true>(A, g.prow, g.nz, 0, g.lw, T.twr);
+  lds_barrier();
+  lds_fft<true, false, false>(A, g.prow, g.nz, g.f_nz, g.ls, T.twr);
 }
 
-// z transforms over the de-aliased columns kx = 0..NKX; the output row slot
-// of z (or kz) index k is posz[k].
+// z transforms over the de-aliased columns kx = 0..NKX: forward by DIF
+// (natural rows in, row of index k at posz[k] out), inverse by DIT (row of
+// index k at posc[k] in, natural rows out).
 __device__ __forceinline__ void plane_cols_fwd(double2 *A, const Geo &g, const Tables &T) {
-  lds_fft<false, true, false>(A, g.pcol, g.nkx1, g.f_nkx1, g.lw, T.twc);
+  lds_fft<false, true, false>(A, g.pcol, g.nkx1, g.f_nkx1, g.ls, T.twc);
 }
 __device__ __forceinline__ void plane_cols_inv(double2 *A, const Geo &g, const Tables &T) {
-  lds_fft<true, true, false>(A, g.pcol, g.nkx1, g.f_nkx1, g.lw, T.twc);
+  lds_fft<true, true, true>(A, g.pcol, g.nkx1, g.f_nkx1, g.ls, T.twc);
 }
 
 // Row of the unpacked z spectrum that holds packed index kzp (mpi.f:1564-1568).
@@ -291,17 +329,21 @@ __device__ __forceinline__ int kz_row(const Geo &g, int kzp) {
 
 // LDS slot of the forward-transformed mode (kx, packed kz index kzp).
 __device__ __forceinline__ int fwd_slot(const Geo &g, const Tables &T, int kx, int kzp) {
-  return T.posz[kz_row(g, kzp)] * g.lw + kx;
+  return T.posz[kz_row(g, kzp)] * g.ls + kx;
 }
 
 // LDS slot of physical pair m (x = 2m, 2m+1) of row z after the inverse.
-__device__ __forceinline__ int inv_slot(const Geo &g, const Tables &T, int z, int m) { return T.posz[z] * g.lw + m; }
+__device__ __forceinline__ int inv_slot(const Geo &g, const Tables &T, int z, int m) { return z * g.ls + T.posrd[m]; }
+
+// LDS row of the unpacked z spectrum entering the inverse column transform
+// at natural index zr.
+__device__ __forceinline__ int inv_row(const Tables &T, int zr) { return T.posc[zr]; }
 
 // Scatter a de-aliased Fourier plane into A (unpack + zero fill of the
 // columns 0..NKX, mpi.f:1562-1574).  `src` may be null (all zero).  Loads
 // are issued in batches of 8 per thread before any LDS store, so their
 // latencies overlap.
-__device__ void plane_unpack(double2 *A, const Geo &g, const double2 *src) {
+__device__ void plane_unpack(double2 *A, const Geo &g, const Tables &T, const double2 *src) {
   constexpr int B = 8;
   const int total = g.nz * g.nkx1;
   for (int base = threadIdx.x; base < total; base += B * blockDim.x) {
@@ -318,7 +360,7 @@ __device__ void plane_unpack(double2 *A, const Geo &g, const double2 *src) {
           kzp = zr;
         else if (zr >= g.nz - g.nkz)
           kzp = zr - (g.nz - 1 - g.nkz) + g.nkz;
-        slot[b] = zr * g.lw + kx;
+        slot[b] = inv_row(T, zr) * g.ls + kx;
         v[b] = (kzp >= 0 && src) ? src[kzp * g.nkx1 + kx] : make_double2(0.0, 0.0);
       }
     }
@@ -326,30 +368,42 @@ __device__ void plane_unpack(double2 *A, const Geo &g, const double2 *src) {
     for (int b = 0; b < B; ++b)
       if (slot[b] >= 0) A[slot[b]] = v[b];
   }
-  __syncthreads();
+  lds_barrier();
 }
 
-// Physical plane (natural order in HBM) into the DIT input slots.
+// Physical plane (natural order in HBM) into the digit-reversed input slots.
 __device__ void plane_load_phys(double2 *A, const Geo &g, const Tables &T, const double2 *src) {
-  constexpr int B = 8;
-  const int n = g.nz * g.lw;
-  for (int base = threadIdx.x; base < n; base += B * blockDim.x) {
+  constexpr int B = 4;
+  const LoadTile lt(g);
+  const int nw = blockDim.x >> 6;
+  for (int t0 = threadIdx.x >> 6; t0 < lt.ntiles; t0 += B * nw) {
     double2 v[B];
+    int sl[B];
 #pragma unroll
     for (int b = 0; b < B; ++b) {
-      const int e = base + b * blockDim.x;
-      if (e < n) v[b] = src[e];
-    }
-#pragma unroll
-    for (int b = 0; b < B; ++b) {
-      const int e = base + b * blockDim.x;
-      if (e < n) {
-        const int z = (int)fdiv((unsigned)e, g.f_lw), m = e - z * g.lw;
-        A[phys_slot(T, g, z, m)] = v[b];
+      int z, m;
+      sl[b] = -1;
+      const int t = t0 + b * nw;
+      if (t < lt.ntiles && lt.at(g, t, z, m)) {
+        sl[b] = phys_slot(T, g, z, m);
+        v[b] = src[z * g.lw + m];
       }
     }
+#pragma unroll
+    for (int b = 0; b < B; ++b)
+      if (sl[b] >= 0) A[sl[b]] = v[b];
   }
-  __syncthreads();
+  lds_barrier();
+}
+
+// Physical plane from the inverse-transform slots to natural order in HBM.
+__device__ void plane_store_phys(const double2 *A, const Geo &g, const Tables &T, double2 *dst) {
+  const LoadTile lt(g);
+  const int nw = blockDim.x >> 6;
+  for (int t = threadIdx.x >> 6; t < lt.ntiles; t += nw) {
+    int z, m;
+    if (lt.at(g, t, z, m)) dst[z * g.lw + m] = A[inv_slot(g, T, z, m)];
+  }
 }
 
 }  // namespace dh

@@ -37,7 +37,8 @@ struct Geo {
   int nkx, nkz, tnkz;   // header:21-28
   int nkx1, nkz1;       // de-aliased extents NKX+1, TNKZ+1
   int pf;               // Fourier plane elements nkx1*nkz1
-  int lw;               // LDS row width in complex elements = NX/2
+  int lw;               // complex elements per row = NX/2
+  int ls;               // LDS row stride in complex elements (lw + 1: rows start on distinct banks)
   long long pp;         // physical plane elements NX*NZ
   FftPlan prow, pcol;   // row c2c (length NX/2), column c2c (length NZ)
   const double2 *tw_row;   // exp(-2 pi i m/(NX/2)), m < NX/2
@@ -45,10 +46,11 @@ struct Geo {
   const double2 *tw_real;  // exp(-2 pi i k/NX),      k <= NX/2
   const double *kx, *kz, *kx2, *kz2;  // wavenumbers (fft.f:77-100), de-aliased
   const double *dy, *dyf;             // grid spacing (0:NY+1)
-  const int *posr;  // DIT input slot of row index m (length NX/2)
-  const int *posz;  // DIF output slot of column index k (length NZ)
-  unsigned long long f_npair, f_nkx1, f_lw;  // reciprocals of NX/4+1, NKX+1, NX/2
-  int rows_pairs_per_thread;                 // ceil(NZ*(NX/4+1)/threads)
+  const int *posr;   // DIT input slot of row index m (length NX/2)
+  const int *posrd;  // DIF output slot of row index k (length NX/2)
+  const int *posz;   // DIF output slot of column index k (length NZ)
+  const int *posc;   // DIT input slot of column index k (length NZ)
+  unsigned long long f_npair, f_nkx1, f_lw, f_nz;  // reciprocals of NX/4+1, NKX+1, NX/2, NZ
 };
 
 // Device-resident clock (/RUNTIME_VARS/ + /RKW3_PARAMS/).
@@ -71,6 +73,7 @@ constexpr int kMaxPcr = 12;  // PCR steps: NY+2 <= 2^12 rows
 struct RealFactors {
   double *k1, *k2;  // [rk 0..2][field][step][row] elimination multipliers
   double *bfin;     // [rk][field][row] diagonal after the last step
+  double *tl, *tb, *tc, *tr;  // [rk][field][row] Thomas multipliers, pivots, upper diagonal, 1/pivots
   int *isbc;        // [field][row] 1: row takes a constant right-hand side
   double *vconst;   // [field][row]
   int nsteps;       // ceil(log2(NY+2))
@@ -84,7 +87,7 @@ struct Ctx {
   int wlo[kMaxFields], whi[kMaxFields];  // solution write-back rows per field
   int lds_bytes;
   int plane_threads;
-  bool rd_pcr = false;  // REM_DIV by cyclic reduction instead of Thomas sweeps
+  bool cn_pcr = false;  // Crank-Nicolson solves by cyclic reduction instead of Thomas sweeps
   hipStream_t stream = nullptr;
   std::string err;
   bool grid_set = false;
@@ -93,15 +96,16 @@ struct Ctx {
   double dt_cap;  // courant.f:15-31 limits that do not depend on U
   // device buffers
   double2 *tw_row = nullptr, *tw_col = nullptr, *tw_real = nullptr;
-  int *posr = nullptr, *posz = nullptr;
+  int *posr = nullptr, *posrd = nullptr, *posz = nullptr, *posc = nullptr;
   double *kx = nullptr, *kz = nullptr, *kx2 = nullptr, *kz2 = nullptr;
   double *d_dy = nullptr, *d_dyf = nullptr, *d_gyf = nullptr;
   double2 *cu[3] = {}, *cp = nullptr, *cf[3] = {}, *cth = nullptr, *cfth = nullptr;
   double2 *phi = nullptr;       // CR1 after REM_DIV
   double *uphys = nullptr;      // [nfields][NY+2][NZ][NX]: U1,U2,U3,TH
   double *rphys = nullptr;      // [nfields][NY+2][NZ][NX]: R1,R2,R3,RTH
-  double *rd_bp = nullptr;      // REM_DIV scratch pivots [NY+2][pf]
-  double2 *rd_g = nullptr;      // REM_DIV scratch rhs    [NY+2][pf]
+  double *rd_ta = nullptr;      // REM_DIV sweep multipliers [NY+2][pf]
+  double *rd_tb = nullptr;      // REM_DIV sweep pivots      [NY+2][pf]
+  double *rd_tr = nullptr;      // REM_DIV 1/pivots          [NY+2][pf]
   Clock *clock = nullptr;
   RealFactors fac{};
   double *rd_l = nullptr, *rd_u = nullptr;  // 1/(DY DYF) tables for REM_DIV
@@ -130,6 +134,7 @@ hipError_t launch_phase(Ctx &c, int rk, int phase);
 hipError_t launch_rem_div(Ctx &c, int rk_for_cp);     // rk_for_cp<0: no CP update
 hipError_t launch_step_begin(Ctx &c);
 hipError_t launch_end_step(Ctx &c);
-hipError_t launch_pcr_tables(Ctx &c);  // for the DELTA_T in the device clock
+hipError_t launch_pcr_tables(Ctx &c);
+hipError_t launch_rem_div_factors(Ctx &c);  // once per grid  // for the DELTA_T in the device clock
 
 }  // namespace dh

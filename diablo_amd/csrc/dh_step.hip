@@ -3,12 +3,14 @@
 //   k_nonlinear   explicit Fourier right-hand side, nonlinear products and
 //                 their forward transforms, RK combination and the inverse
 //                 transform of the right-hand side (channel.f:77-600);
-//   k_solve       explicit half of the y-viscous term plus the real
+//   k_solve_thomas / k_solve
+//                 explicit half of the y-viscous term plus the real
 //                 tridiagonal Crank-Nicolson solve of every (x,z) column, and
 //                 the CFL minimum (channel.f:606-1029, courant.f:34-43);
-//   k_rem_div     divergence, pressure-Poisson solve per (kx,kz), projection
+//   k_rem_div_thomas
+//                 divergence, pressure-Poisson solve per (kx,kz), projection
 //                 and pressure update (channel.f:1103-1214, 1059-1065);
-//   k_courant_fin the rest of COURANT (courant.f:45-62) on the device.
+//   k_courant_dt  the rest of COURANT (courant.f:45-62) on the device.
 #include "dh_fft_dev.h"
 
 namespace dh {
@@ -140,7 +142,17 @@ __global__ void __launch_bounds__(1024) k_nonlinear(StepArgs a) {
   extern __shared__ double2 A[];
   const Geo &g = a.g;
   const Tables T = load_tables(A, g);
-  const int J = blockIdx.x + 1, c = blockIdx.y;
+  // Work item of this block: workgroups are dealt to the 8 XCDs round-robin,
+  // so give XCD x a contiguous range of (plane, component) items, planes
+  // outer: the blocks sharing an XCD's L2 then read the same U planes.
+  int J, c;
+  {
+    const int nblk = gridDim.x * gridDim.y, lin = blockIdx.x + blockIdx.y * gridDim.x;
+    const int x = lin & 7, q = nblk >> 3, r = nblk & 7;
+    const int w = x * q + min(x, r) + (lin >> 3);
+    J = __builtin_amdgcn_readfirstlane(w / (int)gridDim.y + 1);
+    c = __builtin_amdgcn_readfirstlane(w - (J - 1) * (int)gridDim.y);
+  }
   int jlo = 2, jhi = g.ny;
   if (c >= 3) {
     jlo = a.jstart_th[c - 3];
@@ -165,18 +177,19 @@ __global__ void __launch_bounds__(1024) k_nonlinear(StepArgs a) {
     }
   }
   // three products, forward transformed and accumulated in statement order
-  const int npair = g.nz * g.lw;
+  const LoadTile lt(g);
+  const int nw = nthr >> 6;
   for (int p = 0; p < 3; ++p) {
-    constexpr int PB = 2;  // products per thread per batch: loads issued before LDS stores
-    for (int base = threadIdx.x; base < npair; base += PB * nthr) {
+    constexpr int PB = 1;  // tiles per wave per batch: loads issued before LDS stores
+    for (int t0 = threadIdx.x >> 6; t0 < lt.ntiles; t0 += PB * nw) {
       double2 v[PB];
       int sl[PB];
 #pragma unroll
       for (int b = 0; b < PB; ++b) {
-        const int e = base + b * nthr;
+        int z, xp;
         sl[b] = -1;
-        if (e < npair) {
-          const int z = (int)fdiv((unsigned)e, g.f_lw), xp = e - z * g.lw;
+        const int t = t0 + b * nw;
+        if (t < lt.ntiles && lt.at(g, t, z, xp)) {
           sl[b] = phys_slot(T, g, z, xp);
           v[b] = product_pair(a, c, p, J, (long long)z * g.nx + 2 * xp);
         }
@@ -185,7 +198,7 @@ __global__ void __launch_bounds__(1024) k_nonlinear(StepArgs a) {
       for (int b = 0; b < PB; ++b)
         if (sl[b] >= 0) A[sl[b]] = v[b];
     }
-    __syncthreads();
+    lds_barrier();
     plane_rows_fwd(A, g, T);
     plane_cols_fwd(A, g, T);
     // derivative factor: c==0 takes (ikz, ikx, 1); all others (ikx, ikz, 1)
@@ -205,13 +218,13 @@ __global__ void __launch_bounds__(1024) k_nonlinear(StepArgs a) {
         }
       }
     }
-    __syncthreads();
+    lds_barrier();
   }
   // RK combination (channel.f:88-167, 581-595; scalars 254-270, 723-729),
   // new CF stored, CR scattered into LDS for the inverse transform.
   for (int e = threadIdx.x; e < g.nz * g.nkx1; e += nthr) {
     const int zr = (int)fdiv((unsigned)e, g.f_nkx1), kx = e - zr * g.nkx1;
-    if (zr > g.nkz && zr < g.nz - g.nkz) A[zr * g.lw + kx] = make_double2(0.0, 0.0);
+    if (zr > g.nkz && zr < g.nz - g.nkz) A[inv_row(T, zr) * g.ls + kx] = make_double2(0.0, 0.0);
   }
   double2 *CF = (c < 3) ? a.cf[c] : a.cfth + (c - 3) * a.cfs;
   const long long base = (long long)J * g.pf;
@@ -244,17 +257,13 @@ __global__ void __launch_bounds__(1024) k_nonlinear(StepArgs a) {
       const double2 cfn = acc[q];
       CF[o] = cfn;
       cr = make_double2(cr.x + T5 * cfn.x, cr.y + T5 * cfn.y);
-      A[kz_row(g, kzp) * g.lw + kx] = cr;
+      A[inv_row(T, kz_row(g, kzp)) * g.ls + kx] = cr;
     }
   }
-  __syncthreads();
+  lds_barrier();
   plane_cols_inv(A, g, T);
-  plane_rows_inv<8>(A, g, T);
-  double2 *dst = reinterpret_cast<double2 *>(a.rphys + (long long)c * a.fs + (long long)J * g.pp);
-  for (int e = threadIdx.x; e < npair; e += nthr) {
-    const int z = (int)fdiv((unsigned)e, g.f_lw), m = e - z * g.lw;
-    dst[e] = A[inv_slot(g, T, z, m)];
-  }
+  plane_rows_inv(A, g, T);
+  plane_store_phys(A, g, T, reinterpret_cast<double2 *>(a.rphys + (long long)c * a.fs + (long long)J * g.pp));
 }
 // ---------------------------------------------------------------------------
 // Real Crank-Nicolson systems (matrix build of channel.f:771-1005, BC rows
@@ -268,6 +277,7 @@ struct FactorIn {
   int bcmin[kMaxFields], bcmax[kMaxFields];
   double c1min[kMaxFields], c1max[kMaxFields];
   const double *dy, *dyf;
+  int pcr;  // also build the cyclic-reduction tables
 };
 
 // Row J (0..NY+1) of the system of field f: lower L, diagonal D, upper U,
@@ -369,7 +379,28 @@ __global__ void __launch_bounds__(256) k_pcr_tables(const Clock *clk, FactorIn i
       fac.vconst[(long long)f * nrow + i] = V;
     }
   }
-  __syncthreads();
+  lds_barrier();
+  if (threadIdx.x == 0) {
+    // forward elimination of THOMAS_REAL_MPI (mpi.f:630-703): the matrix is
+    // shared by all columns, so its multipliers and pivots are too
+#pragma clang fp contract(off)
+    const long long o = ((long long)rk * in.nfields + f) * nrow;
+    double bp = b[0];
+    fac.tl[o] = 0.0;
+    fac.tb[o] = bp;
+    fac.tc[o] = c[0];
+    fac.tr[o] = 1.0 / bp;
+    for (int J = 1; J < nrow; ++J) {
+      const double m = -a[J] / bp;
+      bp = b[J] + m * c[J - 1];
+      fac.tl[o + J] = m;
+      fac.tb[o + J] = bp;
+      fac.tc[o + J] = c[J];
+      fac.tr[o + J] = 1.0 / bp;
+    }
+  }
+  lds_barrier();
+  if (!in.pcr) return;
   const long long tb = ((long long)rk * in.nfields + f) * (long long)kMaxPcr * nrow;
   int s = 0;
   for (int d = 1; d < nrow; d *= 2, ++s) {
@@ -385,14 +416,14 @@ __global__ void __launch_bounds__(256) k_pcr_tables(const Clock *clk, FactorIn i
       fac.k1[tb + (long long)s * nrow + i] = k1;
       fac.k2[tb + (long long)s * nrow + i] = k2;
     }
-    __syncthreads();
+    lds_barrier();
     q = 0;
     for (int i = threadIdx.x; i < nrow; i += blockDim.x, ++q) {
       a[i] = na[q];
       b[i] = nb[q];
       c[i] = nc[q];
     }
-    __syncthreads();
+    lds_barrier();
   }
   for (int i = threadIdx.x; i < nrow; i += blockDim.x)
     fac.bfin[((long long)rk * in.nfields + f) * nrow + i] = b[i];
@@ -473,7 +504,7 @@ __global__ void __launch_bounds__(kSolveThreads) k_solve(SolveArgs a) {
     const int i = i0 + q;
     r[q] = (live && q < rows && i < nrow) ? R[(long long)i * pp] : 0.0;
   }
-  __syncthreads();
+  lds_barrier();
   // right-hand sides: boundary constants or R + explicit CN term of U_old
 #pragma unroll
   for (int q = 0; q < kSolveRows; ++q) {
@@ -494,7 +525,7 @@ __global__ void __launch_bounds__(kSolveThreads) k_solve(SolveArgs a) {
       D[i * kSolveCols + lane] = d;
     }
   }
-  __syncthreads();
+  lds_barrier();
   for (int s = 0, dd = 1; s < nst; ++s, dd *= 2) {
     const double *k1 = K1 + s * nrow, *k2 = K2 + s * nrow;
     double nd[kSolveRows];
@@ -508,13 +539,13 @@ __global__ void __launch_bounds__(kSolveThreads) k_solve(SolveArgs a) {
         nd[q] = v;
       }
     }
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int q = 0; q < kSolveRows; ++q) {
       const int i = i0 + q;
       if (q < rows && i < nrow) D[i * kSolveCols + lane] = nd[q];
     }
-    __syncthreads();
+    lds_barrier();
   }
   const bool cour = a.courant && f < 3;
   const double dxf = (f == 0) ? a.dx : a.dz;
@@ -535,10 +566,187 @@ __global__ void __launch_bounds__(kSolveThreads) k_solve(SolveArgs a) {
     // wave minimum by shuffles, then one LDS slot per wave
     for (int o = 32; o > 0; o >>= 1) dtmin = fmin(dtmin, __shfl_down(dtmin, o));
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = dtmin;
-    __syncthreads();
+    lds_barrier();
     if (threadIdx.x == 0) {
       double v = red[0];
       for (int w = 1; w < (int)(blockDim.x >> 6); ++w) v = fmin(v, red[w]);
+      if (v >= 0.0) atomicMin(&a.clk->cfl_min_bits, (unsigned long long)__double_as_longlong(v));
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Real Crank-Nicolson solves by Thomas sweeps, one (x,z) column per lane
+// (channel.f:606-626, 739-747 explicit half; 771-1029 implicit half;
+// THOMAS_REAL_MPI mpi.f:630-703 with the shared multipliers and pivots of
+// k_pcr_tables), in the reference's operation order, and the fused CFL
+// minimum of COURANT (courant.f:34-43).  Block: kTsCols consecutive columns
+// of one field.  All waves build the right-hand sides of a slice of rows
+// into LDS [row][col]; the first wave runs the forward and backward sweeps
+// of its lanes' columns in LDS; all waves then write their slice of U.
+// The sweeps read their operands kTsAhead rows ahead at clamped (uniform)
+// row indices, and the rows they run past the system land in one padding
+// row at each end, so nothing in the recurrence waits on LDS.  LDS stays
+// under 80 KB so that two blocks share a CU.
+// ---------------------------------------------------------------------------
+constexpr int kTsCols = 64;
+constexpr int kTsThreads = 512;
+constexpr int kTsBatch = 9;   // rows per load batch of a wave
+constexpr int kTsAhead = 4;   // rows of sweep operands read ahead
+
+__global__ void __launch_bounds__(kTsThreads) k_solve_thomas(SolveArgs a) {
+#pragma clang fp contract(off)
+  extern __shared__ double sd[];
+  __shared__ double red[kTsThreads / 64];
+  const Geo &g = a.g;
+  const int f = blockIdx.y;
+  const long long pp = g.pp;
+  const int NY = g.ny, nrow = NY + 2;
+  const int rk = a.rk;
+  const double hb = a.clk->h_bar[rk];
+  const double T1 = a.nu * hb / 2.0;
+  const double coef = (f < 3) ? T1 * a.nu_v_scale : (T1 / a.pr[f - 3]) * a.nu_v_scale;
+  // LDS: D [row][kTsCols] for rows -1..NY+2 (row J at J + 1), then TL, TB,
+  // TC, TR [row] for rows 0..NY+1
+  double *D = sd + kTsCols;
+  double *TL = sd + (nrow + 2) * kTsCols, *TB = TL + nrow, *TC = TB + nrow, *TR = TC + nrow;
+  const long long to = ((long long)rk * a.nfields + f) * nrow;
+  for (int i = threadIdx.x; i < nrow; i += blockDim.x) {
+    TL[i] = a.fac.tl[to + i];
+    TB[i] = a.fac.tb[to + i];
+    TC[i] = a.fac.tc[to + i];
+    TR[i] = a.fac.tr[to + i];
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const long long col = (long long)blockIdx.x * kTsCols + lane;
+  const bool live = col < pp;
+  double *U = a.uphys + (long long)f * a.fs + col;
+  const double *R = a.rphys + (long long)f * a.fs + col;
+  const int *isbc = a.fac.isbc + (long long)f * nrow;
+  const double *vc = a.fac.vconst + (long long)f * nrow;
+  const double *dy = g.dy, *dyf = g.dyf;
+  // right-hand sides: the boundary constant, or R plus the explicit half of
+  // the y-viscous term of U_old
+  const int per = (nrow + nw - 1) / nw;
+  const int r0 = w * per, r1 = min(nrow, r0 + per);
+  for (int b0 = r0; b0 < r1; b0 += kTsBatch) {
+    double u[kTsBatch + 2], r[kTsBatch];
+#pragma unroll
+    for (int q = 0; q < kTsBatch + 2; ++q) {
+      const int i = b0 - 1 + q;
+      u[q] = (live && i >= 0 && i < nrow && i <= r1) ? U[(long long)i * pp] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < kTsBatch; ++q) {
+      const int i = b0 + q;
+      r[q] = (live && i < r1) ? R[(long long)i * pp] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < kTsBatch; ++q) {
+      const int i = b0 + q;
+      if (i >= r1) break;
+      double d;
+      if (isbc[i]) {
+        d = vc[i];
+      } else {
+        const double um = u[q], u0 = u[q + 1], up = u[q + 2];
+        double ex;
+        if (f == 1)
+          ex = ((up - u0) / dyf[i] - (u0 - um) / dyf[i - 1]) / dy[i];
+        else
+          ex = ((up - u0) / dy[i + 1] - (u0 - um) / dy[i]) / dyf[i];
+        d = r[q] + coef * ex;
+      }
+      D[i * kTsCols + lane] = d;
+    }
+  }
+  lds_barrier();
+  if (w == 0) {
+    // forward sweep: G_J = G_J + m_J G_{J-1}, J = 1..NY+1
+    const int last = nrow - 1;
+    double gp = D[lane], glast = gp;
+    double dc[kTsAhead], dn[kTsAhead], mc[kTsAhead], mn[kTsAhead];
+#pragma unroll
+    for (int q = 0; q < kTsAhead; ++q) {
+      const int J = min(1 + q, last);
+      dc[q] = D[J * kTsCols + lane];
+      mc[q] = TL[J];
+    }
+    for (int J0 = 1; J0 < nrow; J0 += kTsAhead) {
+#pragma unroll
+      for (int q = 0; q < kTsAhead; ++q) {
+        const int J = min(J0 + kTsAhead + q, last);
+        dn[q] = D[J * kTsCols + lane];
+        mn[q] = TL[J];
+      }
+#pragma unroll
+      for (int q = 0; q < kTsAhead; ++q) {
+        const int J = J0 + q;
+        gp = dc[q] + mc[q] * gp;
+        D[min(J, nrow) * kTsCols + lane] = gp;  // past the last row: padding row
+        glast = (J == last) ? gp : glast;
+        dc[q] = dn[q];
+        mc[q] = mn[q];
+      }
+    }
+    // backward sweep: G_{NY+1} /= B_{NY+1}; G_J = (G_J - C_J G_{J+1}) / B_J,
+    // the quotients formed from the tabulated 1/B_J (same doubles)
+    double x = glast / TB[last];
+    D[last * kTsCols + lane] = x;
+    double bc[kTsAhead], bn[kTsAhead], cc[kTsAhead], cn[kTsAhead], rc[kTsAhead], rn[kTsAhead];
+#pragma unroll
+    for (int q = 0; q < kTsAhead; ++q) {
+      const int J = max(last - 1 - q, 0);
+      dc[q] = D[J * kTsCols + lane];
+      bc[q] = TB[J];
+      cc[q] = TC[J];
+      rc[q] = TR[J];
+    }
+    for (int J0 = last - 1; J0 >= 0; J0 -= kTsAhead) {
+#pragma unroll
+      for (int q = 0; q < kTsAhead; ++q) {
+        const int J = max(J0 - kTsAhead - q, 0);
+        dn[q] = D[J * kTsCols + lane];
+        bn[q] = TB[J];
+        cn[q] = TC[J];
+        rn[q] = TR[J];
+      }
+#pragma unroll
+      for (int q = 0; q < kTsAhead; ++q) {
+        const int J = J0 - q;
+        const double num = dc[q] - cc[q] * x;
+        x = (bc[q] != 0.0) ? div_by_recip(num, bc[q], rc[q]) : 0.0;
+        D[max(J, -1) * kTsCols + lane] = x;  // below row 0: padding row
+        dc[q] = dn[q];
+        bc[q] = bn[q];
+        cc[q] = cn[q];
+        rc[q] = rn[q];
+      }
+    }
+  }
+  lds_barrier();
+  // write-back rows and the CFL minimum of this block
+  const bool cour = a.courant && f < 3;
+  const double dxf = (f == 0) ? a.dx : a.dz;
+  double dtmin = __longlong_as_double(0x7FF0000000000000ll);
+  const int wlo = max(a.wlo[f], r0), whi = min(a.whi[f], r1 - 1);
+  if (live) {
+    for (int i = wlo; i <= whi; ++i) {
+      const double x = D[i * kTsCols + lane];
+      U[(long long)i * pp] = x;
+      if (cour && i >= 1 && i <= NY) {
+        const double dl = (f == 1) ? dy[i] : dxf;
+        dtmin = fmin(dtmin, a.cfl * dl / fabs(x));
+      }
+    }
+  }
+  if (cour) {
+    for (int o = 32; o > 0; o >>= 1) dtmin = fmin(dtmin, __shfl_down(dtmin, o));
+    if (lane == 0) red[w] = dtmin;
+    lds_barrier();
+    if (threadIdx.x == 0) {
+      double v = red[0];
+      for (int q = 1; q < nw; ++q) v = fmin(v, red[q]);
       if (v >= 0.0) atomicMin(&a.clk->cfl_min_bits, (unsigned long long)__double_as_longlong(v));
     }
   }
@@ -572,13 +780,9 @@ __global__ void k_courant_dt(Clock *clk, double dt_cap, int variable_dt, int upd
 }
 
 // ---------------------------------------------------------------------------
-// REM_DIV_CHAN (channel.f:1103-1214 with APPLY_BC_REM_DIV_MPI mpi.f:1239-1278)
-// and the CP update (channel.f:1059-1065).  Each block holds the Poisson
-// systems of kRdModes consecutive (kx,kz) modes, rows 1..NY, in LDS and
-// solves them by parallel cyclic reduction; then projects CU and updates CP.
+// REM_DIV_CHAN (channel.f:1103-1214): divergence, pressure-Poisson solve per
+// (kx,kz) mode, projection; plus the CP update of channel.f:1059-1065.
 // ---------------------------------------------------------------------------
-constexpr int kRdModes = 8;  // rows per thread <= 9: NY <= 9 * 256 / kRdModes
-
 struct RemDivArgs {
   Geo g;
   Clock *clk;
@@ -588,129 +792,70 @@ struct RemDivArgs {
   double2 *cp;
   double2 *phi;
   const double *rl, *ru;  // 1/(DY(J)DYF(J)), 1/(DY(J+1)DYF(J))
+  double *ta, *tb, *tr;   // sweep multipliers, pivots, 1/pivots [J][mode]
 };
 
-__global__ void __launch_bounds__(256) k_rem_div(RemDivArgs a) {
-  extern __shared__ double sr[];
+// Pivots of the pressure-Poisson systems (channel.f:1112-1165 with the BC
+// rows of APPLY_BC_REM_DIV_MPI, mpi.f:1239-1278): the matrix of mode (kx,kz)
+// depends only on the grid and k^2, so the multipliers and pivots of the
+// forward sweep of THOMAS_FORWARD_COMPLEX_MPI (mpi.f:708-764) are computed
+// once per grid, in the same operation order.  ta/tb: [J][mode], J = 2..NY.
+__global__ void __launch_bounds__(256) k_rem_div_factors(RemDivArgs a) {
+#pragma clang fp contract(off)
   const Geo &g = a.g;
   const int NY = g.ny;
-  const int n = NY;  // rows 1..NY stored at 0..NY-1
-  double *A = sr, *B = sr + n * kRdModes, *C = sr + 2 * n * kRdModes;
-  double2 *D = reinterpret_cast<double2 *>(sr + 3 * n * kRdModes);
-  const int lane = threadIdx.x % kRdModes, rg = threadIdx.x / kRdModes;
-  const int ngrp = blockDim.x / kRdModes;
-  const int m = blockIdx.x * kRdModes + lane;
-  const bool live = m < g.pf;
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= g.pf) return;
   const long long pf = g.pf;
-  int kx = 0, kzp = 0;
-  if (live) {
-    kzp = (int)fdiv((unsigned)m, g.f_nkx1);
-    kx = m - kzp * g.nkx1;
-  }
-  const double kxv = g.kx[kx], kzv = g.kz[kzp], kx2 = g.kx2[kx], kz2 = g.kz2[kzp];
+  const int kzp = (int)fdiv((unsigned)m, g.f_nkx1), kx = m - kzp * g.nkx1;
+  const double kx2 = g.kx2[kx], kz2 = g.kz2[kzp];
   const bool mean = (kx == 0 && kzp == 0);
-  // system rows J = 1..NY (channel.f:1126-1143, BC rows mpi.f:1244-1272)
-  for (int r = rg; r < n; r += ngrp) {
-    const int J = r + 1;
-    double L, Dg, Uu;
-    double2 V = make_double2(0.0, 0.0);
-    if (J == 1) {
-      L = 0.; Dg = 1.; Uu = mean ? 0. : -1.;
-    } else if (J == NY) {
-      L = 1.; Dg = -1.; Uu = 0.;
-    } else {
+  // row 1: D=1, U=0 (mean mode) or D=1, U=-1
+  double bprev = 1.0, uprev = mean ? 0.0 : -1.0;
+  for (int J = 2; J <= NY; ++J) {
+    double L, D, U;
+    if (J < NY) {
       L = a.rl[J];
-      Uu = a.ru[J];
-      Dg = -kx2 - kz2 - Uu - L;
-      if (live) {
-        const double2 c1 = a.cu[0][pf * J + m], c3 = a.cu[2][pf * J + m];
-        const double2 c2 = a.cu[1][pf * J + m], c2p = a.cu[1][pf * (J + 1) + m];
-        const double2 t1 = cmul_ik(kxv, c1), t3 = cmul_ik(kzv, c3);
-        const double d = g.dyf[J];
-        V = make_double2(t1.x + (c2p.x - c2.x) / d + t3.x, t1.y + (c2p.y - c2.y) / d + t3.y);
-      }
+      U = a.ru[J];
+      D = -kx2 - kz2 - U - L;
+    } else {  // row NY: L=1, D=-1, U=0
+      L = 1.;
+      D = -1.;
+      U = 0.;
     }
-    A[r * kRdModes + lane] = L;
-    B[r * kRdModes + lane] = Dg;
-    C[r * kRdModes + lane] = Uu;
-    D[r * kRdModes + lane] = V;
-  }
-  __syncthreads();
-  for (int dd = 1; dd < n; dd *= 2) {
-    double na[9], nb[9], nc[9];
-    double2 nd[9];
-    int q = 0;
-    for (int r = rg; r < n; r += ngrp, ++q) {
-      const int o = r * kRdModes + lane;
-      const bool lo = r - dd >= 0, hi = r + dd < n;
-      const int ol = (r - dd) * kRdModes + lane, oh = (r + dd) * kRdModes + lane;
-      const double k1 = lo ? A[o] / B[ol] : 0.0;
-      const double k2 = hi ? C[o] / B[oh] : 0.0;
-      na[q] = lo ? -A[ol] * k1 : 0.0;
-      nc[q] = hi ? -C[oh] * k2 : 0.0;
-      nb[q] = B[o] - (lo ? C[ol] * k1 : 0.0) - (hi ? A[oh] * k2 : 0.0);
-      double2 v = D[o];
-      if (lo) v = make_double2(v.x - k1 * D[ol].x, v.y - k1 * D[ol].y);
-      if (hi) v = make_double2(v.x - k2 * D[oh].x, v.y - k2 * D[oh].y);
-      nd[q] = v;
-    }
-    __syncthreads();
-    q = 0;
-    for (int r = rg; r < n; r += ngrp, ++q) {
-      const int o = r * kRdModes + lane;
-      A[o] = na[q];
-      B[o] = nb[q];
-      C[o] = nc[q];
-      D[o] = nd[q];
-    }
-    __syncthreads();
-  }
-  // phi = d / b (held in D), then the projection and pressure update
-  for (int r = rg; r < n; r += ngrp) {
-    const int o = r * kRdModes + lane;
-    const double b = B[o];
-    const double2 v = D[o];
-    D[o] = (b != 0.0) ? make_double2(v.x / b, v.y / b) : make_double2(0.0, 0.0);
-  }
-  __syncthreads();
-  if (!live) return;
-  const double hb = (a.rk < 0) ? 1.0 : (a.cp_old ? a.clk->h_cp : a.clk->h_bar[a.rk]);
-  for (int r = rg; r < n; r += ngrp) {
-    const int J = r + 1;
-    const double2 phi = D[r * kRdModes + lane];
-    a.phi[pf * J + m] = phi;
-    double2 t = cmul_ik(kxv, phi), u = a.cu[0][pf * J + m];
-    a.cu[0][pf * J + m] = make_double2(u.x - t.x, u.y - t.y);
-    t = cmul_ik(kzv, phi);
-    u = a.cu[2][pf * J + m];
-    a.cu[2][pf * J + m] = make_double2(u.x - t.x, u.y - t.y);
-    if (J >= 2) {
-      const double2 pm = D[(r - 1) * kRdModes + lane];
-      const double d = g.dy[J];
-      u = a.cu[1][pf * J + m];
-      a.cu[1][pf * J + m] = make_double2(u.x - (phi.x - pm.x) / d, u.y - (phi.y - pm.y) / d);
-      if (a.rk >= 0) {
-        double2 p = a.cp[pf * J + m];
-        a.cp[pf * J + m] = make_double2(p.x + phi.x / hb, p.y + phi.y / hb);
-      }
-    }
+    const double A = (bprev != 0.0) ? -L / bprev : 0.0;
+    const double B = D + A * uprev;
+    a.ta[pf * J + m] = A;
+    a.tb[pf * J + m] = B;
+    a.tr[pf * J + m] = 1.0 / B;
+    bprev = B;
+    uprev = U;
   }
 }
 
-// REM_DIV_CHAN with the reference's Thomas sweeps (THOMAS_FORWARD/BACKWARD_
-// COMPLEX_MPI, mpi.f:708-865) in their operation order, one thread per
-// (kx,kz) mode: all threads stage the right-hand sides of kRdtModes modes
-// into LDS with coalesced loads, one thread per mode runs the sequential
-// sweeps out of LDS, and all threads apply the projection and the CP update.
-constexpr int kRdtModes = 32;
+// One pressure-Poisson solve per (kx,kz) mode.  All threads stage the
+// divergence (channel.f:1137-1143) and the mode's multipliers and pivots of
+// kRdtModes modes into LDS; one lane per mode runs the forward and backward
+// sweeps (mpi.f:736-747, 838-856); then all threads project U and update CP
+// (channel.f:1185-1210, 1059-1065).  The sweeps read their operands
+// kRdtAhead rows ahead without bounds tests (kRdtPad rows of padding below
+// and above rows 0..NY), so nothing in the recurrence waits on LDS.
+constexpr int kRdtModes = 8;
+constexpr int kRdtAhead = 4;
+constexpr int kRdtPad = 8;  // >= 2 * kRdtAhead
 
 __global__ void __launch_bounds__(256) k_rem_div_thomas(RemDivArgs a) {
+#pragma clang fp contract(off)
   extern __shared__ double st[];
   const Geo &g = a.g;
-  const int NY = g.ny;
-  double2 *Gs = reinterpret_cast<double2 *>(st);                  // [J][mode], J = 0..NY
-  double *Bs = st + 2 * (NY + 1) * kRdtModes;                       // [J][mode]
-  double *rl = Bs + (NY + 1) * kRdtModes, *ru = rl + (NY + 1);       // [J]
+  const int NY = g.ny, ntot = NY + 1 + 2 * kRdtPad;
+  // LDS, row J at index J + kRdtPad: G [row][mode] complex, A, B and 1/B
+  // [row][mode], U [row]
+  double2 *Gs = reinterpret_cast<double2 *>(st) + kRdtPad * kRdtModes;
+  double *As = st + 2 * ntot * kRdtModes + kRdtPad * kRdtModes;
+  double *Bs = As + ntot * kRdtModes;
+  double *Rs = Bs + ntot * kRdtModes;
+  double *ru = st + 5 * ntot * kRdtModes + kRdtPad;
   const int lane = threadIdx.x % kRdtModes, rg = threadIdx.x / kRdtModes;
   const int ngrp = blockDim.x / kRdtModes;
   const int m = blockIdx.x * kRdtModes + lane;
@@ -721,88 +866,125 @@ __global__ void __launch_bounds__(256) k_rem_div_thomas(RemDivArgs a) {
     kzp = (int)fdiv((unsigned)m, g.f_nkx1);
     kx = m - kzp * g.nkx1;
   }
-  const double kxv = g.kx[kx], kzv = g.kz[kzp], kx2 = g.kx2[kx], kz2 = g.kz2[kzp];
+  const double kxv = g.kx[kx], kzv = g.kz[kzp];
   const bool mean = (kx == 0 && kzp == 0);
-  for (int j = threadIdx.x; j <= NY; j += blockDim.x) {
-    rl[j] = a.rl[j];
-    ru[j] = a.ru[j];
-  }
-  // right-hand sides of rows 2..NY-1 (channel.f:1137-1143); rows 1, NY are BC
-  // rows.  Batches of 4 rows per thread keep 16 loads in flight.
-  for (int J0 = 2 + rg; J0 < NY; J0 += 4 * ngrp) {
+  for (int j = (int)threadIdx.x - kRdtPad; j <= NY + kRdtPad; j += blockDim.x)
+    ru[j] = (j >= 0 && j <= NY) ? a.ru[j] : 0.0;
+  // right-hand sides of rows 2..NY-1 (rows 1, NY are BC rows with zero
+  // right-hand side) and the sweep tables of rows 2..NY, in batches of 4
+  // rows per thread
+  for (int J0 = 2 + rg; J0 <= NY; J0 += 4 * ngrp) {
     double2 c1[4], c2[4], c2p[4], c3[4];
+    double ta[4], tb[4], tr[4];
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
       const int J = J0 + b * ngrp;
-      if (live && J < NY) {
-        c1[b] = a.cu[0][pf * J + m];
-        c3[b] = a.cu[2][pf * J + m];
-        c2[b] = a.cu[1][pf * J + m];
-        c2p[b] = a.cu[1][pf * (J + 1) + m];
+      if (live && J <= NY) {
+        ta[b] = a.ta[pf * J + m];
+        tb[b] = a.tb[pf * J + m];
+        tr[b] = a.tr[pf * J + m];
+        if (J < NY) {
+          c1[b] = a.cu[0][pf * J + m];
+          c3[b] = a.cu[2][pf * J + m];
+          c2[b] = a.cu[1][pf * J + m];
+          c2p[b] = a.cu[1][pf * (J + 1) + m];
+        }
       }
     }
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
       const int J = J0 + b * ngrp;
-      if (J < NY) {
+      if (J <= NY) {
         double2 V = make_double2(0.0, 0.0);
-        if (live) {
+        if (live && J < NY) {
           const double2 t1 = cmul_ik(kxv, c1[b]), t3 = cmul_ik(kzv, c3[b]);
           const double d = g.dyf[J];
           V = make_double2(t1.x + (c2p[b].x - c2[b].x) / d + t3.x, t1.y + (c2p[b].y - c2[b].y) / d + t3.y);
         }
         Gs[J * kRdtModes + lane] = V;
+        As[J * kRdtModes + lane] = live ? ta[b] : 0.0;
+        Bs[J * kRdtModes + lane] = live ? tb[b] : 1.0;
+        Rs[J * kRdtModes + lane] = live ? tr[b] : 1.0;
       }
     }
   }
-  __syncthreads();
-  if (rg == 0 && live) {
-    // forward, J = 2..NY (mpi.f:736-747); row 1: D=1, U=0 (mean) or -1, G=0
-    double bprev = 1.0, uprev = mean ? 0.0 : -1.0;
-    double2 gprev = make_double2(0.0, 0.0);
-    for (int J = 2; J <= NY; ++J) {
-      double L, D, U;
-      double2 V;
-      if (J < NY) {
-        L = rl[J];
-        U = ru[J];
-        D = -kx2 - kz2 - U - L;
-        V = Gs[J * kRdtModes + lane];
-      } else {
-        L = 1.;
-        D = -1.;
-        U = 0.;
-        V = make_double2(0.0, 0.0);
-      }
-      const double A = (bprev != 0.0) ? -L / bprev : 0.0;
-      const double B = D + A * uprev;
-      const double2 G = make_double2(V.x + A * gprev.x, V.y + A * gprev.y);
-      Bs[J * kRdtModes + lane] = B;
-      Gs[J * kRdtModes + lane] = G;
-      bprev = B;
-      uprev = U;
-      gprev = G;
+  lds_barrier();
+  if (rg == 0) {
+    // forward: G_J = V_J + A_J G_{J-1}, J = 2..NY, G_1 = 0
+    double2 gp = make_double2(0.0, 0.0), gny = gp;
+    double2 vc[kRdtAhead], vn[kRdtAhead];
+    double ac[kRdtAhead], an[kRdtAhead];
+#pragma unroll
+    for (int q = 0; q < kRdtAhead; ++q) {
+      vc[q] = Gs[(2 + q) * kRdtModes + lane];
+      ac[q] = As[(2 + q) * kRdtModes + lane];
     }
-    // backward, J = NY..1 (mpi.f:838-856)
-    double2 x = (bprev != 0.0) ? make_double2(gprev.x / bprev, gprev.y / bprev) : make_double2(0.0, 0.0);
+    for (int J0 = 2; J0 <= NY; J0 += kRdtAhead) {
+#pragma unroll
+      for (int q = 0; q < kRdtAhead; ++q) {
+        vn[q] = Gs[(J0 + kRdtAhead + q) * kRdtModes + lane];
+        an[q] = As[(J0 + kRdtAhead + q) * kRdtModes + lane];
+      }
+#pragma unroll
+      for (int q = 0; q < kRdtAhead; ++q) {
+        const int J = J0 + q;
+        gp = make_double2(vc[q].x + ac[q] * gp.x, vc[q].y + ac[q] * gp.y);
+        Gs[J * kRdtModes + lane] = gp;  // rows above NY land in the padding
+        // component-wise selects: a select of double2 values goes through scratch
+        gny.x = (J == NY) ? gp.x : gny.x;
+        gny.y = (J == NY) ? gp.y : gny.y;
+        vc[q] = vn[q];
+        ac[q] = an[q];
+      }
+    }
+    // backward: X_NY = G_NY / B_NY; X_J = (G_J - U_J X_{J+1}) / B_J,
+    // J = NY-1..2; row 1: B=1, G=0, U = 0 (mean mode) or -1
+    const double bny = Bs[NY * kRdtModes + lane];
+    double2 x;
+    x.x = (bny != 0.0) ? gny.x / bny : 0.0;
+    x.y = (bny != 0.0) ? gny.y / bny : 0.0;
     Gs[NY * kRdtModes + lane] = x;
-    for (int J = NY - 1; J >= 1; --J) {
-      double B, U;
-      double2 G;
-      if (J >= 2) {
-        B = Bs[J * kRdtModes + lane];
-        G = Gs[J * kRdtModes + lane];
-        U = ru[J];
-      } else {
-        B = 1.0;
-        G = make_double2(0.0, 0.0);
-        U = mean ? 0.0 : -1.0;
-      }
-      x = (B != 0.0) ? make_double2((G.x - U * x.x) / B, (G.y - U * x.y) / B) : make_double2(0.0, 0.0);
-      Gs[J * kRdtModes + lane] = x;
+    double2 x2 = x;
+    double2 wc[kRdtAhead], wn[kRdtAhead];
+    double bc[kRdtAhead], bn[kRdtAhead], rc[kRdtAhead], rn[kRdtAhead], uc[kRdtAhead], un[kRdtAhead];
+#pragma unroll
+    for (int q = 0; q < kRdtAhead; ++q) {
+      wc[q] = Gs[(NY - 1 - q) * kRdtModes + lane];
+      bc[q] = Bs[(NY - 1 - q) * kRdtModes + lane];
+      rc[q] = Rs[(NY - 1 - q) * kRdtModes + lane];
+      uc[q] = ru[NY - 1 - q];
     }
+    for (int J0 = NY - 1; J0 >= 2; J0 -= kRdtAhead) {
+#pragma unroll
+      for (int q = 0; q < kRdtAhead; ++q) {
+        const int J = J0 - kRdtAhead - q;
+        wn[q] = Gs[J * kRdtModes + lane];
+        bn[q] = Bs[J * kRdtModes + lane];
+        rn[q] = Rs[J * kRdtModes + lane];
+        un[q] = ru[J];
+      }
+#pragma unroll
+      for (int q = 0; q < kRdtAhead; ++q) {
+        const int J = J0 - q;
+        const double B = bc[q], U = uc[q];
+        // (G_J - U_J X_{J+1}) / B_J, quotients formed from the tabulated 1/B_J
+        const double xr = div_by_recip(wc[q].x - U * x.x, B, rc[q]);
+        const double xi = div_by_recip(wc[q].y - U * x.y, B, rc[q]);
+        x.x = (B != 0.0) ? xr : 0.0;
+        x.y = (B != 0.0) ? xi : 0.0;
+        Gs[max(J, 1) * kRdtModes + lane] = x;  // row 1 is rewritten below
+        x2.x = (J == 2) ? x.x : x2.x;
+        x2.y = (J == 2) ? x.y : x2.y;
+        wc[q] = wn[q];
+        bc[q] = bn[q];
+        rc[q] = rn[q];
+        uc[q] = un[q];
+      }
+    }
+    const double U1 = mean ? 0.0 : -1.0;
+    Gs[1 * kRdtModes + lane] = make_double2((0.0 - U1 * x2.x) / 1.0, (0.0 - U1 * x2.y) / 1.0);
   }
-  __syncthreads();
+  lds_barrier();
   if (!live) return;
   const double hb = (a.rk < 0) ? 1.0 : (a.cp_old ? a.clk->h_cp : a.clk->h_bar[a.rk]);
   for (int J0 = 1 + rg; J0 <= NY; J0 += 4 * ngrp) {
@@ -904,6 +1086,7 @@ static FactorIn make_factor_in(const Ctx &c) {
   }
   in.dy = c.d_dy;
   in.dyf = c.d_dyf;
+  in.pcr = c.cn_pcr ? 1 : 0;
   return in;
 }
 
@@ -923,6 +1106,11 @@ static hipError_t launch_nonlinear(Ctx &c, int rk) {
 static size_t solve_lds(const Ctx &c) {
   const size_t nrow = c.g.ny + 2;
   return (nrow * kSolveCols + 2 * (size_t)c.fac.nsteps * nrow + 4 * nrow) * sizeof(double) + nrow * sizeof(int);
+}
+
+static size_t solve_thomas_lds(const Ctx &c) {
+  const size_t nrow = c.g.ny + 2;
+  return ((nrow + 2) * kTsCols + 4 * nrow) * sizeof(double);
 }
 
 static hipError_t launch_solve(Ctx &c, int rk, int courant) {
@@ -946,16 +1134,22 @@ static hipError_t launch_solve(Ctx &c, int rk, int courant) {
   a.cfl = c.p.cfl;
   a.dx = c.p.lx / c.g.nx;
   a.dz = c.p.lz / c.g.nz;
-  const int nb = (int)((c.g.pp + kSolveCols - 1) / kSolveCols);
-  hipLaunchKernelGGL(k_solve, dim3(nb, c.nfields), dim3(kSolveThreads), solve_lds(c), c.stream, a);
+  if (c.cn_pcr) {
+    const int nb = (int)((c.g.pp + kSolveCols - 1) / kSolveCols);
+    hipLaunchKernelGGL(k_solve, dim3(nb, c.nfields), dim3(kSolveThreads), solve_lds(c), c.stream, a);
+  } else {
+    const int nb = (int)((c.g.pp + kTsCols - 1) / kTsCols);
+    hipLaunchKernelGGL(k_solve_thomas, dim3(nb, c.nfields), dim3(kTsThreads), solve_thomas_lds(c), c.stream, a);
+  }
   return hipGetLastError();
 }
 
 static size_t rem_div_thomas_lds(const Ctx &c) {
-  return (size_t)(c.g.ny + 1) * kRdtModes * 3 * sizeof(double) + 2 * (size_t)(c.g.ny + 1) * sizeof(double);
+  const size_t ntot = c.g.ny + 1 + 2 * kRdtPad;
+  return ntot * kRdtModes * 5 * sizeof(double) + ntot * sizeof(double);
 }
 
-hipError_t launch_rem_div(Ctx &c, int rk) {
+static RemDivArgs make_rem_div_args(Ctx &c, int rk) {
   RemDivArgs a{};
   a.g = c.g;
   a.clk = c.clock;
@@ -966,14 +1160,22 @@ hipError_t launch_rem_div(Ctx &c, int rk) {
   a.phi = c.phi;
   a.rl = c.rd_l;
   a.ru = c.rd_u;
-  if (c.rd_pcr) {
-    const int nb = (c.g.pf + kRdModes - 1) / kRdModes;
-    const size_t shm = (size_t)c.g.ny * kRdModes * 5 * sizeof(double);
-    hipLaunchKernelGGL(k_rem_div, dim3(nb), dim3(256), shm, c.stream, a);
-  } else {
-    const int nb = (c.g.pf + kRdtModes - 1) / kRdtModes;
-    hipLaunchKernelGGL(k_rem_div_thomas, dim3(nb), dim3(256), rem_div_thomas_lds(c), c.stream, a);
-  }
+  a.ta = c.rd_ta;
+  a.tb = c.rd_tb;
+  a.tr = c.rd_tr;
+  return a;
+}
+
+hipError_t launch_rem_div_factors(Ctx &c) {
+  RemDivArgs a = make_rem_div_args(c, -1);
+  hipLaunchKernelGGL(k_rem_div_factors, dim3((c.g.pf + 255) / 256), dim3(256), 0, c.stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_rem_div(Ctx &c, int rk) {
+  RemDivArgs a = make_rem_div_args(c, rk);
+  const int nb = (c.g.pf + kRdtModes - 1) / kRdtModes;
+  hipLaunchKernelGGL(k_rem_div_thomas, dim3(nb), dim3(256), rem_div_thomas_lds(c), c.stream, a);
   return hipGetLastError();
 }
 
@@ -1031,6 +1233,9 @@ hipError_t prepare_step_kernels(Ctx &c) {
       hipFuncSetAttribute((const void *)k_nonlinear, hipFuncAttributeMaxDynamicSharedMemorySize, c.lds_bytes);
   if (e != hipSuccess) return e;
   e = hipFuncSetAttribute((const void *)k_solve, hipFuncAttributeMaxDynamicSharedMemorySize, (int)solve_lds(c));
+  if (e != hipSuccess) return e;
+  e = hipFuncSetAttribute((const void *)k_solve_thomas, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)solve_thomas_lds(c));
   if (e != hipSuccess) return e;
   return hipFuncSetAttribute((const void *)k_rem_div_thomas, hipFuncAttributeMaxDynamicSharedMemorySize,
                              (int)rem_div_thomas_lds(c));
