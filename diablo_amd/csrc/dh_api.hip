@@ -107,7 +107,7 @@ hipError_t dalloc(T **p, size_t count) {
 void free_all(Ctx &c) {
   void *ptrs[] = {c.tw_row, c.tw_col, c.tw_real, c.posr, c.posrd, c.posz, c.posc, c.kx, c.kz, c.kx2, c.kz2, c.d_dy, c.d_dyf, c.d_gyf, c.cu[0], c.cu[1],
                   c.cu[2], c.cp, c.cf[0], c.cf[1], c.cf[2], c.cth, c.cfth, c.phi, c.uphys, c.rphys, c.rd_ta,
-                  c.rd_tb, c.clock, c.fac.k1, c.fac.k2, c.fac.bfin, c.fac.tl, c.fac.tb, c.fac.tc, c.fac.tr, c.fac.isbc, c.rd_tr, c.fac.vconst, c.rd_l, c.rd_u};
+                  c.rd_tb, c.clock, c.fac.pf, c.fac.pb, c.fac.tl, c.fac.tb, c.fac.tc, c.fac.tr, c.fac.isbc, c.rd_tr, c.fac.vconst, c.rd_l, c.rd_u};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c.step_graph) (void)hipGraphExecDestroy(c.step_graph);
@@ -141,7 +141,7 @@ double dt_cap_of(const Ctx &c) {
 }
 
 int rebuild_tables(Ctx &c) {
-  HIPCHK(launch_pcr_tables(c));
+  HIPCHK(launch_cn_tables(c));
   HIPCHK(hipStreamSynchronize(c.stream));
   return DH_OK;
 }
@@ -193,7 +193,7 @@ int dh_create(const dh_params *p, dh_ctx **out) {
   c.lds_bytes = g.nz * g.ls * (int)sizeof(double2) + (2 * g.lw + g.nz + 1) * (int)sizeof(double2) +
                 (2 * g.lw + 2 * g.nz) * (int)sizeof(int);
   if (c.lds_bytes > 160 * 1024) return reject("x-z plane too large for the LDS-resident plane kernels");
-  if (g.ny + 2 > 288) return reject("NY+2 > 288 rows is beyond the column solver's register tiles");
+  if (g.ny + 2 > 8 * kSegRows) return reject("NY+2 is beyond the column solver's register segments (8 x 36 rows)");
   int thr = (g.nz * g.lw + 7) / 8;
   thr = ((thr + 63) / 64) * 64;
   if (thr < 64) thr = 64;
@@ -202,8 +202,6 @@ int dh_create(const dh_params *p, dh_ctx **out) {
   if (thr * 4 < g.pf) return reject("plane has too many Fourier modes");
   c.plane_threads = thr;
   {
-    const char *cn = std::getenv("DIABLO_CNSOLVE");
-    c.cn_pcr = cn && std::strcmp(cn, "pcr") == 0;
   }
   g.f_npair = recip(g.lw / 2 + 1);
   g.f_nkx1 = recip(g.nkx1);
@@ -272,15 +270,14 @@ int dh_create(const dh_params *p, dh_ctx **out) {
   ACHK(dalloc(&c.rd_l, nyy));
   ACHK(dalloc(&c.rd_u, nyy));
   ACHK(dalloc(&c.clock, 1));
-  ACHK(dalloc(&c.fac.k1, 3 * (size_t)c.nfields * kMaxPcr * nyy));
-  ACHK(dalloc(&c.fac.k2, 3 * (size_t)c.nfields * kMaxPcr * nyy));
-  ACHK(dalloc(&c.fac.bfin, 3 * (size_t)c.nfields * nyy));
+  ACHK(dalloc(&c.fac.pf, 3 * (size_t)c.nfields * nyy));
+  ACHK(dalloc(&c.fac.pb, 3 * (size_t)c.nfields * nyy));
   ACHK(dalloc(&c.fac.tl, 3 * (size_t)c.nfields * nyy));
   ACHK(dalloc(&c.fac.tb, 3 * (size_t)c.nfields * nyy));
   ACHK(dalloc(&c.fac.tc, 3 * (size_t)c.nfields * nyy));
   ACHK(dalloc(&c.fac.tr, 3 * (size_t)c.nfields * nyy));
-  c.fac.nsteps = 0;
-  while ((1 << c.fac.nsteps) < (int)nyy) ++c.fac.nsteps;
+  c.fac.nseg = (nyy <= 4 * (size_t)kSegRows) ? 4 : 8;
+  c.fac.seg_rows = (int)((nyy + c.fac.nseg - 1) / c.fac.nseg);
   ACHK(dalloc(&c.fac.isbc, (size_t)c.nfields * nyy));
   ACHK(dalloc(&c.fac.vconst, (size_t)c.nfields * nyy));
   {

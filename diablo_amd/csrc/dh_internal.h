@@ -69,14 +69,20 @@ struct Clock {
 // Parallel-cyclic-reduction tables of the y-implicit (Crank-Nicolson)
 // matrices, one set per RK substep and field; with LES off each matrix
 // depends only on j, so every (x,z) column of a field shares it.
-constexpr int kMaxPcr = 12;  // PCR steps: NY+2 <= 2^12 rows
+// Rows of one column held by one lane of the Crank-Nicolson solve; a column
+// is split into 4 (NY+2 <= 144) or 8 such segments.
+constexpr int kSegRows = 36;
+
+// Tables of the y-implicit (Crank-Nicolson) matrices, one set per RK
+// substep and field; with LES off each matrix depends only on j, so every
+// (x,z) column of a field shares them.
 struct RealFactors {
-  double *k1, *k2;  // [rk 0..2][field][step][row] elimination multipliers
-  double *bfin;     // [rk][field][row] diagonal after the last step
   double *tl, *tb, *tc, *tr;  // [rk][field][row] Thomas multipliers, pivots, upper diagonal, 1/pivots
-  int *isbc;        // [field][row] 1: row takes a constant right-hand side
-  double *vconst;   // [field][row]
-  int nsteps;       // ceil(log2(NY+2))
+  double *pf, *pb;            // [rk][field][row] segment carry factors of the two sweeps
+  int *isbc;                  // [field][row] 1: row takes a constant right-hand side
+  double *vconst;             // [field][row]
+  int seg_rows;               // rows per register segment of a column
+  int nseg;                   // segments per column (4 or 8)
 };
 
 struct Ctx {
@@ -87,7 +93,6 @@ struct Ctx {
   int wlo[kMaxFields], whi[kMaxFields];  // solution write-back rows per field
   int lds_bytes;
   int plane_threads;
-  bool cn_pcr = false;  // Crank-Nicolson solves by cyclic reduction instead of Thomas sweeps
   hipStream_t stream = nullptr;
   std::string err;
   bool grid_set = false;
@@ -134,7 +139,7 @@ hipError_t launch_phase(Ctx &c, int rk, int phase);
 hipError_t launch_rem_div(Ctx &c, int rk_for_cp);     // rk_for_cp<0: no CP update
 hipError_t launch_step_begin(Ctx &c);
 hipError_t launch_end_step(Ctx &c);
-hipError_t launch_pcr_tables(Ctx &c);
+hipError_t launch_cn_tables(Ctx &c);
 hipError_t launch_rem_div_factors(Ctx &c);  // once per grid  // for the DELTA_T in the device clock
 
 }  // namespace dh

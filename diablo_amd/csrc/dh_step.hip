@@ -1,12 +1,12 @@
 // Kernels of one RK3/Crank-Nicolson substep (RK_CHAN_1, channel.f:62-1073)
 // other than the plain plane transforms:
 //   k_nonlinear   explicit Fourier right-hand side, nonlinear products and
-//                 their forward transforms, RK combination and the inverse
-//                 transform of the right-hand side (channel.f:77-600);
-//   k_solve_thomas / k_solve
-//                 explicit half of the y-viscous term plus the real
-//                 tridiagonal Crank-Nicolson solve of every (x,z) column, and
-//                 the CFL minimum (channel.f:606-1029, courant.f:34-43);
+//                 their forward transforms, RK combination, explicit half of
+//                 the y-viscous term and the inverse transform of the
+//                 right-hand side (channel.f:77-626);
+//   k_solve_seg   real tridiagonal Crank-Nicolson solve of every (x,z)
+//                 column, and the CFL minimum (channel.f:771-1029,
+//                 courant.f:34-43);
 //   k_rem_div_thomas
 //                 divergence, pressure-Poisson solve per (kx,kz), projection
 //                 and pressure update (channel.f:1103-1214, 1059-1065);
@@ -30,7 +30,7 @@ struct StepArgs {
   double *rphys;
   long long fs;   // physical field stride (NY+2)*NX*NZ
   long long cfs;  // Fourier field stride (NY+2)*pf
-  double nu, beta, i_ro, grav_x, grav_y, grav_z, px0, omega0, amp_omega0;
+  double nu, nu_v_scale, beta, i_ro, grav_x, grav_y, grav_z, px0, omega0, amp_omega0;
   int f_type, n_th;
   double ri[DH_MAX_TH], pr[DH_MAX_TH], drhodx[DH_MAX_TH], drhodz[DH_MAX_TH];
   int jstart_th[DH_MAX_TH], jend_th[DH_MAX_TH];
@@ -227,14 +227,33 @@ __global__ void __launch_bounds__(1024) k_nonlinear(StepArgs a) {
     if (zr > g.nkz && zr < g.nz - g.nkz) A[inv_row(T, zr) * g.ls + kx] = make_double2(0.0, 0.0);
   }
   double2 *CF = (c < 3) ? a.cf[c] : a.cfth + (c - 3) * a.cfs;
+  const double2 *CC = (c < 3) ? a.cu[c] : a.cth + (c - 3) * a.cfs;
   const long long base = (long long)J * g.pf;
+  // explicit half of the Crank-Nicolson y-viscous term (channel.f:606-626,
+  // 739-747), applied to the Fourier coefficients: the y-difference operator
+  // commutes with the x-z transform, so adding it to CR before the inverse
+  // transform gives R + T1 NU_V_SCALE d/dy(d/dy U) of the reference
+  const double coefv = (c < 3) ? (a.nu * hb / 2.0) * a.nu_v_scale : ((a.nu * hb / 2.0) / a.pr[c - 3]) * a.nu_v_scale;
+  double ey0, ey1, ey2;  // weights of C(J+1)-C(J), C(J)-C(J-1) and the outer divisor
+  if (c == 1) {
+    ey0 = g.dyf[J];
+    ey1 = g.dyf[J - 1];
+    ey2 = g.dy[J];
+  } else {
+    ey0 = g.dy[J + 1];
+    ey1 = g.dy[J];
+    ey2 = g.dyf[J];
+  }
 #pragma unroll
   for (int q = 0; q < kMaxQ; ++q) {
     const int m = threadIdx.x + q * nthr;
     if (m < g.pf) {
       const int kzp = (int)fdiv((unsigned)m, g.f_nkx1), kx = m - kzp * g.nkx1;
       const long long o = base + m;
-      double2 cr = (c < 3) ? a.cu[c][o] : a.cth[(c - 3) * a.cfs + o];
+      const double2 cm1 = CC[o - g.pf], cp1 = CC[o + g.pf];
+      double2 cr = CC[o];
+      const double2 ev = make_double2(coefv * (((cp1.x - cr.x) / ey0 - (cr.x - cm1.x) / ey1) / ey2),
+                                      coefv * (((cp1.y - cr.y) / ey0 - (cr.y - cm1.y) / ey1) / ey2));
       if (rk > 0) {
         double2 old = CF[o];
         cr = make_double2(cr.x + T3 * old.x, cr.y + T3 * old.y);
@@ -257,6 +276,7 @@ __global__ void __launch_bounds__(1024) k_nonlinear(StepArgs a) {
       const double2 cfn = acc[q];
       CF[o] = cfn;
       cr = make_double2(cr.x + T5 * cfn.x, cr.y + T5 * cfn.y);
+      cr = make_double2(cr.x + ev.x, cr.y + ev.y);
       A[inv_row(T, kz_row(g, kzp)) * g.ls + kx] = cr;
     }
   }
@@ -277,7 +297,7 @@ struct FactorIn {
   int bcmin[kMaxFields], bcmax[kMaxFields];
   double c1min[kMaxFields], c1max[kMaxFields];
   const double *dy, *dyf;
-  int pcr;  // also build the cyclic-reduction tables
+  int seg_rows;  // rows per register segment of the solve
 };
 
 // Row J (0..NY+1) of the system of field f: lower L, diagonal D, upper U,
@@ -286,6 +306,7 @@ struct FactorIn {
 // then the BC rows overwrite.
 __host__ __device__ inline void real_row(const FactorIn &in, int f, double temp1, int J, double &L, double &D,
                                          double &U, double &V, int &isbc) {
+#pragma clang fp contract(off)
   const int NY = in.ny;
   const double *DY = in.dy, *DYF = in.dyf;
   L = 0.;
@@ -355,13 +376,16 @@ __host__ __device__ inline void real_row(const FactorIn &in, int f, double temp1
   }
 }
 
-// Parallel cyclic reduction tables of the shared matrix of (rk, field):
-// step s (distance 2^s) eliminates x_{i-2^s}, x_{i+2^s} from row i with the
-// multipliers k1 = a_i/b_{i-d}, k2 = c_i/b_{i+d}; after all steps x_i =
-// d_i / b_i.  Rows outside 0..NY+1 are the identity.  One block per (rk,
-// field); dynamic LDS 3*(NY+2) doubles.  The constant right-hand sides of
-// the boundary rows (dt independent) are written by the rk=0 blocks.
-__global__ void __launch_bounds__(256) k_pcr_tables(const Clock *clk, FactorIn in, RealFactors fac) {
+// Tables of the shared matrix of (rk, field), one block each: rows from
+// real_row, the forward elimination of THOMAS_REAL_MPI (mpi.f:630-703) in
+// its operation order (multipliers TL, pivots TB, upper diagonal TC, 1/TB),
+// and the carry factors of the segmented sweeps of k_solve_seg:
+//   PF_J = prod_{i=a..J} TL_i          (a: first row of J's segment)
+//   PB_J = prod_{i=J..b-1} -TC_i/TB_i  (b-1: last row of J's segment)
+// The constant right-hand sides of the boundary rows (dt independent) are
+// written by the rk=0 blocks.
+__global__ void __launch_bounds__(256) k_cn_tables(const Clock *clk, FactorIn in, RealFactors fac) {
+#pragma clang fp contract(off)
   extern __shared__ double sm[];
   const int nrow = in.ny + 2;
   double *a = sm, *b = sm + nrow, *c = sm + 2 * nrow;
@@ -379,68 +403,52 @@ __global__ void __launch_bounds__(256) k_pcr_tables(const Clock *clk, FactorIn i
       fac.vconst[(long long)f * nrow + i] = V;
     }
   }
-  lds_barrier();
-  if (threadIdx.x == 0) {
-    // forward elimination of THOMAS_REAL_MPI (mpi.f:630-703): the matrix is
-    // shared by all columns, so its multipliers and pivots are too
-#pragma clang fp contract(off)
-    const long long o = ((long long)rk * in.nfields + f) * nrow;
-    double bp = b[0];
-    fac.tl[o] = 0.0;
-    fac.tb[o] = bp;
-    fac.tc[o] = c[0];
-    fac.tr[o] = 1.0 / bp;
-    for (int J = 1; J < nrow; ++J) {
-      const double m = -a[J] / bp;
-      bp = b[J] + m * c[J - 1];
-      fac.tl[o + J] = m;
-      fac.tb[o + J] = bp;
-      fac.tc[o + J] = c[J];
-      fac.tr[o + J] = 1.0 / bp;
-    }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  const long long o = ((long long)rk * in.nfields + f) * nrow;
+  const int sr = in.seg_rows;
+  double bp = b[0], pf = 0.0;
+  fac.tl[o] = 0.0;
+  fac.tb[o] = bp;
+  fac.tc[o] = c[0];
+  fac.tr[o] = 1.0 / bp;
+  fac.pf[o] = 0.0;
+  for (int J = 1; J < nrow; ++J) {
+    const double m = -a[J] / bp;
+    bp = b[J] + m * c[J - 1];
+    fac.tl[o + J] = m;
+    fac.tb[o + J] = bp;
+    fac.tc[o + J] = c[J];
+    fac.tr[o + J] = 1.0 / bp;
+    pf = (J % sr == 0) ? m : pf * m;
+    fac.pf[o + J] = pf;
   }
-  lds_barrier();
-  if (!in.pcr) return;
-  const long long tb = ((long long)rk * in.nfields + f) * (long long)kMaxPcr * nrow;
-  int s = 0;
-  for (int d = 1; d < nrow; d *= 2, ++s) {
-    double na[8], nb[8], nc[8];
-    int q = 0;
-    for (int i = threadIdx.x; i < nrow; i += blockDim.x, ++q) {
-      const bool lo = i - d >= 0, hi = i + d < nrow;
-      const double k1 = lo ? a[i] / b[i - d] : 0.0;
-      const double k2 = hi ? c[i] / b[i + d] : 0.0;
-      na[q] = lo ? -a[i - d] * k1 : 0.0;
-      nc[q] = hi ? -c[i + d] * k2 : 0.0;
-      nb[q] = b[i] - (lo ? c[i - d] * k1 : 0.0) - (hi ? a[i + d] * k2 : 0.0);
-      fac.k1[tb + (long long)s * nrow + i] = k1;
-      fac.k2[tb + (long long)s * nrow + i] = k2;
-    }
-    lds_barrier();
-    q = 0;
-    for (int i = threadIdx.x; i < nrow; i += blockDim.x, ++q) {
-      a[i] = na[q];
-      b[i] = nb[q];
-      c[i] = nc[q];
-    }
-    lds_barrier();
+  double pb = 0.0;
+  for (int J = nrow - 1; J >= 0; --J) {
+    const double tb = fac.tb[o + J];
+    const double fj = (tb != 0.0) ? -c[J] / tb : 0.0;
+    pb = (J % sr == sr - 1 || J == nrow - 1) ? fj : fj * pb;
+    fac.pb[o + J] = pb;
   }
-  for (int i = threadIdx.x; i < nrow; i += blockDim.x)
-    fac.bfin[((long long)rk * in.nfields + f) * nrow + i] = b[i];
 }
 
 // ---------------------------------------------------------------------------
-// Real Crank-Nicolson solves of every (x,z) column (channel.f:606-626 and
-// 739-747 explicit half; 771-1029 implicit half), solved by applying the
-// shared PCR tables to the right-hand sides held in LDS; fused CFL minimum
-// of COURANT (courant.f:34-43).  Block: kSolveCols consecutive x columns of
-// one field; LDS [row][col].
+// Real Crank-Nicolson solves (channel.f:771-1029; the explicit half of the
+// y-viscous term is already in R, added in Fourier space by k_nonlinear) and
+// the CFL minimum of COURANT (courant.f:34-43).  THOMAS_REAL_MPI's sweeps
+// (mpi.f:630-703, 767-809) run on columns split into NS segments of
+// seg_rows rows, each held in the registers of one lane (lane = segment *
+// 64/NS + column):
+//   forward:  each segment sweeps from a zero carry, h_J = d_J + TL_J h_{J-1};
+//             the carries g_{a-1} pass up the segments by shuffles, and
+//             g_J = h_J + PF_J g_{a-1};
+//   backward: each segment sweeps from a zero carry, y_J = (g_J - TC_J
+//             y_{J+1}) / TB_J (quotient from 1/TB_J, the same double as the
+//             division); the carries x_b pass down, and x_J = y_J + PB_J x_b.
+// The first segment's forward values and the last segment's backward values
+// are the reference's own; the others differ from a single sweep by the
+// rounding of the carry terms.
 // ---------------------------------------------------------------------------
-constexpr int kSolveCols = 32;
-constexpr int kSolveThreads = 1024;
-constexpr int kSolveGroups = kSolveThreads / kSolveCols;  // row groups
-constexpr int kSolveRows = 9;  // contiguous rows per group: NY+2 <= 9 * 32
-
 struct SolveArgs {
   Geo g;
   Clock *clk;
@@ -451,291 +459,118 @@ struct SolveArgs {
   RealFactors fac;
   int nfields;
   int wlo[kMaxFields], whi[kMaxFields];
-  double nu, nu_v_scale, pr[DH_MAX_TH];
   int courant;  // 1: accumulate the CFL minimum of U1..U3
   double cfl, dx, dz;
 };
 
-__global__ void __launch_bounds__(kSolveThreads) k_solve(SolveArgs a) {
-  extern __shared__ double sd[];
-  __shared__ double red[kSolveThreads / 64];
+constexpr int kSegThreads = 256;
+
+template <int NS>
+__global__ void __launch_bounds__(kSegThreads) k_solve_seg(SolveArgs a) {
+#pragma clang fp contract(off)
+  constexpr int CW = 64 / NS;  // columns per wave
+  constexpr int SR = kSegRows;
+  // tables in LDS, padded to NS * seg_rows rows
+  __shared__ double TL[NS * SR], TB[NS * SR], TC[NS * SR], TR[NS * SR], PF[NS * SR], PB[NS * SR], VC[NS * SR];
+  __shared__ int IB[NS * SR];
+  __shared__ double red[kSegThreads / 64];
   const Geo &g = a.g;
   const int f = blockIdx.y;
   const long long pp = g.pp;
   const int NY = g.ny, nrow = NY + 2;
-  const int rk = a.rk;
-  const int nst = a.fac.nsteps;
-  const double hb = a.clk->h_bar[rk];
-  const double T1 = a.nu * hb / 2.0;
-  const double coef = (f < 3) ? T1 * a.nu_v_scale : (T1 / a.pr[f - 3]) * a.nu_v_scale;
-  // LDS: right-hand sides [row][col], then this (rk, field)'s tables
-  double *D = sd;
-  double *K1 = D + nrow * kSolveCols, *K2 = K1 + nst * nrow;
-  double *BF = K2 + nst * nrow, *VC = BF + nrow, *DYs = VC + nrow, *DYFs = DYs + nrow;
-  int *IB = reinterpret_cast<int *>(DYFs + nrow);
-  const long long tb = ((long long)rk * a.nfields + f) * (long long)kMaxPcr * nrow;
-  for (int i = threadIdx.x; i < nst * nrow; i += blockDim.x) {
-    K1[i] = a.fac.k1[tb + i];
-    K2[i] = a.fac.k2[tb + i];
+  const int sr = a.fac.seg_rows;
+  const long long to = ((long long)a.rk * a.nfields + f) * nrow;
+  for (int i = threadIdx.x; i < NS * sr; i += blockDim.x) {
+    const bool in = i < nrow;
+    TL[i] = in ? a.fac.tl[to + i] : 0.0;
+    TB[i] = in ? a.fac.tb[to + i] : 1.0;
+    TC[i] = in ? a.fac.tc[to + i] : 0.0;
+    TR[i] = in ? a.fac.tr[to + i] : 1.0;
+    PF[i] = in ? a.fac.pf[to + i] : 0.0;
+    PB[i] = in ? a.fac.pb[to + i] : 0.0;
+    VC[i] = in ? a.fac.vconst[(long long)f * nrow + i] : 0.0;
+    IB[i] = in ? a.fac.isbc[(long long)f * nrow + i] : 1;
   }
-  for (int i = threadIdx.x; i < nrow; i += blockDim.x) {
-    BF[i] = a.fac.bfin[((long long)rk * a.nfields + f) * nrow + i];
-    VC[i] = a.fac.vconst[(long long)f * nrow + i];
-    IB[i] = a.fac.isbc[(long long)f * nrow + i];
-    DYs[i] = g.dy[i];
-    DYFs[i] = g.dyf[i];
-  }
-  const int lane = threadIdx.x % kSolveCols, rg = threadIdx.x / kSolveCols;
-  const int rows = (nrow + kSolveGroups - 1) / kSolveGroups;
-  const int i0 = rg * rows;
-  const long long col = (long long)blockIdx.x * kSolveCols + lane;
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int seg = lane / CW, cl = lane - seg * CW;
+  const long long col = ((long long)blockIdx.x * (blockDim.x >> 6) + w) * CW + cl;
   const bool live = col < pp;
+  const int r0 = seg * sr;
+  const int nr = min(sr, max(nrow - r0, 0));
   double *U = a.uphys + (long long)f * a.fs + col;
   const double *R = a.rphys + (long long)f * a.fs + col;
-  // U_old rows i0-1 .. i0+rows and R rows i0 .. i0+rows-1, all loads in flight
-  double u[kSolveRows + 2], r[kSolveRows];
+  // right-hand sides: the boundary constant or R
+  double v[SR];
 #pragma unroll
-  for (int q = 0; q < kSolveRows + 2; ++q) {
-    const int i = i0 - 1 + q;
-    u[q] = (live && q <= rows + 1 && i >= 0 && i < nrow) ? U[(long long)i * pp] : 0.0;
+  for (int q = 0; q < SR; ++q) {
+    const int i = r0 + q;
+    v[q] = (live && q < nr && !IB[i]) ? R[(long long)i * pp] : 0.0;
   }
 #pragma unroll
-  for (int q = 0; q < kSolveRows; ++q) {
-    const int i = i0 + q;
-    r[q] = (live && q < rows && i < nrow) ? R[(long long)i * pp] : 0.0;
+  for (int q = 0; q < SR; ++q) {
+    const int i = r0 + q;
+    if (q < nr && IB[i]) v[q] = VC[i];
   }
-  lds_barrier();
-  // right-hand sides: boundary constants or R + explicit CN term of U_old
+  // forward sweep within the segment from a zero carry
+  double hp = 0.0, hend = 0.0;
 #pragma unroll
-  for (int q = 0; q < kSolveRows; ++q) {
-    const int i = i0 + q;
-    if (q < rows && i < nrow) {
-      double d = 0.0;
-      if (IB[i]) {
-        d = VC[i];
-      } else {
-        const double um = u[q], u0 = u[q + 1], up = u[q + 2];
-        double ex;
-        if (f == 1)
-          ex = ((up - u0) / DYFs[i] - (u0 - um) / DYFs[i - 1]) / DYs[i];
-        else
-          ex = ((up - u0) / DYs[i + 1] - (u0 - um) / DYs[i]) / DYFs[i];
-        d = r[q] + coef * ex;
-      }
-      D[i * kSolveCols + lane] = d;
+  for (int q = 0; q < SR; ++q) {
+    const int i = r0 + q;
+    const double h = v[q] + TL[i] * hp;
+    v[q] = h;
+    hp = h;
+    hend = (q == nr - 1) ? h : hend;
+  }
+  // carries up the segments: g_in(s) = g_end(s-1), g_end = h_end + PF_end g_in
+  const double pfe = (nr > 0) ? PF[r0 + nr - 1] : 0.0;
+  double gin = 0.0, gend = hend;
+#pragma unroll
+  for (int t = 1; t < NS; ++t) {
+    const double prev = __shfl_up(gend, CW);
+    if (seg == t) {
+      gin = prev;
+      gend = hend + pfe * gin;
     }
   }
-  lds_barrier();
-  for (int s = 0, dd = 1; s < nst; ++s, dd *= 2) {
-    const double *k1 = K1 + s * nrow, *k2 = K2 + s * nrow;
-    double nd[kSolveRows];
 #pragma unroll
-    for (int q = 0; q < kSolveRows; ++q) {
-      const int i = i0 + q;
-      if (q < rows && i < nrow) {
-        double v = D[i * kSolveCols + lane];
-        if (i - dd >= 0) v -= k1[i] * D[(i - dd) * kSolveCols + lane];
-        if (i + dd < nrow) v -= k2[i] * D[(i + dd) * kSolveCols + lane];
-        nd[q] = v;
-      }
-    }
-    lds_barrier();
+  for (int q = 0; q < SR; ++q) v[q] = (seg == 0) ? v[q] : v[q] + PF[r0 + q] * gin;
+  // backward sweep within the segment from a zero carry
+  double yn = 0.0, ystart = 0.0;
 #pragma unroll
-    for (int q = 0; q < kSolveRows; ++q) {
-      const int i = i0 + q;
-      if (q < rows && i < nrow) D[i * kSolveCols + lane] = nd[q];
-    }
-    lds_barrier();
+  for (int q = SR - 1; q >= 0; --q) {
+    const int i = r0 + q;
+    const double b = TB[i];
+    const double y = (b != 0.0) ? div_by_recip(v[q] - TC[i] * yn, b, TR[i]) : 0.0;
+    const bool in = q < nr;
+    v[q] = y;
+    yn = in ? y : 0.0;
   }
+  ystart = v[0];
+  // carries down the segments: x_in(s) = x_start(s+1), x_start = y_start + PB_start x_in
+  const double pbs = PB[r0];
+  double xin = 0.0, xstart = ystart;
+#pragma unroll
+  for (int t = NS - 2; t >= 0; --t) {
+    const double nxt = __shfl_down(xstart, CW);
+    if (seg == t) {
+      xin = nxt;
+      xstart = ystart + pbs * xin;
+    }
+  }
+  // write-back rows and the CFL minimum
   const bool cour = a.courant && f < 3;
   const double dxf = (f == 0) ? a.dx : a.dz;
   double dtmin = __longlong_as_double(0x7FF0000000000000ll);
   const int wlo = a.wlo[f], whi = a.whi[f];
 #pragma unroll
-  for (int q = 0; q < kSolveRows; ++q) {
-    const int i = i0 + q;
-    if (!live || q >= rows || i >= nrow || i < wlo || i > whi) continue;
-    const double x = D[i * kSolveCols + lane] / BF[i];
-    U[(long long)i * pp] = x;
-    if (cour && i >= 1 && i <= NY) {
-      const double dl = (f == 1) ? DYs[i] : dxf;
-      dtmin = fmin(dtmin, a.cfl * dl / fabs(x));
-    }
-  }
-  if (cour) {
-    // wave minimum by shuffles, then one LDS slot per wave
-    for (int o = 32; o > 0; o >>= 1) dtmin = fmin(dtmin, __shfl_down(dtmin, o));
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = dtmin;
-    lds_barrier();
-    if (threadIdx.x == 0) {
-      double v = red[0];
-      for (int w = 1; w < (int)(blockDim.x >> 6); ++w) v = fmin(v, red[w]);
-      if (v >= 0.0) atomicMin(&a.clk->cfl_min_bits, (unsigned long long)__double_as_longlong(v));
-    }
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Real Crank-Nicolson solves by Thomas sweeps, one (x,z) column per lane
-// (channel.f:606-626, 739-747 explicit half; 771-1029 implicit half;
-// THOMAS_REAL_MPI mpi.f:630-703 with the shared multipliers and pivots of
-// k_pcr_tables), in the reference's operation order, and the fused CFL
-// minimum of COURANT (courant.f:34-43).  Block: kTsCols consecutive columns
-// of one field.  All waves build the right-hand sides of a slice of rows
-// into LDS [row][col]; the first wave runs the forward and backward sweeps
-// of its lanes' columns in LDS; all waves then write their slice of U.
-// The sweeps read their operands kTsAhead rows ahead at clamped (uniform)
-// row indices, and the rows they run past the system land in one padding
-// row at each end, so nothing in the recurrence waits on LDS.  LDS stays
-// under 80 KB so that two blocks share a CU.
-// ---------------------------------------------------------------------------
-constexpr int kTsCols = 64;
-constexpr int kTsThreads = 512;
-constexpr int kTsBatch = 9;   // rows per load batch of a wave
-constexpr int kTsAhead = 4;   // rows of sweep operands read ahead
-
-__global__ void __launch_bounds__(kTsThreads) k_solve_thomas(SolveArgs a) {
-#pragma clang fp contract(off)
-  extern __shared__ double sd[];
-  __shared__ double red[kTsThreads / 64];
-  const Geo &g = a.g;
-  const int f = blockIdx.y;
-  const long long pp = g.pp;
-  const int NY = g.ny, nrow = NY + 2;
-  const int rk = a.rk;
-  const double hb = a.clk->h_bar[rk];
-  const double T1 = a.nu * hb / 2.0;
-  const double coef = (f < 3) ? T1 * a.nu_v_scale : (T1 / a.pr[f - 3]) * a.nu_v_scale;
-  // LDS: D [row][kTsCols] for rows -1..NY+2 (row J at J + 1), then TL, TB,
-  // TC, TR [row] for rows 0..NY+1
-  double *D = sd + kTsCols;
-  double *TL = sd + (nrow + 2) * kTsCols, *TB = TL + nrow, *TC = TB + nrow, *TR = TC + nrow;
-  const long long to = ((long long)rk * a.nfields + f) * nrow;
-  for (int i = threadIdx.x; i < nrow; i += blockDim.x) {
-    TL[i] = a.fac.tl[to + i];
-    TB[i] = a.fac.tb[to + i];
-    TC[i] = a.fac.tc[to + i];
-    TR[i] = a.fac.tr[to + i];
-  }
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  const long long col = (long long)blockIdx.x * kTsCols + lane;
-  const bool live = col < pp;
-  double *U = a.uphys + (long long)f * a.fs + col;
-  const double *R = a.rphys + (long long)f * a.fs + col;
-  const int *isbc = a.fac.isbc + (long long)f * nrow;
-  const double *vc = a.fac.vconst + (long long)f * nrow;
-  const double *dy = g.dy, *dyf = g.dyf;
-  // right-hand sides: the boundary constant, or R plus the explicit half of
-  // the y-viscous term of U_old
-  const int per = (nrow + nw - 1) / nw;
-  const int r0 = w * per, r1 = min(nrow, r0 + per);
-  for (int b0 = r0; b0 < r1; b0 += kTsBatch) {
-    double u[kTsBatch + 2], r[kTsBatch];
-#pragma unroll
-    for (int q = 0; q < kTsBatch + 2; ++q) {
-      const int i = b0 - 1 + q;
-      u[q] = (live && i >= 0 && i < nrow && i <= r1) ? U[(long long)i * pp] : 0.0;
-    }
-#pragma unroll
-    for (int q = 0; q < kTsBatch; ++q) {
-      const int i = b0 + q;
-      r[q] = (live && i < r1) ? R[(long long)i * pp] : 0.0;
-    }
-#pragma unroll
-    for (int q = 0; q < kTsBatch; ++q) {
-      const int i = b0 + q;
-      if (i >= r1) break;
-      double d;
-      if (isbc[i]) {
-        d = vc[i];
-      } else {
-        const double um = u[q], u0 = u[q + 1], up = u[q + 2];
-        double ex;
-        if (f == 1)
-          ex = ((up - u0) / dyf[i] - (u0 - um) / dyf[i - 1]) / dy[i];
-        else
-          ex = ((up - u0) / dy[i + 1] - (u0 - um) / dy[i]) / dyf[i];
-        d = r[q] + coef * ex;
-      }
-      D[i * kTsCols + lane] = d;
-    }
-  }
-  lds_barrier();
-  if (w == 0) {
-    // forward sweep: G_J = G_J + m_J G_{J-1}, J = 1..NY+1
-    const int last = nrow - 1;
-    double gp = D[lane], glast = gp;
-    double dc[kTsAhead], dn[kTsAhead], mc[kTsAhead], mn[kTsAhead];
-#pragma unroll
-    for (int q = 0; q < kTsAhead; ++q) {
-      const int J = min(1 + q, last);
-      dc[q] = D[J * kTsCols + lane];
-      mc[q] = TL[J];
-    }
-    for (int J0 = 1; J0 < nrow; J0 += kTsAhead) {
-#pragma unroll
-      for (int q = 0; q < kTsAhead; ++q) {
-        const int J = min(J0 + kTsAhead + q, last);
-        dn[q] = D[J * kTsCols + lane];
-        mn[q] = TL[J];
-      }
-#pragma unroll
-      for (int q = 0; q < kTsAhead; ++q) {
-        const int J = J0 + q;
-        gp = dc[q] + mc[q] * gp;
-        D[min(J, nrow) * kTsCols + lane] = gp;  // past the last row: padding row
-        glast = (J == last) ? gp : glast;
-        dc[q] = dn[q];
-        mc[q] = mn[q];
-      }
-    }
-    // backward sweep: G_{NY+1} /= B_{NY+1}; G_J = (G_J - C_J G_{J+1}) / B_J,
-    // the quotients formed from the tabulated 1/B_J (same doubles)
-    double x = glast / TB[last];
-    D[last * kTsCols + lane] = x;
-    double bc[kTsAhead], bn[kTsAhead], cc[kTsAhead], cn[kTsAhead], rc[kTsAhead], rn[kTsAhead];
-#pragma unroll
-    for (int q = 0; q < kTsAhead; ++q) {
-      const int J = max(last - 1 - q, 0);
-      dc[q] = D[J * kTsCols + lane];
-      bc[q] = TB[J];
-      cc[q] = TC[J];
-      rc[q] = TR[J];
-    }
-    for (int J0 = last - 1; J0 >= 0; J0 -= kTsAhead) {
-#pragma unroll
-      for (int q = 0; q < kTsAhead; ++q) {
-        const int J = max(J0 - kTsAhead - q, 0);
-        dn[q] = D[J * kTsCols + lane];
-        bn[q] = TB[J];
-        cn[q] = TC[J];
-        rn[q] = TR[J];
-      }
-#pragma unroll
-      for (int q = 0; q < kTsAhead; ++q) {
-        const int J = J0 - q;
-        const double num = dc[q] - cc[q] * x;
-        x = (bc[q] != 0.0) ? div_by_recip(num, bc[q], rc[q]) : 0.0;
-        D[max(J, -1) * kTsCols + lane] = x;  // below row 0: padding row
-        dc[q] = dn[q];
-        bc[q] = bn[q];
-        cc[q] = cn[q];
-        rc[q] = rn[q];
-      }
-    }
-  }
-  lds_barrier();
-  // write-back rows and the CFL minimum of this block
-  const bool cour = a.courant && f < 3;
-  const double dxf = (f == 0) ? a.dx : a.dz;
-  double dtmin = __longlong_as_double(0x7FF0000000000000ll);
-  const int wlo = max(a.wlo[f], r0), whi = min(a.whi[f], r1 - 1);
-  if (live) {
-    for (int i = wlo; i <= whi; ++i) {
-      const double x = D[i * kTsCols + lane];
+  for (int q = 0; q < SR; ++q) {
+    const int i = r0 + q;
+    const double x = (seg == NS - 1) ? v[q] : v[q] + PB[i] * xin;
+    if (live && q < nr && i >= wlo && i <= whi) {
       U[(long long)i * pp] = x;
       if (cour && i >= 1 && i <= NY) {
-        const double dl = (f == 1) ? dy[i] : dxf;
+        const double dl = (f == 1) ? g.dy[i] : dxf;
         dtmin = fmin(dtmin, a.cfl * dl / fabs(x));
       }
     }
@@ -743,17 +578,17 @@ __global__ void __launch_bounds__(kTsThreads) k_solve_thomas(SolveArgs a) {
   if (cour) {
     for (int o = 32; o > 0; o >>= 1) dtmin = fmin(dtmin, __shfl_down(dtmin, o));
     if (lane == 0) red[w] = dtmin;
-    lds_barrier();
+    __syncthreads();
     if (threadIdx.x == 0) {
-      double v = red[0];
-      for (int q = 1; q < nw; ++q) v = fmin(v, red[q]);
-      if (v >= 0.0) atomicMin(&a.clk->cfl_min_bits, (unsigned long long)__double_as_longlong(v));
+      double m = red[0];
+      for (int q = 1; q < (int)(blockDim.x >> 6); ++q) m = fmin(m, red[q]);
+      if (m >= 0.0) atomicMin(&a.clk->cfl_min_bits, (unsigned long long)__double_as_longlong(m));
     }
   }
 }
 
 // COURANT (courant.f:45-62) from the CFL minimum of the solve kernel; the
-// PCR tables for the new DELTA_T are rebuilt right after by k_pcr_tables.
+// solve tables for the new DELTA_T are rebuilt right after by k_cn_tables.
 __global__ void k_courant_dt(Clock *clk, double dt_cap, int variable_dt, int update_dt) {
   if (threadIdx.x != 0) return;
   Clock &c = *clk;
@@ -1040,6 +875,7 @@ static StepArgs make_step_args(Ctx &c, int rk) {
   a.cfs = (long long)(c.g.ny + 2) * c.g.pf;
   const dh_params &p = c.p;
   a.nu = p.nu;
+  a.nu_v_scale = p.nu_v_scale;
   a.beta = p.beta;
   a.i_ro = p.i_ro;
   a.grav_x = p.grav_x;
@@ -1086,14 +922,14 @@ static FactorIn make_factor_in(const Ctx &c) {
   }
   in.dy = c.d_dy;
   in.dyf = c.d_dyf;
-  in.pcr = c.cn_pcr ? 1 : 0;
+  in.seg_rows = c.fac.seg_rows;
   return in;
 }
 
-hipError_t launch_pcr_tables(Ctx &c) {
+hipError_t launch_cn_tables(Ctx &c) {
   FactorIn in = make_factor_in(c);
   const size_t shm = (size_t)3 * (c.g.ny + 2) * sizeof(double);
-  hipLaunchKernelGGL(k_pcr_tables, dim3(3 * c.nfields), dim3(256), shm, c.stream, c.clock, in, c.fac);
+  hipLaunchKernelGGL(k_cn_tables, dim3(3 * c.nfields), dim3(256), shm, c.stream, c.clock, in, c.fac);
   return hipGetLastError();
 }
 
@@ -1101,16 +937,6 @@ static hipError_t launch_nonlinear(Ctx &c, int rk) {
   StepArgs a = make_step_args(c, rk);
   hipLaunchKernelGGL(k_nonlinear, dim3(c.g.ny, c.nfields), dim3(c.plane_threads), c.lds_bytes, c.stream, a);
   return hipGetLastError();
-}
-
-static size_t solve_lds(const Ctx &c) {
-  const size_t nrow = c.g.ny + 2;
-  return (nrow * kSolveCols + 2 * (size_t)c.fac.nsteps * nrow + 4 * nrow) * sizeof(double) + nrow * sizeof(int);
-}
-
-static size_t solve_thomas_lds(const Ctx &c) {
-  const size_t nrow = c.g.ny + 2;
-  return ((nrow + 2) * kTsCols + 4 * nrow) * sizeof(double);
 }
 
 static hipError_t launch_solve(Ctx &c, int rk, int courant) {
@@ -1127,20 +953,16 @@ static hipError_t launch_solve(Ctx &c, int rk, int courant) {
     a.wlo[f] = c.wlo[f];
     a.whi[f] = c.whi[f];
   }
-  a.nu = c.p.nu;
-  a.nu_v_scale = c.p.nu_v_scale;
-  for (int n = 0; n < DH_MAX_TH; ++n) a.pr[n] = c.p.pr[n];
   a.courant = courant;
   a.cfl = c.p.cfl;
   a.dx = c.p.lx / c.g.nx;
   a.dz = c.p.lz / c.g.nz;
-  if (c.cn_pcr) {
-    const int nb = (int)((c.g.pp + kSolveCols - 1) / kSolveCols);
-    hipLaunchKernelGGL(k_solve, dim3(nb, c.nfields), dim3(kSolveThreads), solve_lds(c), c.stream, a);
-  } else {
-    const int nb = (int)((c.g.pp + kTsCols - 1) / kTsCols);
-    hipLaunchKernelGGL(k_solve_thomas, dim3(nb, c.nfields), dim3(kTsThreads), solve_thomas_lds(c), c.stream, a);
-  }
+  const int cols = (kSegThreads / 64) * (64 / c.fac.nseg);  // columns per block
+  const int nb = (int)((c.g.pp + cols - 1) / cols);
+  if (c.fac.nseg == 4)
+    hipLaunchKernelGGL(k_solve_seg<4>, dim3(nb, c.nfields), dim3(kSegThreads), 0, c.stream, a);
+  else
+    hipLaunchKernelGGL(k_solve_seg<8>, dim3(nb, c.nfields), dim3(kSegThreads), 0, c.stream, a);
   return hipGetLastError();
 }
 
@@ -1184,7 +1006,7 @@ static hipError_t launch_courant(Ctx &c) {
                      c.p.update_dt);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  return launch_pcr_tables(c);
+  return launch_cn_tables(c);
 }
 
 // Kernels of one substep in launch order (channel.f:386-1065).
@@ -1231,11 +1053,6 @@ hipError_t launch_end_step(Ctx &c) {
 hipError_t prepare_step_kernels(Ctx &c) {
   hipError_t e =
       hipFuncSetAttribute((const void *)k_nonlinear, hipFuncAttributeMaxDynamicSharedMemorySize, c.lds_bytes);
-  if (e != hipSuccess) return e;
-  e = hipFuncSetAttribute((const void *)k_solve, hipFuncAttributeMaxDynamicSharedMemorySize, (int)solve_lds(c));
-  if (e != hipSuccess) return e;
-  e = hipFuncSetAttribute((const void *)k_solve_thomas, hipFuncAttributeMaxDynamicSharedMemorySize,
-                          (int)solve_thomas_lds(c));
   if (e != hipSuccess) return e;
   return hipFuncSetAttribute((const void *)k_rem_div_thomas, hipFuncAttributeMaxDynamicSharedMemorySize,
                              (int)rem_div_thomas_lds(c));
