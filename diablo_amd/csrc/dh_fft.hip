@@ -14,6 +14,7 @@ struct PlaneBatch {
 };
 
 // Fourier -> physical for planes j0 + blockIdx.x of field blockIdx.y.
+template <bool REG>
 __global__ void __launch_bounds__(1024) k_plane_inv(Geo g, PlaneBatch b) {
   extern __shared__ double2 A[];
   const Tables T = load_tables(A, g);
@@ -21,19 +22,30 @@ __global__ void __launch_bounds__(1024) k_plane_inv(Geo g, PlaneBatch b) {
   const long long j = b.j0 + blockIdx.x;
   plane_unpack(A, g, T, b.four[f] + j * g.pf);
   plane_cols_inv(A, g, T);
-  plane_rows_inv(A, g, T);
-  plane_store_phys(A, g, T, reinterpret_cast<double2 *>(b.phys[f] + j * g.pp));
+  double2 *dst = reinterpret_cast<double2 *>(b.phys[f] + j * g.pp);
+  if constexpr (REG) {
+    rows_inv_reg64(A, g, T, [&](int z, int m, double2 v) { dst[z * 64 + m] = v; });
+  } else {
+    plane_rows_inv(A, g, T);
+    plane_store_phys(A, g, T, dst);
+  }
 }
 
 // physical -> Fourier (x r2c with 1/NX and truncation, z c2c with 1/NZ and
 // packing of kz = -NKZ..-1 after 0..NKZ).
+template <bool REG>
 __global__ void __launch_bounds__(1024) k_plane_fwd(Geo g, PlaneBatch b) {
   extern __shared__ double2 A[];
   const Tables T = load_tables(A, g);
   const int f = blockIdx.y;
   const long long j = b.j0 + blockIdx.x;
-  plane_load_phys(A, g, T, reinterpret_cast<const double2 *>(b.phys[f] + j * g.pp));
-  plane_rows_fwd(A, g, T);
+  const double2 *src = reinterpret_cast<const double2 *>(b.phys[f] + j * g.pp);
+  if constexpr (REG) {
+    rows_fwd_reg64(A, g, T, [&](int z, int m) { return src[z * 64 + m]; });
+  } else {
+    plane_load_phys(A, g, T, src);
+    plane_rows_fwd(A, g, T);
+  }
   plane_cols_fwd(A, g, T);
   double2 *dst = const_cast<double2 *>(b.four[f]) + j * g.pf;
   for (int m = threadIdx.x; m < g.pf; m += blockDim.x) {
@@ -52,9 +64,10 @@ hipError_t launch_fft_to_phys(Ctx &c, const double2 *four, double *phys, int npl
   b.four[0] = four;
   b.phys[0] = phys;
   b.j0 = 0;
-  hipError_t e = set_lds((const void *)k_plane_inv, c.lds_bytes);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_plane_inv, dim3(nplanes, 1), dim3(c.plane_threads), c.lds_bytes, c.stream, c.g, b);
+  if (c.reg_rows)
+    hipLaunchKernelGGL(k_plane_inv<true>, dim3(nplanes, 1), dim3(c.plane_threads), c.lds_bytes, c.stream, c.g, b);
+  else
+    hipLaunchKernelGGL(k_plane_inv<false>, dim3(nplanes, 1), dim3(c.plane_threads), c.lds_bytes, c.stream, c.g, b);
   return hipGetLastError();
 }
 
@@ -63,9 +76,10 @@ hipError_t launch_fft_to_four(Ctx &c, const double *phys, double2 *four, int npl
   b.four[0] = four;
   b.phys[0] = const_cast<double *>(phys);
   b.j0 = 0;
-  hipError_t e = set_lds((const void *)k_plane_fwd, c.lds_bytes);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_plane_fwd, dim3(nplanes, 1), dim3(c.plane_threads), c.lds_bytes, c.stream, c.g, b);
+  if (c.reg_rows)
+    hipLaunchKernelGGL(k_plane_fwd<true>, dim3(nplanes, 1), dim3(c.plane_threads), c.lds_bytes, c.stream, c.g, b);
+  else
+    hipLaunchKernelGGL(k_plane_fwd<false>, dim3(nplanes, 1), dim3(c.plane_threads), c.lds_bytes, c.stream, c.g, b);
   return hipGetLastError();
 }
 
@@ -80,8 +94,12 @@ hipError_t launch_state_to_phys(Ctx &c) {
     b.phys[f] = c.uphys + f * fs;
   }
   b.j0 = 0;
-  hipLaunchKernelGGL(k_plane_inv, dim3(c.g.ny + 2, c.nfields), dim3(c.plane_threads), c.lds_bytes, c.stream, c.g,
-                     b);
+  if (c.reg_rows)
+    hipLaunchKernelGGL(k_plane_inv<true>, dim3(c.g.ny + 2, c.nfields), dim3(c.plane_threads), c.lds_bytes, c.stream,
+                       c.g, b);
+  else
+    hipLaunchKernelGGL(k_plane_inv<false>, dim3(c.g.ny + 2, c.nfields), dim3(c.plane_threads), c.lds_bytes, c.stream,
+                       c.g, b);
   return hipGetLastError();
 }
 
@@ -97,16 +115,25 @@ hipError_t launch_fields_to_four(Ctx &c, int f0, int f1) {
   }
   b.j0 = 0;
   if (f1 <= f0) return hipSuccess;
-  hipLaunchKernelGGL(k_plane_fwd, dim3(c.g.ny + 2, f1 - f0), dim3(c.plane_threads), c.lds_bytes, c.stream, c.g, b);
+  if (c.reg_rows)
+    hipLaunchKernelGGL(k_plane_fwd<true>, dim3(c.g.ny + 2, f1 - f0), dim3(c.plane_threads), c.lds_bytes, c.stream,
+                       c.g, b);
+  else
+    hipLaunchKernelGGL(k_plane_fwd<false>, dim3(c.g.ny + 2, f1 - f0), dim3(c.plane_threads), c.lds_bytes, c.stream,
+                       c.g, b);
   return hipGetLastError();
 }
 
 hipError_t launch_state_to_four(Ctx &c) { return launch_fields_to_four(c, 0, c.nfields); }
 
 hipError_t prepare_fft_kernels(Ctx &c) {
-  hipError_t e = set_lds((const void *)k_plane_inv, c.lds_bytes);
-  if (e != hipSuccess) return e;
-  return set_lds((const void *)k_plane_fwd, c.lds_bytes);
+  const void *fns[] = {(const void *)k_plane_inv<true>, (const void *)k_plane_inv<false>,
+                       (const void *)k_plane_fwd<true>, (const void *)k_plane_fwd<false>};
+  for (const void *fn : fns) {
+    hipError_t e = set_lds(fn, c.lds_bytes);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 }  // namespace dh

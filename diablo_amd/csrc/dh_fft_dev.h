@@ -210,6 +210,7 @@ __device__ __forceinline__ void fft_stage(double2 *A, const FftStage &st, int N,
 
 template <bool INV, bool COLS, bool DIT>
 __device__ void lds_fft(double2 *A, const FftPlan &pl, int ntrans, unsigned long long fnt, int ls, const double2 *tw) {
+#pragma unroll 1
   for (int s = 0; s < pl.nst; ++s) {
     const FftStage &st = DIT ? pl.dit[s] : pl.dif[s];
     switch (st.R) {
@@ -384,9 +385,12 @@ __device__ void plane_load_phys(double2 *A, const Geo &g, const Tables &T, const
       int z, m;
       sl[b] = -1;
       const int t = t0 + b * nw;
+      v[b] = make_double2(0.0, 0.0);
       if (t < lt.ntiles && lt.at(g, t, z, m)) {
         sl[b] = phys_slot(T, g, z, m);
-        v[b] = src[z * g.lw + m];
+        const double2 *q = src + z * g.lw + m;
+        v[b].x = q->x;
+        v[b].y = q->y;
       }
     }
 #pragma unroll
@@ -394,6 +398,117 @@ __device__ void plane_load_phys(double2 *A, const Geo &g, const Tables &T, const
       if (sl[b] >= 0) A[sl[b]] = v[b];
   }
   lds_barrier();
+}
+
+// ---------------------------------------------------------------------------
+// Register-resident row transforms for NX = 128 (64 complex pairs per row).
+// A wave owns a slab of 8 rows; lane = 8 * row + t.  The 64-point complex
+// DFT is split 64 = 8 x 8 (index m = t + 8r, frequency k = k1 + 8 k2): an
+// 8-point DFT over r in each lane's registers, the twiddle W64^{t k1}, an
+// 8 x 8 transpose through the wave's own rows of the LDS plane, and an
+// 8-point DFT over t.  The even/odd split of the real transform exchanges
+// Z_k and Z_{64-k} through the same rows.  Only lanes of one wave touch a
+// slab, and LDS operations of a wave complete in order, so the exchanges
+// need no workgroup barrier.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+}
+
+// Forward x transform of every row, physical pairs from load(z, m) (global
+// memory or computed products) to X_k / NX for k <= NKX in natural slots of
+// the LDS rows (FFT_XZ_TO_FOURIER's r2c, mpi.f:1490-1506).
+template <class Load>
+__device__ void rows_fwd_reg64(double2 *A, const Geo &g, const Tables &T, Load load) {
+  const int lane = threadIdx.x & 63, rho = lane >> 3, t = lane & 7;
+  const int nw = blockDim.x >> 6;
+  for (int slab = threadIdx.x >> 6; slab * 8 < g.nz; slab += nw) {
+    const int z = slab * 8 + rho;
+    const bool zin = z < g.nz;
+    double2 *row = A + z * g.ls;
+    double2 v[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      v[r] = make_double2(0.0, 0.0);
+      if (zin) v[r] = load(z, t + 8 * r);
+    }
+    dft_small<8, false>(v);
+#pragma unroll
+    for (int k1 = 1; k1 < 8; ++k1) v[k1] = cmul(v[k1], T.twr[t * k1]);
+    // transpose: lane t's Y_t[k1] to slot 8 k1 + t; lane k1 reads slots 8 k1 + t'
+#pragma unroll
+    for (int k1 = 0; k1 < 8; ++k1) row[8 * k1 + t] = v[k1];
+    wave_lds_sync();
+#pragma unroll
+    for (int tt = 0; tt < 8; ++tt) v[tt] = row[8 * t + tt];  // lane plays k1 = t
+    dft_small<8, false>(v);  // v[k2] = Z_{t + 8 k2}
+    wave_lds_sync();
+#pragma unroll
+    for (int k2 = 0; k2 < 8; ++k2) row[t + 8 * k2] = v[k2];
+    wave_lds_sync();
+    double2 zm[8];
+#pragma unroll
+    for (int k2 = 0; k2 < 8; ++k2) zm[k2] = row[(64 - t - 8 * k2) & 63];
+    wave_lds_sync();
+    // X_k = (Z_k + conj Z_{64-k})/2 - i w^k (Z_k - conj Z_{64-k})/2, w = e^{-2 pi i/NX}
+#pragma unroll
+    for (int k2 = 0; k2 < 8; ++k2) {
+      const int k = t + 8 * k2;
+      const double2 zk = v[k2], zmk = zm[k2];
+      double2 a = make_double2(0.5 * (zk.x + zmk.x), 0.5 * (zk.y - zmk.y));
+      double2 b = make_double2(0.5 * (zk.x - zmk.x), 0.5 * (zk.y + zmk.y));
+      double2 wb = cmul(T.twx[k], b);
+      if (k <= g.nkx) row[k] = make_double2((a.x + wb.y) / g.nx, (a.y - wb.x) / g.nx);
+    }
+  }
+  lds_barrier();
+}
+
+// Inverse x transform of every row (FFT_XZ_TO_PHYSICAL's c2r,
+// mpi.f:1600-1604): X_k (k <= NKX) in natural slots of the LDS rows to the
+// physical pairs, handed to store(z, m, value).
+template <class Store>
+__device__ void rows_inv_reg64(double2 *A, const Geo &g, const Tables &T, Store store) {
+  const int lane = threadIdx.x & 63, rho = lane >> 3, t = lane & 7;
+  const int nw = blockDim.x >> 6;
+  for (int slab = threadIdx.x >> 6; slab * 8 < g.nz; slab += nw) {
+    const int z = slab * 8 + rho;
+    const bool zin = z < g.nz;
+    double2 *row = A + z * g.ls;
+    // Z_k = (X_k + conj X_{64-k}) + i w^k (X_k - conj X_{64-k}), w = e^{+2 pi i/NX},
+    // for k = t + 8 k2 (X_k = 0 above NKX; Im X_0 ignored)
+    double2 v[8];
+#pragma unroll
+    for (int k2 = 0; k2 < 8; ++k2) {
+      const int k = t + 8 * k2, km = (64 - k) & 63;
+      double2 xk = make_double2(0.0, 0.0), xm = make_double2(0.0, 0.0);
+      if (k <= g.nkx) xk = row[k];
+      if (k != 0 && km <= g.nkx) xm = row[km];
+      if (k == 0) xk.y = 0.0;
+      double2 a = make_double2(xk.x + xm.x, xk.y - xm.y);
+      double2 b = make_double2(xk.x - xm.x, xk.y + xm.y);
+      double2 wb = cmul(cconj(T.twx[k]), b);
+      v[k2] = make_double2(a.x - wb.y, a.y + wb.x);
+    }
+    wave_lds_sync();
+    // c_{t' + 8r} = sum_k1 W8^{-r k1} W64^{-t' k1} sum_k2 W8^{-t' k2} Z_{k1 + 8 k2}
+    dft_small<8, true>(v);  // lane plays k1 = t: v[t'] = sum_k2 Z W8^{-t' k2}
+#pragma unroll
+    for (int tt = 1; tt < 8; ++tt) v[tt] = cmul(v[tt], cconj(T.twr[tt * t]));
+#pragma unroll
+    for (int tt = 0; tt < 8; ++tt) row[8 * tt + t] = v[tt];
+    wave_lds_sync();
+#pragma unroll
+    for (int k1 = 0; k1 < 8; ++k1) v[k1] = row[8 * t + k1];  // lane plays t' = t
+    dft_small<8, true>(v);  // v[r] = c_{t + 8r}
+    wave_lds_sync();
+    if (zin) {
+#pragma unroll
+      for (int r = 0; r < 8; ++r) store(z, t + 8 * r, v[r]);
+    }
+  }
 }
 
 // Physical plane from the inverse-transform slots to natural order in HBM.

@@ -93,6 +93,8 @@ struct Ctx {
   int wlo[kMaxFields], whi[kMaxFields];  // solution write-back rows per field
   int lds_bytes;
   int plane_threads;
+  bool reg_rows = false;     // plane kernels: row transforms in registers (NX = 128, NZ a multiple of 8)
+  bool reg_rows_nl = false;  // the same in k_nonlinear
   hipStream_t stream = nullptr;
   std::string err;
   bool grid_set = false;

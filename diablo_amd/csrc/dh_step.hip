@@ -38,58 +38,88 @@ struct StepArgs {
 
 __device__ __forceinline__ double kpow(double x, double beta) { return (beta == 1.0) ? x : pow(x, beta); }
 
-// Physical product p (0..2) of component c at plane J for the two samples
-// stored in one complex LDS slot (x = 2m, 2m+1).  channel.f:410-571 (c<3)
-// and 669-720 (scalars).
-__device__ __forceinline__ double2 product_pair(const StepArgs &a, int c, int p, int J, long long off) {
+// Physical product p (0..2) of component c at plane J (channel.f:410-571
+// for c<3, 669-720 for scalars), resolved once per block into one of five
+// forms over up to five planes, each evaluated for the two samples stored in
+// one complex slot (x = 2m, 2m+1):
+//   0: s0 s1                                  U3U1, TH U1, TH U3
+//   1: s0 s0                                  U1U1, U3U3
+//   2: (s0 s3 + s1 s3 - s1 s4 - s2 s4)/(2 DYF(J))
+//      with s0..s2 = C(J+1), C(J), C(J-1), s3, s4 = U2(J+1), U2(J):
+//      d/dy(C U2) on the staggered grid, C = U1, U3 or TH
+//   3: ((DYF(J) s0 + DYF(J-1) s1) / (2 DY(J))) s2   interp_y(U1 or U3) U2
+//   4: (0.25 (s1+s0)^2 - 0.25 (s1+s2)^2) / DY(J)    d/dy(U2 U2)
+struct Prod {
+  const double *s[5];
+  int kind;
+  double w0, w1, w2;
+};
+
+__device__ __forceinline__ Prod make_prod(const StepArgs &a, int c, int p, int J) {
   const long long pp = a.g.pp;
   const double *U1 = a.uphys, *U2 = a.uphys + a.fs, *U3 = a.uphys + 2 * a.fs;
-  auto ld = [&](const double *F, int j) -> double2 {
-    return *reinterpret_cast<const double2 *>(F + (long long)j * pp + off);
-  };
-  const double *dy = a.g.dy, *dyf = a.g.dyf;
+  auto pl = [&](const double *F, int j) { return F + (long long)j * pp; };
+  Prod q{};
   if (c == 0 || c == 2) {
     const double *Uc = (c == 0) ? U1 : U3;
-    if (p == 0) {  // U3*U1
-      double2 u1 = ld(U1, J), u3 = ld(U3, J);
-      return make_double2(u3.x * u1.x, u3.y * u1.y);
-    } else if (p == 1) {  // U1*U1 or U3*U3
-      double2 u = ld(Uc, J);
-      return make_double2(u.x * u.x, u.y * u.y);
-    } else {  // d/dy (Uc U2), staggered
-      double2 up = ld(Uc, J + 1), u0 = ld(Uc, J), um = ld(Uc, J - 1);
-      double2 vp = ld(U2, J + 1), v0 = ld(U2, J);
-      const double den = 2.0 * dyf[J];
-      return make_double2((up.x * vp.x + u0.x * vp.x - u0.x * v0.x - um.x * v0.x) / den,
-                          (up.y * vp.y + u0.y * vp.y - u0.y * v0.y - um.y * v0.y) / den);
+    if (p == 0) {
+      q.kind = 0; q.s[0] = pl(U3, J); q.s[1] = pl(U1, J);
+    } else if (p == 1) {
+      q.kind = 1; q.s[0] = pl(Uc, J);
+    } else {
+      q.kind = 2; q.s[0] = pl(Uc, J + 1); q.s[1] = pl(Uc, J); q.s[2] = pl(Uc, J - 1);
+      q.s[3] = pl(U2, J + 1); q.s[4] = pl(U2, J); q.w0 = 2.0 * a.g.dyf[J];
     }
   } else if (c == 1) {
-    if (p <= 1) {  // interp_y(U1 or U3) * U2
+    if (p <= 1) {
       const double *Uc = (p == 0) ? U1 : U3;
-      double2 u0 = ld(Uc, J), um = ld(Uc, J - 1), v = ld(U2, J);
-      const double den = 2. * dy[J];
-      return make_double2(((dyf[J] * u0.x + dyf[J - 1] * um.x) / den) * v.x,
-                          ((dyf[J] * u0.y + dyf[J - 1] * um.y) / den) * v.y);
-    } else {  // d/dy (U2 U2)
-      double2 vp = ld(U2, J + 1), v0 = ld(U2, J), vm = ld(U2, J - 1);
-      double ax = v0.x + vp.x, bx = v0.x + vm.x, ay = v0.y + vp.y, by = v0.y + vm.y;
-      return make_double2((0.25 * (ax * ax) - 0.25 * (bx * bx)) / dy[J], (0.25 * (ay * ay) - 0.25 * (by * by)) / dy[J]);
+      q.kind = 3; q.s[0] = pl(Uc, J); q.s[1] = pl(Uc, J - 1); q.s[2] = pl(U2, J);
+      q.w0 = a.g.dyf[J]; q.w1 = a.g.dyf[J - 1]; q.w2 = 2. * a.g.dy[J];
+    } else {
+      q.kind = 4; q.s[0] = pl(U2, J + 1); q.s[1] = pl(U2, J); q.s[2] = pl(U2, J - 1); q.w0 = a.g.dy[J];
     }
   } else {
     const double *TH = a.uphys + (long long)c * a.fs;
-    if (p == 0) {
-      double2 t = ld(TH, J), u = ld(U1, J);
-      return make_double2(t.x * u.x, t.y * u.y);
-    } else if (p == 1) {
-      double2 t = ld(TH, J), u = ld(U3, J);
-      return make_double2(t.x * u.x, t.y * u.y);
+    if (p <= 1) {
+      q.kind = 0; q.s[0] = pl(TH, J); q.s[1] = pl(p == 0 ? U1 : U3, J);
     } else {
-      double2 tp = ld(TH, J + 1), t0 = ld(TH, J), tm = ld(TH, J - 1);
-      double2 vp = ld(U2, J + 1), v0 = ld(U2, J);
-      const double den = 2.0 * dyf[J];
-      return make_double2((tp.x * vp.x + t0.x * vp.x - t0.x * v0.x - tm.x * v0.x) / den,
-                          (tp.y * vp.y + t0.y * vp.y - t0.y * v0.y - tm.y * v0.y) / den);
+      q.kind = 2; q.s[0] = pl(TH, J + 1); q.s[1] = pl(TH, J); q.s[2] = pl(TH, J - 1);
+      q.s[3] = pl(U2, J + 1); q.s[4] = pl(U2, J); q.w0 = 2.0 * a.g.dyf[J];
     }
+  }
+  return q;
+}
+
+template <int KIND>
+__device__ __forceinline__ double2 eval_prod(const Prod &q, long long off) {
+  auto ld = [&](int i) { return *reinterpret_cast<const double2 *>(q.s[i] + off); };
+  if constexpr (KIND == 0) {
+    const double2 u = ld(0), v = ld(1);
+    return make_double2(u.x * v.x, u.y * v.y);
+  } else if constexpr (KIND == 1) {
+    const double2 u = ld(0);
+    return make_double2(u.x * u.x, u.y * u.y);
+  } else if constexpr (KIND == 2) {
+    const double2 up = ld(0), u0 = ld(1), um = ld(2), vp = ld(3), v0 = ld(4);
+    return make_double2((up.x * vp.x + u0.x * vp.x - u0.x * v0.x - um.x * v0.x) / q.w0,
+                        (up.y * vp.y + u0.y * vp.y - u0.y * v0.y - um.y * v0.y) / q.w0);
+  } else if constexpr (KIND == 3) {
+    const double2 u0 = ld(0), um = ld(1), v = ld(2);
+    return make_double2(((q.w0 * u0.x + q.w1 * um.x) / q.w2) * v.x, ((q.w0 * u0.y + q.w1 * um.y) / q.w2) * v.y);
+  } else {
+    const double2 vp = ld(0), v0 = ld(1), vm = ld(2);
+    const double ax = v0.x + vp.x, bx = v0.x + vm.x, ay = v0.y + vp.y, by = v0.y + vm.y;
+    return make_double2((0.25 * (ax * ax) - 0.25 * (bx * bx)) / q.w0, (0.25 * (ay * ay) - 0.25 * (by * by)) / q.w0);
+  }
+}
+
+__device__ __forceinline__ double2 eval_prod(const Prod &q, long long off) {
+  switch (q.kind) {
+    case 0: return eval_prod<0>(q, off);
+    case 1: return eval_prod<1>(q, off);
+    case 2: return eval_prod<2>(q, off);
+    case 3: return eval_prod<3>(q, off);
+    default: return eval_prod<4>(q, off);
   }
 }
 
@@ -138,6 +168,7 @@ __device__ __forceinline__ double2 cf_init(const StepArgs &a, int c, int J, int 
 }
 
 // Fused explicit pass for plane J (1..NY) and component c (U1,U2,U3,TH...).
+template <bool REG>
 __global__ void __launch_bounds__(1024) k_nonlinear(StepArgs a) {
   extern __shared__ double2 A[];
   const Geo &g = a.g;
@@ -180,26 +211,17 @@ __global__ void __launch_bounds__(1024) k_nonlinear(StepArgs a) {
   const LoadTile lt(g);
   const int nw = nthr >> 6;
   for (int p = 0; p < 3; ++p) {
-    constexpr int PB = 1;  // tiles per wave per batch: loads issued before LDS stores
-    for (int t0 = threadIdx.x >> 6; t0 < lt.ntiles; t0 += PB * nw) {
-      double2 v[PB];
-      int sl[PB];
-#pragma unroll
-      for (int b = 0; b < PB; ++b) {
+    const Prod pr = make_prod(a, c, p, J);
+    if constexpr (REG) {
+      rows_fwd_reg64(A, g, T, [&](int z, int xp) { return eval_prod(pr, (long long)z * g.nx + 2 * xp); });
+    } else {
+      for (int t0 = threadIdx.x >> 6; t0 < lt.ntiles; t0 += nw) {
         int z, xp;
-        sl[b] = -1;
-        const int t = t0 + b * nw;
-        if (t < lt.ntiles && lt.at(g, t, z, xp)) {
-          sl[b] = phys_slot(T, g, z, xp);
-          v[b] = product_pair(a, c, p, J, (long long)z * g.nx + 2 * xp);
-        }
+        if (lt.at(g, t0, z, xp)) A[phys_slot(T, g, z, xp)] = eval_prod(pr, (long long)z * g.nx + 2 * xp);
       }
-#pragma unroll
-      for (int b = 0; b < PB; ++b)
-        if (sl[b] >= 0) A[sl[b]] = v[b];
+      lds_barrier();
+      plane_rows_fwd(A, g, T);
     }
-    lds_barrier();
-    plane_rows_fwd(A, g, T);
     plane_cols_fwd(A, g, T);
     // derivative factor: c==0 takes (ikz, ikx, 1); all others (ikx, ikz, 1)
 #pragma unroll
@@ -282,8 +304,13 @@ __global__ void __launch_bounds__(1024) k_nonlinear(StepArgs a) {
   }
   lds_barrier();
   plane_cols_inv(A, g, T);
-  plane_rows_inv(A, g, T);
-  plane_store_phys(A, g, T, reinterpret_cast<double2 *>(a.rphys + (long long)c * a.fs + (long long)J * g.pp));
+  double2 *dst = reinterpret_cast<double2 *>(a.rphys + (long long)c * a.fs + (long long)J * g.pp);
+  if constexpr (REG) {
+    rows_inv_reg64(A, g, T, [&](int z, int m, double2 v) { dst[z * 64 + m] = v; });
+  } else {
+    plane_rows_inv(A, g, T);
+    plane_store_phys(A, g, T, dst);
+  }
 }
 // ---------------------------------------------------------------------------
 // Real Crank-Nicolson systems (matrix build of channel.f:771-1005, BC rows
@@ -935,7 +962,10 @@ hipError_t launch_cn_tables(Ctx &c) {
 
 static hipError_t launch_nonlinear(Ctx &c, int rk) {
   StepArgs a = make_step_args(c, rk);
-  hipLaunchKernelGGL(k_nonlinear, dim3(c.g.ny, c.nfields), dim3(c.plane_threads), c.lds_bytes, c.stream, a);
+  if (c.reg_rows_nl)
+    hipLaunchKernelGGL(k_nonlinear<true>, dim3(c.g.ny, c.nfields), dim3(c.plane_threads), c.lds_bytes, c.stream, a);
+  else
+    hipLaunchKernelGGL(k_nonlinear<false>, dim3(c.g.ny, c.nfields), dim3(c.plane_threads), c.lds_bytes, c.stream, a);
   return hipGetLastError();
 }
 
@@ -1052,7 +1082,9 @@ hipError_t launch_end_step(Ctx &c) {
 
 hipError_t prepare_step_kernels(Ctx &c) {
   hipError_t e =
-      hipFuncSetAttribute((const void *)k_nonlinear, hipFuncAttributeMaxDynamicSharedMemorySize, c.lds_bytes);
+      hipFuncSetAttribute((const void *)k_nonlinear<true>, hipFuncAttributeMaxDynamicSharedMemorySize, c.lds_bytes);
+  if (e != hipSuccess) return e;
+  e = hipFuncSetAttribute((const void *)k_nonlinear<false>, hipFuncAttributeMaxDynamicSharedMemorySize, c.lds_bytes);
   if (e != hipSuccess) return e;
   return hipFuncSetAttribute((const void *)k_rem_div_thomas, hipFuncAttributeMaxDynamicSharedMemorySize,
                              (int)rem_div_thomas_lds(c));
