@@ -15,7 +15,8 @@
 
 namespace dh {
 
-constexpr int kMaxQ = 4;  // Fourier modes owned per thread in k_nonlinear (pf <= kMaxQ*threads)
+constexpr int kMaxQ = 4;  // Fourier modes owned per thread in the 1024-thread k_nonlinear
+constexpr int kNlRegThreads = 512, kNlRegQ = 8;  // the register-row variant
 
 struct StepArgs {
   Geo g;
@@ -168,8 +169,10 @@ __device__ __forceinline__ double2 cf_init(const StepArgs &a, int c, int J, int 
 }
 
 // Fused explicit pass for plane J (1..NY) and component c (U1,U2,U3,TH...).
-template <bool REG>
-__global__ void __launch_bounds__(1024) k_nonlinear(StepArgs a) {
+// REG: row transforms in registers, 512 threads (twice the register budget
+// of the 1024-thread LDS variant), NQ Fourier modes owned per thread.
+template <bool REG, int NQ>
+__global__ void __launch_bounds__(REG ? 512 : 1024) k_nonlinear(StepArgs a) {
   extern __shared__ double2 A[];
   const Geo &g = a.g;
   const Tables T = load_tables(A, g);
@@ -198,9 +201,9 @@ __global__ void __launch_bounds__(1024) k_nonlinear(StepArgs a) {
   const double time = a.clk->time;
   const int nthr = blockDim.x;
 
-  double2 acc[kMaxQ];
+  double2 acc[NQ];
 #pragma unroll
-  for (int q = 0; q < kMaxQ; ++q) {
+  for (int q = 0; q < NQ; ++q) {
     const int m = threadIdx.x + q * nthr;
     if (m < g.pf) {
       const int kzp = (int)fdiv((unsigned)m, g.f_nkx1), kx = m - kzp * g.nkx1;
@@ -225,7 +228,7 @@ __global__ void __launch_bounds__(1024) k_nonlinear(StepArgs a) {
     plane_cols_fwd(A, g, T);
     // derivative factor: c==0 takes (ikz, ikx, 1); all others (ikx, ikz, 1)
 #pragma unroll
-    for (int q = 0; q < kMaxQ; ++q) {
+    for (int q = 0; q < NQ; ++q) {
       const int m = threadIdx.x + q * nthr;
       if (m < g.pf) {
         const int kzp = (int)fdiv((unsigned)m, g.f_nkx1), kx = m - kzp * g.nkx1;
@@ -267,7 +270,7 @@ __global__ void __launch_bounds__(1024) k_nonlinear(StepArgs a) {
     ey2 = g.dyf[J];
   }
 #pragma unroll
-  for (int q = 0; q < kMaxQ; ++q) {
+  for (int q = 0; q < NQ; ++q) {
     const int m = threadIdx.x + q * nthr;
     if (m < g.pf) {
       const int kzp = (int)fdiv((unsigned)m, g.f_nkx1), kx = m - kzp * g.nkx1;
@@ -963,9 +966,11 @@ hipError_t launch_cn_tables(Ctx &c) {
 static hipError_t launch_nonlinear(Ctx &c, int rk) {
   StepArgs a = make_step_args(c, rk);
   if (c.reg_rows_nl)
-    hipLaunchKernelGGL(k_nonlinear<true>, dim3(c.g.ny, c.nfields), dim3(c.plane_threads), c.lds_bytes, c.stream, a);
+    hipLaunchKernelGGL((k_nonlinear<true, kNlRegQ>), dim3(c.g.ny, c.nfields), dim3(kNlRegThreads), c.lds_bytes,
+                       c.stream, a);
   else
-    hipLaunchKernelGGL(k_nonlinear<false>, dim3(c.g.ny, c.nfields), dim3(c.plane_threads), c.lds_bytes, c.stream, a);
+    hipLaunchKernelGGL((k_nonlinear<false, kMaxQ>), dim3(c.g.ny, c.nfields), dim3(c.plane_threads), c.lds_bytes,
+                       c.stream, a);
   return hipGetLastError();
 }
 
@@ -1081,10 +1086,11 @@ hipError_t launch_end_step(Ctx &c) {
 }
 
 hipError_t prepare_step_kernels(Ctx &c) {
-  hipError_t e =
-      hipFuncSetAttribute((const void *)k_nonlinear<true>, hipFuncAttributeMaxDynamicSharedMemorySize, c.lds_bytes);
+  hipError_t e = hipFuncSetAttribute((const void *)k_nonlinear<true, kNlRegQ>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, c.lds_bytes);
   if (e != hipSuccess) return e;
-  e = hipFuncSetAttribute((const void *)k_nonlinear<false>, hipFuncAttributeMaxDynamicSharedMemorySize, c.lds_bytes);
+  e = hipFuncSetAttribute((const void *)k_nonlinear<false, kMaxQ>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          c.lds_bytes);
   if (e != hipSuccess) return e;
   return hipFuncSetAttribute((const void *)k_rem_div_thomas, hipFuncAttributeMaxDynamicSharedMemorySize,
                              (int)rem_div_thomas_lds(c));
