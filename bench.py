@@ -58,7 +58,21 @@ def phase_bytes(p, rk):
     return b
 
 
-def cpu_baseline(p, gy, ic, budget_s=12.0, max_steps=6):
+def measured_traffic():
+    """Device-memory bytes per launch by phase, from the FETCH_SIZE and
+    WRITE_SIZE counter runs summarised in profiles/ (scaled by the reading
+    of a plain copy of known size); {} when the file is absent."""
+    path = os.path.join(ROOT, "profiles", "traffic_per_launch.txt")
+    out = {}
+    if os.path.exists(path):
+        for line in open(path):
+            parts = line.split()
+            if len(parts) == 2 and not line.startswith("#"):
+                out[parts[0]] = int(parts[1])
+    return out
+
+
+def cpu_baseline(p, gy, ic, budget_s=12.0, max_steps=60):
     from oracle import dor
     from tests import helpers
     o, *_ = helpers.make_oracle(dor, "channel_re180")
@@ -171,7 +185,8 @@ def main():
                                "tanh grid CS=1.75, FP64, CFL=0.5 variable dt",
                    "nx": p["nx"], "ny": p["ny"], "nz": p["nz"], "n_th": p["n_th"]},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic().get(dominant),
                      "bytes_per_launch": int(byts), "mean_ms": round(mean_ms, 5)},
         "step_roofline": {"bytes_per_step": B, "achieved": round(step_gbs, 1), "peak": HBM_PEAK_GBS,
                           "unit": "GB/s", "frac": round(step_gbs / HBM_PEAK_GBS, 4)},

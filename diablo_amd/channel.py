@@ -73,6 +73,7 @@ EXPORTS = {
     "dh_profile_substep": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, _DP, ctypes.c_int32]),
     "dh_phase_name": (ctypes.c_char_p, [ctypes.c_int32]),
     "dh_rem_div": (ctypes.c_int, [ctypes.c_void_p]),
+    "dh_save_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]),
     "dh_create_flow_chan": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_double, ctypes.c_uint64,
                                            ctypes.c_double, ctypes.c_double]),
     "dh_create_th_chan": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
@@ -84,6 +85,10 @@ EXPORTS = {
 }
 
 _lib = None
+
+
+# profiles of dh_save_stats, in the order of the DH_ST_* enum
+STATS = ("ume", "vme", "wme", "urms", "vrms", "wrms", "uv", "uw", "wv", "dudy", "dwdy", "shear")
 
 
 def load_library(path=LIB_PATH):
@@ -243,6 +248,16 @@ class Channel:
 
     def rem_div(self):
         self._check(self.lib.dh_rem_div(self.h), "dh_rem_div")
+
+    def save_stats(self, ly):
+        """SAVE_STATS_CHAN's plane statistics on the device (save_stats.f:136-262):
+        dict of (NY+2,) arrays in the order of STATS, plus 'ubulk'."""
+        out = np.zeros((len(STATS), self.ny + 2))
+        ub = ctypes.c_double(0.0)
+        self._check(self.lib.dh_save_stats(self.h, ly, out.ctypes.data, ctypes.byref(ub)), "dh_save_stats")
+        d = {k: out[i].copy() for i, k in enumerate(STATS)}
+        d["ubulk"] = ub.value
+        return d
 
     def create_flow(self, ic_type, seed, amp, ymask=0.0, ubulk0=0.0):
         self._check(self.lib.dh_create_flow_chan(self.h, ic_type, ubulk0, seed, amp, ymask), "dh_create_flow_chan")

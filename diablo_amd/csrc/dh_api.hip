@@ -107,11 +107,14 @@ hipError_t dalloc(T **p, size_t count) {
 void free_all(Ctx &c) {
   void *ptrs[] = {c.tw_row, c.tw_col, c.tw_real, c.posr, c.posrd, c.posz, c.posc, c.kx, c.kz, c.kx2, c.kz2, c.d_dy, c.d_dyf, c.d_gyf, c.cu[0], c.cu[1],
                   c.cu[2], c.cp, c.cf[0], c.cf[1], c.cf[2], c.cth, c.cfth, c.phi, c.uphys, c.rphys, c.rd_ta,
-                  c.rd_tb, c.clock, c.fac.pf, c.fac.pb, c.fac.tl, c.fac.tb, c.fac.tc, c.fac.tr, c.fac.isbc, c.rd_tr, c.fac.vconst, c.rd_l, c.rd_u};
+                  c.rd_tb, c.clock, c.fac.pf, c.fac.pb, c.fac.tl, c.fac.tb, c.fac.tc, c.fac.tr, c.fac.isbc, c.rd_tr, c.fac.vconst, c.rd_l, c.rd_u, c.stats};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c.step_graph) (void)hipGraphExecDestroy(c.step_graph);
   if (c.stream) (void)hipStreamDestroy(c.stream);
+  if (c.side) (void)hipStreamDestroy(c.side);
+  if (c.ev_fork) (void)hipEventDestroy(c.ev_fork);
+  if (c.ev_join) (void)hipEventDestroy(c.ev_join);
 }
 
 // courant.f:15-31: the part of the time-step bound independent of U.
@@ -141,7 +144,7 @@ double dt_cap_of(const Ctx &c) {
 }
 
 int rebuild_tables(Ctx &c) {
-  HIPCHK(launch_cn_tables(c));
+  HIPCHK(launch_cn_tables(c, c.stream));
   HIPCHK(hipStreamSynchronize(c.stream));
   return DH_OK;
 }
@@ -239,6 +242,9 @@ int dh_create(const dh_params *p, dh_ctx **out) {
     return DH_EHIP;                         \
   }
   ACHK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+  ACHK(hipStreamCreateWithFlags(&c.side, hipStreamNonBlocking));
+  ACHK(hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming));
+  ACHK(hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming));
   const size_t nyy = (size_t)g.ny + 2;
   const size_t cf_elems = nyy * g.pf;
   const size_t ph_elems = nyy * (size_t)g.pp;
@@ -269,6 +275,7 @@ int dh_create(const dh_params *p, dh_ctx **out) {
   ACHK(dalloc(&c.rd_ta, cf_elems));
   ACHK(dalloc(&c.rd_tb, cf_elems));
   ACHK(dalloc(&c.rd_tr, cf_elems));
+  ACHK(dalloc(&c.stats, (size_t)DH_NSTATS * nyy));
   ACHK(dalloc(&c.rd_l, nyy));
   ACHK(dalloc(&c.rd_u, nyy));
   ACHK(dalloc(&c.clock, 1));
@@ -605,6 +612,7 @@ int dh_profile_substep(dh_ctx *h, int32_t rk_step, int32_t nrep, double *ms, int
       HIPCHK(launch_phase(c, rk_step - 1, ph));
       HIPCHK(hipEventRecord(ev[ph + 1], c.stream));
     }
+    HIPCHK(join_side(c));
     HIPCHK(hipEventSynchronize(ev[kNumPhases]));
     for (int ph = 0; ph < kNumPhases; ++ph) {
       float t = 0.f;
@@ -659,6 +667,22 @@ int dh_rem_div(dh_ctx *h) {
   Ctx &c = h->c;
   if (!c.grid_set) return fail(c, DH_ESTATE, "dh_set_grid first");
   HIPCHK(launch_rem_div(c, -1));
+  return DH_OK;
+}
+
+int dh_save_stats(dh_ctx *h, double ly, double *out, double *ubulk) {
+  if (!h || !out || !ubulk || !(ly > 0.0)) return DH_EARG;
+  Ctx &c = h->c;
+  if (!c.grid_set) return fail(c, DH_ESTATE, "dh_set_grid first");
+  const int ny2 = c.g.ny + 2;
+  HIPCHK(launch_stats(c, c.stats));
+  HIPCHK(hipMemcpyAsync(out, c.stats, sizeof(double) * DH_NSTATS * ny2, hipMemcpyDeviceToHost, c.stream));
+  HIPCHK(hipStreamSynchronize(c.stream));
+  // INTEGRATE_Y_VAR (mpi.f:1610-1632), in its order
+  const double *ume = out + DH_ST_UME * ny2;
+  double res = 0.;
+  for (int j = 2; j <= c.g.ny; ++j) res = res + 0.5 * (ume[j] + ume[j - 1]) * c.dy[j];
+  *ubulk = res / ly;
   return DH_OK;
 }
 

@@ -96,6 +96,9 @@ struct Ctx {
   bool reg_rows = false;     // plane kernels: row transforms in registers (NX = 128, NZ a multiple of 8)
   bool reg_rows_nl = false;  // the same in k_nonlinear
   hipStream_t stream = nullptr;
+  hipStream_t side = nullptr;          // table rebuild after COURANT, overlapped with the substep's tail
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  bool tables_pending = false;
   std::string err;
   bool grid_set = false;
   // host copies
@@ -113,6 +116,7 @@ struct Ctx {
   double *rd_ta = nullptr;      // REM_DIV sweep multipliers [NY+2][pf]
   double *rd_tb = nullptr;      // REM_DIV sweep pivots      [NY+2][pf]
   double *rd_tr = nullptr;      // REM_DIV 1/pivots          [NY+2][pf]
+  double *stats = nullptr;      // dh_save_stats profiles [DH_NSTATS][NY+2]
   Clock *clock = nullptr;
   RealFactors fac{};
   double *rd_l = nullptr, *rd_u = nullptr;  // 1/(DY DYF) tables for REM_DIV
@@ -124,6 +128,7 @@ struct Ctx {
 // ---- launchers (dh_fft.hip) ----
 hipError_t prepare_fft_kernels(Ctx &c);
 hipError_t launch_fft_to_phys(Ctx &c, const double2 *four, double *phys, int nplanes);
+hipError_t launch_stats(Ctx &c, double *out_dev);  // profiles of dh_save_stats into out_dev
 hipError_t launch_fft_to_four(Ctx &c, const double *phys, double2 *four, int nplanes);
 hipError_t launch_state_to_phys(Ctx &c);
 hipError_t launch_state_to_four(Ctx &c);
@@ -141,7 +146,8 @@ hipError_t launch_phase(Ctx &c, int rk, int phase);
 hipError_t launch_rem_div(Ctx &c, int rk_for_cp);     // rk_for_cp<0: no CP update
 hipError_t launch_step_begin(Ctx &c);
 hipError_t launch_end_step(Ctx &c);
-hipError_t launch_cn_tables(Ctx &c);
+hipError_t launch_cn_tables(Ctx &c, hipStream_t st);
+hipError_t join_side(Ctx &c);  // main stream waits for a pending table rebuild
 hipError_t launch_rem_div_factors(Ctx &c);  // once per grid  // for the DELTA_T in the device clock
 
 }  // namespace dh

@@ -434,31 +434,38 @@ __global__ void __launch_bounds__(256) k_cn_tables(const Clock *clk, FactorIn in
     }
   }
   __syncthreads();
-  if (threadIdx.x != 0) return;
-  const long long o = ((long long)rk * in.nfields + f) * nrow;
+  // the forward elimination is a recurrence: one thread, operands in LDS
+  double *tl = sm + 3 * nrow, *tb = sm + 4 * nrow, *tr = sm + 5 * nrow, *pf = sm + 6 * nrow, *pb = sm + 7 * nrow;
   const int sr = in.seg_rows;
-  double bp = b[0], pf = 0.0;
-  fac.tl[o] = 0.0;
-  fac.tb[o] = bp;
-  fac.tc[o] = c[0];
-  fac.tr[o] = 1.0 / bp;
-  fac.pf[o] = 0.0;
-  for (int J = 1; J < nrow; ++J) {
-    const double m = -a[J] / bp;
-    bp = b[J] + m * c[J - 1];
-    fac.tl[o + J] = m;
-    fac.tb[o + J] = bp;
-    fac.tc[o + J] = c[J];
-    fac.tr[o + J] = 1.0 / bp;
-    pf = (J % sr == 0) ? m : pf * m;
-    fac.pf[o + J] = pf;
+  if (threadIdx.x == 0) {
+    double bp = b[0], pfv = 0.0;
+    tl[0] = 0.0;
+    tb[0] = bp;
+    pf[0] = 0.0;
+    for (int J = 1; J < nrow; ++J) {
+      const double m = -a[J] / bp;
+      bp = b[J] + m * c[J - 1];
+      tl[J] = m;
+      tb[J] = bp;
+      pfv = (J % sr == 0) ? m : pfv * m;
+      pf[J] = pfv;
+    }
+    double pbv = 0.0;
+    for (int J = nrow - 1; J >= 0; --J) {
+      const double fj = (tb[J] != 0.0) ? -c[J] / tb[J] : 0.0;
+      pbv = (J % sr == sr - 1 || J == nrow - 1) ? fj : fj * pbv;
+      pb[J] = pbv;
+    }
   }
-  double pb = 0.0;
-  for (int J = nrow - 1; J >= 0; --J) {
-    const double tb = fac.tb[o + J];
-    const double fj = (tb != 0.0) ? -c[J] / tb : 0.0;
-    pb = (J % sr == sr - 1 || J == nrow - 1) ? fj : fj * pb;
-    fac.pb[o + J] = pb;
+  __syncthreads();
+  const long long o = ((long long)rk * in.nfields + f) * nrow;
+  for (int i = threadIdx.x; i < nrow; i += blockDim.x) {
+    fac.tl[o + i] = tl[i];
+    fac.tb[o + i] = tb[i];
+    fac.tc[o + i] = c[i];
+    fac.tr[o + i] = 1.0 / tb[i];
+    fac.pf[o + i] = pf[i];
+    fac.pb[o + i] = pb[i];
   }
 }
 
@@ -956,10 +963,10 @@ static FactorIn make_factor_in(const Ctx &c) {
   return in;
 }
 
-hipError_t launch_cn_tables(Ctx &c) {
+hipError_t launch_cn_tables(Ctx &c, hipStream_t st) {
   FactorIn in = make_factor_in(c);
-  const size_t shm = (size_t)3 * (c.g.ny + 2) * sizeof(double);
-  hipLaunchKernelGGL(k_cn_tables, dim3(3 * c.nfields), dim3(256), shm, c.stream, c.clock, in, c.fac);
+  const size_t shm = (size_t)8 * (c.g.ny + 2) * sizeof(double);
+  hipLaunchKernelGGL(k_cn_tables, dim3(3 * c.nfields), dim3(256), shm, st, c.clock, in, c.fac);
   return hipGetLastError();
 }
 
@@ -1036,12 +1043,20 @@ hipError_t launch_rem_div(Ctx &c, int rk) {
   return hipGetLastError();
 }
 
+// COURANT, then the solve tables for the new DELTA_T on the side stream: the
+// next solve is a substep away, so the rebuild runs beside this substep's
+// transform and pressure solve and is joined at the end of the substep.
 static hipError_t launch_courant(Ctx &c) {
   hipLaunchKernelGGL(k_courant_dt, dim3(1), dim3(64), 0, c.stream, c.clock, c.dt_cap, c.p.variable_dt,
                      c.p.update_dt);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  return launch_cn_tables(c);
+  if ((e = hipEventRecord(c.ev_fork, c.stream)) != hipSuccess) return e;
+  if ((e = hipStreamWaitEvent(c.side, c.ev_fork, 0)) != hipSuccess) return e;
+  if ((e = launch_cn_tables(c, c.side)) != hipSuccess) return e;
+  if ((e = hipEventRecord(c.ev_join, c.side)) != hipSuccess) return e;
+  c.tables_pending = true;
+  return hipSuccess;
 }
 
 // Kernels of one substep in launch order (channel.f:386-1065).
@@ -1061,12 +1076,18 @@ hipError_t launch_phase(Ctx &c, int rk, int phase) {
   return hipErrorInvalidValue;
 }
 
+hipError_t join_side(Ctx &c) {
+  if (!c.tables_pending) return hipSuccess;
+  c.tables_pending = false;
+  return hipStreamWaitEvent(c.stream, c.ev_join, 0);
+}
+
 hipError_t launch_substep(Ctx &c, int rk) {
   for (int ph = 0; ph < kNumPhases; ++ph) {
     hipError_t e = launch_phase(c, rk, ph);
     if (e != hipSuccess) return e;
   }
-  return hipSuccess;
+  return join_side(c);
 }
 
 __global__ void k_step_begin(Clock *clk) {

@@ -141,6 +141,23 @@ int dh_create_th_chan(dh_ctx *ctx, int32_t ic_type);
  * planes of CU1..3 in Fourier space. */
 int dh_apply_bc_vel(dh_ctx *ctx);
 
+/* Plane statistics of SAVE_STATS_CHAN (save_stats.f:136-262) on the device
+ * from the current Fourier state, which is left unchanged (the physical
+ * velocity buffers are overwritten; the next substep recomputes them):
+ * out[q * (NY+2) + j], j = 0..NY+1, for q in the order
+ *   UME VME WME (plane means, :136-150)  URMS VRMS WRMS (:166-192)
+ *   UV UW WV (Reynolds stresses, :197-232)  DUDY DWDY (:235-238)
+ *   SHEAR (mean square shear, :241-262)
+ * with rows the reference leaves unset at 0, and *ubulk =
+ * INTEGRATE_Y_VAR(UME) over the domain height ly (mpi.f:1610-1632).
+ * Plane sums are reduced in FP64 on the device. */
+enum {
+  DH_ST_UME = 0, DH_ST_VME, DH_ST_WME, DH_ST_URMS, DH_ST_VRMS, DH_ST_WRMS,
+  DH_ST_UV, DH_ST_UW, DH_ST_WV, DH_ST_DUDY, DH_ST_DWDY, DH_ST_SHEAR,
+  DH_NSTATS
+};
+int dh_save_stats(dh_ctx *ctx, double ly, double *out, double *ubulk);
+
 /* FFT_XZ_TO_FOURIER (mpi.f:1477-1542) of nplanes consecutive physical
  * planes [nplanes][NZ][NX] into de-aliased Fourier planes
  * [nplanes][TNKZ+1][NKX+1] (device pointers). */

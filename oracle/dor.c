@@ -1226,3 +1226,71 @@ void dor_create_th_chan(dor_state *s, int ic_type) {
     memcpy(s->CTH + s->cfield * N, s->CS1, sizeof(cplx) * s->cfield);
   }
 }
+
+/* SAVE_STATS_CHAN subset, save_stats.f:136-262 (one rank, RANKZ=0): mean
+ * profiles from the (0,0) modes (:136-150), the velocities transformed to
+ * physical space from a copy of CU (the reference saves CU in CR and
+ * transforms CU in place, :130-160), rms (:166-192), Reynolds stresses on
+ * the GY/GYF grids (:197-232), mean gradients (:235-238) and the mean
+ * square shear (:241-262); UBULK by INTEGRATE_Y_VAR (mpi.f:1610-1632). */
+void dor_save_stats(dor_state *s, double ly, double *out, double *ubulk) {
+#define R3D(a, i, k, j) a[RIDX(s, i, k, j)]
+  const int NY = s->NY, NZP = s->NZP, NXM = s->NXM, NX = s->NX, NZ = s->NZ;
+  const double *DY = s->DY, *DYF = s->DYF, *GYF = s->GYF;
+  const size_t ny2 = (size_t)NY + 2;
+  memset(out, 0, sizeof(double) * ny2 * DOR_NSTATS);
+  double *ume = out + DOR_ST_UME * ny2, *vme = out + DOR_ST_VME * ny2, *wme = out + DOR_ST_WME * ny2;
+  double *urms = out + DOR_ST_URMS * ny2, *vrms = out + DOR_ST_VRMS * ny2, *wrms = out + DOR_ST_WRMS * ny2;
+  double *uv = out + DOR_ST_UV * ny2, *uw = out + DOR_ST_UW * ny2, *wv = out + DOR_ST_WV * ny2;
+  double *dudy = out + DOR_ST_DUDY * ny2, *dwdy = out + DOR_ST_DWDY * ny2, *shear = out + DOR_ST_SHEAR * ny2;
+  for (size_t j = 0; j < ny2; ++j) {
+    ume[j] = creal(s->CU1[CIDX(s, 0, 0, j)]);
+    vme[j] = creal(s->CU2[CIDX(s, 0, 0, j)]);
+    wme[j] = creal(s->CU3[CIDX(s, 0, 0, j)]);
+  }
+  /* INTEGRATE_Y_VAR(UME): RES = sum_{j=2..NY} 0.5 (VAR(j)+VAR(j-1)) DY(j), / LY */
+  {
+    double res = 0.;
+    for (int j = 2; j <= NY; ++j) res = res + 0.5 * (ume[j] + ume[j - 1]) * DY[j];
+    *ubulk = res / ly;
+  }
+  cplx *C[3];
+  const cplx *src[3] = {s->CU1, s->CU2, s->CU3};
+  for (int c = 0; c < 3; ++c) {
+    C[c] = malloc(sizeof(cplx) * s->cfield);
+    memcpy(C[c], src[c], sizeof(cplx) * s->cfield);
+    dor_fft_xz_to_physical(s, (double *)C[c], 0, NY + 1);
+  }
+  const double *U1 = PHYS(C[0]), *U2 = PHYS(C[1]), *U3 = PHYS(C[2]);
+  for (int j = 1; j <= NY; ++j) {
+    double su = 0., sv = 0., sw = 0., suv = 0., suw = 0., swv = 0., ssh = 0.;
+    for (int k = 0; k <= NZP - 1; ++k)
+      for (int i = 0; i <= NXM; ++i) {
+        const double u = R3D(U1, i, k, j), w = R3D(U3, i, k, j);
+        su = su + (u - ume[j]) * (u - ume[j]);
+        sv = sv + 0.5 * ((R3D(U2, i, k, j) - vme[j]) * (R3D(U2, i, k, j) - vme[j]) +
+                         (R3D(U2, i, k, j + 1) - vme[j + 1]) * (R3D(U2, i, k, j + 1) - vme[j + 1]));
+        sw = sw + (w - wme[j]) * (w - wme[j]);
+        suv = suv + (DYF[j - 1] * (u - ume[j]) + DYF[j] * (R3D(U1, i, k, j - 1) - ume[j - 1])) / (2.0 * DY[j]) *
+                        (R3D(U2, i, k, j) - vme[j]);
+        swv = swv + (DYF[j - 1] * (w - wme[j]) + DYF[j] * (R3D(U3, i, k, j - 1) - wme[j - 1])) / (2.0 * DY[j]) *
+                        (R3D(U2, i, k, j) - vme[j]);
+        suw = suw + (u - ume[j]) * (w - wme[j]);
+        const double a = (R3D(U1, i, k, j + 1) - R3D(U1, i, k, j - 1)) / (2.0 * DYF[j]);
+        const double b = (R3D(U3, i, k, j + 1) - R3D(U3, i, k, j - 1)) / (2.0 * DYF[j]);
+        ssh = ssh + a * a + b * b;
+      }
+    const double n = (double)NZ * (double)NX;
+    urms[j] = sqrt(su / n);
+    vrms[j] = sqrt(sv / n);
+    wrms[j] = sqrt(sw / n);
+    uv[j] = suv / n;
+    uw[j] = suw / n;
+    wv[j] = swv / n;
+    dudy[j] = (ume[j] - ume[j - 1]) / (GYF[j] - GYF[j - 1]);
+    dwdy[j] = (wme[j] - wme[j - 1]) / (GYF[j] - GYF[j - 1]);
+    shear[j] = ssh / (double)(NX * NZ);
+  }
+  for (int c = 0; c < 3; ++c) free(C[c]);
+#undef R3D
+}

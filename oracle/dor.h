@@ -112,6 +112,16 @@ void dor_create_flow_chan(dor_state *s, int ic_type, double ubulk0,
 void dor_create_th_chan(dor_state *s, int ic_type);
 void dor_apply_bc_vel(dor_state *s);
 
+/* Plane statistics of SAVE_STATS_CHAN (save_stats.f:136-262) from the
+ * current Fourier state, without touching it: out[q*(NY+2) + j] for
+ * q = DOR_ST_* and j = 0..NY+1 (rows the reference leaves unset are 0),
+ * *ubulk = INTEGRATE_Y_VAR of UME over the height ly (mpi.f:1610-1632). */
+enum {
+  DOR_ST_UME = 0, DOR_ST_VME, DOR_ST_WME, DOR_ST_URMS, DOR_ST_VRMS, DOR_ST_WRMS,
+  DOR_ST_UV, DOR_ST_UW, DOR_ST_WV, DOR_ST_DUDY, DOR_ST_DWDY, DOR_ST_SHEAR,
+  DOR_NSTATS
+};
+void dor_save_stats(dor_state *s, double ly, double *out, double *ubulk);
 /* Deterministic generator used for ICs (SplitMix64 + Box-Muller). */
 double dor_gauss(uint64_t seed, uint64_t counter);
 

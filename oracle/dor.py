@@ -48,6 +48,10 @@ class DorParams(ctypes.Structure):
 _lib = None
 
 
+# plane statistics of dor_save_stats, in its order
+STATS = ("ume", "vme", "wme", "urms", "vrms", "wrms", "uv", "uw", "wv", "dudy", "dwdy", "shear")
+
+
 def load():
     global _lib
     if _lib is None:
@@ -78,6 +82,7 @@ def load():
         lib.dor_create_flow_chan.argtypes = [vp, ip, ctypes.c_double, ctypes.c_uint64, ctypes.c_double, ctypes.c_double]
         lib.dor_create_th_chan.argtypes = [vp, ip]
         lib.dor_apply_bc_vel.argtypes = [vp]
+        lib.dor_save_stats.argtypes = [vp, ctypes.c_double, dp, dp]
         lib.dor_gauss.restype = ctypes.c_double
         lib.dor_gauss.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
         _lib = lib
@@ -221,3 +226,12 @@ class Oracle:
 
     def apply_bc_vel(self):
         self.lib.dor_apply_bc_vel(self.h)
+
+    def save_stats(self, ly):
+        """SAVE_STATS_CHAN's plane statistics (dict of (NY+2,) arrays) and UBULK."""
+        out = np.zeros((len(STATS), self.ny + 2))
+        ub = ctypes.c_double(0.0)
+        self.lib.dor_save_stats(self.h, ly, _dptr(out), ctypes.byref(ub))
+        d = {k: out[i].copy() for i, k in enumerate(STATS)}
+        d["ubulk"] = ub.value
+        return d
