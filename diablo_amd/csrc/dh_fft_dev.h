@@ -420,19 +420,29 @@ __device__ __forceinline__ void wave_lds_sync() {
 // Forward x transform of every row, physical pairs from load(z, m) (global
 // memory or computed products) to X_k / NX for k <= NKX in natural slots of
 // the LDS rows (FFT_XZ_TO_FOURIER's r2c, mpi.f:1490-1506).
-template <class Load>
-__device__ void rows_fwd_reg64(double2 *A, const Geo &g, const Tables &T, Load load) {
+template <class Fetch, class Eval>
+__device__ void rows_fwd_reg64(double2 *A, const Geo &g, const Tables &T, Fetch fetch, Eval eval) {
   const int lane = threadIdx.x & 63, rho = lane >> 3, t = lane & 7;
   const int nw = blockDim.x >> 6;
   for (int slab = threadIdx.x >> 6; slab * 8 < g.nz; slab += nw) {
-    const int z = slab * 8 + rho;
-    const bool zin = z < g.nz;
+    const int z = slab * 8 + rho;  // < NZ: register rows require NZ % 8 == 0
     double2 *row = A + z * g.ls;
+    // the loads of a group of G samples are all issued before any arithmetic
+    // on them; left to itself the scheduler interleaves loads and arithmetic
+    // sample by sample and the memory latency is exposed eight times per
+    // slab.  Samples wider than three complex values go in two groups (the
+    // register file holds the raw values of one group).
+    using Raw = decltype(fetch(0, 0));
+    constexpr int G = sizeof(Raw) > 48 ? 4 : 8;
     double2 v[8];
 #pragma unroll
-    for (int r = 0; r < 8; ++r) {
-      v[r] = make_double2(0.0, 0.0);
-      if (zin) v[r] = load(z, t + 8 * r);
+    for (int r0 = 0; r0 < 8; r0 += G) {
+      Raw raw[G];
+#pragma unroll
+      for (int r = 0; r < G; ++r) raw[r] = fetch(z, t + 8 * (r0 + r));
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int r = 0; r < G; ++r) v[r0 + r] = eval(raw[r]);
     }
     dft_small<8, false>(v);
 #pragma unroll
@@ -464,6 +474,11 @@ __device__ void rows_fwd_reg64(double2 *A, const Geo &g, const Tables &T, Load l
     }
   }
   lds_barrier();
+}
+
+template <class Load>
+__device__ __forceinline__ void rows_fwd_reg64(double2 *A, const Geo &g, const Tables &T, Load load) {
+  rows_fwd_reg64(A, g, T, load, [](double2 v) { return v; });
 }
 
 // Inverse x transform of every row (FFT_XZ_TO_PHYSICAL's c2r,

@@ -11,6 +11,8 @@
 //                 divergence, pressure-Poisson solve per (kx,kz), projection
 //                 and pressure update (channel.f:1103-1214, 1059-1065);
 //   k_courant_dt  the rest of COURANT (courant.f:45-62) on the device.
+#include <type_traits>
+
 #include "dh_fft_dev.h"
 
 namespace dh {
@@ -91,27 +93,47 @@ __device__ __forceinline__ Prod make_prod(const StepArgs &a, int c, int p, int J
   return q;
 }
 
+// The samples a form reads (planes s[0..n-1] at one offset), loaded apart
+// from the arithmetic so that a caller can issue many loads ahead of it.
 template <int KIND>
-__device__ __forceinline__ double2 eval_prod(const Prod &q, long long off) {
-  auto ld = [&](int i) { return *reinterpret_cast<const double2 *>(q.s[i] + off); };
+struct ProdRaw {
+  static constexpr int n = (KIND == 1) ? 1 : (KIND == 0) ? 2 : (KIND == 2) ? 5 : 3;
+  double2 s[n];
+};
+
+template <int KIND>
+__device__ __forceinline__ ProdRaw<KIND> fetch_prod(const Prod &q, long long off) {
+  ProdRaw<KIND> r;
+#pragma unroll
+  for (int i = 0; i < ProdRaw<KIND>::n; ++i) r.s[i] = *reinterpret_cast<const double2 *>(q.s[i] + off);
+  return r;
+}
+
+template <int KIND>
+__device__ __forceinline__ double2 eval_raw(const Prod &q, const ProdRaw<KIND> &r) {
   if constexpr (KIND == 0) {
-    const double2 u = ld(0), v = ld(1);
+    const double2 u = r.s[0], v = r.s[1];
     return make_double2(u.x * v.x, u.y * v.y);
   } else if constexpr (KIND == 1) {
-    const double2 u = ld(0);
+    const double2 u = r.s[0];
     return make_double2(u.x * u.x, u.y * u.y);
   } else if constexpr (KIND == 2) {
-    const double2 up = ld(0), u0 = ld(1), um = ld(2), vp = ld(3), v0 = ld(4);
+    const double2 up = r.s[0], u0 = r.s[1], um = r.s[2], vp = r.s[3], v0 = r.s[4];
     return make_double2((up.x * vp.x + u0.x * vp.x - u0.x * v0.x - um.x * v0.x) / q.w0,
                         (up.y * vp.y + u0.y * vp.y - u0.y * v0.y - um.y * v0.y) / q.w0);
   } else if constexpr (KIND == 3) {
-    const double2 u0 = ld(0), um = ld(1), v = ld(2);
+    const double2 u0 = r.s[0], um = r.s[1], v = r.s[2];
     return make_double2(((q.w0 * u0.x + q.w1 * um.x) / q.w2) * v.x, ((q.w0 * u0.y + q.w1 * um.y) / q.w2) * v.y);
   } else {
-    const double2 vp = ld(0), v0 = ld(1), vm = ld(2);
+    const double2 vp = r.s[0], v0 = r.s[1], vm = r.s[2];
     const double ax = v0.x + vp.x, bx = v0.x + vm.x, ay = v0.y + vp.y, by = v0.y + vm.y;
     return make_double2((0.25 * (ax * ax) - 0.25 * (bx * bx)) / q.w0, (0.25 * (ay * ay) - 0.25 * (by * by)) / q.w0);
   }
+}
+
+template <int KIND>
+__device__ __forceinline__ double2 eval_prod(const Prod &q, long long off) {
+  return eval_raw<KIND>(q, fetch_prod<KIND>(q, off));
 }
 
 __device__ __forceinline__ double2 eval_prod(const Prod &q, long long off) {
@@ -168,6 +190,67 @@ __device__ __forceinline__ double2 cf_init(const StepArgs &a, int c, int J, int 
   }
 }
 
+// cf_init for the NQ modes m = threadIdx.x + q nthr of one thread.  For the
+// velocity components without scalars and with BETA = 1 (the channel
+// configuration) every load is issued before the arithmetic, so the modes'
+// memory latencies overlap; other configurations take cf_init mode by mode.
+// Indices beyond the plane are clamped for the loads and their results
+// never stored.
+template <int NQ>
+__device__ __forceinline__ void cf_init_all(const StepArgs &a, int c, int J, int nthr, double2 (&acc)[NQ]) {
+  const Geo &g = a.g;
+  if (c >= 3 || a.n_th > 0 || a.beta != 1.0) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int m = threadIdx.x + q * nthr;
+      if (m < g.pf) {
+        const int kzp = (int)fdiv((unsigned)m, g.f_nkx1), kx = m - kzp * g.nkx1;
+        acc[q] = cf_init(a, c, J, kx, kzp, m);
+      }
+    }
+    return;
+  }
+  const double2 *U = a.cu[c], *W = a.cu[c == 0 ? 2 : 0];
+  const long long base = (long long)J * g.pf;
+  double2 u[NQ], w[NQ];
+  double k2[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    const int m = min((int)threadIdx.x + q * nthr, g.pf - 1);
+    const int kzp = (int)fdiv((unsigned)m, g.f_nkx1), kx = m - kzp * g.nkx1;
+    u[q] = U[base + m];
+    k2[q] = g.kx2[kx] + g.kz2[kzp];
+  }
+  if (c != 1) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) w[q] = W[base + min((int)threadIdx.x + q * nthr, g.pf - 1)];
+  }
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    const double f = a.nu * k2[q];  // kpow(k2, 1) = k2
+    double2 cf = make_double2(-(f * u[q].x), -(f * u[q].y));
+    if (c == 0) {
+      cf = make_double2(cf.x + a.i_ro * w[q].x, cf.y + a.i_ro * w[q].y);
+    } else if (c == 2) {
+      cf = make_double2(cf.x - a.i_ro * w[q].x, cf.y - a.i_ro * w[q].y);
+    }
+    acc[q] = cf;
+  }
+}
+
+// Forward row transforms of product pr, its form resolved once so that each
+// slab's loads are straight-line code ahead of the arithmetic.
+template <class Fn>
+__device__ __forceinline__ void with_prod_kind(int kind, Fn fn) {
+  switch (kind) {
+    case 0: fn(std::integral_constant<int, 0>{}); break;
+    case 1: fn(std::integral_constant<int, 1>{}); break;
+    case 2: fn(std::integral_constant<int, 2>{}); break;
+    case 3: fn(std::integral_constant<int, 3>{}); break;
+    default: fn(std::integral_constant<int, 4>{}); break;
+  }
+}
+
 // Fused explicit pass for plane J (1..NY) and component c (U1,U2,U3,TH...).
 // REG: row transforms in registers, 512 threads (twice the register budget
 // of the 1024-thread LDS variant), NQ Fourier modes owned per thread.
@@ -202,21 +285,19 @@ __global__ void __launch_bounds__(REG ? 512 : 1024) k_nonlinear(StepArgs a) {
   const int nthr = blockDim.x;
 
   double2 acc[NQ];
-#pragma unroll
-  for (int q = 0; q < NQ; ++q) {
-    const int m = threadIdx.x + q * nthr;
-    if (m < g.pf) {
-      const int kzp = (int)fdiv((unsigned)m, g.f_nkx1), kx = m - kzp * g.nkx1;
-      acc[q] = cf_init(a, c, J, kx, kzp, m);
-    }
-  }
+  cf_init_all<NQ>(a, c, J, nthr, acc);
   // three products, forward transformed and accumulated in statement order
   const LoadTile lt(g);
   const int nw = nthr >> 6;
   for (int p = 0; p < 3; ++p) {
     const Prod pr = make_prod(a, c, p, J);
     if constexpr (REG) {
-      rows_fwd_reg64(A, g, T, [&](int z, int xp) { return eval_prod(pr, (long long)z * g.nx + 2 * xp); });
+      with_prod_kind(pr.kind, [&](auto K) {
+        constexpr int KIND = decltype(K)::value;
+        rows_fwd_reg64(
+            A, g, T, [&](int z, int xp) { return fetch_prod<KIND>(pr, (long long)z * g.nx + 2 * xp); },
+            [&](const ProdRaw<KIND> &r) { return eval_raw<KIND>(pr, r); });
+      });
     } else {
       for (int t0 = threadIdx.x >> 6; t0 < lt.ntiles; t0 += nw) {
         int z, xp;
@@ -226,7 +307,18 @@ __global__ void __launch_bounds__(REG ? 512 : 1024) k_nonlinear(StepArgs a) {
       plane_rows_fwd(A, g, T);
     }
     plane_cols_fwd(A, g, T);
-    // derivative factor: c==0 takes (ikz, ikx, 1); all others (ikx, ikz, 1)
+    // derivative factor: c==0 takes (ikz, ikx, 1); all others (ikx, ikz, 1).
+    // The wavenumbers of all modes are loaded ahead of the LDS reads.
+    const bool use_kz = (c == 0) ? (p == 0) : (p == 1);
+    double kq[NQ];
+    if (p != 2) {
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        const int m = min((int)threadIdx.x + q * nthr, g.pf - 1);
+        const int kzp = (int)fdiv((unsigned)m, g.f_nkx1), kx = m - kzp * g.nkx1;
+        kq[q] = use_kz ? g.kz[kzp] : g.kx[kx];
+      }
+    }
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
       const int m = threadIdx.x + q * nthr;
@@ -237,9 +329,7 @@ __global__ void __launch_bounds__(REG ? 512 : 1024) k_nonlinear(StepArgs a) {
         if (p == 2) {
           acc[q] = csub(acc[q], s);
         } else {
-          const bool use_kz = (c == 0) ? (p == 0) : (p == 1);
-          const double k = use_kz ? g.kz[kzp] : g.kx[kx];
-          acc[q] = csub(acc[q], cmul_ik(k, s));
+          acc[q] = csub(acc[q], cmul_ik(kq[q], s));
         }
       }
     }
@@ -269,40 +359,69 @@ __global__ void __launch_bounds__(REG ? 512 : 1024) k_nonlinear(StepArgs a) {
     ey1 = g.dy[J];
     ey2 = g.dyf[J];
   }
+  // modes in batches of NB: the batch's loads (indices clamped to the plane)
+  // are all issued before its arithmetic (smaller batches in the
+  // 1024-thread variant, whose register budget is half as large)
+  constexpr int NB = REG ? 4 : 2;
+  const double dyJ = g.dy[J];
+  static_assert(NQ % NB == 0, "mode batches must tile the modes of a thread");
 #pragma unroll
-  for (int q = 0; q < NQ; ++q) {
-    const int m = threadIdx.x + q * nthr;
-    if (m < g.pf) {
-      const int kzp = (int)fdiv((unsigned)m, g.f_nkx1), kx = m - kzp * g.nkx1;
-      const long long o = base + m;
-      const double2 cm1 = CC[o - g.pf], cp1 = CC[o + g.pf];
-      double2 cr = CC[o];
-      const double2 ev = make_double2(coefv * (((cp1.x - cr.x) / ey0 - (cr.x - cm1.x) / ey1) / ey2),
-                                      coefv * (((cp1.y - cr.y) / ey0 - (cr.y - cm1.y) / ey1) / ey2));
-      if (rk > 0) {
-        double2 old = CF[o];
-        cr = make_double2(cr.x + T3 * old.x, cr.y + T3 * old.y);
-      }
+  for (int q0 = 0; q0 < NQ; q0 += NB) {
+    double2 cm1[NB], cp1[NB], cc0[NB], old[NB], p0[NB], pm[NB];
+    double kk[NB];  // KX (U1) or KZ (U3) of the pressure gradient
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const int mc = min((int)threadIdx.x + (q0 + b) * nthr, g.pf - 1);
+      const long long o = base + mc;
+      cm1[b] = CC[o - g.pf];
+      cp1[b] = CC[o + g.pf];
+      cc0[b] = CC[o];
       if (c == 0 || c == 2) {
-        const double k = (c == 0) ? g.kx[kx] : g.kz[kzp];
-        double2 ikp = cmul_ik(k, a.cp[o]);
-        cr = make_double2(cr.x - T4 * ikp.x, cr.y - T4 * ikp.y);
-        if (c == 0 && kx == 0 && kzp == 0) {
-          if (a.f_type == 1)
-            cr.x = cr.x - T4 * a.px0;
-          else if (a.f_type == 2)
-            cr.x = cr.x - T4 * (a.px0 + a.amp_omega0 * cos(a.omega0 * time));
-        }
-      } else if (c == 1) {
-        double2 p0 = a.cp[o], pm = a.cp[o - g.pf];
-        double2 cs = make_double2((p0.x - pm.x) / g.dy[J], (p0.y - pm.y) / g.dy[J]);
-        cr = make_double2(cr.x - T4 * cs.x, cr.y - T4 * cs.y);
+        const int kzp = (int)fdiv((unsigned)mc, g.f_nkx1), kx = mc - kzp * g.nkx1;
+        kk[b] = (c == 0) ? g.kx[kx] : g.kz[kzp];
       }
-      const double2 cfn = acc[q];
-      CF[o] = cfn;
-      cr = make_double2(cr.x + T5 * cfn.x, cr.y + T5 * cfn.y);
-      cr = make_double2(cr.x + ev.x, cr.y + ev.y);
-      A[inv_row(T, kz_row(g, kzp)) * g.ls + kx] = cr;
+    }
+    if (rk > 0) {
+#pragma unroll
+      for (int b = 0; b < NB; ++b) old[b] = CF[base + min((int)threadIdx.x + (q0 + b) * nthr, g.pf - 1)];
+    }
+    if (c < 3) {
+#pragma unroll
+      for (int b = 0; b < NB; ++b) p0[b] = a.cp[base + min((int)threadIdx.x + (q0 + b) * nthr, g.pf - 1)];
+    }
+    if (c == 1) {
+#pragma unroll
+      for (int b = 0; b < NB; ++b) pm[b] = a.cp[base - g.pf + min((int)threadIdx.x + (q0 + b) * nthr, g.pf - 1)];
+    }
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const int m = threadIdx.x + (q0 + b) * nthr;
+      if (m < g.pf) {
+        const int kzp = (int)fdiv((unsigned)m, g.f_nkx1), kx = m - kzp * g.nkx1;
+        const long long o = base + m;
+        double2 cr = cc0[b];
+        const double2 ev = make_double2(coefv * (((cp1[b].x - cr.x) / ey0 - (cr.x - cm1[b].x) / ey1) / ey2),
+                                        coefv * (((cp1[b].y - cr.y) / ey0 - (cr.y - cm1[b].y) / ey1) / ey2));
+        if (rk > 0) cr = make_double2(cr.x + T3 * old[b].x, cr.y + T3 * old[b].y);
+        if (c == 0 || c == 2) {
+          double2 ikp = cmul_ik(kk[b], p0[b]);
+          cr = make_double2(cr.x - T4 * ikp.x, cr.y - T4 * ikp.y);
+          if (c == 0 && kx == 0 && kzp == 0) {
+            if (a.f_type == 1)
+              cr.x = cr.x - T4 * a.px0;
+            else if (a.f_type == 2)
+              cr.x = cr.x - T4 * (a.px0 + a.amp_omega0 * cos(a.omega0 * time));
+          }
+        } else if (c == 1) {
+          double2 cs = make_double2((p0[b].x - pm[b].x) / dyJ, (p0[b].y - pm[b].y) / dyJ);
+          cr = make_double2(cr.x - T4 * cs.x, cr.y - T4 * cs.y);
+        }
+        const double2 cfn = acc[q0 + b];
+        CF[o] = cfn;
+        cr = make_double2(cr.x + T5 * cfn.x, cr.y + T5 * cfn.y);
+        cr = make_double2(cr.x + ev.x, cr.y + ev.y);
+        A[inv_row(T, kz_row(g, kzp)) * g.ls + kx] = cr;
+      }
     }
   }
   lds_barrier();
