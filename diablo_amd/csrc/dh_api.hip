@@ -107,7 +107,7 @@ hipError_t dalloc(T **p, size_t count) {
 void free_all(Ctx &c) {
   void *ptrs[] = {c.tw_row, c.tw_col, c.tw_real, c.posr, c.posrd, c.posz, c.posc, c.kx, c.kz, c.kx2, c.kz2, c.d_dy, c.d_dyf, c.d_gyf, c.cu[0], c.cu[1],
                   c.cu[2], c.cp, c.cf[0], c.cf[1], c.cf[2], c.cth, c.cfth, c.phi, c.uphys, c.rphys, c.rd_ta,
-                  c.rd_tb, c.clock, c.fac.pf, c.fac.pb, c.fac.tl, c.fac.tb, c.fac.tc, c.fac.tr, c.fac.isbc, c.rd_tr, c.fac.vconst, c.rd_l, c.rd_u, c.stats};
+                  c.rd_tb, c.clock, c.fac.pf, c.fac.pb, c.fac.tl, c.fac.tb, c.fac.tc, c.fac.tr, c.fac.isbc, c.rd_tr, c.fac.vconst, c.rd_l, c.rd_u, c.stats, c.cun[0], c.cun[1], c.cun[2]};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c.step_graph) (void)hipGraphExecDestroy(c.step_graph);
@@ -115,6 +115,9 @@ void free_all(Ctx &c) {
   if (c.side) (void)hipStreamDestroy(c.side);
   if (c.ev_fork) (void)hipEventDestroy(c.ev_fork);
   if (c.ev_join) (void)hipEventDestroy(c.ev_join);
+  if (c.branch) (void)hipStreamDestroy(c.branch);
+  if (c.ev_nl01) (void)hipEventDestroy(c.ev_nl01);
+  if (c.ev_branch_done) (void)hipEventDestroy(c.ev_branch_done);
 }
 
 // courant.f:15-31: the part of the time-step bound independent of U.
@@ -245,6 +248,9 @@ int dh_create(const dh_params *p, dh_ctx **out) {
   ACHK(hipStreamCreateWithFlags(&c.side, hipStreamNonBlocking));
   ACHK(hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming));
   ACHK(hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming));
+  ACHK(hipStreamCreateWithFlags(&c.branch, hipStreamNonBlocking));
+  ACHK(hipEventCreateWithFlags(&c.ev_nl01, hipEventDisableTiming));
+  ACHK(hipEventCreateWithFlags(&c.ev_branch_done, hipEventDisableTiming));
   const size_t nyy = (size_t)g.ny + 2;
   const size_t cf_elems = nyy * g.pf;
   const size_t ph_elems = nyy * (size_t)g.pp;
@@ -268,6 +274,7 @@ int dh_create(const dh_params *p, dh_ctx **out) {
   }
   ACHK(dalloc(&c.cp, cf_elems));
   ACHK(dalloc(&c.phi, cf_elems));
+  for (int i = 0; i < 3; ++i) ACHK(dalloc(&c.cun[i], cf_elems));
   ACHK(dalloc(&c.cth, cf_elems * (p->n_th > 0 ? p->n_th : 1)));
   ACHK(dalloc(&c.cfth, cf_elems * (p->n_th > 0 ? p->n_th : 1)));
   ACHK(dalloc(&c.uphys, ph_elems * c.nfields));

@@ -11,6 +11,9 @@ struct PlaneBatch {
   const double2 *four[kMaxFields];  // Fourier source/destination per field
   double *phys[kMaxFields];         // physical destination/source per field
   int j0;                           // first plane
+  // forward only: planes alo..ahi of field f come from alt[f] when set
+  const double *alt[kMaxFields];
+  int alo[kMaxFields], ahi[kMaxFields];
 };
 
 // Fourier -> physical for planes j0 + blockIdx.x of field blockIdx.y.
@@ -39,7 +42,8 @@ __global__ void __launch_bounds__(1024) k_plane_fwd(Geo g, PlaneBatch b) {
   const Tables T = load_tables(A, g);
   const int f = blockIdx.y;
   const long long j = b.j0 + blockIdx.x;
-  const double2 *src = reinterpret_cast<const double2 *>(b.phys[f] + j * g.pp);
+  const double *ph = (b.alt[f] && j >= b.alo[f] && j <= b.ahi[f]) ? b.alt[f] : b.phys[f];
+  const double2 *src = reinterpret_cast<const double2 *>(ph + j * g.pp);
   if constexpr (REG) {
     rows_fwd_reg64(A, g, T, [&](int z, int m) { return src[z * 64 + m]; });
   } else {
@@ -125,6 +129,36 @@ hipError_t launch_fields_to_four(Ctx &c, int f0, int f1) {
 }
 
 hipError_t launch_state_to_four(Ctx &c) { return launch_fields_to_four(c, 0, c.nfields); }
+
+// Fields f0..f1-1 back to Fourier space at the end of a substep
+// (channel.f:1043-1050): the solve left planes wlo..whi of a velocity field
+// in its R array (the other planes keep the values transformed at the start
+// of the substep); velocity goes to CUN, for REM_DIV to project into CU, and
+// scalars straight to CTH.
+hipError_t launch_step_to_four(Ctx &c, int f0, int f1, hipStream_t st) {
+  if (f1 <= f0) return hipSuccess;
+  PlaneBatch b{};
+  const long long fs = (long long)(c.g.ny + 2) * c.g.pp;
+  const long long cfs = (long long)(c.g.ny + 2) * c.g.pf;
+  for (int f = f0; f < f1; ++f) {
+    const int i = f - f0;
+    b.phys[i] = c.uphys + f * fs;
+    if (f < 3) {
+      b.four[i] = c.cun[f];
+      b.alt[i] = c.rphys + f * fs;
+      b.alo[i] = c.wlo[f];
+      b.ahi[i] = c.whi[f];
+    } else {
+      b.four[i] = c.cth + (f - 3) * cfs;
+    }
+  }
+  b.j0 = 0;
+  if (c.reg_rows)
+    hipLaunchKernelGGL(k_plane_fwd<true>, dim3(c.g.ny + 2, f1 - f0), dim3(c.plane_threads), c.lds_bytes, st, c.g, b);
+  else
+    hipLaunchKernelGGL(k_plane_fwd<false>, dim3(c.g.ny + 2, f1 - f0), dim3(c.plane_threads), c.lds_bytes, st, c.g, b);
+  return hipGetLastError();
+}
 
 hipError_t prepare_fft_kernels(Ctx &c) {
   const void *fns[] = {(const void *)k_plane_inv<true>, (const void *)k_plane_inv<false>,

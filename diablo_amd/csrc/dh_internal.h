@@ -99,6 +99,11 @@ struct Ctx {
   hipStream_t side = nullptr;          // table rebuild after COURANT, overlapped with the substep's tail
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   bool tables_pending = false;
+  // branch of a substep: the last velocity component's (and the scalars')
+  // nonlinear pass, solve and forward transform, beside the solve and
+  // transform of components 0 and 1
+  hipStream_t branch = nullptr;
+  hipEvent_t ev_nl01 = nullptr, ev_branch_done = nullptr;
   std::string err;
   bool grid_set = false;
   // host copies
@@ -111,6 +116,7 @@ struct Ctx {
   double *d_dy = nullptr, *d_dyf = nullptr, *d_gyf = nullptr;
   double2 *cu[3] = {}, *cp = nullptr, *cf[3] = {}, *cth = nullptr, *cfth = nullptr;
   double2 *phi = nullptr;       // CR1 after REM_DIV
+  double2 *cun[3] = {};         // velocity back in Fourier space before REM_DIV projects it into CU
   double *uphys = nullptr;      // [nfields][NY+2][NZ][NX]: U1,U2,U3,TH
   double *rphys = nullptr;      // [nfields][NY+2][NZ][NX]: R1,R2,R3,RTH
   double *rd_ta = nullptr;      // REM_DIV sweep multipliers [NY+2][pf]
@@ -133,6 +139,7 @@ hipError_t launch_fft_to_four(Ctx &c, const double *phys, double2 *four, int npl
 hipError_t launch_state_to_phys(Ctx &c);
 hipError_t launch_state_to_four(Ctx &c);
 hipError_t launch_fields_to_four(Ctx &c, int f0, int f1);
+hipError_t launch_step_to_four(Ctx &c, int f0, int f1, hipStream_t st);  // end of a substep (into CUN / CTH)
 // ---- launchers (dh_ics.hip) ----
 hipError_t launch_apply_bc_vel(Ctx &c);
 hipError_t launch_create_flow(Ctx &c, int ic_type, double ubulk0, unsigned long long seed, double amp, double ymask);

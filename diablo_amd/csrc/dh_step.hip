@@ -33,6 +33,7 @@ struct StepArgs {
   double *rphys;
   long long fs;   // physical field stride (NY+2)*NX*NZ
   long long cfs;  // Fourier field stride (NY+2)*pf
+  int c0, j0;     // component of blockIdx.y = 0 and plane of the first block row
   double nu, nu_v_scale, beta, i_ro, grav_x, grav_y, grav_z, px0, omega0, amp_omega0;
   int f_type, n_th;
   double ri[DH_MAX_TH], pr[DH_MAX_TH], drhodx[DH_MAX_TH], drhodz[DH_MAX_TH];
@@ -267,8 +268,9 @@ __global__ void __launch_bounds__(REG ? 512 : 1024) k_nonlinear(StepArgs a) {
     const int nblk = gridDim.x * gridDim.y, lin = blockIdx.x + blockIdx.y * gridDim.x;
     const int x = lin & 7, q = nblk >> 3, r = nblk & 7;
     const int w = x * q + min(x, r) + (lin >> 3);
-    J = __builtin_amdgcn_readfirstlane(w / (int)gridDim.y + 1);
-    c = __builtin_amdgcn_readfirstlane(w - (J - 1) * (int)gridDim.y);
+    const int jr = w / (int)gridDim.y;
+    J = __builtin_amdgcn_readfirstlane(a.j0 + jr);
+    c = __builtin_amdgcn_readfirstlane(a.c0 + w - jr * (int)gridDim.y);
   }
   int jlo = 2, jhi = g.ny;
   if (c >= 3) {
@@ -609,8 +611,9 @@ struct SolveArgs {
   Geo g;
   Clock *clk;
   int rk;
-  double *uphys;
+  double *dst[kMaxFields];  // solution of field f (may be the field's own R: each lane reads its rows first)
   const double *rphys;
+  int f0;                   // field of blockIdx.y = 0
   long long fs;
   RealFactors fac;
   int nfields;
@@ -631,7 +634,7 @@ __global__ void __launch_bounds__(kSegThreads) k_solve_seg(SolveArgs a) {
   __shared__ int IB[NS * SR];
   __shared__ double red[kSegThreads / 64];
   const Geo &g = a.g;
-  const int f = blockIdx.y;
+  const int f = a.f0 + blockIdx.y;
   const long long pp = g.pp;
   const int NY = g.ny, nrow = NY + 2;
   const int sr = a.fac.seg_rows;
@@ -654,7 +657,7 @@ __global__ void __launch_bounds__(kSegThreads) k_solve_seg(SolveArgs a) {
   const bool live = col < pp;
   const int r0 = seg * sr;
   const int nr = min(sr, max(nrow - r0, 0));
-  double *U = a.uphys + (long long)f * a.fs + col;
+  double *U = a.dst[f] + col;
   const double *R = a.rphys + (long long)f * a.fs + col;
   // right-hand sides: the boundary constant or R
   double v[SR];
@@ -779,7 +782,8 @@ struct RemDivArgs {
   Clock *clk;
   int rk;       // <0: no CP update
   int cp_old;   // 1: COURANT ran in this substep, divide by the saved H_BAR
-  double2 *cu[3];
+  const double2 *cin[3];  // velocity before the projection (may equal cu)
+  double2 *cu[3];         // projected velocity
   double2 *cp;
   double2 *phi;
   const double *rl, *ru;  // 1/(DY(J)DYF(J)), 1/(DY(J+1)DYF(J))
@@ -875,10 +879,10 @@ __global__ void __launch_bounds__(256) k_rem_div_thomas(RemDivArgs a) {
         tb[b] = a.tb[pf * J + m];
         tr[b] = a.tr[pf * J + m];
         if (J < NY) {
-          c1[b] = a.cu[0][pf * J + m];
-          c3[b] = a.cu[2][pf * J + m];
-          c2[b] = a.cu[1][pf * J + m];
-          c2p[b] = a.cu[1][pf * (J + 1) + m];
+          c1[b] = a.cin[0][pf * J + m];
+          c3[b] = a.cin[2][pf * J + m];
+          c2[b] = a.cin[1][pf * J + m];
+          c2p[b] = a.cin[1][pf * (J + 1) + m];
         }
       }
     }
@@ -984,9 +988,9 @@ __global__ void __launch_bounds__(256) k_rem_div_thomas(RemDivArgs a) {
     for (int b = 0; b < 4; ++b) {
       const int J = J0 + b * ngrp;
       if (J <= NY) {
-        u1[b] = a.cu[0][pf * J + m];
-        u3[b] = a.cu[2][pf * J + m];
-        u2[b] = a.cu[1][pf * J + m];
+        u1[b] = a.cin[0][pf * J + m];
+        u3[b] = a.cin[2][pf * J + m];
+        u2[b] = a.cin[1][pf * J + m];
         if (a.rk >= 0) pc[b] = a.cp[pf * J + m];
       }
     }
@@ -1007,6 +1011,18 @@ __global__ void __launch_bounds__(256) k_rem_div_thomas(RemDivArgs a) {
         if (a.rk >= 0) a.cp[pf * J + m] = make_double2(pc[b].x + phi.x / hb, pc[b].y + phi.y / hb);
       }
     }
+  }
+  // planes the projection leaves alone (the ghost planes 0 and NY+1, and
+  // plane 1 of CU2) pass from the transformed velocity unchanged
+  if (a.cin[0] != a.cu[0] && rg == 0) {
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const long long o = (long long)pf * (i ? NY + 1 : 0) + m;
+      a.cu[0][o] = a.cin[0][o];
+      a.cu[1][o] = a.cin[1][o];
+      a.cu[2][o] = a.cin[2][o];
+    }
+    a.cu[1][pf + m] = a.cin[1][pf + m];
   }
 }
 
@@ -1029,6 +1045,8 @@ static StepArgs make_step_args(Ctx &c, int rk) {
   a.rphys = c.rphys;
   a.fs = (long long)(c.g.ny + 2) * c.g.pp;
   a.cfs = (long long)(c.g.ny + 2) * c.g.pf;
+  a.c0 = 0;
+  a.j0 = 1;
   const dh_params &p = c.p;
   a.nu = p.nu;
   a.nu_v_scale = p.nu_v_scale;
@@ -1089,25 +1107,43 @@ hipError_t launch_cn_tables(Ctx &c, hipStream_t st) {
   return hipGetLastError();
 }
 
-static hipError_t launch_nonlinear(Ctx &c, int rk) {
+// Nonlinear pass of components c0..c1-1 on stream st: one block per plane
+// J of the components' ranges (JSTART..JEND, channel.f:77-626) and component.
+static hipError_t launch_nonlinear_range(Ctx &c, int rk, int c0, int c1, hipStream_t st) {
+  if (c1 <= c0) return hipSuccess;
   StepArgs a = make_step_args(c, rk);
+  int jlo = c.g.ny, jhi = 1;
+  for (int k = c0; k < c1; ++k) {
+    jlo = min(jlo, k < 3 ? 2 : c.jstart_th[k - 3]);
+    jhi = max(jhi, k < 3 ? c.g.ny : c.jend_th[k - 3]);
+  }
+  if (jhi < jlo) return hipSuccess;
+  a.c0 = c0;
+  a.j0 = jlo;
+  const dim3 grid(jhi - jlo + 1, c1 - c0);
   if (c.reg_rows_nl)
-    hipLaunchKernelGGL((k_nonlinear<true, kNlRegQ>), dim3(c.g.ny, c.nfields), dim3(kNlRegThreads), c.lds_bytes,
-                       c.stream, a);
+    hipLaunchKernelGGL((k_nonlinear<true, kNlRegQ>), grid, dim3(kNlRegThreads), c.lds_bytes, st, a);
   else
-    hipLaunchKernelGGL((k_nonlinear<false, kMaxQ>), dim3(c.g.ny, c.nfields), dim3(c.plane_threads), c.lds_bytes,
-                       c.stream, a);
+    hipLaunchKernelGGL((k_nonlinear<false, kMaxQ>), grid, dim3(c.plane_threads), c.lds_bytes, st, a);
   return hipGetLastError();
 }
 
-static hipError_t launch_solve(Ctx &c, int rk, int courant) {
+static hipError_t launch_nonlinear(Ctx &c, int rk) { return launch_nonlinear_range(c, rk, 0, c.nfields, c.stream); }
+
+// Crank-Nicolson solve of fields f0..f1-1 on stream st.  A velocity
+// field's solution overwrites its own R (other components' nonlinear
+// passes may still read the physical velocity); a scalar's goes to its
+// physical field, which only its own nonlinear pass reads.
+static hipError_t launch_solve_range(Ctx &c, int rk, int courant, int f0, int f1, hipStream_t st) {
+  if (f1 <= f0) return hipSuccess;
   SolveArgs a{};
   a.g = c.g;
   a.clk = c.clock;
   a.rk = rk;
-  a.uphys = c.uphys;
   a.rphys = c.rphys;
   a.fs = (long long)(c.g.ny + 2) * c.g.pp;
+  for (int f = 0; f < c.nfields; ++f) a.dst[f] = (f < 3 ? c.rphys : c.uphys) + f * a.fs;
+  a.f0 = f0;
   a.fac = c.fac;
   a.nfields = c.nfields;
   for (int f = 0; f < c.nfields; ++f) {
@@ -1121,10 +1157,14 @@ static hipError_t launch_solve(Ctx &c, int rk, int courant) {
   const int cols = (kSegThreads / 64) * (64 / c.fac.nseg);  // columns per block
   const int nb = (int)((c.g.pp + cols - 1) / cols);
   if (c.fac.nseg == 4)
-    hipLaunchKernelGGL(k_solve_seg<4>, dim3(nb, c.nfields), dim3(kSegThreads), 0, c.stream, a);
+    hipLaunchKernelGGL(k_solve_seg<4>, dim3(nb, f1 - f0), dim3(kSegThreads), 0, st, a);
   else
-    hipLaunchKernelGGL(k_solve_seg<8>, dim3(nb, c.nfields), dim3(kSegThreads), 0, c.stream, a);
+    hipLaunchKernelGGL(k_solve_seg<8>, dim3(nb, f1 - f0), dim3(kSegThreads), 0, st, a);
   return hipGetLastError();
+}
+
+static hipError_t launch_solve(Ctx &c, int rk, int courant) {
+  return launch_solve_range(c, rk, courant, 0, c.nfields, c.stream);
 }
 
 static size_t rem_div_thomas_lds(const Ctx &c) {
@@ -1138,7 +1178,10 @@ static RemDivArgs make_rem_div_args(Ctx &c, int rk) {
   a.clk = c.clock;
   a.rk = rk;
   a.cp_old = (rk == 2 && c.p.variable_dt) ? 1 : 0;
-  for (int i = 0; i < 3; ++i) a.cu[i] = c.cu[i];
+  for (int i = 0; i < 3; ++i) {
+    a.cin[i] = (rk >= 0) ? c.cun[i] : c.cu[i];  // a substep projects the transformed velocity
+    a.cu[i] = c.cu[i];
+  }
   a.cp = c.cp;
   a.phi = c.phi;
   a.rl = c.rd_l;
@@ -1189,7 +1232,7 @@ hipError_t launch_phase(Ctx &c, int rk, int phase) {
     case 1: return launch_nonlinear(c, rk);
     case 2: return launch_solve(c, rk, courant);
     case 3: return courant ? launch_courant(c) : hipSuccess;
-    case 4: return launch_state_to_four(c);
+    case 4: return launch_step_to_four(c, 0, c.nfields, c.stream);
     case 5: return launch_rem_div(c, rk);
   }
   return hipErrorInvalidValue;
@@ -1201,11 +1244,34 @@ hipError_t join_side(Ctx &c) {
   return hipStreamWaitEvent(c.stream, c.ev_join, 0);
 }
 
+// One substep.  The nonlinear pass of U1 and U2 fills one wave of blocks
+// (one per plane and component); the last velocity component's nonlinear
+// pass, solve and transform back (then the scalars' nonlinear pass and
+// solve) run as a branch beside the solve and transform of U1 and U2, which
+// use the compute units that pass leaves idle.  The scalars' transform back,
+// COURANT and REM_DIV follow on the context stream once the branch has
+// joined.  The kernels and their arithmetic are those of launch_phase.
 hipError_t launch_substep(Ctx &c, int rk) {
-  for (int ph = 0; ph < kNumPhases; ++ph) {
-    hipError_t e = launch_phase(c, rk, ph);
-    if (e != hipSuccess) return e;
-  }
+  const int courant = (c.p.variable_dt && rk == 2) ? 1 : 0;
+  const int nv = c.nfields < 3 ? c.nfields : 3;
+  hipError_t e;
+#define SCHK(expr) \
+  if ((e = (expr)) != hipSuccess) return e;
+  SCHK(launch_state_to_phys(c));
+  SCHK(launch_nonlinear_range(c, rk, 0, 2, c.stream));
+  SCHK(hipEventRecord(c.ev_nl01, c.stream));
+  SCHK(hipStreamWaitEvent(c.branch, c.ev_nl01, 0));
+  SCHK(launch_nonlinear_range(c, rk, 2, c.nfields, c.branch));
+  SCHK(launch_solve_range(c, rk, courant, 2, c.nfields, c.branch));
+  SCHK(launch_step_to_four(c, 2, nv, c.branch));
+  SCHK(hipEventRecord(c.ev_branch_done, c.branch));
+  SCHK(launch_solve_range(c, rk, courant, 0, 2, c.stream));
+  SCHK(launch_step_to_four(c, 0, 2, c.stream));
+  SCHK(hipStreamWaitEvent(c.stream, c.ev_branch_done, 0));
+  if (courant) SCHK(launch_courant(c));
+  SCHK(launch_step_to_four(c, 3, c.nfields, c.stream));
+  SCHK(launch_rem_div(c, rk));
+#undef SCHK
   return join_side(c);
 }
 
