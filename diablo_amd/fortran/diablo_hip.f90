@@ -90,6 +90,13 @@ module diablo_hip
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
     end function dh_rem_div
+    integer(c_int) function dh_save_stats(ctx, ly, out, ubulk) bind(C, name='dh_save_stats')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: ly
+      real(c_double), intent(out) :: out(*)
+      real(c_double), intent(out) :: ubulk
+    end function dh_save_stats
   end interface
 
 contains

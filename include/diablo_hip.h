@@ -167,8 +167,9 @@ int dh_fft_xz_to_physical_dev(dh_ctx *ctx, const double *four, double *phys, int
 
 /* Device pointer of one state field in the device layout, for tests and
  * benchmarks: which = 0..2 CU1..3, 3 CP, 4..6 CF1..3, 7 phi (CR1 after
- * REM_DIV), 8 + n CTH(n), 8 + N_TH + n CFTH(n); 100+c physical U(c) /
- * TH (c>=3) as left by the last substep's solve. */
+ * REM_DIV), 8 + n CTH(n), 8 + N_TH + n CFTH(n); 100+c the physical
+ * field c: for c < 3 the velocity transformed at the start of the last
+ * substep (its solve writes over R), for c >= 3 TH as left by its solve. */
 double *dh_field_dev(dh_ctx *ctx, int32_t which);
 /* HIP stream (hipStream_t) the context runs on. */
 void *dh_stream(dh_ctx *ctx);
