@@ -837,7 +837,7 @@ __global__ void __launch_bounds__(256) k_rem_div_factors(RemDivArgs a) {
 // and above rows 0..NY), so nothing in the recurrence waits on LDS.
 constexpr int kRdtModes = 8;
 constexpr int kRdtAhead = 4;
-constexpr int kRdtPad = 8;  // >= 2 * kRdtAhead
+constexpr int kRdtPad = 16;  // >= 4 * kRdtAhead - 1: the sweeps read up to 4 kRdtAhead - 1 rows past their range
 
 __global__ void __launch_bounds__(256) k_rem_div_thomas(RemDivArgs a) {
 #pragma clang fp contract(off)
@@ -905,76 +905,89 @@ __global__ void __launch_bounds__(256) k_rem_div_thomas(RemDivArgs a) {
   }
   lds_barrier();
   if (rg == 0) {
-    // forward: G_J = V_J + A_J G_{J-1}, J = 2..NY, G_1 = 0
+    // forward: G_J = V_J + A_J G_{J-1}, J = 2..NY, G_1 = 0.  Two register
+    // sets of kRdtAhead rows alternate: one set's rows are computed while
+    // the other's LDS reads are in flight (a rotation of one set would make
+    // every read wait before the first row that follows it).
     double2 gp = make_double2(0.0, 0.0), gny = gp;
-    double2 vc[kRdtAhead], vn[kRdtAhead];
-    double ac[kRdtAhead], an[kRdtAhead];
-#pragma unroll
-    for (int q = 0; q < kRdtAhead; ++q) {
-      vc[q] = Gs[(2 + q) * kRdtModes + lane];
-      ac[q] = As[(2 + q) * kRdtModes + lane];
-    }
-    for (int J0 = 2; J0 <= NY; J0 += kRdtAhead) {
+    double2 va[kRdtAhead], vb[kRdtAhead];
+    double aa[kRdtAhead], ab[kRdtAhead];
+    // the empty asm after each read group keeps the compiler from sinking the
+    // reads to their first use in the next group
+    auto fwd_load = [&](double2(&v)[kRdtAhead], double(&ac)[kRdtAhead], int J0) {
 #pragma unroll
       for (int q = 0; q < kRdtAhead; ++q) {
-        vn[q] = Gs[(J0 + kRdtAhead + q) * kRdtModes + lane];
-        an[q] = As[(J0 + kRdtAhead + q) * kRdtModes + lane];
+        v[q] = Gs[(J0 + q) * kRdtModes + lane];
+        ac[q] = As[(J0 + q) * kRdtModes + lane];
       }
+      asm volatile("" ::: "memory");
+    };
+    auto fwd_rows = [&](const double2(&v)[kRdtAhead], const double(&ac)[kRdtAhead], int J0) {
 #pragma unroll
       for (int q = 0; q < kRdtAhead; ++q) {
         const int J = J0 + q;
-        gp = make_double2(vc[q].x + ac[q] * gp.x, vc[q].y + ac[q] * gp.y);
+        gp = make_double2(v[q].x + ac[q] * gp.x, v[q].y + ac[q] * gp.y);
         Gs[J * kRdtModes + lane] = gp;  // rows above NY land in the padding
         // component-wise selects: a select of double2 values goes through scratch
         gny.x = (J == NY) ? gp.x : gny.x;
         gny.y = (J == NY) ? gp.y : gny.y;
-        vc[q] = vn[q];
-        ac[q] = an[q];
       }
+    };
+    fwd_load(va, aa, 2);
+    fwd_load(vb, ab, 2 + kRdtAhead);
+    for (int J0 = 2; J0 <= NY; J0 += 2 * kRdtAhead) {
+      fwd_rows(va, aa, J0);
+      fwd_load(va, aa, J0 + 2 * kRdtAhead);
+      fwd_rows(vb, ab, J0 + kRdtAhead);
+      fwd_load(vb, ab, J0 + 3 * kRdtAhead);
     }
     // backward: X_NY = G_NY / B_NY; X_J = (G_J - U_J X_{J+1}) / B_J,
-    // J = NY-1..2; row 1: B=1, G=0, U = 0 (mean mode) or -1
+    // J = NY-1..2; row 1: B=1, G=0, U = 0 (mean mode) or -1.  The rows the
+    // forward sweep wrote are complete before these reads (in-order LDS of
+    // one lane).
     const double bny = Bs[NY * kRdtModes + lane];
     double2 x;
     x.x = (bny != 0.0) ? gny.x / bny : 0.0;
     x.y = (bny != 0.0) ? gny.y / bny : 0.0;
     Gs[NY * kRdtModes + lane] = x;
     double2 x2 = x;
-    double2 wc[kRdtAhead], wn[kRdtAhead];
-    double bc[kRdtAhead], bn[kRdtAhead], rc[kRdtAhead], rn[kRdtAhead], uc[kRdtAhead], un[kRdtAhead];
-#pragma unroll
-    for (int q = 0; q < kRdtAhead; ++q) {
-      wc[q] = Gs[(NY - 1 - q) * kRdtModes + lane];
-      bc[q] = Bs[(NY - 1 - q) * kRdtModes + lane];
-      rc[q] = Rs[(NY - 1 - q) * kRdtModes + lane];
-      uc[q] = ru[NY - 1 - q];
-    }
-    for (int J0 = NY - 1; J0 >= 2; J0 -= kRdtAhead) {
-#pragma unroll
-      for (int q = 0; q < kRdtAhead; ++q) {
-        const int J = J0 - kRdtAhead - q;
-        wn[q] = Gs[J * kRdtModes + lane];
-        bn[q] = Bs[J * kRdtModes + lane];
-        rn[q] = Rs[J * kRdtModes + lane];
-        un[q] = ru[J];
-      }
+    double2 wa[kRdtAhead], wb[kRdtAhead];
+    double ba[kRdtAhead], bb[kRdtAhead], ra[kRdtAhead], rb[kRdtAhead], ua[kRdtAhead], ub[kRdtAhead];
+    auto bwd_load = [&](double2(&w)[kRdtAhead], double(&b)[kRdtAhead], double(&r)[kRdtAhead],
+                        double(&u)[kRdtAhead], int J0) {
 #pragma unroll
       for (int q = 0; q < kRdtAhead; ++q) {
         const int J = J0 - q;
-        const double B = bc[q], U = uc[q];
+        w[q] = Gs[J * kRdtModes + lane];
+        b[q] = Bs[J * kRdtModes + lane];
+        r[q] = Rs[J * kRdtModes + lane];
+        u[q] = ru[J];
+      }
+      asm volatile("" ::: "memory");
+    };
+    auto bwd_rows = [&](const double2(&w)[kRdtAhead], const double(&b)[kRdtAhead], const double(&r)[kRdtAhead],
+                        const double(&u)[kRdtAhead], int J0) {
+#pragma unroll
+      for (int q = 0; q < kRdtAhead; ++q) {
+        const int J = J0 - q;
+        const double B = b[q], U = u[q];
         // (G_J - U_J X_{J+1}) / B_J, quotients formed from the tabulated 1/B_J
-        const double xr = div_by_recip(wc[q].x - U * x.x, B, rc[q]);
-        const double xi = div_by_recip(wc[q].y - U * x.y, B, rc[q]);
+        const double xr = div_by_recip(w[q].x - U * x.x, B, r[q]);
+        const double xi = div_by_recip(w[q].y - U * x.y, B, r[q]);
         x.x = (B != 0.0) ? xr : 0.0;
         x.y = (B != 0.0) ? xi : 0.0;
         Gs[max(J, 1) * kRdtModes + lane] = x;  // row 1 is rewritten below
         x2.x = (J == 2) ? x.x : x2.x;
         x2.y = (J == 2) ? x.y : x2.y;
-        wc[q] = wn[q];
-        bc[q] = bn[q];
-        rc[q] = rn[q];
-        uc[q] = un[q];
       }
+    };
+    bwd_load(wa, ba, ra, ua, NY - 1);
+    bwd_load(wb, bb, rb, ub, NY - 1 - kRdtAhead);
+    for (int J0 = NY - 1; J0 >= 2; J0 -= 2 * kRdtAhead) {
+      bwd_rows(wa, ba, ra, ua, J0);
+      bwd_load(wa, ba, ra, ua, J0 - 2 * kRdtAhead);
+      bwd_rows(wb, bb, rb, ub, J0 - kRdtAhead);
+      bwd_load(wb, bb, rb, ub, J0 - 3 * kRdtAhead);
     }
     const double U1 = mean ? 0.0 : -1.0;
     Gs[1 * kRdtModes + lane] = make_double2((0.0 - U1 * x2.x) / 1.0, (0.0 - U1 * x2.y) / 1.0);
