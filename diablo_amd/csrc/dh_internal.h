@@ -72,6 +72,10 @@ struct Clock {
 // Rows of one column held by one lane of the Crank-Nicolson solve; a column
 // is split into 4 (NY+2 <= 144) or 8 such segments.
 constexpr int kSegRows = 36;
+// Segments of one (kx,kz) mode's pressure-Poisson sweep over rows 2..NY, one
+// lane each, and the most rows a segment holds in registers.
+constexpr int kRdsSeg = 8;
+constexpr int kRdsRows = 17;
 
 // Tables of the y-implicit (Crank-Nicolson) matrices, one set per RK
 // substep and field; with LES off each matrix depends only on j, so every
@@ -122,6 +126,9 @@ struct Ctx {
   double *rd_ta = nullptr;      // REM_DIV sweep multipliers [NY+2][pf]
   double *rd_tb = nullptr;      // REM_DIV sweep pivots      [NY+2][pf]
   double *rd_tr = nullptr;      // REM_DIV 1/pivots          [NY+2][pf]
+  double *rd_pf = nullptr;      // REM_DIV segment carry factors, forward  [NY+2][pf]
+  double *rd_pb = nullptr;      // REM_DIV segment carry factors, backward [NY+2][pf]
+  int rd_seg = 0;               // rows per REM_DIV sweep segment (0: one sequential sweep per mode)
   double *stats = nullptr;      // dh_save_stats profiles [DH_NSTATS][NY+2]
   Clock *clock = nullptr;
   RealFactors fac{};
