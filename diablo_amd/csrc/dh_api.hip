@@ -43,11 +43,11 @@ void digrev(int n, const int *rad, int nst, std::vector<int> &idx_of_slot) {
   for (int s = 0; s < n; ++s) idx_of_slot[s] = s / S + R * sub[s % S];
 }
 
-bool make_plan(int n, FftPlan &p, std::vector<int> &dit_in_slot, std::vector<int> &dif_out_slot) {
+bool make_plan(int n, FftPlan &p, std::vector<int> &dit_in_slot, std::vector<int> &dif_out_slot, bool radix16) {
   int rad[kMaxStages];
   int nst = 0, m = n;
-  for (int R : {8, 4, 2, 3})
-    while (m % R == 0 && nst < kMaxStages) {
+  for (int R : {16, 8, 4, 2, 3})
+    while (m % R == 0 && nst < kMaxStages && (R != 16 || radix16)) {
       rad[nst++] = R;
       m /= R;
     }
@@ -193,7 +193,9 @@ int dh_create(const dh_params *p, dh_ctx **out) {
   g.pp = (long long)p->nx * p->nz;
   g.ls = g.lw + 1;
   std::vector<int> posr, posrd, posc, posz;
-  if (!make_plan(g.lw, g.prow, posr, posrd) || !make_plan(g.nz, g.pcol, posc, posz))
+  // radix 16 only along z: its 16-point butterflies cost registers that the
+  // row transforms of the 1024-thread kernels do not have
+  if (!make_plan(g.lw, g.prow, posr, posrd, false) || !make_plan(g.nz, g.pcol, posc, posz, true))
     return reject("NX/2 and NZ must factor into 2 and 3");
   c.nfields = 3 + p->n_th;
   c.lds_bytes = g.nz * g.ls * (int)sizeof(double2) + (2 * g.lw + g.nz + 1) * (int)sizeof(double2) +

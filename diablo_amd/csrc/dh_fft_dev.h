@@ -116,6 +116,39 @@ __device__ __forceinline__ void dft_small(double2 *v) {
     v[6] = csub(e[2], o2);
     v[3] = cadd(e[3], o3);
     v[7] = csub(e[3], o3);
+  } else if constexpr (R == 16) {
+    // 16 = 4 x 4: DFT4 over n1 of x[4 n1 + n2], twiddle W16^{n2 k1}, DFT4
+    // over n2; output X[k1 + 4 k2] in natural order
+    double2 y[4][4];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2) {
+      double2 t[4] = {v[n2], v[4 + n2], v[8 + n2], v[12 + n2]};
+      dft_small<4, INV>(t);
+#pragma unroll
+      for (int k1 = 0; k1 < 4; ++k1) y[n2][k1] = t[k1];
+    }
+    const double c1 = 0.92387953251128675613, s1 = 0.38268343236508977173, h = 0.70710678118654752440;
+    // W16^j = cos(2 pi j/16) -/+ i sin(2 pi j/16) for j = 1, 2, 3, 4, 6, 9
+    const double sg = INV ? 1.0 : -1.0;
+    auto tw = [&](double2 a, double c, double sn) {  // a * (c + i sg sn)
+      return make_double2(a.x * c - a.y * (sg * sn), a.x * (sg * sn) + a.y * c);
+    };
+    y[1][1] = tw(y[1][1], c1, s1);
+    y[1][2] = tw(y[1][2], h, h);
+    y[1][3] = tw(y[1][3], s1, c1);
+    y[2][1] = tw(y[2][1], h, h);
+    y[2][2] = cmul_mi<INV>(y[2][2]);
+    y[2][3] = tw(y[2][3], -h, h);
+    y[3][1] = tw(y[3][1], s1, c1);
+    y[3][2] = tw(y[3][2], -h, h);
+    y[3][3] = tw(y[3][3], -c1, -s1);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+      double2 t[4] = {y[0][k1], y[1][k1], y[2][k1], y[3][k1]};
+      dft_small<4, INV>(t);
+#pragma unroll
+      for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = t[k2];
+    }
   }
 }
 
@@ -214,6 +247,7 @@ __device__ void lds_fft(double2 *A, const FftPlan &pl, int ntrans, unsigned long
   for (int s = 0; s < pl.nst; ++s) {
     const FftStage &st = DIT ? pl.dit[s] : pl.dif[s];
     switch (st.R) {
+      case 16: fft_stage<16, INV, COLS, DIT>(A, st, pl.n, ntrans, fnt, ls, tw); break;
       case 8: fft_stage<8, INV, COLS, DIT>(A, st, pl.n, ntrans, fnt, ls, tw); break;
       case 4: fft_stage<4, INV, COLS, DIT>(A, st, pl.n, ntrans, fnt, ls, tw); break;
       case 3: fft_stage<3, INV, COLS, DIT>(A, st, pl.n, ntrans, fnt, ls, tw); break;

@@ -22,7 +22,7 @@ struct FftStage {
   unsigned long long fS, fnb;
 };
 
-// Radix plan of one 1-D transform length (radices 8,4,2,3), as decimation
+// Radix plan of one 1-D transform length (radices 16,8,4,2,3), as decimation
 // in time (L grows) and decimation in frequency (L shrinks).
 struct FftPlan {
   int n;
