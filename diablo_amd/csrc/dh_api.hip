@@ -107,7 +107,7 @@ hipError_t dalloc(T **p, size_t count) {
 void free_all(Ctx &c) {
   void *ptrs[] = {c.tw_row, c.tw_col, c.tw_real, c.posr, c.posrd, c.posz, c.posc, c.kx, c.kz, c.kx2, c.kz2, c.d_dy, c.d_dyf, c.d_gyf, c.cu[0], c.cu[1],
                   c.cu[2], c.cp, c.cf[0], c.cf[1], c.cf[2], c.cth, c.cfth, c.phi, c.uphys, c.rphys, c.rd_ta,
-                  c.rd_tb, c.clock, c.fac.pf, c.fac.pb, c.fac.tl, c.fac.tb, c.fac.tc, c.fac.tr, c.fac.isbc, c.rd_tr, c.fac.vconst, c.rd_l, c.rd_u, c.stats, c.cun[0], c.cun[1], c.cun[2], c.rd_pf, c.rd_pb};
+                  c.rd_tb, c.clock, c.fac.pf, c.fac.pb, c.fac.tl, c.fac.tb, c.fac.tc, c.fac.tr, c.fac.isbc, c.rd_tr, c.fac.vconst, c.rd_l, c.rd_u, c.stats, c.cun[0], c.cun[1], c.cun[2]};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c.step_graph) (void)hipGraphExecDestroy(c.step_graph);
@@ -284,8 +284,6 @@ int dh_create(const dh_params *p, dh_ctx **out) {
   ACHK(dalloc(&c.rd_ta, cf_elems));
   ACHK(dalloc(&c.rd_tb, cf_elems));
   ACHK(dalloc(&c.rd_tr, cf_elems));
-  ACHK(dalloc(&c.rd_pf, cf_elems));
-  ACHK(dalloc(&c.rd_pb, cf_elems));
   ACHK(dalloc(&c.stats, (size_t)DH_NSTATS * nyy));
   ACHK(dalloc(&c.rd_l, nyy));
   ACHK(dalloc(&c.rd_u, nyy));

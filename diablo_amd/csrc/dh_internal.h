@@ -126,8 +126,6 @@ struct Ctx {
   double *rd_ta = nullptr;      // REM_DIV sweep multipliers [NY+2][pf]
   double *rd_tb = nullptr;      // REM_DIV sweep pivots      [NY+2][pf]
   double *rd_tr = nullptr;      // REM_DIV 1/pivots          [NY+2][pf]
-  double *rd_pf = nullptr;      // REM_DIV segment carry factors, forward  [NY+2][pf]
-  double *rd_pb = nullptr;      // REM_DIV segment carry factors, backward [NY+2][pf]
   int rd_seg = 0;               // rows per REM_DIV sweep segment (0: one sequential sweep per mode)
   double *stats = nullptr;      // dh_save_stats profiles [DH_NSTATS][NY+2]
   Clock *clock = nullptr;

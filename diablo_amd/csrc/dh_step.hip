@@ -788,7 +788,6 @@ struct RemDivArgs {
   double2 *phi;
   const double *rl, *ru;  // 1/(DY(J)DYF(J)), 1/(DY(J+1)DYF(J))
   double *ta, *tb, *tr;   // sweep multipliers, pivots, 1/pivots [J][mode]
-  double *tpf, *tpb;      // segment carry factors of the two sweeps [J][mode]
   int seg;                // rows per sweep segment; 0: one sequential sweep
 };
 
@@ -828,26 +827,6 @@ __global__ void __launch_bounds__(256) k_rem_div_factors(RemDivArgs a) {
     bprev = B;
     uprev = U;
   }
-  if (a.seg <= 0) return;
-  // carry factors of the segments [2 + s seg, 1 + (s+1) seg] (capped at NY):
-  // forward G_J = h_J + PF_J G_{r0-1} with PF_J = A_{r0} ... A_J; backward
-  // X_J = y_J + PB_J X_{r1+1} with PB_J = (-U_J/B_J) ... (-U_{r1}/B_{r1}),
-  // 0 below a zero pivot (the sweep sets X = 0 there)
-  double p = 1.0;
-  for (int J = 2; J <= NY; ++J) {
-    const double A = a.ta[pf * J + m];
-    p = ((J - 2) % a.seg == 0) ? A : p * A;
-    a.tpf[pf * J + m] = p;
-  }
-  p = 1.0;
-  for (int J = NY; J >= 2; --J) {
-    const double B = a.tb[pf * J + m];
-    const double U = (J < NY) ? a.ru[J] : 0.0;
-    const bool top = (J == NY) || ((J - 1) % a.seg == 0);  // last row of its segment
-    const double q = (B != 0.0) ? -U / B : 0.0;
-    p = top ? q : ((B != 0.0) ? q * p : 0.0);
-    a.tpb[pf * J + m] = p;
-  }
 }
 
 // One pressure-Poisson solve per (kx,kz) mode.  All threads stage the
@@ -863,18 +842,23 @@ constexpr int kRdtModes = 8;
 // [2 + s seg, 1 + (s+1) seg] (capped at NY), segment s on lane s*kRdtModes
 // + mode of the first wave.  Each segment runs the recurrence from a zero
 // carry in registers; the carries G_{r0-1} (up) and X_{r1+1} (down) then pass
-// between the segments by shuffles, and the tabulated factors PF, PB add
-// them to every row: G_J = h_J + PF_J G_{r0-1}, X_J = y_J + PB_J X_{r1+1}.
-// The top segment takes no carry down, so its rows equal the sequential
-// sweep's; the others differ from it by rounding.  Row 1 is rewritten last.
+// between the segments by shuffles and are added to every row:
+//   G_J = h_J + PF_J G_{r0-1},  PF_J = A_{r0} ... A_J;
+//   X_J = y_J + PB_J X_{r1+1},  PB_J = (-U_J/B_J) ... (-U_{r1}/B_{r1}),
+// PB = 0 at and below a zero pivot (the sweep sets X = 0 there), U_NY = 0.
+// The factors are products of the tabulated multipliers and of quotients
+// formed from the tabulated reciprocal pivots.  The top segment takes no
+// carry down, so its rows equal the sequential sweep's; the others differ
+// from it by rounding.  Row 1 is rewritten last.
 __device__ __forceinline__ void rem_div_sweep_seg(int seg, int s, int lane, int NY, bool mean, double2 *Gs,
                                                   const double *As, const double *Bs, const double *Rs,
-                                                  const double *ru, const double *PFs, const double *PBs) {
+                                                  const double *ru) {
 #pragma clang fp contract(off)
   const int r0 = 2 + s * seg, r1 = min(NY, r0 + seg - 1);
   const bool any = r0 <= r1;
   double2 h[kRdsRows];
   double2 hp = make_double2(0.0, 0.0);
+  double pfe = 1.0;
 #pragma unroll
   for (int q = 0; q < kRdsRows; ++q) {
     const int J = r0 + q;
@@ -882,10 +866,10 @@ __device__ __forceinline__ void rem_div_sweep_seg(int seg, int s, int lane, int 
       const double2 v = Gs[J * kRdtModes + lane];
       const double A = As[J * kRdtModes + lane];
       hp = make_double2(v.x + A * hp.x, v.y + A * hp.y);
+      pfe = (q == 0) ? A : pfe * A;
     }
     h[q] = hp;
   }
-  const double pfe = any ? PFs[r1 * kRdtModes + lane] : 1.0;
   const double2 hend = any ? hp : make_double2(0.0, 0.0);
   double2 gin = make_double2(0.0, 0.0), gend = hend;  // G_1 = 0 below segment 0
 #pragma unroll
@@ -896,17 +880,24 @@ __device__ __forceinline__ void rem_div_sweep_seg(int seg, int s, int lane, int 
       gend = make_double2(hend.x + pfe * gin.x, hend.y + pfe * gin.y);
     }
   }
+  double pf = 1.0;
 #pragma unroll
   for (int q = 0; q < kRdsRows; ++q) {
     const int J = r0 + q;
     if (J <= r1) {
-      const double P = PFs[J * kRdtModes + lane];
-      h[q] = make_double2(h[q].x + P * gin.x, h[q].y + P * gin.y);
+      const double A = As[J * kRdtModes + lane];
+      pf = (q == 0) ? A : pf * A;
+      h[q] = make_double2(h[q].x + pf * gin.x, h[q].y + pf * gin.y);
     }
   }
   // backward from a zero carry: X_J = (G_J - U_J X_{J+1}) / B_J, X = 0 where
   // B = 0; at J = NY the carry term U X_{NY+1} is 0 and X_NY = G_NY / B_NY
+  auto pb_step = [&](int J, double B, double R, double U, double pb) {
+    const double qv = (B != 0.0) ? div_by_recip(-(J < NY ? U : 0.0), B, R) : 0.0;
+    return (J == r1) ? qv : ((B != 0.0) ? qv * pb : 0.0);
+  };
   double2 yn = make_double2(0.0, 0.0);
+  double pbs = 1.0;
 #pragma unroll
   for (int q = kRdsRows - 1; q >= 0; --q) {
     const int J = r0 + q;
@@ -917,9 +908,9 @@ __device__ __forceinline__ void rem_div_sweep_seg(int seg, int s, int lane, int 
       yn.x = (B != 0.0) ? xr : 0.0;
       yn.y = (B != 0.0) ? xi : 0.0;
       h[q] = yn;
+      pbs = pb_step(J, B, R, U, pbs);
     }
   }
-  const double pbs = any ? PBs[r0 * kRdtModes + lane] : 1.0;
   const double2 ystart = any ? h[0] : make_double2(0.0, 0.0);
   double2 xin = make_double2(0.0, 0.0), xstart = ystart;  // nothing above row NY
 #pragma unroll
@@ -931,12 +922,13 @@ __device__ __forceinline__ void rem_div_sweep_seg(int seg, int s, int lane, int 
     }
   }
   const double U1 = mean ? 0.0 : -1.0;
+  double pb = 1.0;
 #pragma unroll
-  for (int q = 0; q < kRdsRows; ++q) {
+  for (int q = kRdsRows - 1; q >= 0; --q) {
     const int J = r0 + q;
     if (J <= r1) {
-      const double P = PBs[J * kRdtModes + lane];
-      const double2 x = make_double2(h[q].x + P * xin.x, h[q].y + P * xin.y);
+      pb = pb_step(J, Bs[J * kRdtModes + lane], Rs[J * kRdtModes + lane], ru[J], pb);
+      const double2 x = make_double2(h[q].x + pb * xin.x, h[q].y + pb * xin.y);
       Gs[J * kRdtModes + lane] = x;
       // row 1: B=1, G=0, U = 0 (mean mode) or -1
       if (J == 2) Gs[1 * kRdtModes + lane] = make_double2((0.0 - U1 * x.x) / 1.0, (0.0 - U1 * x.y) / 1.0);
@@ -959,9 +951,6 @@ __global__ void __launch_bounds__(256) k_rem_div_thomas(RemDivArgs a) {
   double *Bs = As + ntot * kRdtModes;
   double *Rs = Bs + ntot * kRdtModes;
   double *ru = st + 5 * ntot * kRdtModes + kRdtPad;
-  // segment carry factors [row][mode] (segmented sweep only)
-  double *PFs = st + 5 * ntot * kRdtModes + ntot + kRdtPad * kRdtModes;
-  double *PBs = PFs + ntot * kRdtModes;
   const int lane = threadIdx.x % kRdtModes, rg = threadIdx.x / kRdtModes;
   const int ngrp = blockDim.x / kRdtModes;
   const int m = blockIdx.x * kRdtModes + lane;
@@ -981,7 +970,7 @@ __global__ void __launch_bounds__(256) k_rem_div_thomas(RemDivArgs a) {
   // rows per thread
   for (int J0 = 2 + rg; J0 <= NY; J0 += 4 * ngrp) {
     double2 c1[4], c2[4], c2p[4], c3[4];
-    double ta[4], tb[4], tr[4], tpf[4], tpb[4];
+    double ta[4], tb[4], tr[4];
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
       const int J = J0 + b * ngrp;
@@ -989,10 +978,6 @@ __global__ void __launch_bounds__(256) k_rem_div_thomas(RemDivArgs a) {
         ta[b] = a.ta[pf * J + m];
         tb[b] = a.tb[pf * J + m];
         tr[b] = a.tr[pf * J + m];
-        if (a.seg > 0) {
-          tpf[b] = a.tpf[pf * J + m];
-          tpb[b] = a.tpb[pf * J + m];
-        }
         if (J < NY) {
           c1[b] = a.cin[0][pf * J + m];
           c3[b] = a.cin[2][pf * J + m];
@@ -1015,16 +1000,12 @@ __global__ void __launch_bounds__(256) k_rem_div_thomas(RemDivArgs a) {
         As[J * kRdtModes + lane] = live ? ta[b] : 0.0;
         Bs[J * kRdtModes + lane] = live ? tb[b] : 1.0;
         Rs[J * kRdtModes + lane] = live ? tr[b] : 1.0;
-        if (a.seg > 0) {
-          PFs[J * kRdtModes + lane] = live ? tpf[b] : 0.0;
-          PBs[J * kRdtModes + lane] = live ? tpb[b] : 0.0;
-        }
       }
     }
   }
   lds_barrier();
   if (a.seg > 0) {
-    if (rg < kRdsSeg) rem_div_sweep_seg(a.seg, rg, lane, NY, mean, Gs, As, Bs, Rs, ru, PFs, PBs);
+    if (rg < kRdsSeg) rem_div_sweep_seg(a.seg, rg, lane, NY, mean, Gs, As, Bs, Rs, ru);
   } else if (rg == 0) {
     // forward: G_J = V_J + A_J G_{J-1}, J = 2..NY, G_1 = 0.  Two register
     // sets of kRdtAhead rows alternate: one set's rows are computed while
@@ -1303,7 +1284,7 @@ static hipError_t launch_solve(Ctx &c, int rk, int courant) {
 
 static size_t rem_div_thomas_lds(const Ctx &c) {
   const size_t ntot = c.g.ny + 1 + 2 * kRdtPad;
-  return ntot * kRdtModes * 7 * sizeof(double) + ntot * sizeof(double);
+  return ntot * kRdtModes * 5 * sizeof(double) + ntot * sizeof(double);
 }
 
 static RemDivArgs make_rem_div_args(Ctx &c, int rk) {
@@ -1323,8 +1304,6 @@ static RemDivArgs make_rem_div_args(Ctx &c, int rk) {
   a.ta = c.rd_ta;
   a.tb = c.rd_tb;
   a.tr = c.rd_tr;
-  a.tpf = c.rd_pf;
-  a.tpb = c.rd_pb;
   a.seg = c.rd_seg;
   return a;
 }
