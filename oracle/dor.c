@@ -1006,6 +1006,119 @@ void dor_rem_div_chan(dor_state *s) {
 #undef C3
 }
 
+/* POISSON_P_CHAN channel.f:1217-1453, USE_MPI branch: boundary rows of
+ * APPLY_BC_POISSON_MPI mpi.f:1280-1310 and THOMAS_*_COMPLEX_MPI
+ * mpi.f:708-865.  CP(J), J=1..NY, from CU (and CTH): the Poisson equation
+ * for the pressure with right-hand side (dui/dxj)(duj/dxi) plus buoyancy.
+ * As in the reference, CF1..CF3 (and their physical views) and CS1 are
+ * used as scratch and hold junk on return. */
+void dor_poisson_p_chan(dor_state *s) {
+  const int NY = s->NY, NXP = s->NXP, TNKZ = s->TNKZ, NZP = s->NZP, NXM = s->NXM;
+  const double *DY = s->DY, *DYF = s->DYF, *GYF = s->GYF, *KX = s->KX, *KZ = s->KZ, *KX2 = s->KX2, *KZ2 = s->KZ2;
+  double *MATL_C = s->MATL_C, *MATD_C = s->MATD_C, *MATU_C = s->MATU_C;
+  cplx *VEC_C = s->VEC_C;
+  cplx *CU1 = s->CU1, *CU2 = s->CU2, *CU3 = s->CU3, *CF1 = s->CF1, *CF2 = s->CF2, *CF3 = s->CF3, *CS1 = s->CS1;
+  double *F1 = PHYS(CF1), *F2 = PHYS(CF2), *F3 = PHYS(CF3);
+#define MC(i, j) ((size_t)(i) + (size_t)(NXP + 1) * (size_t)(j))
+#define C3(a, i, k, j) a[CIDX(s, i, k, j)]
+#define R3D(a, i, k, j) a[RIDX(s, i, k, j)]
+  /* diagonal terms: (dU1/dx)^2 + (dU2/dy)^2 + (dU3/dz)^2, :1230-1270 */
+  for (int J = 2; J <= NY; ++J)
+    for (int K = 0; K <= TNKZ; ++K)
+      for (int i = 0; i <= NXP - 1; ++i) {
+        C3(CF1, i, K, J) = (I * KX[i]) * C3(CU1, i, K, J);
+        C3(CF2, i, K, J) = (C3(CU2, i, K, J + 1) - C3(CU2, i, K, J)) / DYF[J];
+        C3(CF3, i, K, J) = (I * KZ[K]) * C3(CU3, i, K, J);
+      }
+  dor_fft_xz_to_physical(s, F1, 0, NY + 1);
+  dor_fft_xz_to_physical(s, F2, 0, NY + 1);
+  dor_fft_xz_to_physical(s, F3, 0, NY + 1);
+  for (int J = 2; J <= NY; ++J)
+    for (int K = 0; K <= NZP - 1; ++K)
+      for (int i = 0; i <= NXM; ++i) {
+        R3D(F1, i, K, J) = R3D(F1, i, K, J) * R3D(F1, i, K, J);
+        R3D(F2, i, K, J) = R3D(F2, i, K, J) * R3D(F2, i, K, J);
+        R3D(F3, i, K, J) = R3D(F3, i, K, J) * R3D(F3, i, K, J);
+      }
+  dor_fft_xz_to_fourier(s, F1, 0, NY + 1);
+  dor_fft_xz_to_fourier(s, F2, 0, NY + 1);
+  dor_fft_xz_to_fourier(s, F3, 0, NY + 1);
+  for (int J = 2; J <= NY; ++J)
+    for (int K = 0; K <= TNKZ; ++K)
+      for (int i = 0; i <= NXP - 1; ++i) C3(CS1, i, K, J) = C3(CF1, i, K, J) + C3(CF2, i, K, J) + C3(CF3, i, K, J);
+  /* off-diagonal terms 2 (a)(b), :1280-1390 */
+  for (int term = 0; term < 3; ++term) {
+    for (int J = 2; J <= NY; ++J)
+      for (int K = 0; K <= TNKZ; ++K)
+        for (int i = 0; i <= NXP - 1; ++i) {
+          if (term == 0) {
+            C3(CF1, i, K, J) = (C3(CU1, i, K, J + 1) - C3(CU1, i, K, J - 1)) / (2. * DYF[J]);
+            C3(CF2, i, K, J) = (I * KX[i]) * 0.5 * (C3(CU2, i, K, J) + C3(CU2, i, K, J + 1));
+          } else if (term == 1) {
+            C3(CF1, i, K, J) = (C3(CU3, i, K, J + 1) - C3(CU3, i, K, J - 1)) / (2. * DYF[J]);
+            C3(CF2, i, K, J) = (I * KZ[K]) * 0.5 * (C3(CU2, i, K, J) + C3(CU2, i, K, J + 1));
+          } else {
+            C3(CF1, i, K, J) = (I * KZ[K]) * C3(CU1, i, K, J);
+            C3(CF2, i, K, J) = (I * KX[i]) * C3(CU3, i, K, J);
+          }
+        }
+    dor_fft_xz_to_physical(s, F1, 0, NY + 1);
+    dor_fft_xz_to_physical(s, F2, 0, NY + 1);
+    for (int J = 2; J <= NY; ++J)
+      for (int K = 0; K <= NZP - 1; ++K)
+        for (int i = 0; i <= NXM; ++i) R3D(F1, i, K, J) = 2. * R3D(F1, i, K, J) * R3D(F2, i, K, J);
+    dor_fft_xz_to_fourier(s, F1, 0, NY + 1);
+    for (int J = 2; J <= NY; ++J)
+      for (int K = 0; K <= TNKZ; ++K)
+        for (int i = 0; i <= NXP - 1; ++i) C3(CS1, i, K, J) = C3(CS1, i, K, J) + C3(CF1, i, K, J);
+  }
+  /* buoyancy, skipping the (0,0) mode, :1394-1406 */
+  for (int N = 0; N < s->NTH; ++N) {
+    const cplx *CTH = s->CTH + s->cfield * N;
+    const double RI = s->p.ri[N];
+    for (int J = 2; J <= NY; ++J)
+      for (int K = 0; K <= TNKZ; ++K)
+        for (int i = 0; i <= NXP - 1; ++i)
+          if (i != 0 || K != 0)
+            C3(CS1, i, K, J) =
+                C3(CS1, i, K, J) + RI * (C3(CTH, i, K, J + 1) - C3(CTH, i, K, J - 1)) / (GYF[J + 1] - GYF[J - 1]);
+  }
+  /* the tridiagonal systems, :1410-1444 */
+  for (int J = 0; J <= NY + 1; ++J)
+    for (int i = 0; i <= NXP - 1; ++i) {
+      MATL_C[MC(i, J)] = 0.;
+      MATD_C[MC(i, J)] = 1.;
+      MATU_C[MC(i, J)] = 0.;
+      VEC_C[MC(i, J)] = 0.;
+    }
+  for (int K = 0; K <= TNKZ; ++K) {
+    for (int J = 2; J <= NY; ++J)
+      for (int i = 0; i <= NXP - 1; ++i) {
+        MATL_C[MC(i, J)] = 1. / (DY[J] * DYF[J]);
+        MATD_C[MC(i, J)] = -KX2[i] - KZ2[K] - 1. / (DY[J + 1] * DYF[J]) - 1. / (DY[J] * DYF[J]);
+        MATU_C[MC(i, J)] = 1. / (DY[J + 1] * DYF[J]);
+        VEC_C[MC(i, J)] = -1. * C3(CS1, i, K, J);
+      }
+    /* APPLY_BC_POISSON_MPI */
+    for (int i = 0; i <= NXP - 1; ++i) {
+      if (i == 0 && K == 0) {
+        MATD_C[MC(i, 1)] = 1.; MATU_C[MC(i, 1)] = 0.; VEC_C[MC(i, 1)] = 0.;
+      } else {
+        MATD_C[MC(i, 1)] = 1.; MATU_C[MC(i, 1)] = -1.; VEC_C[MC(i, 1)] = 0.;
+      }
+    }
+    for (int i = 0; i <= NXP - 1; ++i) {
+      MATD_C[MC(i, NY)] = -1.; MATL_C[MC(i, NY)] = 1.; VEC_C[MC(i, NY)] = 0.;
+    }
+    dor_thomas_complex(MATL_C, MATD_C, MATU_C, (double *)VEC_C, NY, NXP);
+    for (int J = 1; J <= NY; ++J)
+      for (int i = 0; i <= NXP - 1; ++i) C3(s->CP, i, K, J) = VEC_C[MC(i, J)];
+  }
+#undef MC
+#undef C3
+#undef R3D
+}
+
 /* COURANT courant.f:1-65 (get_minimum_mpi is the identity on one rank) */
 void dor_courant(dor_state *s) {
   const int NY = s->NY, NZP = s->NZP, NXM = s->NXM;

@@ -86,6 +86,9 @@ void dor_fft_xz_to_fourier(dor_state *s, double *v, int jmin, int jmax);
 void dor_fft_xz_to_physical(dor_state *s, double *vv, int jmin, int jmax);
 void dor_rk_chan_1(dor_state *s, int rk_step);
 void dor_rem_div_chan(dor_state *s);
+/* POISSON_P_CHAN (channel.f:1217-1453): CP(J), J=1..NY, from CU and CTH;
+ * CF1..3 and CS1 are left holding scratch, as in the reference. */
+void dor_poisson_p_chan(dor_state *s);
 void dor_courant(dor_state *s);
 /* One full time step as diablo.f:66-110 (3 substeps, then TIME+=DELTA_T). */
 void dor_time_step(dor_state *s);

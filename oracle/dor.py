@@ -74,6 +74,7 @@ def load():
             getattr(lib, f).argtypes = [vp, dp, ip, ip]
         lib.dor_rk_chan_1.argtypes = [vp, ip]
         lib.dor_rem_div_chan.argtypes = [vp]
+        lib.dor_poisson_p_chan.argtypes = [vp]
         lib.dor_courant.argtypes = [vp]
         lib.dor_time_step.argtypes = [vp]
         lib.dor_thomas_real.argtypes = [dp, dp, dp, dp, ip, ip]
@@ -210,6 +211,10 @@ class Oracle:
 
     def rem_div(self):
         self.lib.dor_rem_div_chan(self.h)
+
+    def poisson_p(self):
+        """POISSON_P_CHAN: CP from CU (and CTH); CF1..3 and CS1 become scratch."""
+        self.lib.dor_poisson_p_chan(self.h)
 
     def courant(self):
         self.lib.dor_courant(self.h)
