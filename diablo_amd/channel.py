@@ -73,6 +73,7 @@ EXPORTS = {
     "dh_profile_substep": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, _DP, ctypes.c_int32]),
     "dh_phase_name": (ctypes.c_char_p, [ctypes.c_int32]),
     "dh_rem_div": (ctypes.c_int, [ctypes.c_void_p]),
+    "dh_poisson_p_chan": (ctypes.c_int, [ctypes.c_void_p]),
     "dh_save_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]),
     "dh_create_flow_chan": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_double, ctypes.c_uint64,
                                            ctypes.c_double, ctypes.c_double]),
@@ -248,6 +249,10 @@ class Channel:
 
     def rem_div(self):
         self._check(self.lib.dh_rem_div(self.h), "dh_rem_div")
+
+    def poisson_p(self):
+        """POISSON_P_CHAN (channel.f:1217-1453): CP on planes 1..NY from CU and CTH."""
+        self._check(self.lib.dh_poisson_p_chan(self.h), "dh_poisson_p_chan")
 
     def save_stats(self, ly):
         """SAVE_STATS_CHAN's plane statistics on the device (save_stats.f:136-262):

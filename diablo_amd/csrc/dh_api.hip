@@ -684,6 +684,14 @@ int dh_rem_div(dh_ctx *h) {
   return DH_OK;
 }
 
+int dh_poisson_p_chan(dh_ctx *h) {
+  if (!h) return DH_EARG;
+  Ctx &c = h->c;
+  if (!c.grid_set) return fail(c, DH_ESTATE, "dh_set_grid first");
+  HIPCHK(launch_poisson_p(c));
+  return DH_OK;
+}
+
 int dh_save_stats(dh_ctx *h, double ly, double *out, double *ubulk) {
   if (!h || !out || !ubulk || !(ly > 0.0)) return DH_EARG;
   Ctx &c = h->c;

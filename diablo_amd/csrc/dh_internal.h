@@ -160,6 +160,8 @@ hipError_t launch_step_begin(Ctx &c);
 hipError_t launch_end_step(Ctx &c);
 hipError_t launch_cn_tables(Ctx &c, hipStream_t st);
 hipError_t join_side(Ctx &c);  // main stream waits for a pending table rebuild
-hipError_t launch_rem_div_factors(Ctx &c);  // once per grid  // for the DELTA_T in the device clock
+hipError_t launch_rem_div_factors(Ctx &c);
+// ---- launchers (dh_poisson.hip) ----
+hipError_t launch_poisson_p(Ctx &c);  // POISSON_P_CHAN into CP  // once per grid  // for the DELTA_T in the device clock
 
 }  // namespace dh

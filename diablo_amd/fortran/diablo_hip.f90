@@ -90,6 +90,10 @@ module diablo_hip
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
     end function dh_rem_div
+    integer(c_int) function dh_poisson_p_chan(ctx) bind(C, name='dh_poisson_p_chan')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function dh_poisson_p_chan
     integer(c_int) function dh_save_stats(ctx, ly, out, ubulk) bind(C, name='dh_save_stats')
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx

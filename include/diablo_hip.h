@@ -127,6 +127,14 @@ int dh_synchronize(dh_ctx *ctx);
  * state; leaves phi in the context's phi buffer. */
 int dh_rem_div(dh_ctx *ctx);
 
+/* POISSON_P_CHAN (channel.f:1217-1453 with APPLY_BC_POISSON_MPI
+ * mpi.f:1280-1310 and THOMAS_*_COMPLEX_MPI mpi.f:708-865): CP on planes
+ * 1..NY from the current CU (and CTH), as the driver computes it at a
+ * restart that has no pressure (hdf5.f:537).  CU, CF and CTH are left
+ * unchanged (the reference overwrites CF1..3 as scratch); the context's
+ * phi buffer and physical velocity buffers are overwritten. */
+int dh_poisson_p_chan(dh_ctx *ctx);
+
 /* CREATE_FLOW_CHAN (set_ics.f:2-200) on the device: IC_TYPE 0 (laminar
  * Poiseuille, 3/2 UBULK0 (1-GYF^2)), 2 (Couette), 3 (tanh shear layer), plus
  * Gaussian noise of standard deviation amp on U1..U3 wherever |GYF| < ymask

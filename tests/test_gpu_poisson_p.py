@@ -1,0 +1,56 @@
+"""POISSON_P_CHAN on the device (dh_poisson_p_chan) against the oracle's
+restatement of channel.f:1217-1453 (needs an MI355X).
+
+The state is the deck's initial condition advanced two RK3 steps by the
+oracle, so every term of the right-hand side is non-zero (the small deck
+also has a scalar with buoyancy).  CP must agree to relative L2
+<= max(1e-11, 30 x floor), the floor being the oracle's own change when its
+input velocity is perturbed at 1e-16: the right-hand side passes through
+nine transforms, and the lowest (kx,kz) Poisson modes are ill-conditioned
+on the stretched grid.  CU and CTH must come back unchanged."""
+import numpy as np
+import pytest
+
+from diablo_amd import channel
+from tests import helpers
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def dor():
+    from oracle import dor as d
+    d.load()
+    return d
+
+
+@pytest.mark.parametrize("name", ["small_case", "kh_shear_layer", "channel_re180"])
+def test_poisson_p_matches_oracle(dor, name):
+    print(f"[poisson_p] {name}", flush=True)
+    o, p, gy, ic = helpers.make_oracle(dor, name)
+    o.set_clock(0.0, p["delta_t"], 0)
+    o.time_step(2)
+    st0 = helpers.oracle_state(dor, o)
+    ch = channel.Channel(p, gy)
+    ch.upload(st0)
+    o2, *_ = helpers.make_oracle(dor, name)
+    o2.set_clock(0.0, p["delta_t"], 0)
+    o2.time_step(2)
+    helpers.perturb(dor, o2, 1e-16)
+    o.poisson_p()
+    o2.poisson_p()
+    ch.poisson_p()
+    st = ch.download()
+    keys = ("cu1", "cu2", "cu3", "cp")
+    errs = helpers.field_errors(dor, o, st, keys=keys)
+    floor = helpers.field_errors(dor, o, helpers.oracle_state(dor, o2), keys=("cp",))
+    print("  err", {k: f"{v:.1e}" for k, v in errs.items()}, " floor cp", f"{floor['cp']:.1e}", flush=True)
+    assert errs["cp"] <= max(1e-11, 30 * floor["cp"]), (errs, floor)
+    for k in ("cu1", "cu2", "cu3"):
+        assert errs[k] == 0.0, (k, errs[k])
+    for n in range(o.n_th):
+        assert errs[f"cth{n}"] == 0.0
+    # the pressure did change from the uploaded one
+    assert helpers.rel_l2(helpers.dealiased(st["cp"], o.nkx, o.tnkz),
+                          helpers.dealiased(st0["cp"], o.nkx, o.tnkz)) > 1e-6
+    ch.close()
