@@ -74,6 +74,9 @@ EXPORTS = {
     "dh_phase_name": (ctypes.c_char_p, [ctypes.c_int32]),
     "dh_rem_div": (ctypes.c_int, [ctypes.c_void_p]),
     "dh_poisson_p_chan": (ctypes.c_int, [ctypes.c_void_p]),
+    "dh_h5_read_f64": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_void_p, ctypes.c_int64,
+                                      ctypes.POINTER(ctypes.c_int64), ctypes.c_char_p, ctypes.c_int32]),
+    "dh_set_grid_h5": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
     "dh_save_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]),
     "dh_create_flow_chan": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_double, ctypes.c_uint64,
                                            ctypes.c_double, ctypes.c_double]),
@@ -141,6 +144,23 @@ def grid_metrics(gy):
     return GY, GYF, DY, DYF
 
 
+def read_h5_f64(path, dataset):
+    """A float64 dataset of an HDF5 grid or flow file through the library's
+    reader (dh_h5_read_f64; ReadGridHDF5's subset, hdf5.f:548-710)."""
+    lib = load_library()
+    n = ctypes.c_int64(0)
+    err = ctypes.create_string_buffer(256)
+    rc = lib.dh_h5_read_f64(str(path).encode(), dataset.encode(), None, 0, ctypes.byref(n), err, 256)
+    if rc != 0 and b"too small" not in err.value:
+        raise DiabloError(f"dh_h5_read_f64({path}, {dataset}): {err.value.decode()} ({rc})")
+    out = np.zeros(max(n.value, 1))
+    rc = lib.dh_h5_read_f64(str(path).encode(), dataset.encode(), out.ctypes.data, out.size, ctypes.byref(n),
+                            err, 256)
+    if rc != 0:
+        raise DiabloError(f"dh_h5_read_f64({path}, {dataset}): {err.value.decode()} ({rc})")
+    return out[: n.value].copy()
+
+
 def _ptr(a):
     if a is None:
         return None
@@ -193,6 +213,11 @@ class Channel:
         self.GY, self.GYF, self.DY, self.DYF = grid_metrics(np.asarray(gy, dtype=np.float64))
         self._check(self.lib.dh_set_grid(self.h, _ptr(self.GY), _ptr(self.GYF), _ptr(self.DY), _ptr(self.DYF)),
                     "dh_set_grid")
+
+    def set_grid_h5(self, path):
+        """CREATE_GRID_CHAN from a grid file (/grids/y, ReadGridHDF5 hdf5.f:548-710)."""
+        self._check(self.lib.dh_set_grid_h5(self.h, str(path).encode()), "dh_set_grid_h5")
+        self.GY, self.GYF, self.DY, self.DYF = grid_metrics(read_h5_f64(path, "/grids/y"))
 
     def set_clock(self, time, delta_t, time_step):
         self._check(self.lib.dh_set_clock(self.h, time, delta_t, time_step), "dh_set_clock")

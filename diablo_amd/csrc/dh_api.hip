@@ -402,6 +402,28 @@ int dh_set_grid(dh_ctx *h, const double *gy, const double *gyf, const double *dy
   return rebuild_tables(c);
 }
 
+int dh_set_grid_h5(dh_ctx *h, const char *path) {
+  if (!h || !path) return DH_EARG;
+  Ctx &c = h->c;
+  const int NY = c.g.ny;
+  std::vector<double> y(NY + 2, 0.0);
+  int64_t n = 0;
+  char err[256];
+  int r = dh_h5_read_f64(path, "/grids/y", y.data(), NY + 2, &n, err, (int32_t)sizeof(err));
+  if (r != DH_OK) return fail(c, r, err);
+  if (n != NY + 1) return fail(c, DH_EARG, "Grid file and program do not match: /grids/y must hold NY+1 values");
+  // hdf5.f:690-705 and channel.f:1646-1652 on one rank
+  std::vector<double> GY(NY + 2, 0.0), GYF(NY + 2, 0.0), DY(NY + 2, 0.0), DYF(NY + 2, 0.0);
+  for (int j = 1; j <= NY + 1; ++j) GY[j] = y[j - 1];
+  for (int j = 1; j <= NY; ++j) GYF[j] = 0.5 * (GY[j] + GY[j + 1]);
+  GYF[0] = 2.0 * GYF[1] - GYF[2];
+  GYF[NY + 1] = 2.0 * GYF[NY] - GYF[NY - 1];
+  for (int j = 1; j <= NY + 1; ++j) DY[j] = GYF[j] - GYF[j - 1];
+  for (int j = 1; j <= NY; ++j) DYF[j] = GY[j + 1] - GY[j];
+  DYF[NY + 1] = DYF[NY];
+  return dh_set_grid(h, GY.data(), GYF.data(), DY.data(), DYF.data());
+}
+
 int dh_set_clock(dh_ctx *h, double time, double delta_t, int32_t time_step) {
   if (!h) return DH_EARG;
   Ctx &c = h->c;

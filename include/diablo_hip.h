@@ -89,6 +89,23 @@ const char *dh_last_error(dh_ctx *ctx);
 
 /* /GRID_PARAMS/ (header:205-207): GY, GYF, DY, DYF each (0:NY+1), as built by
  * CREATE_GRID_CHAN (channel.f:1564-1655) + ReadGridHDF5 (hdf5.f:548-710). */
+/* Reads the float64 dataset `dataset` (e.g. "/grids/y") of an HDF5 file
+ * into out (cap values) and sets *n to its number of values.  Covers what
+ * the reference's grid generator (pre_process/create_grid_h5.m) writes and
+ * ReadGridHDF5 (hdf5.f:548-710) reads: superblock version 0 or 1,
+ * symbol-table groups, contiguous or compact little-endian float64 data;
+ * anything else returns DH_ELAYOUT with the reason in err (errlen bytes).
+ * Needs no context and no GPU. */
+int dh_h5_read_f64(const char *path, const char *dataset, double *out, int64_t cap, int64_t *n, char *err,
+                   int32_t errlen);
+
+/* CREATE_GRID_CHAN (channel.f:1564-1655) with ReadGridHDF5 (hdf5.f:548-710):
+ * GY(1..NY+1) from /grids/y of the file at `path`, which must hold NY+1
+ * values ("Grid file and program do not match" otherwise); GYF(1..NY) =
+ * (GY(J)+GY(J+1))/2 with the ghosts GYF(0), GYF(NY+1) extrapolated; DY, DYF
+ * as channel.f:1646-1652; then dh_set_grid. */
+int dh_set_grid_h5(dh_ctx *ctx, const char *path);
+
 int dh_set_grid(dh_ctx *ctx, const double *gy, const double *gyf, const double *dy, const double *dyf);
 
 /* Copy the host state in / out (the COMMON arrays RK_CHAN_1 reads and
