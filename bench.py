@@ -124,6 +124,53 @@ def copy_calibration(mib, reps=5):
                       "best_ms": round(best * 1e3, 4)}), flush=True)
 
 
+def ensemble_bench(n, steps, warmup):
+    """Uneven shear-layer instances (NX in {64..256}, NY = NX + 1, NZ = 4)
+    stepped as one graph; aggregate sum(NX NY NZ) steps / wall, next to the
+    same instances run one after another (each with its own step graph)."""
+    from diablo_amd import channel, configs
+    decks = configs.kh_ensemble(n)
+    chs = []
+    for p, gy, ic in decks:
+        ch = channel.Channel(p, gy)
+        ch.initial_condition(ic)
+        ch.set_clock(0.0, p["delta_t"], 0)
+        chs.append(ch)
+    pts = sum(p["nx"] * p["ny"] * p["nz"] for p, _, _ in decks)
+    ens = channel.Ensemble(chs)
+    ens.time_step(warmup)
+    ens.synchronize()
+    t0 = time.perf_counter()
+    ens.time_step(steps)
+    ens.synchronize()
+    wall = time.perf_counter() - t0
+    # the same instances one at a time
+    for ch in chs:
+        ch.time_step(1)
+        ch.synchronize()
+    t1 = time.perf_counter()
+    for ch in chs:
+        ch.time_step(steps)
+        ch.synchronize()
+    wall_seq = time.perf_counter() - t1
+    sizes = {}
+    for p, _, _ in decks:
+        sizes[p["nx"]] = sizes.get(p["nx"], 0) + 1
+    print(json.dumps({
+        "metric": f"Mpts·RK3-steps/s, KH shear-layer ensemble of {n} uneven instances",
+        "value": round(pts * steps / wall / 1e6, 3), "unit": UNIT, "n_gpus": 1, "steps": steps, "warmup": warmup,
+        "ms_per_step": round(wall / steps * 1e3, 4), "higher_is_better": True, "dtype": "f64",
+        "data": "synthetic (seeded KH tanh shear layer + noise per instance)",
+        "config": {"workload": "kh_ensemble", "instances": n, "nx_counts": sizes, "nz": 4,
+                   "points_per_step": pts},
+        "one_at_a_time": {"value": round(pts * steps / wall_seq / 1e6, 3),
+                          "ms_per_step": round(wall_seq / steps * 1e3, 4)},
+    }), flush=True)
+    ens.close()
+    for ch in chs:
+        ch.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -131,6 +178,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-reps", type=int, default=10)
+    ap.add_argument("--ensemble", type=int, default=0, metavar="N",
+                    help="only time N uneven shear-layer instances stepped as one graph")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -140,6 +189,9 @@ def main():
         sys.exit(2)
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.ensemble:
+        ensemble_bench(args.ensemble, args.steps, args.warmup)
         return
 
     from diablo_amd import channel, configs

@@ -77,6 +77,10 @@ EXPORTS = {
     "dh_h5_read_f64": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_void_p, ctypes.c_int64,
                                       ctypes.POINTER(ctypes.c_int64), ctypes.c_char_p, ctypes.c_int32]),
     "dh_set_grid_h5": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
+    "dh_ensemble_create": (ctypes.c_int, [ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p]),
+    "dh_ensemble_time_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "dh_ensemble_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
+    "dh_ensemble_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "dh_save_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]),
     "dh_create_flow_chan": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_double, ctypes.c_uint64,
                                            ctypes.c_double, ctypes.c_double]),
@@ -321,3 +325,42 @@ class Channel:
 def make_channel(name, **kw):
     p, gy, ic = configs.CONFIGS[name](**kw)
     return Channel(p, gy), p, gy, ic
+
+
+class Ensemble:
+    """Independent Channel instances of any sizes stepped together
+    (dh_ensemble_*): each member's step graph on its own stream, all members
+    advanced step by step so their kernels overlap.  The members stay usable
+    on their own between ensemble steps and must outlive it."""
+
+    def __init__(self, members):
+        self.lib = load_library()
+        self.members = list(members)
+        arr = (ctypes.c_void_p * len(self.members))(*[m.h.value for m in self.members])
+        h = ctypes.c_void_p()
+        rc = self.lib.dh_ensemble_create(len(self.members), arr, ctypes.byref(h))
+        if rc != DH_OK:
+            raise DiabloError(f"dh_ensemble_create failed ({rc})")
+        self.h = h
+
+    def time_step(self, n=1):
+        rc = self.lib.dh_ensemble_time_step(self.h, n)
+        if rc != DH_OK:
+            raise DiabloError(f"dh_ensemble_time_step failed ({rc})")
+
+    def synchronize(self):
+        rc = self.lib.dh_ensemble_synchronize(self.h)
+        if rc != DH_OK:
+            raise DiabloError(f"dh_ensemble_synchronize failed ({rc})")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.dh_ensemble_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+

@@ -77,6 +77,7 @@ typedef struct dh_host_state {
 } dh_host_state;
 
 typedef struct dh_ctx dh_ctx;
+typedef struct dh_ens dh_ens;
 
 /* INIT_FFT (fft.f:55-143: wavenumbers, transform set-up), INIT_CHAN_MPI
  * index rules on one rank (mpi.f:1013-1081) and the RKW3 constants
@@ -138,6 +139,21 @@ int dh_profile_substep(dh_ctx *ctx, int32_t rk_step, int32_t nrep, double *ms, i
 const char *dh_phase_name(int32_t phase);
 /* Wait for all work queued on the context. */
 int dh_synchronize(dh_ctx *ctx);
+
+/* Ensembles of independent instances of any sizes (many shear layers at
+ * once): no counterpart in the reference, which runs one flow per process.
+ * The ensemble steps existing contexts (it does not own them; they must
+ * outlive it and each may appear once).  dh_ensemble_time_step runs nsteps
+ * iterations of diablo.f:66-110 on every member: each member's captured
+ * step graph on the member's own stream, every member's step launched
+ * before the next step of any, so members overlap on the device.  Work
+ * queued on a member stays ordered with it, so each member's own entry
+ * points remain usable between calls, and results equal those of
+ * dh_time_step on each member alone. */
+int dh_ensemble_create(int32_t n, dh_ctx *const *members, dh_ens **out);
+int dh_ensemble_time_step(dh_ens *ens, int32_t nsteps);
+int dh_ensemble_synchronize(dh_ens *ens);
+int dh_ensemble_destroy(dh_ens *ens);
 
 /* REM_DIV_CHAN (channel.f:1103-1214 with APPLY_BC_REM_DIV_MPI
  * mpi.f:1239-1278 and THOMAS_*_COMPLEX_MPI mpi.f:708-865) on the device

@@ -1,0 +1,76 @@
+"""Graph-captured ensembles of uneven shear-layer instances
+(dh_ensemble_*, needs an MI355X).
+
+Three KH instances of different sizes (NX = 64, 96, 128, NY = NX + 1,
+NZ = 4) are stepped together; each must come out bit-identical to the same
+instance stepped alone with dh_time_step, since the ensemble graph runs the
+same kernels in the same order.  The NX = 64 member is also compared with
+the oracle at its roundoff floor (this size runs the LDS row transforms)."""
+import numpy as np
+import pytest
+
+from diablo_amd import channel, configs
+from tests import helpers
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def dor():
+    from oracle import dor as d
+    d.load()
+    return d
+
+
+def _decks():
+    out = []
+    for i, (nx, ri, re) in enumerate([(64, 0.05, 800.0), (96, 0.2, 3000.0), (128, 0.1, 1500.0)]):
+        p, gy, ic = configs.kh_shear_layer(nx=nx, ny=nx + 1, nz=4, ri=ri, re=re)
+        out.append((p, gy, dict(ic, seed=200 + i)))
+    return out
+
+
+def _make(p, gy, ic):
+    ch = channel.Channel(p, gy)
+    ch.initial_condition(ic)
+    ch.set_clock(0.0, p["delta_t"], 0)
+    return ch
+
+
+def test_ensemble_equals_members_alone(dor):
+    print("[ensemble] 3 KH instances, NX = 64, 96, 128", flush=True)
+    decks = _decks()
+    alone = [_make(*d) for d in decks]
+    together = [_make(*d) for d in decks]
+    for ch in alone:
+        ch.time_step(3)
+    ens = channel.Ensemble(together)
+    ens.time_step(2)
+    ens.time_step(1)  # a second call replays the same graph
+    for a, b in zip(alone, together):
+        sa, sb = a.download(), b.download()
+        for k in ("cu1", "cu2", "cu3", "cp", "cf1", "cf2", "cf3", "cth"):
+            assert np.array_equal(sa[k], sb[k]), (a.nx, k)
+        assert a.clock() == b.clock()
+    # member 0 (NX = 64) against the oracle: ICs and 3 steps
+    p, gy, ic = decks[0]
+    o = dor.Oracle(p)
+    o.set_grid_gy(gy)
+    o.create_th(ic["th_ic_type"])
+    o.create_flow(ic["ic_type"], ic["seed"], ic["amp"], ic["ymask"], ic["ubulk0"])
+    o2 = dor.Oracle(p)
+    o2.set_grid_gy(gy)
+    o2.create_th(ic["th_ic_type"])
+    o2.create_flow(ic["ic_type"], ic["seed"], ic["amp"], ic["ymask"], ic["ubulk0"])
+    helpers.perturb(dor, o2, 1e-16)
+    for x in (o, o2):
+        x.set_clock(0.0, p["delta_t"], 0)
+        x.time_step(3)
+    errs = helpers.field_errors(dor, o, together[0].download())
+    floor = helpers.field_errors(dor, o, helpers.oracle_state(dor, o2))
+    print("  member NX=64 vs oracle", {k: f"{v:.1e}" for k, v in errs.items()}, flush=True)
+    bad = helpers.within(errs, floor, 1e-10)
+    assert not bad, bad
+    ens.close()
+    for ch in alone + together:
+        ch.close()
