@@ -117,6 +117,7 @@ void free_all(Ctx &c) {
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c.step_graph) (void)hipGraphExecDestroy(c.step_graph);
+  if (c.steps_graph) (void)hipGraphExecDestroy(c.steps_graph);
   if (c.stream) (void)hipStreamDestroy(c.stream);
   if (c.side) (void)hipStreamDestroy(c.side);
   if (c.ev_fork) (void)hipEventDestroy(c.ev_fork);
@@ -599,11 +600,37 @@ int dh_rk_substep(dh_ctx *h, int32_t rk_step) {
   return DH_OK;
 }
 
-static int enqueue_step(Ctx &c) {
-  HIPCHK(launch_step_begin(c));
-  for (int rk = 0; rk < 3; ++rk) HIPCHK(launch_substep(c, rk));
-  HIPCHK(launch_end_step(c));
+// nsteps iterations of diablo.f:66-110 without I/O: TIME_STEP += 1, the
+// three substeps, TIME += DELTA_T; the end of one step and the start of the
+// next share one launch.
+static int enqueue_steps(Ctx &c, int nsteps) {
+  HIPCHK(launch_step_boundary(c, false, true));
+  for (int s = 0; s < nsteps; ++s) {
+    for (int rk = 0; rk < 3; ++rk) HIPCHK(launch_substep(c, rk));
+    HIPCHK(launch_step_boundary(c, true, s + 1 < nsteps));
+  }
   return DH_OK;
+}
+
+static int enqueue_step(Ctx &c) { return enqueue_steps(c, 1); }
+
+// Captures nsteps steps into *exec; false (and a cleared error) if the
+// capture or instantiation fails, in which case the steps run uncaptured.
+static bool capture_steps(Ctx &c, int nsteps, hipGraphExec_t *exec) {
+  hipGraph_t graph = nullptr;
+  hipError_t e = hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal);
+  int r = DH_OK;
+  if (e == hipSuccess) {
+    r = enqueue_steps(c, nsteps);
+    e = hipStreamEndCapture(c.stream, &graph);
+  }
+  bool ok = false;
+  if (e == hipSuccess && r == DH_OK && graph) {
+    ok = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0) == hipSuccess;
+  }
+  if (graph) (void)hipGraphDestroy(graph);
+  if (!ok) (void)hipGetLastError();
+  return ok;
 }
 
 int dh_time_step(dh_ctx *h, int32_t nsteps) {
@@ -611,29 +638,18 @@ int dh_time_step(dh_ctx *h, int32_t nsteps) {
   Ctx &c = h->c;
   if (!c.grid_set) return fail(c, DH_ESTATE, "dh_set_grid first");
   if (nsteps == 0) return DH_OK;
-  if (!c.graph_ok) {
-    // capture one step; fall back to direct launches if capture fails
-    hipGraph_t graph = nullptr;
-    hipError_t e = hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal);
-    int r = DH_OK;
-    if (e == hipSuccess) {
-      r = enqueue_step(c);
-      e = hipStreamEndCapture(c.stream, &graph);
-    }
-    if (e == hipSuccess && r == DH_OK && graph) {
-      e = hipGraphInstantiate(&c.step_graph, graph, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(graph);
-      if (e == hipSuccess) c.graph_ok = true;
-    } else {
-      (void)hipGetLastError();
-      if (graph) (void)hipGraphDestroy(graph);
-    }
+  if (nsteps >= kStepsPerGraph && !c.steps_graph_ok) c.steps_graph_ok = capture_steps(c, kStepsPerGraph, &c.steps_graph);
+  if (!c.graph_ok) c.graph_ok = capture_steps(c, 1, &c.step_graph);
+  int left = nsteps;
+  while (c.steps_graph_ok && left >= kStepsPerGraph) {
+    HIPCHK(hipGraphLaunch(c.steps_graph, c.stream));
+    left -= kStepsPerGraph;
   }
-  for (int s = 0; s < nsteps; ++s) {
+  for (; left > 0; --left) {
     if (c.graph_ok) {
       HIPCHK(hipGraphLaunch(c.step_graph, c.stream));
     } else {
-      int r = enqueue_step(c);
+      const int r = enqueue_step(c);
       if (r != DH_OK) return r;
     }
   }

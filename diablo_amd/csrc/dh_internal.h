@@ -72,6 +72,7 @@ struct Clock {
 // Rows of one column held by one lane of the Crank-Nicolson solve; a column
 // is split into 4 (NY+2 <= 144) or 8 such segments.
 constexpr int kSegRows = 36;
+constexpr int kStepsPerGraph = 8;  // time steps captured in one graph for long runs
 // Segments of one (kx,kz) mode's pressure-Poisson sweep over rows 2..NY, one
 // lane each, and the most rows a segment holds in registers.
 constexpr int kRdsSeg = 8;
@@ -131,9 +132,9 @@ struct Ctx {
   Clock *clock = nullptr;
   RealFactors fac{};
   double *rd_l = nullptr, *rd_u = nullptr;  // 1/(DY DYF) tables for REM_DIV
-  // graph of one full time step
-  hipGraphExec_t step_graph = nullptr;
-  bool graph_ok = false;
+  // graphs of one and of kStepsPerGraph full time steps
+  hipGraphExec_t step_graph = nullptr, steps_graph = nullptr;
+  bool graph_ok = false, steps_graph_ok = false;
 };
 
 // ---- launchers (dh_fft.hip) ----
@@ -156,8 +157,7 @@ constexpr int kNumPhases = 6;
 extern const char *const kPhaseNames[kNumPhases];
 hipError_t launch_phase(Ctx &c, int rk, int phase);
 hipError_t launch_rem_div(Ctx &c, int rk_for_cp);     // rk_for_cp<0: no CP update
-hipError_t launch_step_begin(Ctx &c);
-hipError_t launch_end_step(Ctx &c);
+hipError_t launch_step_boundary(Ctx &c, bool end_prev, bool begin_next);  // TIME += DELTA_T, TIME_STEP += 1
 hipError_t launch_cn_tables(Ctx &c, hipStream_t st);
 hipError_t join_side(Ctx &c);  // main stream waits for a pending table rebuild
 hipError_t launch_rem_div_factors(Ctx &c);

@@ -1391,19 +1391,17 @@ hipError_t launch_substep(Ctx &c, int rk) {
   return join_side(c);
 }
 
-__global__ void k_step_begin(Clock *clk) {
-  if (threadIdx.x == 0) clk->time_step += 1;
-}
-__global__ void k_step_end(Clock *clk) {
-  if (threadIdx.x == 0) clk->time = clk->time + clk->dt;
+// Step bookkeeping of diablo.f:66-110 between substeps: TIME = TIME +
+// DELTA_T ending a step (with the DELTA_T COURANT set in it), then
+// TIME_STEP = TIME_STEP + 1 starting the next; one launch serves both.
+__global__ void k_step_boundary(Clock *clk, int end_prev, int begin_next) {
+  if (threadIdx.x != 0) return;
+  if (end_prev) clk->time = clk->time + clk->dt;
+  if (begin_next) clk->time_step += 1;
 }
 
-hipError_t launch_step_begin(Ctx &c) {
-  hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(64), 0, c.stream, c.clock);
-  return hipGetLastError();
-}
-hipError_t launch_end_step(Ctx &c) {
-  hipLaunchKernelGGL(k_step_end, dim3(1), dim3(64), 0, c.stream, c.clock);
+hipError_t launch_step_boundary(Ctx &c, bool end_prev, bool begin_next) {
+  hipLaunchKernelGGL(k_step_boundary, dim3(1), dim3(64), 0, c.stream, c.clock, end_prev ? 1 : 0, begin_next ? 1 : 0);
   return hipGetLastError();
 }
 
