@@ -111,7 +111,7 @@ hipError_t dalloc(T **p, size_t count) {
 }
 
 void free_all(Ctx &c) {
-  void *ptrs[] = {c.tw_row, c.tw_col, c.tw_real, c.posr, c.posrd, c.posz, c.posc, c.kx, c.kz, c.kx2, c.kz2, c.d_dy, c.d_dyf, c.d_gyf, c.cu[0], c.cu[1],
+  void *ptrs[] = {c.tw_row, c.tw_col, c.tw_real, c.posr, c.posrd, c.posz, c.posc, c.kx, c.kz, c.kx2, c.kz2, c.d_dy, c.d_dyf, c.d_gyf, c.d_rdy, c.d_rdyf, c.cu[0], c.cu[1],
                   c.cu[2], c.cp, c.cf[0], c.cf[1], c.cf[2], c.cth, c.cfth, c.phi, c.uphys, c.rphys, c.rd_ta,
                   c.rd_tb, c.clock, c.fac.pf, c.fac.pb, c.fac.tl, c.fac.tb, c.fac.tc, c.fac.tr, c.fac.isbc, c.rd_tr, c.fac.vconst, c.rd_l, c.rd_u, c.stats, c.cun[0], c.cun[1], c.cun[2]};
   for (void *p : ptrs)
@@ -199,6 +199,10 @@ int dh_create(const dh_params *p, dh_ctx **out) {
   g.lw = p->nx / 2;
   g.pp = (long long)p->nx * p->nz;
   g.ls = g.lw + 1;
+  g.dnx = p->nx;
+  g.dnz = p->nz;
+  g.rnx = 1.0 / g.dnx;
+  g.rnz = 1.0 / g.dnz;
   std::vector<int> posr, posrd, posc, posz;
   // radix 16 only along z: its 16-point butterflies cost registers that the
   // row transforms of the 1024-thread kernels do not have
@@ -277,6 +281,8 @@ int dh_create(const dh_params *p, dh_ctx **out) {
   ACHK(dalloc(&c.d_dy, nyy));
   ACHK(dalloc(&c.d_dyf, nyy));
   ACHK(dalloc(&c.d_gyf, nyy));
+  ACHK(dalloc(&c.d_rdy, nyy));
+  ACHK(dalloc(&c.d_rdyf, nyy));
   for (int i = 0; i < 3; ++i) {
     ACHK(dalloc(&c.cu[i], cf_elems));
     ACHK(dalloc(&c.cf[i], cf_elems));
@@ -347,6 +353,8 @@ int dh_create(const dh_params *p, dh_ctx **out) {
   g.kz2 = c.kz2;
   g.dy = c.d_dy;
   g.dyf = c.d_dyf;
+  g.rdy = c.d_rdy;
+  g.rdyf = c.d_rdyf;
   {
     Clock ck{};
     ck.time = 0.0;
@@ -395,6 +403,15 @@ int dh_set_grid(dh_ctx *h, const double *gy, const double *gyf, const double *dy
   HIPCHK(hipMemcpy(c.d_dy, dy, nyy * 8, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c.d_dyf, dyf, nyy * 8, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c.d_gyf, gyf, nyy * 8, hipMemcpyHostToDevice));
+  {
+    std::vector<double> r(nyy), rf(nyy);
+    for (int j = 0; j < nyy; ++j) {
+      r[j] = (c.dy[j] != 0.0) ? 1.0 / c.dy[j] : 0.0;
+      rf[j] = (c.dyf[j] != 0.0) ? 1.0 / c.dyf[j] : 0.0;
+    }
+    HIPCHK(hipMemcpy(c.d_rdy, r.data(), nyy * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c.d_rdyf, rf.data(), nyy * 8, hipMemcpyHostToDevice));
+  }
   // channel.f:1126-1134 coefficient tables
   std::vector<double> rl(nyy, 0.0), ru(nyy, 0.0);
   for (int j = 1; j <= c.g.ny; ++j) {

@@ -52,10 +52,11 @@ __global__ void __launch_bounds__(1024) k_plane_fwd(Geo g, PlaneBatch b) {
   }
   plane_cols_fwd(A, g, T);
   double2 *dst = const_cast<double2 *>(b.four[f]) + j * g.pf;
+  const double nzd = g.dnz, rnz = g.rnz;  // V / NZ as the correctly rounded quotient
   for (int m = threadIdx.x; m < g.pf; m += blockDim.x) {
     const int kzp = (int)fdiv((unsigned)m, g.f_nkx1), kx = m - kzp * g.nkx1;
     const double2 v = A[fwd_slot(g, T, kx, kzp)];
-    dst[m] = make_double2(v.x / g.nz, v.y / g.nz);
+    dst[m] = make_double2(div_by_recip(v.x, nzd, rnz), div_by_recip(v.y, nzd, rnz));
   }
 }
 

@@ -290,6 +290,7 @@ __device__ void plane_rows_fwd(double2 *A, const Geo &g, const Tables &T) {
   // even/odd split: X_k = (Z_k + conj(Z_{h-k}))/2 - i w^-k (Z_k - conj(Z_{h-k}))/2
   const int npair = h / 2 + 1;  // k = 0..h/2 paired with h-k
   const int total = g.nz * npair;
+  const double nxd = g.dnx, rnx = g.rnx;  // X / NX as the correctly rounded quotient
   for (int e = threadIdx.x; e < total; e += blockDim.x) {
     const int k = (int)fdiv((unsigned)e, g.f_nz), z = e - k * g.nz;
     double2 *row = A + z * g.ls;
@@ -301,14 +302,14 @@ __device__ void plane_rows_fwd(double2 *A, const Geo &g, const Tables &T) {
       double2 b = make_double2(0.5 * (zk.x - zm.x), 0.5 * (zk.y + zm.y));  // (Z_k - conj Z_{h-k})/2
       double2 wb = cmul(T.twx[k], b);                                     // w = e^{-2 pi i k/NX}
       double2 X = make_double2(a.x + wb.y, a.y - wb.x);                  // a - i*w*b
-      if (k <= g.nkx) row[k] = make_double2(X.x / g.nx, X.y / g.nx);
+      if (k <= g.nkx) row[k] = make_double2(div_by_recip(X.x, nxd, rnx), div_by_recip(X.y, nxd, rnx));
     }
     if (k != 0 && k2 != k && k2 <= g.nkx) {
       double2 a = make_double2(0.5 * (zm.x + zk.x), 0.5 * (zm.y - zk.y));
       double2 b = make_double2(0.5 * (zm.x - zk.x), 0.5 * (zm.y + zk.y));
       double2 wb = cmul(T.twx[k2], b);
       double2 X = make_double2(a.x + wb.y, a.y - wb.x);
-      row[k2] = make_double2(X.x / g.nx, X.y / g.nx);
+      row[k2] = make_double2(div_by_recip(X.x, nxd, rnx), div_by_recip(X.y, nxd, rnx));
     }
   }
   lds_barrier();
@@ -458,6 +459,7 @@ template <class Fetch, class Eval>
 __device__ void rows_fwd_reg64(double2 *A, const Geo &g, const Tables &T, Fetch fetch, Eval eval) {
   const int lane = threadIdx.x & 63, rho = lane >> 3, t = lane & 7;
   const int nw = blockDim.x >> 6;
+  const double nxd = g.dnx, rnx = g.rnx;  // X / NX as the correctly rounded quotient
   for (int slab = threadIdx.x >> 6; slab * 8 < g.nz; slab += nw) {
     const int z = slab * 8 + rho;  // < NZ: register rows require NZ % 8 == 0
     double2 *row = A + z * g.ls;
@@ -504,7 +506,7 @@ __device__ void rows_fwd_reg64(double2 *A, const Geo &g, const Tables &T, Fetch 
       double2 a = make_double2(0.5 * (zk.x + zmk.x), 0.5 * (zk.y - zmk.y));
       double2 b = make_double2(0.5 * (zk.x - zmk.x), 0.5 * (zk.y + zmk.y));
       double2 wb = cmul(T.twx[k], b);
-      if (k <= g.nkx) row[k] = make_double2((a.x + wb.y) / g.nx, (a.y - wb.x) / g.nx);
+      if (k <= g.nkx) row[k] = make_double2(div_by_recip(a.x + wb.y, nxd, rnx), div_by_recip(a.y - wb.x, nxd, rnx));
     }
   }
   lds_barrier();

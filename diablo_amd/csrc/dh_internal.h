@@ -40,12 +40,14 @@ struct Geo {
   int lw;               // complex elements per row = NX/2
   int ls;               // LDS row stride in complex elements (lw + 1: rows start on distinct banks)
   long long pp;         // physical plane elements NX*NZ
+  double dnx, dnz, rnx, rnz;  // NX, NZ as doubles and their correctly rounded reciprocals
   FftPlan prow, pcol;   // row c2c (length NX/2), column c2c (length NZ)
   const double2 *tw_row;   // exp(-2 pi i m/(NX/2)), m < NX/2
   const double2 *tw_col;   // exp(-2 pi i m/NZ),     m < NZ
   const double2 *tw_real;  // exp(-2 pi i k/NX),      k <= NX/2
   const double *kx, *kz, *kx2, *kz2;  // wavenumbers (fft.f:77-100), de-aliased
   const double *dy, *dyf;             // grid spacing (0:NY+1)
+  const double *rdy, *rdyf;           // their correctly rounded reciprocals (0 where the spacing is 0)
   const int *posr;   // DIT input slot of row index m (length NX/2)
   const int *posrd;  // DIF output slot of row index k (length NX/2)
   const int *posz;   // DIF output slot of column index k (length NZ)
@@ -118,7 +120,7 @@ struct Ctx {
   double2 *tw_row = nullptr, *tw_col = nullptr, *tw_real = nullptr;
   int *posr = nullptr, *posrd = nullptr, *posz = nullptr, *posc = nullptr;
   double *kx = nullptr, *kz = nullptr, *kx2 = nullptr, *kz2 = nullptr;
-  double *d_dy = nullptr, *d_dyf = nullptr, *d_gyf = nullptr;
+  double *d_dy = nullptr, *d_dyf = nullptr, *d_gyf = nullptr, *d_rdy = nullptr, *d_rdyf = nullptr;
   double2 *cu[3] = {}, *cp = nullptr, *cf[3] = {}, *cth = nullptr, *cfth = nullptr;
   double2 *phi = nullptr;       // CR1 after REM_DIV
   double2 *cun[3] = {};         // velocity back in Fourier space before REM_DIV projects it into CU

@@ -57,6 +57,7 @@ struct Prod {
   const double *s[5];
   int kind;
   double w0, w1, w2;
+  double d, r;  // divisor of forms 2-4 and its correctly rounded reciprocal
 };
 
 __device__ __forceinline__ Prod make_prod(const StepArgs &a, int c, int p, int J) {
@@ -91,6 +92,9 @@ __device__ __forceinline__ Prod make_prod(const StepArgs &a, int c, int p, int J
       q.s[3] = pl(U2, J + 1); q.s[4] = pl(U2, J); q.w0 = 2.0 * a.g.dyf[J];
     }
   }
+  // reciprocals from the tables (a power-of-two factor scales RN(1/x) exactly)
+  q.d = (q.kind == 3) ? q.w2 : q.w0;
+  q.r = (q.kind == 2) ? 0.5 * a.g.rdyf[J] : (q.kind == 3) ? 0.5 * a.g.rdy[J] : (q.kind == 4) ? a.g.rdy[J] : 0.0;
   return q;
 }
 
@@ -120,15 +124,17 @@ __device__ __forceinline__ double2 eval_raw(const Prod &q, const ProdRaw<KIND> &
     return make_double2(u.x * u.x, u.y * u.y);
   } else if constexpr (KIND == 2) {
     const double2 up = r.s[0], u0 = r.s[1], um = r.s[2], vp = r.s[3], v0 = r.s[4];
-    return make_double2((up.x * vp.x + u0.x * vp.x - u0.x * v0.x - um.x * v0.x) / q.w0,
-                        (up.y * vp.y + u0.y * vp.y - u0.y * v0.y - um.y * v0.y) / q.w0);
+    return make_double2(div_by_recip(up.x * vp.x + u0.x * vp.x - u0.x * v0.x - um.x * v0.x, q.d, q.r),
+                        div_by_recip(up.y * vp.y + u0.y * vp.y - u0.y * v0.y - um.y * v0.y, q.d, q.r));
   } else if constexpr (KIND == 3) {
     const double2 u0 = r.s[0], um = r.s[1], v = r.s[2];
-    return make_double2(((q.w0 * u0.x + q.w1 * um.x) / q.w2) * v.x, ((q.w0 * u0.y + q.w1 * um.y) / q.w2) * v.y);
+    return make_double2(div_by_recip(q.w0 * u0.x + q.w1 * um.x, q.d, q.r) * v.x,
+                        div_by_recip(q.w0 * u0.y + q.w1 * um.y, q.d, q.r) * v.y);
   } else {
     const double2 vp = r.s[0], v0 = r.s[1], vm = r.s[2];
     const double ax = v0.x + vp.x, bx = v0.x + vm.x, ay = v0.y + vp.y, by = v0.y + vm.y;
-    return make_double2((0.25 * (ax * ax) - 0.25 * (bx * bx)) / q.w0, (0.25 * (ay * ay) - 0.25 * (by * by)) / q.w0);
+    return make_double2(div_by_recip(0.25 * (ax * ax) - 0.25 * (bx * bx), q.d, q.r),
+                        div_by_recip(0.25 * (ay * ay) - 0.25 * (by * by), q.d, q.r));
   }
 }
 
@@ -291,6 +297,9 @@ __global__ void __launch_bounds__(REG ? 512 : 1024) k_nonlinear(StepArgs a) {
   // three products, forward transformed and accumulated in statement order
   const LoadTile lt(g);
   const int nw = nthr >> 6;
+  // quotients by per-block constants take their correctly rounded
+  // reciprocals (div_by_recip): the same doubles as the divisions
+  const double nzd = g.dnz, rnz = g.rnz;
   for (int p = 0; p < 3; ++p) {
     const Prod pr = make_prod(a, c, p, J);
     if constexpr (REG) {
@@ -327,7 +336,7 @@ __global__ void __launch_bounds__(REG ? 512 : 1024) k_nonlinear(StepArgs a) {
       if (m < g.pf) {
         const int kzp = (int)fdiv((unsigned)m, g.f_nkx1), kx = m - kzp * g.nkx1;
         double2 s = A[fwd_slot(g, T, kx, kzp)];
-        s = make_double2(s.x / g.nz, s.y / g.nz);
+        s = make_double2(div_by_recip(s.x, nzd, rnz), div_by_recip(s.y, nzd, rnz));
         if (p == 2) {
           acc[q] = csub(acc[q], s);
         } else {
@@ -352,20 +361,27 @@ __global__ void __launch_bounds__(REG ? 512 : 1024) k_nonlinear(StepArgs a) {
   // transform gives R + T1 NU_V_SCALE d/dy(d/dy U) of the reference
   const double coefv = (c < 3) ? (a.nu * hb / 2.0) * a.nu_v_scale : ((a.nu * hb / 2.0) / a.pr[c - 3]) * a.nu_v_scale;
   double ey0, ey1, ey2;  // weights of C(J+1)-C(J), C(J)-C(J-1) and the outer divisor
+  double rey0, rey1, rey2;
   if (c == 1) {
     ey0 = g.dyf[J];
     ey1 = g.dyf[J - 1];
     ey2 = g.dy[J];
+    rey0 = g.rdyf[J];
+    rey1 = g.rdyf[J - 1];
+    rey2 = g.rdy[J];
   } else {
     ey0 = g.dy[J + 1];
     ey1 = g.dy[J];
     ey2 = g.dyf[J];
+    rey0 = g.rdy[J + 1];
+    rey1 = g.rdy[J];
+    rey2 = g.rdyf[J];
   }
   // modes in batches of NB: the batch's loads (indices clamped to the plane)
   // are all issued before its arithmetic (smaller batches in the
   // 1024-thread variant, whose register budget is half as large)
   constexpr int NB = REG ? 4 : 2;
-  const double dyJ = g.dy[J];
+  const double dyJ = g.dy[J], rdyJ = g.rdy[J];
   static_assert(NQ % NB == 0, "mode batches must tile the modes of a thread");
 #pragma unroll
   for (int q0 = 0; q0 < NQ; q0 += NB) {
@@ -402,8 +418,11 @@ __global__ void __launch_bounds__(REG ? 512 : 1024) k_nonlinear(StepArgs a) {
         const int kzp = (int)fdiv((unsigned)m, g.f_nkx1), kx = m - kzp * g.nkx1;
         const long long o = base + m;
         double2 cr = cc0[b];
-        const double2 ev = make_double2(coefv * (((cp1[b].x - cr.x) / ey0 - (cr.x - cm1[b].x) / ey1) / ey2),
-                                        coefv * (((cp1[b].y - cr.y) / ey0 - (cr.y - cm1[b].y) / ey1) / ey2));
+        const double2 ev = make_double2(
+            coefv * div_by_recip(div_by_recip(cp1[b].x - cr.x, ey0, rey0) - div_by_recip(cr.x - cm1[b].x, ey1, rey1),
+                                 ey2, rey2),
+            coefv * div_by_recip(div_by_recip(cp1[b].y - cr.y, ey0, rey0) - div_by_recip(cr.y - cm1[b].y, ey1, rey1),
+                                 ey2, rey2));
         if (rk > 0) cr = make_double2(cr.x + T3 * old[b].x, cr.y + T3 * old[b].y);
         if (c == 0 || c == 2) {
           double2 ikp = cmul_ik(kk[b], p0[b]);
@@ -415,7 +434,7 @@ __global__ void __launch_bounds__(REG ? 512 : 1024) k_nonlinear(StepArgs a) {
               cr.x = cr.x - T4 * (a.px0 + a.amp_omega0 * cos(a.omega0 * time));
           }
         } else if (c == 1) {
-          double2 cs = make_double2((p0[b].x - pm[b].x) / dyJ, (p0[b].y - pm[b].y) / dyJ);
+          double2 cs = make_double2(div_by_recip(p0[b].x - pm[b].x, dyJ, rdyJ), div_by_recip(p0[b].y - pm[b].y, dyJ, rdyJ));
           cr = make_double2(cr.x - T4 * cs.x, cr.y - T4 * cs.y);
         }
         const double2 cfn = acc[q0 + b];
@@ -993,8 +1012,9 @@ __global__ void __launch_bounds__(256) k_rem_div_thomas(RemDivArgs a) {
         double2 V = make_double2(0.0, 0.0);
         if (live && J < NY) {
           const double2 t1 = cmul_ik(kxv, c1[b]), t3 = cmul_ik(kzv, c3[b]);
-          const double d = g.dyf[J];
-          V = make_double2(t1.x + (c2p[b].x - c2[b].x) / d + t3.x, t1.y + (c2p[b].y - c2[b].y) / d + t3.y);
+          const double d = g.dyf[J], rd = g.rdyf[J];
+          V = make_double2(t1.x + div_by_recip(c2p[b].x - c2[b].x, d, rd) + t3.x,
+                           t1.y + div_by_recip(c2p[b].y - c2[b].y, d, rd) + t3.y);
         }
         Gs[J * kRdtModes + lane] = V;
         As[J * kRdtModes + lane] = live ? ta[b] : 0.0;
@@ -1097,6 +1117,7 @@ __global__ void __launch_bounds__(256) k_rem_div_thomas(RemDivArgs a) {
   lds_barrier();
   if (!live) return;
   const double hb = (a.rk < 0) ? 1.0 : (a.cp_old ? a.clk->h_cp : a.clk->h_bar[a.rk]);
+  const double rhb = 1.0 / hb;
   for (int J0 = 1 + rg; J0 <= NY; J0 += 4 * ngrp) {
     double2 u1[4], u2[4], u3[4], pc[4];
 #pragma unroll
@@ -1121,9 +1142,11 @@ __global__ void __launch_bounds__(256) k_rem_div_thomas(RemDivArgs a) {
       a.cu[2][pf * J + m] = make_double2(u3[b].x - t.x, u3[b].y - t.y);
       if (J >= 2) {
         const double2 pm = Gs[(J - 1) * kRdtModes + lane];
-        const double d = g.dy[J];
-        a.cu[1][pf * J + m] = make_double2(u2[b].x - (phi.x - pm.x) / d, u2[b].y - (phi.y - pm.y) / d);
-        if (a.rk >= 0) a.cp[pf * J + m] = make_double2(pc[b].x + phi.x / hb, pc[b].y + phi.y / hb);
+        const double d = g.dy[J], rd = g.rdy[J];
+        a.cu[1][pf * J + m] = make_double2(u2[b].x - div_by_recip(phi.x - pm.x, d, rd),
+                                           u2[b].y - div_by_recip(phi.y - pm.y, d, rd));
+        if (a.rk >= 0)
+          a.cp[pf * J + m] = make_double2(pc[b].x + div_by_recip(phi.x, hb, rhb), pc[b].y + div_by_recip(phi.y, hb, rhb));
       }
     }
   }
