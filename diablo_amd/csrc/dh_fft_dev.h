@@ -454,15 +454,30 @@ __device__ __forceinline__ void wave_lds_sync() {
 
 // Forward x transform of every row, physical pairs from load(z, m) (global
 // memory or computed products) to X_k / NX for k <= NKX in natural slots of
-// the LDS rows (FFT_XZ_TO_FOURIER's r2c, mpi.f:1490-1506).
-template <class Fetch, class Eval>
-__device__ void rows_fwd_reg64(double2 *A, const Geo &g, const Tables &T, Fetch fetch, Eval eval) {
+// the LDS rows (FFT_XZ_TO_FOURIER's r2c, mpi.f:1490-1506).  MERGE: the rows
+// already hold the transform R_k of another product; the result stored is
+// i KX(k) R_k + X_k / NX (kxv = KX), the sum whose z transform is that of
+// both products' contributions.
+template <bool MERGE = false, class Fetch, class Eval>
+__device__ void rows_fwd_reg64(double2 *A, const Geo &g, const Tables &T, Fetch fetch, Eval eval,
+                               const double *kxv = nullptr) {
   const int lane = threadIdx.x & 63, rho = lane >> 3, t = lane & 7;
   const int nw = blockDim.x >> 6;
   const double nxd = g.dnx, rnx = g.rnx;  // X / NX as the correctly rounded quotient
   for (int slab = threadIdx.x >> 6; slab * 8 < g.nz; slab += nw) {
     const int z = slab * 8 + rho;  // < NZ: register rows require NZ % 8 == 0
     double2 *row = A + z * g.ls;
+    // the other product's outputs of this lane, read before the row is reused
+    double2 prev[kRowK2];
+    double kxp[kRowK2];
+    if constexpr (MERGE) {
+#pragma unroll
+      for (int k2 = 0; k2 < kRowK2; ++k2) {
+        const int k = min(t + 8 * k2, g.nkx);
+        prev[k2] = row[k];
+        kxp[k2] = kxv[k];
+      }
+    }
     // the loads of a group of G samples are all issued before any arithmetic
     // on them; left to itself the scheduler interleaves loads and arithmetic
     // sample by sample and the memory latency is exposed eight times per
@@ -494,19 +509,24 @@ __device__ void rows_fwd_reg64(double2 *A, const Geo &g, const Tables &T, Fetch 
 #pragma unroll
     for (int k2 = 0; k2 < 8; ++k2) row[t + 8 * k2] = v[k2];
     wave_lds_sync();
-    double2 zm[8];
+    double2 zm[kRowK2];
 #pragma unroll
-    for (int k2 = 0; k2 < 8; ++k2) zm[k2] = row[(64 - t - 8 * k2) & 63];
+    for (int k2 = 0; k2 < kRowK2; ++k2) zm[k2] = row[(64 - t - 8 * k2) & 63];
     wave_lds_sync();
     // X_k = (Z_k + conj Z_{64-k})/2 - i w^k (Z_k - conj Z_{64-k})/2, w = e^{-2 pi i/NX}
 #pragma unroll
-    for (int k2 = 0; k2 < 8; ++k2) {
+    for (int k2 = 0; k2 < kRowK2; ++k2) {
       const int k = t + 8 * k2;
       const double2 zk = v[k2], zmk = zm[k2];
       double2 a = make_double2(0.5 * (zk.x + zmk.x), 0.5 * (zk.y - zmk.y));
       double2 b = make_double2(0.5 * (zk.x - zmk.x), 0.5 * (zk.y + zmk.y));
       double2 wb = cmul(T.twx[k], b);
-      if (k <= g.nkx) row[k] = make_double2(div_by_recip(a.x + wb.y, nxd, rnx), div_by_recip(a.y - wb.x, nxd, rnx));
+      double2 x = make_double2(div_by_recip(a.x + wb.y, nxd, rnx), div_by_recip(a.y - wb.x, nxd, rnx));
+      if constexpr (MERGE) {
+        const double2 ik = cmul_ik(kxp[k2], prev[k2]);
+        x = make_double2(ik.x + x.x, ik.y + x.y);
+      }
+      if (k <= g.nkx) row[k] = x;
     }
   }
   lds_barrier();
@@ -514,7 +534,7 @@ __device__ void rows_fwd_reg64(double2 *A, const Geo &g, const Tables &T, Fetch 
 
 template <class Load>
 __device__ __forceinline__ void rows_fwd_reg64(double2 *A, const Geo &g, const Tables &T, Load load) {
-  rows_fwd_reg64(A, g, T, load, [](double2 v) { return v; });
+  rows_fwd_reg64<false>(A, g, T, load, [](double2 v) { return v; });
 }
 
 // Inverse x transform of every row (FFT_XZ_TO_PHYSICAL's c2r,

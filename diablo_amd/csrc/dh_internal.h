@@ -79,6 +79,9 @@ constexpr int kStepsPerGraph = 8;  // time steps captured in one graph for long 
 // lane each, and the most rows a segment holds in registers.
 constexpr int kRdsSeg = 8;
 constexpr int kRdsRows = 17;
+// Output k = t + 8 k2 of a register row transform (NX = 128) is stored only
+// for k <= NKX = NX/3 = 42, so k2 < kRowK2 covers every stored output.
+constexpr int kRowK2 = 6;
 
 // Tables of the y-implicit (Crank-Nicolson) matrices, one set per RK
 // substep and field; with LES off each matrix depends only on j, so every

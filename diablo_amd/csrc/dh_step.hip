@@ -245,19 +245,6 @@ __device__ __forceinline__ void cf_init_all(const StepArgs &a, int c, int J, int
   }
 }
 
-// Forward row transforms of product pr, its form resolved once so that each
-// slab's loads are straight-line code ahead of the arithmetic.
-template <class Fn>
-__device__ __forceinline__ void with_prod_kind(int kind, Fn fn) {
-  switch (kind) {
-    case 0: fn(std::integral_constant<int, 0>{}); break;
-    case 1: fn(std::integral_constant<int, 1>{}); break;
-    case 2: fn(std::integral_constant<int, 2>{}); break;
-    case 3: fn(std::integral_constant<int, 3>{}); break;
-    default: fn(std::integral_constant<int, 4>{}); break;
-  }
-}
-
 // Fused explicit pass for plane J (1..NY) and component c (U1,U2,U3,TH...).
 // REG: row transforms in registers, 512 threads (twice the register budget
 // of the 1024-thread LDS variant), NQ Fourier modes owned per thread.
@@ -294,40 +281,24 @@ __global__ void __launch_bounds__(REG ? 512 : 1024) k_nonlinear(StepArgs a) {
 
   double2 acc[NQ];
   cf_init_all<NQ>(a, c, J, nthr, acc);
-  // three products, forward transformed and accumulated in statement order
+  // the three products, forward transformed and subtracted with their
+  // derivative factors: c==0 takes (ikz, ikx, 1), all others (ikx, ikz, 1)
+  // (statement order of channel.f:410-571, 669-720)
   const LoadTile lt(g);
   const int nw = nthr >> 6;
   // quotients by per-block constants take their correctly rounded
   // reciprocals (div_by_recip): the same doubles as the divisions
   const double nzd = g.dnz, rnz = g.rnz;
-  for (int p = 0; p < 3; ++p) {
-    const Prod pr = make_prod(a, c, p, J);
-    if constexpr (REG) {
-      with_prod_kind(pr.kind, [&](auto K) {
-        constexpr int KIND = decltype(K)::value;
-        rows_fwd_reg64(
-            A, g, T, [&](int z, int xp) { return fetch_prod<KIND>(pr, (long long)z * g.nx + 2 * xp); },
-            [&](const ProdRaw<KIND> &r) { return eval_raw<KIND>(pr, r); });
-      });
-    } else {
-      for (int t0 = threadIdx.x >> 6; t0 < lt.ntiles; t0 += nw) {
-        int z, xp;
-        if (lt.at(g, t0, z, xp)) A[phys_slot(T, g, z, xp)] = eval_prod(pr, (long long)z * g.nx + 2 * xp);
-      }
-      lds_barrier();
-      plane_rows_fwd(A, g, T);
-    }
-    plane_cols_fwd(A, g, T);
-    // derivative factor: c==0 takes (ikz, ikx, 1); all others (ikx, ikz, 1).
-    // The wavenumbers of all modes are loaded ahead of the LDS reads.
-    const bool use_kz = (c == 0) ? (p == 0) : (p == 1);
+  // acc -= f (plane / NZ) after the z transform, f = ikz (mode 0), ikx (1)
+  // or 1 (2); the wavenumbers are loaded ahead of the LDS reads
+  auto accumulate = [&](int mode) {
     double kq[NQ];
-    if (p != 2) {
+    if (mode != 2) {
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
         const int m = min((int)threadIdx.x + q * nthr, g.pf - 1);
         const int kzp = (int)fdiv((unsigned)m, g.f_nkx1), kx = m - kzp * g.nkx1;
-        kq[q] = use_kz ? g.kz[kzp] : g.kx[kx];
+        kq[q] = (mode == 0) ? g.kz[kzp] : g.kx[kx];
       }
     }
 #pragma unroll
@@ -337,14 +308,57 @@ __global__ void __launch_bounds__(REG ? 512 : 1024) k_nonlinear(StepArgs a) {
         const int kzp = (int)fdiv((unsigned)m, g.f_nkx1), kx = m - kzp * g.nkx1;
         double2 s = A[fwd_slot(g, T, kx, kzp)];
         s = make_double2(div_by_recip(s.x, nzd, rnz), div_by_recip(s.y, nzd, rnz));
-        if (p == 2) {
-          acc[q] = csub(acc[q], s);
-        } else {
-          acc[q] = csub(acc[q], cmul_ik(kq[q], s));
-        }
+        acc[q] = csub(acc[q], (mode == 2) ? s : cmul_ik(kq[q], s));
       }
     }
     lds_barrier();
+  };
+  const int pz = (c == 0) ? 0 : 1;  // the ikz product; the other of 0, 1 takes ikx
+  if constexpr (REG) {
+    // x transforms in registers.  The ikz product has its own z transform;
+    // the ikx product and the plain y-flux product (p = 2) share one: the
+    // x-transformed ikx product is scaled by i KX and added to the y flux's
+    // (both transforms are linear and KX is constant along z).
+    auto rows = [&](int p, auto merge) {
+      constexpr bool MERGE = decltype(merge)::value;
+      const Prod pr = make_prod(a, c, p, J);
+      // the form is resolved once so that each slab's loads are
+      // straight-line code ahead of the arithmetic
+      auto run = [&](auto K) {
+        constexpr int KIND = decltype(K)::value;
+        rows_fwd_reg64<MERGE>(
+            A, g, T, [&](int z, int xp) { return fetch_prod<KIND>(pr, (long long)z * g.nx + 2 * xp); },
+            [&](const ProdRaw<KIND> &r) { return eval_raw<KIND>(pr, r); }, g.kx);
+      };
+      // forms: the y fluxes (p = 2) are 2 or 4, the others 0, 1 or 3
+      if constexpr (MERGE) {
+        if (pr.kind == 2) run(std::integral_constant<int, 2>{});
+        else run(std::integral_constant<int, 4>{});
+      } else {
+        if (pr.kind == 0) run(std::integral_constant<int, 0>{});
+        else if (pr.kind == 1) run(std::integral_constant<int, 1>{});
+        else run(std::integral_constant<int, 3>{});
+      }
+    };
+    rows(pz, std::false_type{});
+    plane_cols_fwd(A, g, T);
+    accumulate(0);
+    rows(1 - pz, std::false_type{});
+    rows(2, std::true_type{});
+    plane_cols_fwd(A, g, T);
+    accumulate(2);
+  } else {
+    for (int p = 0; p < 3; ++p) {
+      const Prod pr = make_prod(a, c, p, J);
+      for (int t0 = threadIdx.x >> 6; t0 < lt.ntiles; t0 += nw) {
+        int z, xp;
+        if (lt.at(g, t0, z, xp)) A[phys_slot(T, g, z, xp)] = eval_prod(pr, (long long)z * g.nx + 2 * xp);
+      }
+      lds_barrier();
+      plane_rows_fwd(A, g, T);
+      plane_cols_fwd(A, g, T);
+      accumulate(p == 2 ? 2 : (p == pz ? 0 : 1));
+    }
   }
   // RK combination (channel.f:88-167, 581-595; scalars 254-270, 723-729),
   // new CF stored, CR scattered into LDS for the inverse transform.
