@@ -69,6 +69,8 @@ EXPORTS = {
     "dh_get_clock": (ctypes.c_int, [ctypes.c_void_p, _DP, _DP, _DP, ctypes.POINTER(ctypes.c_int32)]),
     "dh_rk_substep": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "dh_time_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "dh_set_mixed_precision": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "dh_mixed_precision": (ctypes.c_int, [ctypes.c_void_p]),
     "dh_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
     "dh_profile_substep": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, _DP, ctypes.c_int32]),
     "dh_phase_name": (ctypes.c_char_p, [ctypes.c_int32]),
@@ -266,6 +268,14 @@ class Channel:
 
     def time_step(self, n=1):
         self._check(self.lib.dh_time_step(self.h, n), "dh_time_step")
+
+    def set_mixed_precision(self, on):
+        """Switch the FP32/FP64 mixed form of the substep on or off."""
+        self._check(self.lib.dh_set_mixed_precision(self.h, 1 if on else 0), "dh_set_mixed_precision")
+
+    @property
+    def mixed_precision(self):
+        return bool(self.lib.dh_mixed_precision(self.h))
 
     def synchronize(self):
         self._check(self.lib.dh_synchronize(self.h), "dh_synchronize")

@@ -113,7 +113,7 @@ hipError_t dalloc(T **p, size_t count) {
 void free_all(Ctx &c) {
   void *ptrs[] = {c.tw_row, c.tw_col, c.tw_real, c.posr, c.posrd, c.posz, c.posc, c.kx, c.kz, c.kx2, c.kz2, c.d_dy, c.d_dyf, c.d_gyf, c.d_rdy, c.d_rdyf, c.cu[0], c.cu[1],
                   c.cu[2], c.cp, c.cf[0], c.cf[1], c.cf[2], c.cth, c.cfth, c.phi, c.uphys, c.rphys, c.rd_ta,
-                  c.rd_tb, c.clock, c.fac.pf, c.fac.pb, c.fac.tl, c.fac.tb, c.fac.tc, c.fac.tr, c.fac.isbc, c.rd_tr, c.fac.vconst, c.rd_l, c.rd_u, c.stats, c.cun[0], c.cun[1], c.cun[2]};
+                  c.rd_tb, c.clock, c.fac.pf, c.fac.pb, c.fac.tl, c.fac.tb, c.fac.tc, c.fac.tr, c.fac.isbc, c.rd_tr, c.fac.vconst, c.rd_l, c.rd_u, c.stats, c.cun[0], c.cun[1], c.cun[2], c.uphys32, c.rphys32, c.fac.bcl, c.fac.bcd, c.fac.bcu};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c.step_graph) (void)hipGraphExecDestroy(c.step_graph);
@@ -211,6 +211,8 @@ int dh_create(const dh_params *p, dh_ctx **out) {
   c.nfields = 3 + p->n_th;
   c.lds_bytes = g.nz * g.ls * (int)sizeof(double2) + (2 * g.lw + g.nz + 1) * (int)sizeof(double2) +
                 (2 * g.lw + 2 * g.nz) * (int)sizeof(int);
+  c.lds_bytes32 = g.nz * g.ls * (int)sizeof(float2) + (2 * g.lw + g.nz + 1) * (int)sizeof(float2) +
+                  (2 * g.lw + 2 * g.nz) * (int)sizeof(int);
   if (c.lds_bytes > 160 * 1024) return reject("x-z plane too large for the LDS-resident plane kernels");
   if (g.ny + 2 > 8 * kSegRows) return reject("NY+2 is beyond the column solver's register segments (8 x 36 rows)");
   int thr = (g.nz * g.lw + 7) / 8;
@@ -316,6 +318,9 @@ int dh_create(const dh_params *p, dh_ctx **out) {
   c.fac.seg_rows = (int)((nyy + c.fac.nseg - 1) / c.fac.nseg);
   ACHK(dalloc(&c.fac.isbc, (size_t)c.nfields * nyy));
   ACHK(dalloc(&c.fac.vconst, (size_t)c.nfields * nyy));
+  ACHK(dalloc(&c.fac.bcl, (size_t)c.nfields * nyy));
+  ACHK(dalloc(&c.fac.bcd, (size_t)c.nfields * nyy));
+  ACHK(dalloc(&c.fac.bcu, (size_t)c.nfields * nyy));
   {
     auto tr = twiddles(g.lw, g.lw), tc = twiddles(g.nz, g.nz), tx = twiddles(g.nx, g.lw + 1);
     ACHK(hipMemcpy(c.tw_row, tr.data(), tr.size() * sizeof(double2), hipMemcpyHostToDevice));
@@ -585,8 +590,15 @@ int dh_upload(dh_ctx *h, const dh_host_state *st) {
     if (st->cth && (r = pack_fourier(c, st->cth, c.cth, nth, "CTH")) != DH_OK) return r;
     if (st->cfth && (r = pack_fourier(c, st->cfth, c.cfth, nth, "CFTH")) != DH_OK) return r;
     if (st->th && (r = pack_phys(c, st->th, c.uphys + 3 * fs, nth)) != DH_OK) return r;
+    if (st->th) c.th_phys32 = false;
   }
   return DH_OK;
+}
+
+// FP32 physical fields widened to FP64 (the mixed form's physical TH).
+__global__ void k_widen(const float *src, double *dst, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    dst[i] = src[i];
 }
 
 int dh_download(dh_ctx *h, dh_host_state *st) {
@@ -603,6 +615,14 @@ int dh_download(dh_ctx *h, dh_host_state *st) {
   if (nth > 0) {
     if (st->cth && (r = unpack_fourier(c, c.cth, st->cth, nth)) != DH_OK) return r;
     if (st->cfth && (r = unpack_fourier(c, c.cfth, st->cfth, nth)) != DH_OK) return r;
+    if (st->th && c.th_phys32) {
+      // the last substep ran in the mixed form: its TH is in the FP32 array
+      hipLaunchKernelGGL(k_widen, dim3(1024), dim3(256), 0, c.stream, c.uphys32 + 3 * fs, c.uphys + 3 * fs,
+                         (long long)nth * fs);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipStreamSynchronize(c.stream));
+      c.th_phys32 = false;
+    }
     if (st->th && (r = unpack_phys(c, c.uphys + 3 * fs, st->th, nth)) != DH_OK) return r;
   }
   return DH_OK;
@@ -614,8 +634,33 @@ int dh_rk_substep(dh_ctx *h, int32_t rk_step) {
   if (rk_step < 1 || rk_step > 3) return fail(c, DH_EARG, "rk_step must be 1..3");
   if (!c.grid_set) return fail(c, DH_ESTATE, "dh_set_grid first");
   HIPCHK(launch_substep(c, rk_step - 1));
+  c.th_phys32 = c.mixed;
   return DH_OK;
 }
+
+static void drop_graphs(Ctx &c) {
+  if (c.step_graph) (void)hipGraphExecDestroy(c.step_graph);
+  if (c.steps_graph) (void)hipGraphExecDestroy(c.steps_graph);
+  c.step_graph = c.steps_graph = nullptr;
+  c.graph_ok = c.steps_graph_ok = false;
+}
+
+int dh_set_mixed_precision(dh_ctx *h, int32_t on) {
+  if (!h || (on != 0 && on != 1)) return DH_EARG;
+  Ctx &c = h->c;
+  if ((on != 0) == c.mixed) return DH_OK;
+  HIPCHK(hipStreamSynchronize(c.stream));
+  if (on && !c.uphys32) {
+    const size_t n = (size_t)(c.g.ny + 2) * (size_t)c.g.pp * (size_t)c.nfields;
+    HIPCHK(dalloc(&c.uphys32, n));
+    HIPCHK(dalloc(&c.rphys32, n));
+  }
+  c.mixed = (on != 0);
+  drop_graphs(c);  // the captured steps launch the other form's kernels
+  return DH_OK;
+}
+
+int dh_mixed_precision(dh_ctx *h) { return h ? (h->c.mixed ? 1 : 0) : DH_EARG; }
 
 // nsteps iterations of diablo.f:66-110 without I/O: TIME_STEP += 1, the
 // three substeps, TIME += DELTA_T; the end of one step and the start of the
@@ -670,6 +715,7 @@ int dh_time_step(dh_ctx *h, int32_t nsteps) {
       if (r != DH_OK) return r;
     }
   }
+  c.th_phys32 = c.mixed;
   return DH_OK;
 }
 
@@ -847,6 +893,8 @@ double *dh_field_dev(dh_ctx *h, int32_t which) {
   if (which >= 8 && which < 8 + nth) return reinterpret_cast<double *>(c.cth + (which - 8) * cfs);
   if (which >= 8 + nth && which < 8 + 2 * nth) return reinterpret_cast<double *>(c.cfth + (which - 8 - nth) * cfs);
   if (which >= 100 && which < 100 + c.nfields) return c.uphys + (which - 100) * fs;
+  if (which >= 200 && which < 200 + c.nfields && c.uphys32)
+    return reinterpret_cast<double *>(c.uphys32 + (which - 200) * fs);
   return nullptr;
 }
 

@@ -7,27 +7,36 @@
 
 namespace dh {
 
+// Physical planes are double (FP64 transforms, C = double2) or float (the
+// mixed form, C = float2); the Fourier side is always FP64.
 struct PlaneBatch {
   const double2 *four[kMaxFields];  // Fourier source/destination per field
-  double *phys[kMaxFields];         // physical destination/source per field
+  void *phys[kMaxFields];           // physical destination/source per field (Sc<C>)
   int j0;                           // first plane
   // forward only: planes alo..ahi of field f come from alt[f] when set
-  const double *alt[kMaxFields];
+  const void *alt[kMaxFields];
   int alo[kMaxFields], ahi[kMaxFields];
+  // forward only, the mixed form: when base[f] is set, planes alo..ahi of
+  // alt[f] hold increments and the destination becomes base + transform;
+  // the other planes are base itself (base may be the destination)
+  const double2 *base[kMaxFields];
+  // inverse only, the mixed form: transform the fluctuations (no plane mean)
+  int drop_mean;
 };
 
 // Fourier -> physical for planes j0 + blockIdx.x of field blockIdx.y.
-template <bool REG>
+template <bool REG, class C>
 __global__ void __launch_bounds__(1024) k_plane_inv(Geo g, PlaneBatch b) {
-  extern __shared__ double2 A[];
-  const Tables T = load_tables(A, g);
+  extern __shared__ double2 smem[];
+  C *A = reinterpret_cast<C *>(smem);
+  const auto T = load_tables(A, g);
   const int f = blockIdx.y;
   const long long j = b.j0 + blockIdx.x;
-  plane_unpack(A, g, T, b.four[f] + j * g.pf);
+  plane_unpack(A, g, T, b.four[f] + j * g.pf, b.drop_mean != 0);
   plane_cols_inv(A, g, T);
-  double2 *dst = reinterpret_cast<double2 *>(b.phys[f] + j * g.pp);
+  C *dst = reinterpret_cast<C *>(static_cast<Sc<C> *>(b.phys[f]) + j * g.pp);
   if constexpr (REG) {
-    rows_inv_reg64(A, g, T, [&](int z, int m, double2 v) { dst[z * 64 + m] = v; });
+    rows_inv_reg64(A, g, T, [&](int z, int m, C v) { dst[z * 64 + m] = v; });
   } else {
     plane_rows_inv(A, g, T);
     plane_store_phys(A, g, T, dst);
@@ -36,14 +45,22 @@ __global__ void __launch_bounds__(1024) k_plane_inv(Geo g, PlaneBatch b) {
 
 // physical -> Fourier (x r2c with 1/NX and truncation, z c2c with 1/NZ and
 // packing of kz = -NKZ..-1 after 0..NKZ).
-template <bool REG>
+template <bool REG, class C>
 __global__ void __launch_bounds__(1024) k_plane_fwd(Geo g, PlaneBatch b) {
-  extern __shared__ double2 A[];
-  const Tables T = load_tables(A, g);
+  extern __shared__ double2 smem[];
+  C *A = reinterpret_cast<C *>(smem);
+  const auto T = load_tables(A, g);
   const int f = blockIdx.y;
   const long long j = b.j0 + blockIdx.x;
-  const double *ph = (b.alt[f] && j >= b.alo[f] && j <= b.ahi[f]) ? b.alt[f] : b.phys[f];
-  const double2 *src = reinterpret_cast<const double2 *>(ph + j * g.pp);
+  const double2 *base = b.base[f] ? b.base[f] + j * g.pf : nullptr;
+  double2 *dst = const_cast<double2 *>(b.four[f]) + j * g.pf;
+  if (base && (j < b.alo[f] || j > b.ahi[f])) {  // no increment on this plane
+    if (base != dst)
+      for (int m = threadIdx.x; m < g.pf; m += blockDim.x) dst[m] = base[m];
+    return;
+  }
+  const void *ph = (b.alt[f] && j >= b.alo[f] && j <= b.ahi[f]) ? b.alt[f] : b.phys[f];
+  const C *src = reinterpret_cast<const C *>(static_cast<const Sc<C> *>(ph) + j * g.pp);
   if constexpr (REG) {
     rows_fwd_reg64(A, g, T, [&](int z, int m) { return src[z * 64 + m]; });
   } else {
@@ -51,12 +68,13 @@ __global__ void __launch_bounds__(1024) k_plane_fwd(Geo g, PlaneBatch b) {
     plane_rows_fwd(A, g, T);
   }
   plane_cols_fwd(A, g, T);
-  double2 *dst = const_cast<double2 *>(b.four[f]) + j * g.pf;
   const double nzd = g.dnz, rnz = g.rnz;  // V / NZ as the correctly rounded quotient
   for (int m = threadIdx.x; m < g.pf; m += blockDim.x) {
     const int kzp = (int)fdiv((unsigned)m, g.f_nkx1), kx = m - kzp * g.nkx1;
-    const double2 v = A[fwd_slot(g, T, kx, kzp)];
-    dst[m] = make_double2(div_by_recip(v.x, nzd, rnz), div_by_recip(v.y, nzd, rnz));
+    const C v = A[fwd_slot(g, T, kx, kzp)];
+    double2 r = make_double2(div_by_recip((double)v.x, nzd, rnz), div_by_recip((double)v.y, nzd, rnz));
+    if (base) r = make_double2(base[m].x + r.x, base[m].y + r.y);
+    dst[m] = r;
   }
 }
 
@@ -64,16 +82,33 @@ static hipError_t set_lds(const void *fn, int bytes) {
   return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
 }
 
+// Launches of the plane kernels in precision C (register rows when the
+// plane allows them).
+template <class C>
+static hipError_t plane_inv(Ctx &c, dim3 grid, hipStream_t st, const PlaneBatch &b) {
+  const int lds = lds_bytes_of<C>(c);
+  if (c.reg_rows)
+    hipLaunchKernelGGL((k_plane_inv<true, C>), grid, dim3(c.plane_threads), lds, st, c.g, b);
+  else
+    hipLaunchKernelGGL((k_plane_inv<false, C>), grid, dim3(c.plane_threads), lds, st, c.g, b);
+  return hipGetLastError();
+}
+template <class C>
+static hipError_t plane_fwd(Ctx &c, dim3 grid, hipStream_t st, const PlaneBatch &b) {
+  const int lds = lds_bytes_of<C>(c);
+  if (c.reg_rows)
+    hipLaunchKernelGGL((k_plane_fwd<true, C>), grid, dim3(c.plane_threads), lds, st, c.g, b);
+  else
+    hipLaunchKernelGGL((k_plane_fwd<false, C>), grid, dim3(c.plane_threads), lds, st, c.g, b);
+  return hipGetLastError();
+}
+
 hipError_t launch_fft_to_phys(Ctx &c, const double2 *four, double *phys, int nplanes) {
   PlaneBatch b{};
   b.four[0] = four;
   b.phys[0] = phys;
   b.j0 = 0;
-  if (c.reg_rows)
-    hipLaunchKernelGGL(k_plane_inv<true>, dim3(nplanes, 1), dim3(c.plane_threads), c.lds_bytes, c.stream, c.g, b);
-  else
-    hipLaunchKernelGGL(k_plane_inv<false>, dim3(nplanes, 1), dim3(c.plane_threads), c.lds_bytes, c.stream, c.g, b);
-  return hipGetLastError();
+  return plane_inv<double2>(c, dim3(nplanes, 1), c.stream, b);
 }
 
 hipError_t launch_fft_to_four(Ctx &c, const double *phys, double2 *four, int nplanes) {
@@ -81,35 +116,27 @@ hipError_t launch_fft_to_four(Ctx &c, const double *phys, double2 *four, int npl
   b.four[0] = four;
   b.phys[0] = const_cast<double *>(phys);
   b.j0 = 0;
-  if (c.reg_rows)
-    hipLaunchKernelGGL(k_plane_fwd<true>, dim3(nplanes, 1), dim3(c.plane_threads), c.lds_bytes, c.stream, c.g, b);
-  else
-    hipLaunchKernelGGL(k_plane_fwd<false>, dim3(nplanes, 1), dim3(c.plane_threads), c.lds_bytes, c.stream, c.g, b);
-  return hipGetLastError();
+  return plane_fwd<double2>(c, dim3(nplanes, 1), c.stream, b);
 }
 
-// All state fields (U1..3, TH) to physical space, planes 0..NY+1
-// (channel.f:386-399).
+// All state fields (U1..3, TH) to the substep's physical arrays, planes
+// 0..NY+1 (channel.f:386-399).
 hipError_t launch_state_to_phys(Ctx &c) {
   PlaneBatch b{};
   const long long fs = (long long)(c.g.ny + 2) * c.g.pp;
   const long long cfs = (long long)(c.g.ny + 2) * c.g.pf;
   for (int f = 0; f < c.nfields; ++f) {
     b.four[f] = (f < 3) ? c.cu[f] : c.cth + (f - 3) * cfs;
-    b.phys[f] = c.uphys + f * fs;
+    b.phys[f] = c.mixed ? (void *)(c.uphys32 + f * fs) : (void *)(c.uphys + f * fs);
   }
   b.j0 = 0;
-  if (c.reg_rows)
-    hipLaunchKernelGGL(k_plane_inv<true>, dim3(c.g.ny + 2, c.nfields), dim3(c.plane_threads), c.lds_bytes, c.stream,
-                       c.g, b);
-  else
-    hipLaunchKernelGGL(k_plane_inv<false>, dim3(c.g.ny + 2, c.nfields), dim3(c.plane_threads), c.lds_bytes, c.stream,
-                       c.g, b);
-  return hipGetLastError();
+  b.drop_mean = c.mixed;  // the mixed form keeps the plane means in FP64
+  const dim3 grid(c.g.ny + 2, c.nfields);
+  return c.mixed ? plane_inv<float2>(c, grid, c.stream, b) : plane_inv<double2>(c, grid, c.stream, b);
 }
 
-// Fields f0..f1-1 (U1,U2,U3,TH...) back to Fourier space, planes 0..NY+1
-// (channel.f:1043-1050).
+// FP64 fields f0..f1-1 (U1,U2,U3,TH...) back to Fourier space, planes
+// 0..NY+1 (channel.f:1043-1050).
 hipError_t launch_fields_to_four(Ctx &c, int f0, int f1) {
   PlaneBatch b{};
   const long long fs = (long long)(c.g.ny + 2) * c.g.pp;
@@ -120,13 +147,7 @@ hipError_t launch_fields_to_four(Ctx &c, int f0, int f1) {
   }
   b.j0 = 0;
   if (f1 <= f0) return hipSuccess;
-  if (c.reg_rows)
-    hipLaunchKernelGGL(k_plane_fwd<true>, dim3(c.g.ny + 2, f1 - f0), dim3(c.plane_threads), c.lds_bytes, c.stream,
-                       c.g, b);
-  else
-    hipLaunchKernelGGL(k_plane_fwd<false>, dim3(c.g.ny + 2, f1 - f0), dim3(c.plane_threads), c.lds_bytes, c.stream,
-                       c.g, b);
-  return hipGetLastError();
+  return plane_fwd<double2>(c, dim3(c.g.ny + 2, f1 - f0), c.stream, b);
 }
 
 hipError_t launch_state_to_four(Ctx &c) { return launch_fields_to_four(c, 0, c.nfields); }
@@ -143,29 +164,40 @@ hipError_t launch_step_to_four(Ctx &c, int f0, int f1, hipStream_t st) {
   const long long cfs = (long long)(c.g.ny + 2) * c.g.pf;
   for (int f = f0; f < f1; ++f) {
     const int i = f - f0;
-    b.phys[i] = c.uphys + f * fs;
-    if (f < 3) {
-      b.four[i] = c.cun[f];
-      b.alt[i] = c.rphys + f * fs;
+    const double2 *four = (f < 3) ? c.cun[f] : c.cth + (f - 3) * cfs;
+    b.four[i] = four;
+    if (c.mixed) {
+      // increments over the state at the start of the substep
+      b.phys[i] = c.rphys32 + f * fs;
+      b.alt[i] = c.rphys32 + f * fs;
+      b.base[i] = (f < 3) ? c.cu[f] : four;
       b.alo[i] = c.wlo[f];
       b.ahi[i] = c.whi[f];
     } else {
-      b.four[i] = c.cth + (f - 3) * cfs;
+      b.phys[i] = c.uphys + f * fs;
+      if (f < 3) {
+        b.alt[i] = c.rphys + f * fs;
+        b.alo[i] = c.wlo[f];
+        b.ahi[i] = c.whi[f];
+      }
     }
   }
   b.j0 = 0;
-  if (c.reg_rows)
-    hipLaunchKernelGGL(k_plane_fwd<true>, dim3(c.g.ny + 2, f1 - f0), dim3(c.plane_threads), c.lds_bytes, st, c.g, b);
-  else
-    hipLaunchKernelGGL(k_plane_fwd<false>, dim3(c.g.ny + 2, f1 - f0), dim3(c.plane_threads), c.lds_bytes, st, c.g, b);
-  return hipGetLastError();
+  const dim3 grid(c.g.ny + 2, f1 - f0);
+  return c.mixed ? plane_fwd<float2>(c, grid, st, b) : plane_fwd<double2>(c, grid, st, b);
 }
 
 hipError_t prepare_fft_kernels(Ctx &c) {
-  const void *fns[] = {(const void *)k_plane_inv<true>, (const void *)k_plane_inv<false>,
-                       (const void *)k_plane_fwd<true>, (const void *)k_plane_fwd<false>};
+  const void *fns[] = {(const void *)k_plane_inv<true, double2>, (const void *)k_plane_inv<false, double2>,
+                       (const void *)k_plane_fwd<true, double2>, (const void *)k_plane_fwd<false, double2>};
   for (const void *fn : fns) {
     hipError_t e = set_lds(fn, c.lds_bytes);
+    if (e != hipSuccess) return e;
+  }
+  const void *fns32[] = {(const void *)k_plane_inv<true, float2>, (const void *)k_plane_inv<false, float2>,
+                         (const void *)k_plane_fwd<true, float2>, (const void *)k_plane_fwd<false, float2>};
+  for (const void *fn : fns32) {
+    hipError_t e = set_lds(fn, c.lds_bytes32);
     if (e != hipSuccess) return e;
   }
   return hipSuccess;

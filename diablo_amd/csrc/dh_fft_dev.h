@@ -33,18 +33,58 @@
 
 namespace dh {
 
-__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
-__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
-  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+// Complex element of a plane: double2 (FP64 transforms) or float2 (the
+// mixed form's FP32 transforms).  Sc<C> is its real type and mkc builds one.
+template <class C>
+struct CxOf;
+template <>
+struct CxOf<double2> {
+  using S = double;
+};
+template <>
+struct CxOf<float2> {
+  using S = float;
+};
+template <class C>
+using Sc = typename CxOf<C>::S;
+template <class C>
+__device__ __forceinline__ C mkc(Sc<C> x, Sc<C> y) {
+  C r;
+  r.x = x;
+  r.y = y;
+  return r;
 }
-__device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+// an FP64 complex value as C (rounded to nearest for float2)
+template <class C>
+__device__ __forceinline__ C cvt(double2 v) {
+  return mkc<C>(static_cast<Sc<C>>(v.x), static_cast<Sc<C>>(v.y));
+}
+
+template <class C>
+__device__ __forceinline__ C cadd(C a, C b) {
+  return mkc<C>(a.x + b.x, a.y + b.y);
+}
+template <class C>
+__device__ __forceinline__ C csub(C a, C b) {
+  return mkc<C>(a.x - b.x, a.y - b.y);
+}
+template <class C>
+__device__ __forceinline__ C cmul(C a, C b) {
+  return mkc<C>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <class C>
+__device__ __forceinline__ C cconj(C a) {
+  return mkc<C>(a.x, -a.y);
+}
 // (i*k) * z with the component order of a COMPLEX*16 product (0,k)*(x,y)
-__device__ __forceinline__ double2 cmul_ik(double k, double2 z) { return make_double2(-(k * z.y), k * z.x); }
+template <class C>
+__device__ __forceinline__ C cmul_ik(Sc<C> k, C z) {
+  return mkc<C>(-(k * z.y), k * z.x);
+}
 // multiply by -i (forward) or +i (inverse)
-template <bool INV>
-__device__ __forceinline__ double2 cmul_mi(double2 a) {
-  return INV ? make_double2(-a.y, a.x) : make_double2(a.y, -a.x);
+template <bool INV, class C>
+__device__ __forceinline__ C cmul_mi(C a) {
+  return INV ? mkc<C>(-a.y, a.x) : mkc<C>(a.y, -a.x);
 }
 
 // Workgroup barrier that orders LDS only: global loads already issued stay
@@ -62,9 +102,10 @@ __device__ __forceinline__ void lds_barrier() {
 // rounded this is the correctly rounded quotient (Markstein's theorem) away
 // from overflow and underflow, i.e. the same double as n / b, in three
 // dependent operations instead of a full division.
-__device__ __forceinline__ double div_by_recip(double n, double b, double r) {
-  const double q0 = n * r;
-  const double e = fma(-b, q0, n);
+template <class S>
+__device__ __forceinline__ S div_by_recip(S n, S b, S r) {
+  const S q0 = n * r;
+  const S e = fma(-b, q0, n);
   return fma(e, r, q0);
 }
 
@@ -73,41 +114,42 @@ __device__ __forceinline__ unsigned fdiv(unsigned n, unsigned long long m) {
   return (unsigned)(((unsigned long long)n * m) >> 32);
 }
 
-template <int R, bool INV>
-__device__ __forceinline__ void dft_small(double2 *v) {
+template <int R, bool INV, class C>
+__device__ __forceinline__ void dft_small(C *v) {
+  using S = Sc<C>;
   if constexpr (R == 2) {
-    double2 a = v[0], b = v[1];
+    C a = v[0], b = v[1];
     v[0] = cadd(a, b);
     v[1] = csub(a, b);
   } else if constexpr (R == 3) {
-    const double c1 = -0.5;
-    const double s1 = INV ? 0.86602540378443864676 : -0.86602540378443864676;
-    double2 t1 = cadd(v[1], v[2]);
-    double2 t2 = csub(v[1], v[2]);
-    double2 m = make_double2(v[0].x + c1 * t1.x, v[0].y + c1 * t1.y);
-    double2 it = make_double2(-s1 * t2.y, s1 * t2.x);  // i*s1*t2
+    const S c1 = -0.5;
+    const S s1 = INV ? 0.86602540378443864676 : -0.86602540378443864676;
+    C t1 = cadd(v[1], v[2]);
+    C t2 = csub(v[1], v[2]);
+    C m = mkc<C>(v[0].x + c1 * t1.x, v[0].y + c1 * t1.y);
+    C it = mkc<C>(-s1 * t2.y, s1 * t2.x);  // i*s1*t2
     v[0] = cadd(v[0], t1);
     v[1] = cadd(m, it);
     v[2] = csub(m, it);
   } else if constexpr (R == 4) {
-    double2 a = cadd(v[0], v[2]), b = csub(v[0], v[2]);
-    double2 c = cadd(v[1], v[3]), d = cmul_mi<INV>(csub(v[1], v[3]));
+    C a = cadd(v[0], v[2]), b = csub(v[0], v[2]);
+    C c = cadd(v[1], v[3]), d = cmul_mi<INV>(csub(v[1], v[3]));
     v[0] = cadd(a, c);
     v[2] = csub(a, c);
     v[1] = cadd(b, d);
     v[3] = csub(b, d);
   } else if constexpr (R == 8) {
-    double2 e[4] = {v[0], v[2], v[4], v[6]};
-    double2 o[4] = {v[1], v[3], v[5], v[7]};
+    C e[4] = {v[0], v[2], v[4], v[6]};
+    C o[4] = {v[1], v[3], v[5], v[7]};
     dft_small<4, INV>(e);
     dft_small<4, INV>(o);
-    const double h = 0.70710678118654752440;
+    const S h = 0.70710678118654752440;
     // w^1 = h(1 -/+ i), w^2 = -/+ i, w^3 = h(-1 -/+ i)
-    double2 o1 = INV ? make_double2(h * (o[1].x - o[1].y), h * (o[1].x + o[1].y))
-                     : make_double2(h * (o[1].x + o[1].y), h * (o[1].y - o[1].x));
-    double2 o2 = cmul_mi<INV>(o[2]);
-    double2 o3 = INV ? make_double2(-h * (o[3].x + o[3].y), h * (o[3].x - o[3].y))
-                     : make_double2(h * (o[3].y - o[3].x), -h * (o[3].x + o[3].y));
+    C o1 = INV ? mkc<C>(h * (o[1].x - o[1].y), h * (o[1].x + o[1].y))
+                     : mkc<C>(h * (o[1].x + o[1].y), h * (o[1].y - o[1].x));
+    C o2 = cmul_mi<INV>(o[2]);
+    C o3 = INV ? mkc<C>(-h * (o[3].x + o[3].y), h * (o[3].x - o[3].y))
+                     : mkc<C>(h * (o[3].y - o[3].x), -h * (o[3].x + o[3].y));
     v[0] = cadd(e[0], o[0]);
     v[4] = csub(e[0], o[0]);
     v[1] = cadd(e[1], o1);
@@ -119,19 +161,19 @@ __device__ __forceinline__ void dft_small(double2 *v) {
   } else if constexpr (R == 16) {
     // 16 = 4 x 4: DFT4 over n1 of x[4 n1 + n2], twiddle W16^{n2 k1}, DFT4
     // over n2; output X[k1 + 4 k2] in natural order
-    double2 y[4][4];
+    C y[4][4];
 #pragma unroll
     for (int n2 = 0; n2 < 4; ++n2) {
-      double2 t[4] = {v[n2], v[4 + n2], v[8 + n2], v[12 + n2]};
+      C t[4] = {v[n2], v[4 + n2], v[8 + n2], v[12 + n2]};
       dft_small<4, INV>(t);
 #pragma unroll
       for (int k1 = 0; k1 < 4; ++k1) y[n2][k1] = t[k1];
     }
-    const double c1 = 0.92387953251128675613, s1 = 0.38268343236508977173, h = 0.70710678118654752440;
+    const S c1 = 0.92387953251128675613, s1 = 0.38268343236508977173, h = 0.70710678118654752440;
     // W16^j = cos(2 pi j/16) -/+ i sin(2 pi j/16) for j = 1, 2, 3, 4, 6, 9
-    const double sg = INV ? 1.0 : -1.0;
-    auto tw = [&](double2 a, double c, double sn) {  // a * (c + i sg sn)
-      return make_double2(a.x * c - a.y * (sg * sn), a.x * (sg * sn) + a.y * c);
+    const S sg = INV ? S(1) : S(-1);
+    auto tw = [&](C a, S c, S sn) {  // a * (c + i sg sn)
+      return mkc<C>(a.x * c - a.y * (sg * sn), a.x * (sg * sn) + a.y * c);
     };
     y[1][1] = tw(y[1][1], c1, s1);
     y[1][2] = tw(y[1][2], h, h);
@@ -144,7 +186,7 @@ __device__ __forceinline__ void dft_small(double2 *v) {
     y[3][3] = tw(y[3][3], -c1, -s1);
 #pragma unroll
     for (int k1 = 0; k1 < 4; ++k1) {
-      double2 t[4] = {y[0][k1], y[1][k1], y[2][k1], y[3][k1]};
+      C t[4] = {y[0][k1], y[1][k1], y[2][k1], y[3][k1]};
       dft_small<4, INV>(t);
 #pragma unroll
       for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = t[k2];
@@ -153,20 +195,23 @@ __device__ __forceinline__ void dft_small(double2 *v) {
 }
 
 // Transform tables staged in LDS behind the plane (see load_tables).
-struct Tables {
-  double2 *twr;  // exp(-2 pi i m/LW), m < LW
-  double2 *twc;  // exp(-2 pi i m/NZ), m < NZ
-  double2 *twx;  // exp(-2 pi i k/NX), k <= LW
+template <class C>
+struct TablesT {
+  C *twr;        // exp(-2 pi i m/LW), m < LW
+  C *twc;        // exp(-2 pi i m/NZ), m < NZ
+  C *twx;        // exp(-2 pi i k/NX), k <= LW
   int *posr;     // DIT input slot of row index m
   int *posrd;    // slot of row output index k after the DIF
   int *posz;     // slot of column output index k after the DIF
   int *posc;     // DIT input slot of column index k
 };
+using Tables = TablesT<double2>;
 
 // Copies the tables into LDS; returns pointers into `smem` after the plane.
-__device__ Tables load_tables(double2 *smem, const Geo &g) {
-  Tables t;
-  double2 *p = smem + g.nz * g.ls;
+template <class C>
+__device__ TablesT<C> load_tables(C *smem, const Geo &g) {
+  TablesT<C> t;
+  C *p = smem + g.nz * g.ls;
   t.twr = p;
   t.twc = p + g.lw;
   t.twx = p + g.lw + g.nz;
@@ -176,16 +221,16 @@ __device__ Tables load_tables(double2 *smem, const Geo &g) {
   t.posz = q + 2 * g.lw;
   t.posc = q + 2 * g.lw + g.nz;
   for (int i = threadIdx.x; i < g.lw; i += blockDim.x) {
-    t.twr[i] = g.tw_row[i];
+    t.twr[i] = cvt<C>(g.tw_row[i]);
     t.posr[i] = g.posr[i];
     t.posrd[i] = g.posrd[i];
   }
   for (int i = threadIdx.x; i < g.nz; i += blockDim.x) {
-    t.twc[i] = g.tw_col[i];
+    t.twc[i] = cvt<C>(g.tw_col[i]);
     t.posz[i] = g.posz[i];
     t.posc[i] = g.posc[i];
   }
-  for (int i = threadIdx.x; i <= g.lw; i += blockDim.x) t.twx[i] = g.tw_real[i];
+  for (int i = threadIdx.x; i <= g.lw; i += blockDim.x) t.twx[i] = cvt<C>(g.tw_real[i]);
   lds_barrier();
   return t;
 }
@@ -197,9 +242,9 @@ __device__ Tables load_tables(double2 *smem, const Geo &g) {
 // cols (COLS=true):  transform t is column t, element i at A[i*ls + t].
 // Consecutive threads take consecutive transforms t (fnt = reciprocal of
 // ntrans) with the same butterfly.
-template <int R, bool INV, bool COLS, bool DIT>
-__device__ __forceinline__ void fft_stage(double2 *A, const FftStage &st, int N, int ntrans,
-                                          unsigned long long fnt, int ls, const double2 *tw) {
+template <int R, bool INV, bool COLS, bool DIT, class C>
+__device__ __forceinline__ void fft_stage(C *A, const FftStage &st, int N, int ntrans, unsigned long long fnt, int ls,
+                                          const C *tw) {
   const int S = st.S;
   const int nb = N / R;
   const int total = ntrans * nb;
@@ -208,7 +253,7 @@ __device__ __forceinline__ void fft_stage(double2 *A, const FftStage &st, int N,
     const int t = gi - b * ntrans;
     const int g = (int)fdiv((unsigned)b, st.fS), j = b - g * S;
     const int i0 = g * st.L + j;
-    double2 *base;
+    C *base;
     int es;
     if (COLS) {
       base = A + t + i0 * ls;
@@ -217,13 +262,13 @@ __device__ __forceinline__ void fft_stage(double2 *A, const FftStage &st, int N,
       base = A + t * ls + i0;
       es = S;
     }
-    double2 v[R];
+    C v[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) v[r] = base[r * es];
     if (DIT && j > 0) {
 #pragma unroll
       for (int r = 1; r < R; ++r) {
-        double2 w = tw[r * j * st.twm];
+        C w = tw[r * j * st.twm];
         v[r] = cmul(v[r], INV ? cconj(w) : w);
       }
     }
@@ -231,7 +276,7 @@ __device__ __forceinline__ void fft_stage(double2 *A, const FftStage &st, int N,
     if (!DIT && j > 0) {
 #pragma unroll
       for (int q = 1; q < R; ++q) {
-        double2 w = tw[q * j * st.twm];
+        C w = tw[q * j * st.twm];
         v[q] = cmul(v[q], INV ? cconj(w) : w);
       }
     }
@@ -241,8 +286,8 @@ __device__ __forceinline__ void fft_stage(double2 *A, const FftStage &st, int N,
   lds_barrier();
 }
 
-template <bool INV, bool COLS, bool DIT>
-__device__ void lds_fft(double2 *A, const FftPlan &pl, int ntrans, unsigned long long fnt, int ls, const double2 *tw) {
+template <bool INV, bool COLS, bool DIT, class C>
+__device__ void lds_fft(C *A, const FftPlan &pl, int ntrans, unsigned long long fnt, int ls, const C *tw) {
 #pragma unroll 1
   for (int s = 0; s < pl.nst; ++s) {
     const FftStage &st = DIT ? pl.dit[s] : pl.dif[s];
@@ -257,7 +302,10 @@ __device__ void lds_fft(double2 *A, const FftPlan &pl, int ntrans, unsigned long
 }
 
 // LDS slot through which physical pair m of row z enters the forward transform.
-__device__ __forceinline__ int phys_slot(const Tables &T, const Geo &g, int z, int m) { return z * g.ls + T.posr[m]; }
+template <class TT>
+__device__ __forceinline__ int phys_slot(const TT &T, const Geo &g, int z, int m) {
+  return z * g.ls + T.posr[m];
+}
 
 // Work split for filling the digit-reversed input slots from natural-order
 // HBM rows: a wave covers a tile of 8 rows x 8 consecutive pairs, lane =
@@ -284,32 +332,34 @@ struct LoadTile {
 // Forward x transform of every row (FFT_XZ_TO_FOURIER's r2c, mpi.f:1490-1506).
 // On entry row z holds (x_{2m}, x_{2m+1}) in slot posr[m]; on exit slot k
 // holds X_k / NX for k <= NKX (natural order).
-__device__ void plane_rows_fwd(double2 *A, const Geo &g, const Tables &T) {
+template <class C>
+__device__ void plane_rows_fwd(C *A, const Geo &g, const TablesT<C> &T) {
+  using S = Sc<C>;
   const int h = g.lw;  // NX/2
   lds_fft<false, false, true>(A, g.prow, g.nz, g.f_nz, g.ls, T.twr);
   // even/odd split: X_k = (Z_k + conj(Z_{h-k}))/2 - i w^-k (Z_k - conj(Z_{h-k}))/2
   const int npair = h / 2 + 1;  // k = 0..h/2 paired with h-k
   const int total = g.nz * npair;
-  const double nxd = g.dnx, rnx = g.rnx;  // X / NX as the correctly rounded quotient
+  const S nxd = S(g.dnx), rnx = S(g.rnx);  // X / NX as the correctly rounded quotient
   for (int e = threadIdx.x; e < total; e += blockDim.x) {
     const int k = (int)fdiv((unsigned)e, g.f_nz), z = e - k * g.nz;
-    double2 *row = A + z * g.ls;
+    C *row = A + z * g.ls;
     const int k2 = (k == 0) ? 0 : h - k;
-    const double2 zk = row[k];
-    const double2 zm = row[k2];
+    const C zk = row[k];
+    const C zm = row[k2];
     {
-      double2 a = make_double2(0.5 * (zk.x + zm.x), 0.5 * (zk.y - zm.y));  // (Z_k + conj Z_{h-k})/2
-      double2 b = make_double2(0.5 * (zk.x - zm.x), 0.5 * (zk.y + zm.y));  // (Z_k - conj Z_{h-k})/2
-      double2 wb = cmul(T.twx[k], b);                                     // w = e^{-2 pi i k/NX}
-      double2 X = make_double2(a.x + wb.y, a.y - wb.x);                  // a - i*w*b
-      if (k <= g.nkx) row[k] = make_double2(div_by_recip(X.x, nxd, rnx), div_by_recip(X.y, nxd, rnx));
+      C a = mkc<C>(S(0.5) * (zk.x + zm.x), S(0.5) * (zk.y - zm.y));  // (Z_k + conj Z_{h-k})/2
+      C b = mkc<C>(S(0.5) * (zk.x - zm.x), S(0.5) * (zk.y + zm.y));  // (Z_k - conj Z_{h-k})/2
+      C wb = cmul(T.twx[k], b);                                     // w = e^{-2 pi i k/NX}
+      C X = mkc<C>(a.x + wb.y, a.y - wb.x);                  // a - i*w*b
+      if (k <= g.nkx) row[k] = mkc<C>(div_by_recip(X.x, nxd, rnx), div_by_recip(X.y, nxd, rnx));
     }
     if (k != 0 && k2 != k && k2 <= g.nkx) {
-      double2 a = make_double2(0.5 * (zm.x + zk.x), 0.5 * (zm.y - zk.y));
-      double2 b = make_double2(0.5 * (zm.x - zk.x), 0.5 * (zm.y + zk.y));
-      double2 wb = cmul(T.twx[k2], b);
-      double2 X = make_double2(a.x + wb.y, a.y - wb.x);
-      row[k2] = make_double2(div_by_recip(X.x, nxd, rnx), div_by_recip(X.y, nxd, rnx));
+      C a = mkc<C>(S(0.5) * (zm.x + zk.x), S(0.5) * (zm.y - zk.y));
+      C b = mkc<C>(S(0.5) * (zm.x - zk.x), S(0.5) * (zm.y + zk.y));
+      C wb = cmul(T.twx[k2], b);
+      C X = mkc<C>(a.x + wb.y, a.y - wb.x);
+      row[k2] = mkc<C>(div_by_recip(X.x, nxd, rnx), div_by_recip(X.y, nxd, rnx));
     }
   }
   lds_barrier();
@@ -319,29 +369,30 @@ __device__ void plane_rows_fwd(double2 *A, const Geo &g, const Tables &T) {
 // On entry slot k of each row holds X_k for k <= NKX (higher modes and the
 // Nyquist term are zero; the imaginary part of the k=0 term is ignored); on
 // exit slot posrd[m] holds (x_{2m}, x_{2m+1}).
-__device__ void plane_rows_inv(double2 *A, const Geo &g, const Tables &T) {
+template <class C>
+__device__ void plane_rows_inv(C *A, const Geo &g, const TablesT<C> &T) {
   const int h = g.lw;
   const int npair = h / 2 + 1;
   const int total = g.nz * npair;
   for (int e = threadIdx.x; e < total; e += blockDim.x) {
     const int k = (int)fdiv((unsigned)e, g.f_nz), z = e - k * g.nz;
-    double2 *row = A + z * g.ls;
+    C *row = A + z * g.ls;
     const int k2 = h - k;  // partner (k=0 pairs with the Nyquist term)
-    double2 xk = (k <= g.nkx) ? row[k] : make_double2(0.0, 0.0);
-    double2 xm = (k2 <= g.nkx && k2 < h) ? row[k2] : make_double2(0.0, 0.0);
+    C xk = (k <= g.nkx) ? row[k] : mkc<C>(0.0, 0.0);
+    C xm = (k2 <= g.nkx && k2 < h) ? row[k2] : mkc<C>(0.0, 0.0);
     if (k == 0) xk.y = 0.0;
     // Z_k = (X_k + conj X_{h-k}) + i w^k (X_k - conj X_{h-k}),  w = e^{+2 pi i/NX}
     {
-      double2 a = make_double2(xk.x + xm.x, xk.y - xm.y);
-      double2 b = make_double2(xk.x - xm.x, xk.y + xm.y);
-      double2 wb = cmul(cconj(T.twx[k]), b);
-      if (k < h) row[k] = make_double2(a.x - wb.y, a.y + wb.x);
+      C a = mkc<C>(xk.x + xm.x, xk.y - xm.y);
+      C b = mkc<C>(xk.x - xm.x, xk.y + xm.y);
+      C wb = cmul(cconj(T.twx[k]), b);
+      if (k < h) row[k] = mkc<C>(a.x - wb.y, a.y + wb.x);
     }
     if (k != 0 && k2 != k) {
-      double2 a = make_double2(xm.x + xk.x, xm.y - xk.y);
-      double2 b = make_double2(xm.x - xk.x, xm.y + xk.y);
-      double2 wb = cmul(cconj(T.twx[k2]), b);
-      row[k2] = make_double2(a.x - wb.y, a.y + wb.x);
+      C a = mkc<C>(xm.x + xk.x, xm.y - xk.y);
+      C b = mkc<C>(xm.x - xk.x, xm.y + xk.y);
+      C wb = cmul(cconj(T.twx[k2]), b);
+      row[k2] = mkc<C>(a.x - wb.y, a.y + wb.x);
     }
   }
   lds_barrier();
@@ -351,10 +402,12 @@ __device__ void plane_rows_inv(double2 *A, const Geo &g, const Tables &T) {
 // z transforms over the de-aliased columns kx = 0..NKX: forward by DIF
 // (natural rows in, row of index k at posz[k] out), inverse by DIT (row of
 // index k at posc[k] in, natural rows out).
-__device__ __forceinline__ void plane_cols_fwd(double2 *A, const Geo &g, const Tables &T) {
+template <class C>
+__device__ __forceinline__ void plane_cols_fwd(C *A, const Geo &g, const TablesT<C> &T) {
   lds_fft<false, true, false>(A, g.pcol, g.nkx1, g.f_nkx1, g.ls, T.twc);
 }
-__device__ __forceinline__ void plane_cols_inv(double2 *A, const Geo &g, const Tables &T) {
+template <class C>
+__device__ __forceinline__ void plane_cols_inv(C *A, const Geo &g, const TablesT<C> &T) {
   lds_fft<true, true, true>(A, g.pcol, g.nkx1, g.f_nkx1, g.ls, T.twc);
 }
 
@@ -364,26 +417,35 @@ __device__ __forceinline__ int kz_row(const Geo &g, int kzp) {
 }
 
 // LDS slot of the forward-transformed mode (kx, packed kz index kzp).
-__device__ __forceinline__ int fwd_slot(const Geo &g, const Tables &T, int kx, int kzp) {
+template <class TT>
+__device__ __forceinline__ int fwd_slot(const Geo &g, const TT &T, int kx, int kzp) {
   return T.posz[kz_row(g, kzp)] * g.ls + kx;
 }
 
 // LDS slot of physical pair m (x = 2m, 2m+1) of row z after the inverse.
-__device__ __forceinline__ int inv_slot(const Geo &g, const Tables &T, int z, int m) { return z * g.ls + T.posrd[m]; }
+template <class TT>
+__device__ __forceinline__ int inv_slot(const Geo &g, const TT &T, int z, int m) {
+  return z * g.ls + T.posrd[m];
+}
 
 // LDS row of the unpacked z spectrum entering the inverse column transform
 // at natural index zr.
-__device__ __forceinline__ int inv_row(const Tables &T, int zr) { return T.posc[zr]; }
+template <class TT>
+__device__ __forceinline__ int inv_row(const TT &T, int zr) {
+  return T.posc[zr];
+}
 
 // Scatter a de-aliased Fourier plane into A (unpack + zero fill of the
-// columns 0..NKX, mpi.f:1562-1574).  `src` may be null (all zero).  Loads
-// are issued in batches of 8 per thread before any LDS store, so their
+// columns 0..NKX, mpi.f:1562-1574).  `src` may be null (all zero);
+// drop_mean leaves out the (0,0) coefficient (the plane mean).  Loads are
+// issued in batches of 8 per thread before any LDS store, so their
 // latencies overlap.
-__device__ void plane_unpack(double2 *A, const Geo &g, const Tables &T, const double2 *src) {
+template <class C>
+__device__ void plane_unpack(C *A, const Geo &g, const TablesT<C> &T, const double2 *src, bool drop_mean = false) {
   constexpr int B = 8;
   const int total = g.nz * g.nkx1;
   for (int base = threadIdx.x; base < total; base += B * blockDim.x) {
-    double2 v[B];
+    C v[B];
     int slot[B];
 #pragma unroll
     for (int b = 0; b < B; ++b) {
@@ -397,7 +459,7 @@ __device__ void plane_unpack(double2 *A, const Geo &g, const Tables &T, const do
         else if (zr >= g.nz - g.nkz)
           kzp = zr - (g.nz - 1 - g.nkz) + g.nkz;
         slot[b] = inv_row(T, zr) * g.ls + kx;
-        v[b] = (kzp >= 0 && src) ? src[kzp * g.nkx1 + kx] : make_double2(0.0, 0.0);
+        v[b] = (kzp >= 0 && src && !(drop_mean && kzp == 0 && kx == 0)) ? cvt<C>(src[kzp * g.nkx1 + kx]) : mkc<C>(0, 0);
       }
     }
 #pragma unroll
@@ -408,22 +470,23 @@ __device__ void plane_unpack(double2 *A, const Geo &g, const Tables &T, const do
 }
 
 // Physical plane (natural order in HBM) into the digit-reversed input slots.
-__device__ void plane_load_phys(double2 *A, const Geo &g, const Tables &T, const double2 *src) {
+template <class C>
+__device__ void plane_load_phys(C *A, const Geo &g, const TablesT<C> &T, const C *src) {
   constexpr int B = 4;
   const LoadTile lt(g);
   const int nw = blockDim.x >> 6;
   for (int t0 = threadIdx.x >> 6; t0 < lt.ntiles; t0 += B * nw) {
-    double2 v[B];
+    C v[B];
     int sl[B];
 #pragma unroll
     for (int b = 0; b < B; ++b) {
       int z, m;
       sl[b] = -1;
       const int t = t0 + b * nw;
-      v[b] = make_double2(0.0, 0.0);
+      v[b] = mkc<C>(0.0, 0.0);
       if (t < lt.ntiles && lt.at(g, t, z, m)) {
         sl[b] = phys_slot(T, g, z, m);
-        const double2 *q = src + z * g.lw + m;
+        const C *q = src + z * g.lw + m;
         v[b].x = q->x;
         v[b].y = q->y;
       }
@@ -458,34 +521,25 @@ __device__ __forceinline__ void wave_lds_sync() {
 // already hold the transform R_k of another product; the result stored is
 // i KX(k) R_k + X_k / NX (kxv = KX), the sum whose z transform is that of
 // both products' contributions.
-template <bool MERGE = false, class Fetch, class Eval>
-__device__ void rows_fwd_reg64(double2 *A, const Geo &g, const Tables &T, Fetch fetch, Eval eval,
+template <bool MERGE = false, class C, class Fetch, class Eval>
+__device__ void rows_fwd_reg64(C *A, const Geo &g, const TablesT<C> &T, Fetch fetch, Eval eval,
                                const double *kxv = nullptr) {
+  using S = Sc<C>;
   const int lane = threadIdx.x & 63, rho = lane >> 3, t = lane & 7;
   const int nw = blockDim.x >> 6;
-  const double nxd = g.dnx, rnx = g.rnx;  // X / NX as the correctly rounded quotient
+  const S nxd = S(g.dnx), rnx = S(g.rnx);  // X / NX as the correctly rounded quotient
   for (int slab = threadIdx.x >> 6; slab * 8 < g.nz; slab += nw) {
     const int z = slab * 8 + rho;  // < NZ: register rows require NZ % 8 == 0
-    double2 *row = A + z * g.ls;
-    // the other product's outputs of this lane, read before the row is reused
-    double2 prev[kRowK2];
-    double kxp[kRowK2];
-    if constexpr (MERGE) {
-#pragma unroll
-      for (int k2 = 0; k2 < kRowK2; ++k2) {
-        const int k = min(t + 8 * k2, g.nkx);
-        prev[k2] = row[k];
-        kxp[k2] = kxv[k];
-      }
-    }
+    C *row = A + z * g.ls;
     // the loads of a group of G samples are all issued before any arithmetic
     // on them; left to itself the scheduler interleaves loads and arithmetic
     // sample by sample and the memory latency is exposed eight times per
-    // slab.  Samples wider than three complex values go in two groups (the
-    // register file holds the raw values of one group).
+    // slab.  Samples wider than three complex values (two complex pairs in
+    // FP32, whose kernels run with half the registers) go in two groups: the
+    // register file holds the raw values of one group.
     using Raw = decltype(fetch(0, 0));
-    constexpr int G = sizeof(Raw) > 48 ? 4 : 8;
-    double2 v[8];
+    constexpr int G = (sizeof(C) == sizeof(double2)) ? (sizeof(Raw) > 48 ? 4 : 8) : (sizeof(Raw) > 16 ? 4 : 8);
+    C v[8];
 #pragma unroll
     for (int r0 = 0; r0 < 8; r0 += G) {
       Raw raw[G];
@@ -494,6 +548,18 @@ __device__ void rows_fwd_reg64(double2 *A, const Geo &g, const Tables &T, Fetch 
       __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int r = 0; r < G; ++r) v[r0 + r] = eval(raw[r]);
+    }
+    // the other product's outputs of this lane, read before the row is
+    // reused (after the samples are consumed, to spare registers)
+    C prev[kRowK2];
+    S kxp[kRowK2];
+    if constexpr (MERGE) {
+#pragma unroll
+      for (int k2 = 0; k2 < kRowK2; ++k2) {
+        const int k = min(t + 8 * k2, g.nkx);
+        prev[k2] = row[k];
+        kxp[k2] = S(kxv[k]);
+      }
     }
     dft_small<8, false>(v);
 #pragma unroll
@@ -509,7 +575,7 @@ __device__ void rows_fwd_reg64(double2 *A, const Geo &g, const Tables &T, Fetch 
 #pragma unroll
     for (int k2 = 0; k2 < 8; ++k2) row[t + 8 * k2] = v[k2];
     wave_lds_sync();
-    double2 zm[kRowK2];
+    C zm[kRowK2];
 #pragma unroll
     for (int k2 = 0; k2 < kRowK2; ++k2) zm[k2] = row[(64 - t - 8 * k2) & 63];
     wave_lds_sync();
@@ -517,14 +583,14 @@ __device__ void rows_fwd_reg64(double2 *A, const Geo &g, const Tables &T, Fetch 
 #pragma unroll
     for (int k2 = 0; k2 < kRowK2; ++k2) {
       const int k = t + 8 * k2;
-      const double2 zk = v[k2], zmk = zm[k2];
-      double2 a = make_double2(0.5 * (zk.x + zmk.x), 0.5 * (zk.y - zmk.y));
-      double2 b = make_double2(0.5 * (zk.x - zmk.x), 0.5 * (zk.y + zmk.y));
-      double2 wb = cmul(T.twx[k], b);
-      double2 x = make_double2(div_by_recip(a.x + wb.y, nxd, rnx), div_by_recip(a.y - wb.x, nxd, rnx));
+      const C zk = v[k2], zmk = zm[k2];
+      C a = mkc<C>(S(0.5) * (zk.x + zmk.x), S(0.5) * (zk.y - zmk.y));
+      C b = mkc<C>(S(0.5) * (zk.x - zmk.x), S(0.5) * (zk.y + zmk.y));
+      C wb = cmul(T.twx[k], b);
+      C x = mkc<C>(div_by_recip(a.x + wb.y, nxd, rnx), div_by_recip(a.y - wb.x, nxd, rnx));
       if constexpr (MERGE) {
-        const double2 ik = cmul_ik(kxp[k2], prev[k2]);
-        x = make_double2(ik.x + x.x, ik.y + x.y);
+        const C ik = cmul_ik(kxp[k2], prev[k2]);
+        x = mkc<C>(ik.x + x.x, ik.y + x.y);
       }
       if (k <= g.nkx) row[k] = x;
     }
@@ -532,36 +598,36 @@ __device__ void rows_fwd_reg64(double2 *A, const Geo &g, const Tables &T, Fetch 
   lds_barrier();
 }
 
-template <class Load>
-__device__ __forceinline__ void rows_fwd_reg64(double2 *A, const Geo &g, const Tables &T, Load load) {
-  rows_fwd_reg64<false>(A, g, T, load, [](double2 v) { return v; });
+template <class C, class Load>
+__device__ __forceinline__ void rows_fwd_reg64(C *A, const Geo &g, const TablesT<C> &T, Load load) {
+  rows_fwd_reg64<false>(A, g, T, load, [](C v) { return v; });
 }
 
 // Inverse x transform of every row (FFT_XZ_TO_PHYSICAL's c2r,
 // mpi.f:1600-1604): X_k (k <= NKX) in natural slots of the LDS rows to the
 // physical pairs, handed to store(z, m, value).
-template <class Store>
-__device__ void rows_inv_reg64(double2 *A, const Geo &g, const Tables &T, Store store) {
+template <class C, class Store>
+__device__ void rows_inv_reg64(C *A, const Geo &g, const TablesT<C> &T, Store store) {
   const int lane = threadIdx.x & 63, rho = lane >> 3, t = lane & 7;
   const int nw = blockDim.x >> 6;
   for (int slab = threadIdx.x >> 6; slab * 8 < g.nz; slab += nw) {
     const int z = slab * 8 + rho;
     const bool zin = z < g.nz;
-    double2 *row = A + z * g.ls;
+    C *row = A + z * g.ls;
     // Z_k = (X_k + conj X_{64-k}) + i w^k (X_k - conj X_{64-k}), w = e^{+2 pi i/NX},
     // for k = t + 8 k2 (X_k = 0 above NKX; Im X_0 ignored)
-    double2 v[8];
+    C v[8];
 #pragma unroll
     for (int k2 = 0; k2 < 8; ++k2) {
       const int k = t + 8 * k2, km = (64 - k) & 63;
-      double2 xk = make_double2(0.0, 0.0), xm = make_double2(0.0, 0.0);
+      C xk = mkc<C>(0.0, 0.0), xm = mkc<C>(0.0, 0.0);
       if (k <= g.nkx) xk = row[k];
       if (k != 0 && km <= g.nkx) xm = row[km];
       if (k == 0) xk.y = 0.0;
-      double2 a = make_double2(xk.x + xm.x, xk.y - xm.y);
-      double2 b = make_double2(xk.x - xm.x, xk.y + xm.y);
-      double2 wb = cmul(cconj(T.twx[k]), b);
-      v[k2] = make_double2(a.x - wb.y, a.y + wb.x);
+      C a = mkc<C>(xk.x + xm.x, xk.y - xm.y);
+      C b = mkc<C>(xk.x - xm.x, xk.y + xm.y);
+      C wb = cmul(cconj(T.twx[k]), b);
+      v[k2] = mkc<C>(a.x - wb.y, a.y + wb.x);
     }
     wave_lds_sync();
     // c_{t' + 8r} = sum_k1 W8^{-r k1} W64^{-t' k1} sum_k2 W8^{-t' k2} Z_{k1 + 8 k2}
@@ -583,7 +649,8 @@ __device__ void rows_inv_reg64(double2 *A, const Geo &g, const Tables &T, Store 
 }
 
 // Physical plane from the inverse-transform slots to natural order in HBM.
-__device__ void plane_store_phys(const double2 *A, const Geo &g, const Tables &T, double2 *dst) {
+template <class C>
+__device__ void plane_store_phys(const C *A, const Geo &g, const TablesT<C> &T, C *dst) {
   const LoadTile lt(g);
   const int nw = blockDim.x >> 6;
   for (int t = threadIdx.x >> 6; t < lt.ntiles; t += nw) {

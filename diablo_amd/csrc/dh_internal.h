@@ -91,6 +91,7 @@ struct RealFactors {
   double *pf, *pb;            // [rk][field][row] segment carry factors of the two sweeps
   int *isbc;                  // [field][row] 1: row takes a constant right-hand side
   double *vconst;             // [field][row]
+  double *bcl, *bcd, *bcu;    // [field][row] coefficients of the boundary rows (isbc = 1)
   int seg_rows;               // rows per register segment of a column
   int nseg;                   // segments per column (4 or 8)
 };
@@ -101,7 +102,8 @@ struct Ctx {
   int nfields;  // 3 + n_th
   int jstart, jend, jstart_th[DH_MAX_TH], jend_th[DH_MAX_TH];
   int wlo[kMaxFields], whi[kMaxFields];  // solution write-back rows per field
-  int lds_bytes;
+  int lds_bytes;     // dynamic LDS of the FP64 plane kernels
+  int lds_bytes32;   // the same for the FP32 transforms of the mixed form
   int plane_threads;
   bool reg_rows = false;     // plane kernels: row transforms in registers (NX = 128, NZ a multiple of 8)
   bool reg_rows_nl = false;  // the same in k_nonlinear
@@ -129,6 +131,11 @@ struct Ctx {
   double2 *cun[3] = {};         // velocity back in Fourier space before REM_DIV projects it into CU
   double *uphys = nullptr;      // [nfields][NY+2][NZ][NX]: U1,U2,U3,TH
   double *rphys = nullptr;      // [nfields][NY+2][NZ][NX]: R1,R2,R3,RTH
+  // mixed form (dh_set_mixed_precision): the substep's transforms and
+  // physical arrays in FP32, allocated when it is first switched on
+  bool mixed = false;
+  float *uphys32 = nullptr, *rphys32 = nullptr;
+  bool th_phys32 = false;  // the physical TH of the last substep is in uphys32
   double *rd_ta = nullptr;      // REM_DIV sweep multipliers [NY+2][pf]
   double *rd_tb = nullptr;      // REM_DIV sweep pivots      [NY+2][pf]
   double *rd_tr = nullptr;      // REM_DIV 1/pivots          [NY+2][pf]
@@ -144,6 +151,11 @@ struct Ctx {
 
 // ---- launchers (dh_fft.hip) ----
 hipError_t prepare_fft_kernels(Ctx &c);
+// dynamic LDS of a plane kernel whose plane holds complex values of type C
+template <class C>
+inline int lds_bytes_of(const Ctx &c) {
+  return sizeof(C) == sizeof(double2) ? c.lds_bytes : c.lds_bytes32;
+}
 hipError_t launch_fft_to_phys(Ctx &c, const double2 *four, double *phys, int nplanes);
 hipError_t launch_stats(Ctx &c, double *out_dev);  // profiles of dh_save_stats into out_dev
 hipError_t launch_fft_to_four(Ctx &c, const double *phys, double2 *four, int nplanes);

@@ -29,8 +29,8 @@ struct StepArgs {
   double2 *cf[3];
   const double2 *cth;
   double2 *cfth;
-  const double *uphys;
-  double *rphys;
+  const void *uphys;  // physical U and R: double, or float in the mixed form
+  void *rphys;
   long long fs;   // physical field stride (NY+2)*NX*NZ
   long long cfs;  // Fourier field stride (NY+2)*pf
   int c0, j0;     // component of blockIdx.y = 0 and plane of the first block row
@@ -53,20 +53,26 @@ __device__ __forceinline__ double kpow(double x, double beta) { return (beta == 
 //      d/dy(C U2) on the staggered grid, C = U1, U3 or TH
 //   3: ((DYF(J) s0 + DYF(J-1) s1) / (2 DY(J))) s2   interp_y(U1 or U3) U2
 //   4: (0.25 (s1+s0)^2 - 0.25 (s1+s2)^2) / DY(J)    d/dy(U2 U2)
+// Planes hold complex pairs C (double2, or float2 in the mixed form) whose
+// real type S = Sc<C> is also the arithmetic type of the forms.
+template <class C>
 struct Prod {
-  const double *s[5];
+  const Sc<C> *s[5];
   int kind;
-  double w0, w1, w2;
-  double d, r;  // divisor of forms 2-4 and its correctly rounded reciprocal
+  Sc<C> w0, w1, w2;
+  Sc<C> d, r;  // divisor of forms 2-4 and its reciprocal (correctly rounded in FP64)
 };
 
-__device__ __forceinline__ Prod make_prod(const StepArgs &a, int c, int p, int J) {
+template <class C>
+__device__ __forceinline__ Prod<C> make_prod(const StepArgs &a, int c, int p, int J) {
+  using S = Sc<C>;
   const long long pp = a.g.pp;
-  const double *U1 = a.uphys, *U2 = a.uphys + a.fs, *U3 = a.uphys + 2 * a.fs;
-  auto pl = [&](const double *F, int j) { return F + (long long)j * pp; };
-  Prod q{};
+  const S *up = static_cast<const S *>(a.uphys);
+  const S *U1 = up, *U2 = up + a.fs, *U3 = up + 2 * a.fs;
+  auto pl = [&](const S *F, int j) { return F + (long long)j * pp; };
+  Prod<C> q{};
   if (c == 0 || c == 2) {
-    const double *Uc = (c == 0) ? U1 : U3;
+    const S *Uc = (c == 0) ? U1 : U3;
     if (p == 0) {
       q.kind = 0; q.s[0] = pl(U3, J); q.s[1] = pl(U1, J);
     } else if (p == 1) {
@@ -77,14 +83,14 @@ __device__ __forceinline__ Prod make_prod(const StepArgs &a, int c, int p, int J
     }
   } else if (c == 1) {
     if (p <= 1) {
-      const double *Uc = (p == 0) ? U1 : U3;
+      const S *Uc = (p == 0) ? U1 : U3;
       q.kind = 3; q.s[0] = pl(Uc, J); q.s[1] = pl(Uc, J - 1); q.s[2] = pl(U2, J);
       q.w0 = a.g.dyf[J]; q.w1 = a.g.dyf[J - 1]; q.w2 = 2. * a.g.dy[J];
     } else {
       q.kind = 4; q.s[0] = pl(U2, J + 1); q.s[1] = pl(U2, J); q.s[2] = pl(U2, J - 1); q.w0 = a.g.dy[J];
     }
   } else {
-    const double *TH = a.uphys + (long long)c * a.fs;
+    const S *TH = up + (long long)c * a.fs;
     if (p <= 1) {
       q.kind = 0; q.s[0] = pl(TH, J); q.s[1] = pl(p == 0 ? U1 : U3, J);
     } else {
@@ -94,62 +100,65 @@ __device__ __forceinline__ Prod make_prod(const StepArgs &a, int c, int p, int J
   }
   // reciprocals from the tables (a power-of-two factor scales RN(1/x) exactly)
   q.d = (q.kind == 3) ? q.w2 : q.w0;
-  q.r = (q.kind == 2) ? 0.5 * a.g.rdyf[J] : (q.kind == 3) ? 0.5 * a.g.rdy[J] : (q.kind == 4) ? a.g.rdy[J] : 0.0;
+  q.r = (S)((q.kind == 2) ? 0.5 * a.g.rdyf[J] : (q.kind == 3) ? 0.5 * a.g.rdy[J] : (q.kind == 4) ? a.g.rdy[J] : 0.0);
   return q;
 }
 
 // The samples a form reads (planes s[0..n-1] at one offset), loaded apart
 // from the arithmetic so that a caller can issue many loads ahead of it.
-template <int KIND>
+template <int KIND, class C>
 struct ProdRaw {
   static constexpr int n = (KIND == 1) ? 1 : (KIND == 0) ? 2 : (KIND == 2) ? 5 : 3;
-  double2 s[n];
+  C s[n];
 };
 
-template <int KIND>
-__device__ __forceinline__ ProdRaw<KIND> fetch_prod(const Prod &q, long long off) {
-  ProdRaw<KIND> r;
+template <int KIND, class C>
+__device__ __forceinline__ ProdRaw<KIND, C> fetch_prod(const Prod<C> &q, long long off) {
+  ProdRaw<KIND, C> r;
 #pragma unroll
-  for (int i = 0; i < ProdRaw<KIND>::n; ++i) r.s[i] = *reinterpret_cast<const double2 *>(q.s[i] + off);
+  for (int i = 0; i < ProdRaw<KIND, C>::n; ++i) r.s[i] = *reinterpret_cast<const C *>(q.s[i] + off);
   return r;
 }
 
-template <int KIND>
-__device__ __forceinline__ double2 eval_raw(const Prod &q, const ProdRaw<KIND> &r) {
+template <int KIND, class C>
+__device__ __forceinline__ C eval_raw(const Prod<C> &q, const ProdRaw<KIND, C> &r) {
+  using S = Sc<C>;
   if constexpr (KIND == 0) {
-    const double2 u = r.s[0], v = r.s[1];
-    return make_double2(u.x * v.x, u.y * v.y);
+    const C u = r.s[0], v = r.s[1];
+    return mkc<C>(u.x * v.x, u.y * v.y);
   } else if constexpr (KIND == 1) {
-    const double2 u = r.s[0];
-    return make_double2(u.x * u.x, u.y * u.y);
+    const C u = r.s[0];
+    return mkc<C>(u.x * u.x, u.y * u.y);
   } else if constexpr (KIND == 2) {
-    const double2 up = r.s[0], u0 = r.s[1], um = r.s[2], vp = r.s[3], v0 = r.s[4];
-    return make_double2(div_by_recip(up.x * vp.x + u0.x * vp.x - u0.x * v0.x - um.x * v0.x, q.d, q.r),
-                        div_by_recip(up.y * vp.y + u0.y * vp.y - u0.y * v0.y - um.y * v0.y, q.d, q.r));
+    const C up = r.s[0], u0 = r.s[1], um = r.s[2], vp = r.s[3], v0 = r.s[4];
+    return mkc<C>(div_by_recip(up.x * vp.x + u0.x * vp.x - u0.x * v0.x - um.x * v0.x, q.d, q.r),
+                  div_by_recip(up.y * vp.y + u0.y * vp.y - u0.y * v0.y - um.y * v0.y, q.d, q.r));
   } else if constexpr (KIND == 3) {
-    const double2 u0 = r.s[0], um = r.s[1], v = r.s[2];
-    return make_double2(div_by_recip(q.w0 * u0.x + q.w1 * um.x, q.d, q.r) * v.x,
-                        div_by_recip(q.w0 * u0.y + q.w1 * um.y, q.d, q.r) * v.y);
+    const C u0 = r.s[0], um = r.s[1], v = r.s[2];
+    return mkc<C>(div_by_recip(q.w0 * u0.x + q.w1 * um.x, q.d, q.r) * v.x,
+                  div_by_recip(q.w0 * u0.y + q.w1 * um.y, q.d, q.r) * v.y);
   } else {
-    const double2 vp = r.s[0], v0 = r.s[1], vm = r.s[2];
-    const double ax = v0.x + vp.x, bx = v0.x + vm.x, ay = v0.y + vp.y, by = v0.y + vm.y;
-    return make_double2(div_by_recip(0.25 * (ax * ax) - 0.25 * (bx * bx), q.d, q.r),
-                        div_by_recip(0.25 * (ay * ay) - 0.25 * (by * by), q.d, q.r));
+    const C vp = r.s[0], v0 = r.s[1], vm = r.s[2];
+    const S ax = v0.x + vp.x, bx = v0.x + vm.x, ay = v0.y + vp.y, by = v0.y + vm.y;
+    const S q4 = S(0.25);
+    return mkc<C>(div_by_recip(q4 * (ax * ax) - q4 * (bx * bx), q.d, q.r),
+                  div_by_recip(q4 * (ay * ay) - q4 * (by * by), q.d, q.r));
   }
 }
 
-template <int KIND>
-__device__ __forceinline__ double2 eval_prod(const Prod &q, long long off) {
-  return eval_raw<KIND>(q, fetch_prod<KIND>(q, off));
+template <int KIND, class C>
+__device__ __forceinline__ C eval_prod(const Prod<C> &q, long long off) {
+  return eval_raw<KIND, C>(q, fetch_prod<KIND, C>(q, off));
 }
 
-__device__ __forceinline__ double2 eval_prod(const Prod &q, long long off) {
+template <class C>
+__device__ __forceinline__ C eval_prod(const Prod<C> &q, long long off) {
   switch (q.kind) {
-    case 0: return eval_prod<0>(q, off);
-    case 1: return eval_prod<1>(q, off);
-    case 2: return eval_prod<2>(q, off);
-    case 3: return eval_prod<3>(q, off);
-    default: return eval_prod<4>(q, off);
+    case 0: return eval_prod<0, C>(q, off);
+    case 1: return eval_prod<1, C>(q, off);
+    case 2: return eval_prod<2, C>(q, off);
+    case 3: return eval_prod<3, C>(q, off);
+    default: return eval_prod<4, C>(q, off);
   }
 }
 
@@ -197,14 +206,15 @@ __device__ __forceinline__ double2 cf_init(const StepArgs &a, int c, int J, int 
   }
 }
 
-// cf_init for the NQ modes m = threadIdx.x + q nthr of one thread.  For the
-// velocity components without scalars and with BETA = 1 (the channel
-// configuration) every load is issued before the arithmetic, so the modes'
-// memory latencies overlap; other configurations take cf_init mode by mode.
-// Indices beyond the plane are clamped for the loads and their results
-// never stored.
-template <int NQ>
-__device__ __forceinline__ void cf_init_all(const StepArgs &a, int c, int J, int nthr, double2 (&acc)[NQ]) {
+// cf_init for the NQ modes m = threadIdx.x + q nthr of one thread, into
+// accumulators of type C (computed in FP64, rounded to FP32 in the mixed
+// form).  For the velocity components without scalars and with BETA = 1
+// (the channel configuration) the loads of a batch of QB modes are issued
+// before their arithmetic, so the modes' memory latencies overlap; other
+// configurations take cf_init mode by mode.  Indices beyond the plane are
+// clamped for the loads and their results never stored.
+template <int NQ, class C>
+__device__ __forceinline__ void cf_init_all(const StepArgs &a, int c, int J, int nthr, C (&acc)[NQ]) {
   const Geo &g = a.g;
   if (c >= 3 || a.n_th > 0 || a.beta != 1.0) {
 #pragma unroll
@@ -212,47 +222,115 @@ __device__ __forceinline__ void cf_init_all(const StepArgs &a, int c, int J, int
       const int m = threadIdx.x + q * nthr;
       if (m < g.pf) {
         const int kzp = (int)fdiv((unsigned)m, g.f_nkx1), kx = m - kzp * g.nkx1;
-        acc[q] = cf_init(a, c, J, kx, kzp, m);
+        acc[q] = cvt<C>(cf_init(a, c, J, kx, kzp, m));
       }
     }
     return;
   }
   const double2 *U = a.cu[c], *W = a.cu[c == 0 ? 2 : 0];
   const long long base = (long long)J * g.pf;
-  double2 u[NQ], w[NQ];
-  double k2[NQ];
+  // all modes at once in FP64; half of them at a time in the mixed form,
+  // whose kernel has half the registers
+  constexpr int QB = (sizeof(C) == sizeof(double2) || NQ < 2) ? NQ : NQ / 2;
 #pragma unroll
-  for (int q = 0; q < NQ; ++q) {
-    const int m = min((int)threadIdx.x + q * nthr, g.pf - 1);
-    const int kzp = (int)fdiv((unsigned)m, g.f_nkx1), kx = m - kzp * g.nkx1;
-    u[q] = U[base + m];
-    k2[q] = g.kx2[kx] + g.kz2[kzp];
-  }
-  if (c != 1) {
+  for (int q0 = 0; q0 < NQ; q0 += QB) {
+    double2 u[QB], w[QB];
+    double k2[QB];
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) w[q] = W[base + min((int)threadIdx.x + q * nthr, g.pf - 1)];
-  }
-#pragma unroll
-  for (int q = 0; q < NQ; ++q) {
-    const double f = a.nu * k2[q];  // kpow(k2, 1) = k2
-    double2 cf = make_double2(-(f * u[q].x), -(f * u[q].y));
-    if (c == 0) {
-      cf = make_double2(cf.x + a.i_ro * w[q].x, cf.y + a.i_ro * w[q].y);
-    } else if (c == 2) {
-      cf = make_double2(cf.x - a.i_ro * w[q].x, cf.y - a.i_ro * w[q].y);
+    for (int b = 0; b < QB; ++b) {
+      const int m = min((int)threadIdx.x + (q0 + b) * nthr, g.pf - 1);
+      const int kzp = (int)fdiv((unsigned)m, g.f_nkx1), kx = m - kzp * g.nkx1;
+      u[b] = U[base + m];
+      k2[b] = g.kx2[kx] + g.kz2[kzp];
     }
-    acc[q] = cf;
+    if (c != 1) {
+#pragma unroll
+      for (int b = 0; b < QB; ++b) w[b] = W[base + min((int)threadIdx.x + (q0 + b) * nthr, g.pf - 1)];
+    }
+#pragma unroll
+    for (int b = 0; b < QB; ++b) {
+      const double f = a.nu * k2[b];  // kpow(k2, 1) = k2
+      double2 cf = make_double2(-(f * u[b].x), -(f * u[b].y));
+      if (c == 0) {
+        cf = make_double2(cf.x + a.i_ro * w[b].x, cf.y + a.i_ro * w[b].y);
+      } else if (c == 2) {
+        cf = make_double2(cf.x - a.i_ro * w[b].x, cf.y - a.i_ro * w[b].y);
+      }
+      acc[q0 + b] = cvt<C>(cf);
+    }
   }
 }
 
 // Fused explicit pass for plane J (1..NY) and component c (U1,U2,U3,TH...).
 // REG: row transforms in registers, 512 threads (twice the register budget
 // of the 1024-thread LDS variant), NQ Fourier modes owned per thread.
-template <bool REG, int NQ>
-__global__ void __launch_bounds__(REG ? 512 : 1024) k_nonlinear(StepArgs a) {
-  extern __shared__ double2 A[];
+// The mixed form transforms only the fluctuations: every field without its
+// (kx,kz) = (0,0) coefficient, the plane mean, which stays FP64.  The three
+// products of a component are bilinear, so their terms with a plane-mean
+// factor are mode-wise products of FP64 Fourier coefficients and need no
+// transform.  mean_terms returns those terms of f0 P0 + f1 P1 + P2 for one
+// mode (f = ikx or ikz as the pass applies them, P in the forms of
+// make_prod): xm, x0, xp are the component's coefficients at J-1, J, J+1,
+// mu[f][j] the means of U1, U2, U3 and the component's scalar at J-1+j.  At
+// the (0,0) mode the fluctuations vanish and only the mean x mean part of
+// the y flux P2 remains.
+__device__ __forceinline__ double2 cscale(double s, double2 z) { return make_double2(s * z.x, s * z.y); }
+
+__device__ double2 mean_terms(const StepArgs &a, int c, int J, long long o, bool m00, double kxv, double kzv,
+                              double2 xm, double2 x0, double2 xp, const double (&mu)[4][3]) {
   const Geo &g = a.g;
-  const Tables T = load_tables(A, g);
+  const long long pf = g.pf;
+  const double2 zero = make_double2(0.0, 0.0);
+  if (m00) xm = x0 = xp = zero;
+  auto ld = [&](int f, int dj) { return m00 ? zero : a.cu[f][o + dj * pf]; };
+  double2 L0, L1, L2;
+  double mm;  // mean x mean part of P2
+  if (c == 1) {  // interp(U1) U2, interp(U3) U2, d/dy(U2 U2)
+    const double2 A0 = ld(0, 0), Am = ld(0, -1), B0 = ld(2, 0), Bm = ld(2, -1);
+    const double w0 = g.dyf[J], w1 = g.dyf[J - 1], r = 0.5 * g.rdy[J];
+    const double2 I1 = cscale(r, cadd(cscale(w0, A0), cscale(w1, Am)));
+    const double2 I3 = cscale(r, cadd(cscale(w0, B0), cscale(w1, Bm)));
+    const double i1 = (w0 * mu[0][1] + w1 * mu[0][0]) * r, i3 = (w0 * mu[2][1] + w1 * mu[2][0]) * r;
+    L0 = cadd(cscale(i1, x0), cscale(mu[1][1], I1));
+    L1 = cadd(cscale(i3, x0), cscale(mu[1][1], I3));
+    const double am = mu[1][1] + mu[1][2], bm = mu[1][1] + mu[1][0];
+    L2 = cscale(g.rdy[J], csub(cscale(0.5 * am, cadd(x0, xp)), cscale(0.5 * bm, cadd(x0, xm))));
+    mm = (0.25 * am * am - 0.25 * bm * bm) * g.rdy[J];
+  } else {
+    // own field C (U1, U3 or TH) with the y flux d/dy(C U2); P0 and P1 are
+    // U3 U1 and U_c U_c for velocity, TH U1 and TH U3 for a scalar
+    const double *uo = (c < 3) ? mu[c] : mu[3];
+    const double2 V0 = ld(1, 0), Vp = ld(1, 1);
+    if (c < 3) {
+      const int other = 2 - c;
+      const double2 Y0 = ld(other, 0);
+      L0 = cadd(cscale(mu[other][1], x0), cscale(uo[1], Y0));
+      L1 = cscale(2.0 * uo[1], x0);
+    } else {
+      const double2 A0 = ld(0, 0), B0 = ld(2, 0);
+      L0 = cadd(cscale(uo[1], A0), cscale(mu[0][1], x0));
+      L1 = cadd(cscale(uo[1], B0), cscale(mu[2][1], x0));
+    }
+    const double r2 = 0.5 * g.rdyf[J];
+    double2 s = cadd(cscale(uo[2], Vp), cscale(mu[1][2], xp));
+    s = cadd(s, cadd(cscale(uo[1], Vp), cscale(mu[1][2], x0)));
+    s = csub(s, cadd(cscale(uo[1], V0), cscale(mu[1][1], x0)));
+    s = csub(s, cadd(cscale(uo[0], V0), cscale(mu[1][1], xm)));
+    L2 = cscale(r2, s);
+    mm = (uo[2] * mu[1][2] + uo[1] * mu[1][2] - uo[1] * mu[1][1] - uo[0] * mu[1][1]) * r2;
+  }
+  if (m00) return make_double2(mm, 0.0);
+  const double f0 = (c == 0) ? kzv : kxv, f1 = (c == 0) ? kxv : kzv;
+  return cadd(cadd(cmul_ik(f0, L0), cmul_ik(f1, L1)), L2);
+}
+
+// C: complex type of the plane and the physical arrays (double2, or float2
+// in the mixed form, whose products and transforms run in FP32 while the
+// Fourier state, the accumulation of CF and the RK combination stay FP64).
+template <bool REG, int NQ, class C>
+__device__ __forceinline__ void nonlinear_pass(const StepArgs &a, C *A) {
+  const Geo &g = a.g;
+  const auto T = load_tables(A, g);
   // Work item of this block: workgroups are dealt to the 8 XCDs round-robin,
   // so give XCD x a contiguous range of (plane, component) items, planes
   // outer: the blocks sharing an XCD's L2 then read the same U planes.
@@ -279,7 +357,7 @@ __global__ void __launch_bounds__(REG ? 512 : 1024) k_nonlinear(StepArgs a) {
   const double time = a.clk->time;
   const int nthr = blockDim.x;
 
-  double2 acc[NQ];
+  C acc[NQ];  // CF of this thread's modes (FP64, or FP32 in the mixed form)
   cf_init_all<NQ>(a, c, J, nthr, acc);
   // the three products, forward transformed and subtracted with their
   // derivative factors: c==0 takes (ikz, ikx, 1), all others (ikx, ikz, 1)
@@ -288,17 +366,19 @@ __global__ void __launch_bounds__(REG ? 512 : 1024) k_nonlinear(StepArgs a) {
   const int nw = nthr >> 6;
   // quotients by per-block constants take their correctly rounded
   // reciprocals (div_by_recip): the same doubles as the divisions
-  const double nzd = g.dnz, rnz = g.rnz;
+  using S = Sc<C>;
+  constexpr bool DELTA = sizeof(C) != sizeof(double2);  // the mixed form transforms increments
+  const S nzd = S(g.dnz), rnz = S(g.rnz);
   // acc -= f (plane / NZ) after the z transform, f = ikz (mode 0), ikx (1)
   // or 1 (2); the wavenumbers are loaded ahead of the LDS reads
   auto accumulate = [&](int mode) {
-    double kq[NQ];
+    S kq[NQ];
     if (mode != 2) {
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
         const int m = min((int)threadIdx.x + q * nthr, g.pf - 1);
         const int kzp = (int)fdiv((unsigned)m, g.f_nkx1), kx = m - kzp * g.nkx1;
-        kq[q] = (mode == 0) ? g.kz[kzp] : g.kx[kx];
+        kq[q] = S((mode == 0) ? g.kz[kzp] : g.kx[kx]);
       }
     }
 #pragma unroll
@@ -306,8 +386,8 @@ __global__ void __launch_bounds__(REG ? 512 : 1024) k_nonlinear(StepArgs a) {
       const int m = threadIdx.x + q * nthr;
       if (m < g.pf) {
         const int kzp = (int)fdiv((unsigned)m, g.f_nkx1), kx = m - kzp * g.nkx1;
-        double2 s = A[fwd_slot(g, T, kx, kzp)];
-        s = make_double2(div_by_recip(s.x, nzd, rnz), div_by_recip(s.y, nzd, rnz));
+        const C s0 = A[fwd_slot(g, T, kx, kzp)];
+        const C s = mkc<C>(div_by_recip(s0.x, nzd, rnz), div_by_recip(s0.y, nzd, rnz));
         acc[q] = csub(acc[q], (mode == 2) ? s : cmul_ik(kq[q], s));
       }
     }
@@ -321,14 +401,14 @@ __global__ void __launch_bounds__(REG ? 512 : 1024) k_nonlinear(StepArgs a) {
     // (both transforms are linear and KX is constant along z).
     auto rows = [&](int p, auto merge) {
       constexpr bool MERGE = decltype(merge)::value;
-      const Prod pr = make_prod(a, c, p, J);
+      const Prod<C> pr = make_prod<C>(a, c, p, J);
       // the form is resolved once so that each slab's loads are
       // straight-line code ahead of the arithmetic
       auto run = [&](auto K) {
         constexpr int KIND = decltype(K)::value;
         rows_fwd_reg64<MERGE>(
-            A, g, T, [&](int z, int xp) { return fetch_prod<KIND>(pr, (long long)z * g.nx + 2 * xp); },
-            [&](const ProdRaw<KIND> &r) { return eval_raw<KIND>(pr, r); }, g.kx);
+            A, g, T, [&](int z, int xp) { return fetch_prod<KIND, C>(pr, (long long)z * g.nx + 2 * xp); },
+            [&](const ProdRaw<KIND, C> &r) { return eval_raw<KIND, C>(pr, r); }, g.kx);
       };
       // forms: the y fluxes (p = 2) are 2 or 4, the others 0, 1 or 3
       if constexpr (MERGE) {
@@ -349,7 +429,7 @@ __global__ void __launch_bounds__(REG ? 512 : 1024) k_nonlinear(StepArgs a) {
     accumulate(2);
   } else {
     for (int p = 0; p < 3; ++p) {
-      const Prod pr = make_prod(a, c, p, J);
+      const Prod<C> pr = make_prod<C>(a, c, p, J);
       for (int t0 = threadIdx.x >> 6; t0 < lt.ntiles; t0 += nw) {
         int z, xp;
         if (lt.at(g, t0, z, xp)) A[phys_slot(T, g, z, xp)] = eval_prod(pr, (long long)z * g.nx + 2 * xp);
@@ -364,7 +444,7 @@ __global__ void __launch_bounds__(REG ? 512 : 1024) k_nonlinear(StepArgs a) {
   // new CF stored, CR scattered into LDS for the inverse transform.
   for (int e = threadIdx.x; e < g.nz * g.nkx1; e += nthr) {
     const int zr = (int)fdiv((unsigned)e, g.f_nkx1), kx = e - zr * g.nkx1;
-    if (zr > g.nkz && zr < g.nz - g.nkz) A[inv_row(T, zr) * g.ls + kx] = make_double2(0.0, 0.0);
+    if (zr > g.nkz && zr < g.nz - g.nkz) A[inv_row(T, zr) * g.ls + kx] = mkc<C>(0, 0);
   }
   double2 *CF = (c < 3) ? a.cf[c] : a.cfth + (c - 3) * a.cfs;
   const double2 *CC = (c < 3) ? a.cu[c] : a.cth + (c - 3) * a.cfs;
@@ -394,8 +474,20 @@ __global__ void __launch_bounds__(REG ? 512 : 1024) k_nonlinear(StepArgs a) {
   // modes in batches of NB: the batch's loads (indices clamped to the plane)
   // are all issued before its arithmetic (smaller batches in the
   // 1024-thread variant, whose register budget is half as large)
-  constexpr int NB = REG ? 4 : 2;
+  constexpr int NB = (REG && sizeof(C) == sizeof(double2)) ? 4 : (sizeof(C) == sizeof(double2) ? 2 : 1);
   const double dyJ = g.dy[J], rdyJ = g.rdy[J];
+  // the mixed form: plane means of U1, U2, U3 and this scalar at J-1..J+1
+  double mu[4][3] = {};
+  if constexpr (DELTA) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const long long pl = (long long)(J - 1 + j) * g.pf;
+      mu[0][j] = a.cu[0][pl].x;
+      mu[1][j] = a.cu[1][pl].x;
+      mu[2][j] = a.cu[2][pl].x;
+      mu[3][j] = (c >= 3) ? CC[pl].x : 0.0;
+    }
+  }
   static_assert(NQ % NB == 0, "mode batches must tile the modes of a thread");
 #pragma unroll
   for (int q0 = 0; q0 < NQ; q0 += NB) {
@@ -431,11 +523,15 @@ __global__ void __launch_bounds__(REG ? 512 : 1024) k_nonlinear(StepArgs a) {
       if (m < g.pf) {
         const int kzp = (int)fdiv((unsigned)m, g.f_nkx1), kx = m - kzp * g.nkx1;
         const long long o = base + m;
-        double2 cr = cc0[b];
+        // the mixed form leaves out CU and takes the explicit y-viscous half
+        // twice: CR - CU + T1 NU_V_SCALE d/dy(d/dy CU) is the right-hand side of
+        // the increment U_new - U_old under the same implicit operator
+        const double2 c0 = cc0[b];
+        double2 cr = DELTA ? make_double2(0.0, 0.0) : c0;
         const double2 ev = make_double2(
-            coefv * div_by_recip(div_by_recip(cp1[b].x - cr.x, ey0, rey0) - div_by_recip(cr.x - cm1[b].x, ey1, rey1),
+            coefv * div_by_recip(div_by_recip(cp1[b].x - c0.x, ey0, rey0) - div_by_recip(c0.x - cm1[b].x, ey1, rey1),
                                  ey2, rey2),
-            coefv * div_by_recip(div_by_recip(cp1[b].y - cr.y, ey0, rey0) - div_by_recip(cr.y - cm1[b].y, ey1, rey1),
+            coefv * div_by_recip(div_by_recip(cp1[b].y - c0.y, ey0, rey0) - div_by_recip(c0.y - cm1[b].y, ey1, rey1),
                                  ey2, rey2));
         if (rk > 0) cr = make_double2(cr.x + T3 * old[b].x, cr.y + T3 * old[b].y);
         if (c == 0 || c == 2) {
@@ -451,23 +547,40 @@ __global__ void __launch_bounds__(REG ? 512 : 1024) k_nonlinear(StepArgs a) {
           double2 cs = make_double2(div_by_recip(p0[b].x - pm[b].x, dyJ, rdyJ), div_by_recip(p0[b].y - pm[b].y, dyJ, rdyJ));
           cr = make_double2(cr.x - T4 * cs.x, cr.y - T4 * cs.y);
         }
-        const double2 cfn = acc[q0 + b];
+        double2 cfn = make_double2(acc[q0 + b].x, acc[q0 + b].y);
+        if constexpr (DELTA)
+          cfn = csub(cfn, mean_terms(a, c, J, o, kx == 0 && kzp == 0, g.kx[kx], g.kz[kzp], cm1[b], c0, cp1[b], mu));
         CF[o] = cfn;
         cr = make_double2(cr.x + T5 * cfn.x, cr.y + T5 * cfn.y);
         cr = make_double2(cr.x + ev.x, cr.y + ev.y);
-        A[inv_row(T, kz_row(g, kzp)) * g.ls + kx] = cr;
+        if (DELTA) cr = make_double2(cr.x + ev.x, cr.y + ev.y);
+        A[inv_row(T, kz_row(g, kzp)) * g.ls + kx] = cvt<C>(cr);
       }
     }
   }
   lds_barrier();
   plane_cols_inv(A, g, T);
-  double2 *dst = reinterpret_cast<double2 *>(a.rphys + (long long)c * a.fs + (long long)J * g.pp);
+  C *dst = reinterpret_cast<C *>(static_cast<Sc<C> *>(a.rphys) + (long long)c * a.fs + (long long)J * g.pp);
   if constexpr (REG) {
-    rows_inv_reg64(A, g, T, [&](int z, int m, double2 v) { dst[z * 64 + m] = v; });
+    rows_inv_reg64(A, g, T, [&](int z, int m, C v) { dst[z * 64 + m] = v; });
   } else {
     plane_rows_inv(A, g, T);
     plane_store_phys(A, g, T, dst);
   }
+}
+template <bool REG, int NQ>
+__global__ void __launch_bounds__(REG ? 512 : 1024) k_nonlinear(StepArgs a) {
+  extern __shared__ double2 smem[];
+  nonlinear_pass<REG, NQ, double2>(a, smem);
+}
+
+// The mixed form: the FP32 plane takes half the LDS and half the registers
+// of a complex value, so two 512-thread blocks share a CU (4 waves per SIMD).
+template <bool REG, int NQ>
+__global__ void __launch_bounds__(REG ? 512 : 1024) __attribute__((amdgpu_waves_per_eu(4)))
+k_nonlinear32(StepArgs a) {
+  extern __shared__ double2 smem[];
+  nonlinear_pass<REG, NQ, float2>(a, reinterpret_cast<float2 *>(smem));
 }
 // ---------------------------------------------------------------------------
 // Real Crank-Nicolson systems (matrix build of channel.f:771-1005, BC rows
@@ -566,8 +679,8 @@ __host__ __device__ inline void real_row(const FactorIn &in, int f, double temp1
 // and the carry factors of the segmented sweeps of k_solve_seg:
 //   PF_J = prod_{i=a..J} TL_i          (a: first row of J's segment)
 //   PB_J = prod_{i=J..b-1} -TC_i/TB_i  (b-1: last row of J's segment)
-// The constant right-hand sides of the boundary rows (dt independent) are
-// written by the rk=0 blocks.
+// The constant right-hand sides of the boundary rows and the rows'
+// coefficients (dt independent) are written by the rk=0 blocks.
 __global__ void __launch_bounds__(256) k_cn_tables(const Clock *clk, FactorIn in, RealFactors fac) {
 #pragma clang fp contract(off)
   extern __shared__ double sm[];
@@ -585,6 +698,9 @@ __global__ void __launch_bounds__(256) k_cn_tables(const Clock *clk, FactorIn in
     if (rk == 0) {
       fac.isbc[(long long)f * nrow + i] = bc;
       fac.vconst[(long long)f * nrow + i] = V;
+      fac.bcl[(long long)f * nrow + i] = bc ? L : 0.0;
+      fac.bcd[(long long)f * nrow + i] = bc ? D : 0.0;
+      fac.bcu[(long long)f * nrow + i] = bc ? U : 0.0;
     }
   }
   __syncthreads();
@@ -644,8 +760,10 @@ struct SolveArgs {
   Geo g;
   Clock *clk;
   int rk;
-  double *dst[kMaxFields];  // solution of field f (may be the field's own R: each lane reads its rows first)
-  const double *rphys;
+  void *dst[kMaxFields];  // solution of field f (may be the field's own R: each lane reads its rows first)
+  const void *rphys;       // physical R and U: double, or float in the mixed form (FP64 arithmetic either way)
+  void *uold;              // mixed form: the fluctuations at the start of the substep (float)
+  const double2 *four[kMaxFields];  // mixed form: Fourier fields, whose (0,0) coefficients are the plane means
   int f0;                   // field of blockIdx.y = 0
   long long fs;
   RealFactors fac;
@@ -657,7 +775,7 @@ struct SolveArgs {
 
 constexpr int kSegThreads = 256;
 
-template <int NS>
+template <int NS, class P>
 __global__ void __launch_bounds__(kSegThreads) k_solve_seg(SolveArgs a) {
 #pragma clang fp contract(off)
   constexpr int CW = 64 / NS;  // columns per wave
@@ -690,8 +808,8 @@ __global__ void __launch_bounds__(kSegThreads) k_solve_seg(SolveArgs a) {
   const bool live = col < pp;
   const int r0 = seg * sr;
   const int nr = min(sr, max(nrow - r0, 0));
-  double *U = a.dst[f] + col;
-  const double *R = a.rphys + (long long)f * a.fs + col;
+  P *U = static_cast<P *>(a.dst[f]) + col;
+  const P *R = static_cast<const P *>(a.rphys) + (long long)f * a.fs + col;
   // right-hand sides: the boundary constant or R
   double v[SR];
 #pragma unroll
@@ -699,10 +817,40 @@ __global__ void __launch_bounds__(kSegThreads) k_solve_seg(SolveArgs a) {
     const int i = r0 + q;
     v[q] = (live && q < nr && !IB[i]) ? R[(long long)i * pp] : 0.0;
   }
+  // the mixed form solves for the increment d = U_new - U_old: interior rows
+  // bring the increment's right-hand side from the nonlinear pass, boundary
+  // rows V - (their row applied to U_old).  Boundary rows are among
+  // 0, 1, 2, NY, NY+1 (real_row), so their five values are formed once.
+  constexpr bool DELTA = sizeof(P) == sizeof(float);
+  P *Uo = nullptr;
+  double bcv[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  if constexpr (DELTA) {
+    Uo = static_cast<P *>(a.uold) + (long long)f * a.fs + col;
+    const long long ro = (long long)f * nrow;
+    // U_old = plane mean (FP64) + fluctuation (FP32)
+    auto uold = [&](int i) { return a.four[f][(long long)i * g.pf].x + (double)Uo[(long long)i * pp]; };
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      const int i = (k < 3) ? k : NY - 3 + k;
+      if (live && i >= r0 && i < r0 + nr && IB[i]) {
+        const double cl = a.fac.bcl[ro + i], cd = a.fac.bcd[ro + i], cu = a.fac.bcu[ro + i];
+        double s = 0.0;
+        if (cl != 0.0 && i >= 1) s += cl * uold(i - 1);
+        if (cd != 0.0) s += cd * uold(i);
+        if (cu != 0.0 && i + 1 < nrow) s += cu * uold(i + 1);
+        bcv[k] = VC[i] - s;
+      }
+    }
+  }
 #pragma unroll
   for (int q = 0; q < SR; ++q) {
     const int i = r0 + q;
-    if (q < nr && IB[i]) v[q] = VC[i];
+    if (q < nr && IB[i]) {
+      if constexpr (DELTA)
+        v[q] = (i == 0) ? bcv[0] : (i == 1) ? bcv[1] : (i == 2) ? bcv[2] : (i == NY) ? bcv[3] : bcv[4];
+      else
+        v[q] = VC[i];
+    }
   }
   // forward sweep within the segment from a zero carry
   double hp = 0.0, hend = 0.0;
@@ -761,9 +909,14 @@ __global__ void __launch_bounds__(kSegThreads) k_solve_seg(SolveArgs a) {
     const double x = (seg == NS - 1) ? v[q] : v[q] + PB[i] * xin;
     if (live && q < nr && i >= wlo && i <= whi) {
       U[(long long)i * pp] = x;
+      double un = x;  // the new field value (U_old + d in the mixed form)
+      if constexpr (DELTA) {
+        if (cour || f >= 3) un = (a.four[f][(long long)i * g.pf].x + (double)Uo[(long long)i * pp]) + x;
+        if (f >= 3) Uo[(long long)i * pp] = (P)un;  // the scalar's physical field persists
+      }
       if (cour && i >= 1 && i <= NY) {
         const double dl = (f == 1) ? g.dy[i] : dxf;
-        dtmin = fmin(dtmin, a.cfl * dl / fabs(x));
+        dtmin = fmin(dtmin, a.cfl * dl / fabs(un));
       }
     }
   }
@@ -1193,8 +1346,8 @@ static StepArgs make_step_args(Ctx &c, int rk) {
   a.cp = c.cp;
   a.cth = c.cth;
   a.cfth = c.cfth;
-  a.uphys = c.uphys;
-  a.rphys = c.rphys;
+  a.uphys = c.mixed ? (const void *)c.uphys32 : (const void *)c.uphys;
+  a.rphys = c.mixed ? (void *)c.rphys32 : (void *)c.rphys;
   a.fs = (long long)(c.g.ny + 2) * c.g.pp;
   a.cfs = (long long)(c.g.ny + 2) * c.g.pf;
   a.c0 = 0;
@@ -1273,10 +1426,16 @@ static hipError_t launch_nonlinear_range(Ctx &c, int rk, int c0, int c1, hipStre
   a.c0 = c0;
   a.j0 = jlo;
   const dim3 grid(jhi - jlo + 1, c1 - c0);
-  if (c.reg_rows_nl)
+  if (c.mixed) {
+    if (c.reg_rows_nl)
+      hipLaunchKernelGGL((k_nonlinear32<true, kNlRegQ>), grid, dim3(kNlRegThreads), c.lds_bytes32, st, a);
+    else
+      hipLaunchKernelGGL((k_nonlinear32<false, kMaxQ>), grid, dim3(c.plane_threads), c.lds_bytes32, st, a);
+  } else if (c.reg_rows_nl) {
     hipLaunchKernelGGL((k_nonlinear<true, kNlRegQ>), grid, dim3(kNlRegThreads), c.lds_bytes, st, a);
-  else
+  } else {
     hipLaunchKernelGGL((k_nonlinear<false, kMaxQ>), grid, dim3(c.plane_threads), c.lds_bytes, st, a);
+  }
   return hipGetLastError();
 }
 
@@ -1292,9 +1451,21 @@ static hipError_t launch_solve_range(Ctx &c, int rk, int courant, int f0, int f1
   a.g = c.g;
   a.clk = c.clock;
   a.rk = rk;
-  a.rphys = c.rphys;
   a.fs = (long long)(c.g.ny + 2) * c.g.pp;
-  for (int f = 0; f < c.nfields; ++f) a.dst[f] = (f < 3 ? c.rphys : c.uphys) + f * a.fs;
+  if (c.mixed) {
+    // increments of every field over their R arrays; scalars' new values
+    // also into their physical fields
+    a.rphys = c.rphys32;
+    a.uold = c.uphys32;
+    const long long cfs = (long long)(c.g.ny + 2) * c.g.pf;
+    for (int f = 0; f < c.nfields; ++f) {
+      a.dst[f] = c.rphys32 + f * a.fs;
+      a.four[f] = (f < 3) ? c.cu[f] : c.cth + (f - 3) * cfs;
+    }
+  } else {
+    a.rphys = c.rphys;
+    for (int f = 0; f < c.nfields; ++f) a.dst[f] = (f < 3 ? c.rphys : c.uphys) + f * a.fs;
+  }
   a.f0 = f0;
   a.fac = c.fac;
   a.nfields = c.nfields;
@@ -1308,10 +1479,17 @@ static hipError_t launch_solve_range(Ctx &c, int rk, int courant, int f0, int f1
   a.dz = c.p.lz / c.g.nz;
   const int cols = (kSegThreads / 64) * (64 / c.fac.nseg);  // columns per block
   const int nb = (int)((c.g.pp + cols - 1) / cols);
-  if (c.fac.nseg == 4)
-    hipLaunchKernelGGL(k_solve_seg<4>, dim3(nb, f1 - f0), dim3(kSegThreads), 0, st, a);
-  else
-    hipLaunchKernelGGL(k_solve_seg<8>, dim3(nb, f1 - f0), dim3(kSegThreads), 0, st, a);
+  const dim3 grid(nb, f1 - f0);
+  if (c.mixed) {
+    if (c.fac.nseg == 4)
+      hipLaunchKernelGGL((k_solve_seg<4, float>), grid, dim3(kSegThreads), 0, st, a);
+    else
+      hipLaunchKernelGGL((k_solve_seg<8, float>), grid, dim3(kSegThreads), 0, st, a);
+  } else if (c.fac.nseg == 4) {
+    hipLaunchKernelGGL((k_solve_seg<4, double>), grid, dim3(kSegThreads), 0, st, a);
+  } else {
+    hipLaunchKernelGGL((k_solve_seg<8, double>), grid, dim3(kSegThreads), 0, st, a);
+  }
   return hipGetLastError();
 }
 
@@ -1448,6 +1626,12 @@ hipError_t prepare_step_kernels(Ctx &c) {
   if (e != hipSuccess) return e;
   e = hipFuncSetAttribute((const void *)k_nonlinear<false, kMaxQ>, hipFuncAttributeMaxDynamicSharedMemorySize,
                           c.lds_bytes);
+  if (e != hipSuccess) return e;
+  e = hipFuncSetAttribute((const void *)k_nonlinear32<true, kNlRegQ>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          c.lds_bytes32);
+  if (e != hipSuccess) return e;
+  e = hipFuncSetAttribute((const void *)k_nonlinear32<false, kMaxQ>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          c.lds_bytes32);
   if (e != hipSuccess) return e;
   return hipFuncSetAttribute((const void *)k_rem_div_thomas, hipFuncAttributeMaxDynamicSharedMemorySize,
                              (int)rem_div_thomas_lds(c));

@@ -130,6 +130,17 @@ int dh_rk_substep(dh_ctx *ctx, int32_t rk_step);
  * hooks: TIME_STEP+=1, RK_CHAN_1 for RK_STEP=1..3, TIME=TIME+DELTA_T.
  * Replays a captured hipGraph of one step after the first call. */
 int dh_time_step(dh_ctx *ctx, int32_t nsteps);
+/* The FP32/FP64 mixed form of RK_CHAN_1 (no counterpart in the reference,
+ * which is FP64 throughout): with on = 1 the substeps of dh_rk_substep and
+ * dh_time_step run the x-z transforms, the nonlinear products and the
+ * physical work arrays in FP32, while the Fourier state (CU, CP, CF, CTH,
+ * CFTH), the accumulation of CF, the RK combination, the Crank-Nicolson and
+ * pressure-Poisson solves and COURANT stay FP64.  on = 0 returns to the FP64
+ * drop-in path.  The first switch allocates two FP32 physical arrays of
+ * (NY+2)*NX*NZ*(3+N_TH) values. */
+int dh_set_mixed_precision(dh_ctx *ctx, int32_t on);
+/* 1 when the mixed form is on, 0 when not. */
+int dh_mixed_precision(dh_ctx *ctx);
 /* Per-phase timing of one substep (the reference's only timer is the
  * wall-clock "Seconds per Iteration" of diablo.f:123-127): runs substep
  * rk_step nrep times with HIP events between its kernels on the context
@@ -210,7 +221,9 @@ int dh_fft_xz_to_physical_dev(dh_ctx *ctx, const double *four, double *phys, int
  * benchmarks: which = 0..2 CU1..3, 3 CP, 4..6 CF1..3, 7 phi (CR1 after
  * REM_DIV), 8 + n CTH(n), 8 + N_TH + n CFTH(n); 100+c the physical
  * field c: for c < 3 the velocity transformed at the start of the last
- * substep (its solve writes over R), for c >= 3 TH as left by its solve. */
+ * substep (its solve writes over R), for c >= 3 TH as left by its solve;
+ * 200+c the same field of the mixed form (float values; null until the
+ * mixed form is first switched on). */
 double *dh_field_dev(dh_ctx *ctx, int32_t which);
 /* HIP stream (hipStream_t) the context runs on. */
 void *dh_stream(dh_ctx *ctx);

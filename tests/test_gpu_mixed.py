@@ -1,0 +1,96 @@
+"""The FP32/FP64 mixed form of the substep (dh_set_mixed_precision) against
+the FP64 oracle (needs an MI355X).
+
+The mixed form transforms only the O(dt) increments U_new - U_old of the
+substep and the fluctuations about the plane means in FP32; the Fourier
+state, the plane means ((kx,kz) = (0,0) coefficients) and every term with a
+mean factor, CF, the RK combination, the solves and COURANT stay FP64.
+After 10 RK3 steps from the same state as the oracle every state field of
+each deck must agree to relative L2 <= 1e-5 (the gate of the mixed form),
+and DELTA_T to 1e-5 relative: the small all-terms deck (scalar, rotation,
+mixed BCs), the KH shear layer with free-slip walls, and the Re_tau=180
+channel (128x129x128, variable dt)."""
+import numpy as np
+import pytest
+
+from diablo_amd import channel
+from tests import helpers
+
+pytestmark = pytest.mark.gpu
+
+STATE = ("cu1", "cu2", "cu3", "cp")
+BAR = 1e-5
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def _fmt(d):
+    return {k: f"{v:.1e}" for k, v in d.items()}
+
+
+@pytest.fixture(scope="module")
+def dor():
+    from oracle import dor as d
+    d.load()
+    return d
+
+
+@pytest.mark.parametrize("name", ["small_case", "kh_shear_layer", "channel_re180"])
+def test_mixed_ten_steps(dor, name):
+    steps = 10
+    _say(f"[mixed] {steps} steps {name}")
+    o, p, gy, ic = helpers.make_oracle(dor, name)
+    o2, *_ = helpers.make_oracle(dor, name)
+    helpers.perturb(dor, o2, 1e-16)
+    ch = channel.Channel(p, gy)
+    ch.upload(helpers.oracle_state(dor, o))
+    ch.set_mixed_precision(True)
+    assert ch.mixed_precision
+    ch.set_clock(0.0, p["delta_t"], 0)
+    o.time_step(steps)
+    o2.time_step(steps)
+    ch.time_step(steps)
+    t_o, dt_o, ts_o = o.clock()
+    t_g, dt_g, hb, ts_g = ch.clock()
+    _say(f"  time oracle {t_o!r} mixed {t_g!r}; dt oracle {dt_o!r} mixed {dt_g!r}")
+    assert ts_g == ts_o == steps
+    assert abs(dt_g - dt_o) <= BAR * dt_o
+    st = ch.download()
+    keys = STATE + (("th",) if p["n_th"] > 0 else ())
+    errs = helpers.field_errors(dor, o, st, keys=keys)
+    floor = helpers.field_errors(dor, o, helpers.oracle_state(dor, o2), keys=keys)
+    _say("  err  ", _fmt(errs))
+    _say("  floor", _fmt(floor))
+    bad = {k: e for k, e in errs.items() if not e <= BAR}
+    assert not bad, bad
+    ch.close()
+
+
+def test_switching_back_is_the_plain_path(dor):
+    """Mixed on, a step, off again: the next steps are the FP64 path's,
+    bit for bit (the captured step graphs of the other form are dropped)."""
+    _say("[mixed] switch back small_case")
+    o, p, gy, ic = helpers.make_oracle(dor, "small_case")
+    state = helpers.oracle_state(dor, o)
+    ref = channel.Channel(p, gy)
+    ref.upload(state)
+    ref.set_clock(0.0, p["delta_t"], 0)
+    ref.time_step(3)
+    a = ref.download()
+    ch = channel.Channel(p, gy)
+    ch.upload(state)
+    ch.set_clock(0.0, p["delta_t"], 0)
+    ch.set_mixed_precision(True)
+    ch.time_step(1)
+    ch.set_mixed_precision(False)
+    assert not ch.mixed_precision
+    ch.upload(state)
+    ch.set_clock(0.0, p["delta_t"], 0)
+    ch.time_step(3)
+    b = ch.download()
+    for k in STATE:
+        assert np.array_equal(a[k], b[k]), k
+    ref.close()
+    ch.close()
