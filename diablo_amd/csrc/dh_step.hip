@@ -808,48 +808,32 @@ __global__ void __launch_bounds__(kSegThreads) k_solve_seg(SolveArgs a) {
   const bool live = col < pp;
   const int r0 = seg * sr;
   const int nr = min(sr, max(nrow - r0, 0));
-  P *U = static_cast<P *>(a.dst[f]) + col;
-  const P *R = static_cast<const P *>(a.rphys) + (long long)f * a.fs + col;
+  // rows are addressed from the field's (uniform) base with 32-bit element
+  // offsets i pp + col (a field is below 2^31 elements)
+  P *const Ub = static_cast<P *>(a.dst[f]);
+  const P *const Rb = static_cast<const P *>(a.rphys) + (long long)f * a.fs;
+  const unsigned ppu = (unsigned)pp, colu = (unsigned)col;
+  auto ro_ = [&](int i) { return (unsigned)i * ppu + colu; };
+  // the mixed form solves for the increment d = U_new - U_old: every row's
+  // right-hand side is in R (interior rows from the nonlinear pass,
+  // boundary rows from k_bc_increment)
+  constexpr bool DELTA = sizeof(P) == sizeof(float);
+  P *const Uob = DELTA ? static_cast<P *>(a.uold) + (long long)f * a.fs : nullptr;
   // right-hand sides: the boundary constant or R
   double v[SR];
 #pragma unroll
   for (int q = 0; q < SR; ++q) {
     const int i = r0 + q;
-    v[q] = (live && q < nr && !IB[i]) ? R[(long long)i * pp] : 0.0;
+    v[q] = (live && q < nr && (DELTA || !IB[i])) ? Rb[ro_(i)] : 0.0;
+    // FP32 rows are converted in chunks, so that the loads in flight and
+    // their FP64 values do not both hold all rows' registers
+    if (DELTA && q % 12 == 11) __builtin_amdgcn_sched_barrier(0);
   }
-  // the mixed form solves for the increment d = U_new - U_old: interior rows
-  // bring the increment's right-hand side from the nonlinear pass, boundary
-  // rows V - (their row applied to U_old).  Boundary rows are among
-  // 0, 1, 2, NY, NY+1 (real_row), so their five values are formed once.
-  constexpr bool DELTA = sizeof(P) == sizeof(float);
-  P *Uo = nullptr;
-  double bcv[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-  if constexpr (DELTA) {
-    Uo = static_cast<P *>(a.uold) + (long long)f * a.fs + col;
-    const long long ro = (long long)f * nrow;
-    // U_old = plane mean (FP64) + fluctuation (FP32)
-    auto uold = [&](int i) { return a.four[f][(long long)i * g.pf].x + (double)Uo[(long long)i * pp]; };
+  if constexpr (!DELTA) {
 #pragma unroll
-    for (int k = 0; k < 5; ++k) {
-      const int i = (k < 3) ? k : NY - 3 + k;
-      if (live && i >= r0 && i < r0 + nr && IB[i]) {
-        const double cl = a.fac.bcl[ro + i], cd = a.fac.bcd[ro + i], cu = a.fac.bcu[ro + i];
-        double s = 0.0;
-        if (cl != 0.0 && i >= 1) s += cl * uold(i - 1);
-        if (cd != 0.0) s += cd * uold(i);
-        if (cu != 0.0 && i + 1 < nrow) s += cu * uold(i + 1);
-        bcv[k] = VC[i] - s;
-      }
-    }
-  }
-#pragma unroll
-  for (int q = 0; q < SR; ++q) {
-    const int i = r0 + q;
-    if (q < nr && IB[i]) {
-      if constexpr (DELTA)
-        v[q] = (i == 0) ? bcv[0] : (i == 1) ? bcv[1] : (i == 2) ? bcv[2] : (i == NY) ? bcv[3] : bcv[4];
-      else
-        v[q] = VC[i];
+    for (int q = 0; q < SR; ++q) {
+      const int i = r0 + q;
+      if (q < nr && IB[i]) v[q] = VC[i];
     }
   }
   // forward sweep within the segment from a zero carry
@@ -908,15 +892,31 @@ __global__ void __launch_bounds__(kSegThreads) k_solve_seg(SolveArgs a) {
     const int i = r0 + q;
     const double x = (seg == NS - 1) ? v[q] : v[q] + PB[i] * xin;
     if (live && q < nr && i >= wlo && i <= whi) {
-      U[(long long)i * pp] = x;
-      double un = x;  // the new field value (U_old + d in the mixed form)
-      if constexpr (DELTA) {
-        if (cour || f >= 3) un = (a.four[f][(long long)i * g.pf].x + (double)Uo[(long long)i * pp]) + x;
-        if (f >= 3) Uo[(long long)i * pp] = (P)un;  // the scalar's physical field persists
-      }
-      if (cour && i >= 1 && i <= NY) {
+      Ub[ro_(i)] = x;
+      if (!DELTA && cour && i >= 1 && i <= NY) {
         const double dl = (f == 1) ? g.dy[i] : dxf;
-        dtmin = fmin(dtmin, a.cfl * dl / fabs(un));
+        dtmin = fmin(dtmin, a.cfl * dl / fabs(x));
+      }
+    }
+  }
+  // the mixed form: the new values U_old + d, for COURANT and for the
+  // physical field of a scalar (which persists); in a loop of its own that
+  // reads d back from U (this lane's own stores), so that the rows' values
+  // need not stay in registers
+  if constexpr (DELTA) {
+    if (cour || f >= 3) {
+#pragma unroll 6
+      for (int q = 0; q < SR; ++q) {
+        const int i = r0 + q;
+        if (live && q < nr && i >= wlo && i <= whi) {
+          const double x = (double)Ub[ro_(i)];
+          const double un = (a.four[f][(long long)i * g.pf].x + (double)Uob[ro_(i)]) + x;
+          if (f >= 3) Uob[ro_(i)] = (P)un;
+          if (cour && i >= 1 && i <= NY) {
+            const double dl = (f == 1) ? g.dy[i] : dxf;
+            dtmin = fmin(dtmin, a.cfl * dl / fabs(un));
+          }
+        }
       }
     }
   }
@@ -1441,6 +1441,35 @@ static hipError_t launch_nonlinear_range(Ctx &c, int rk, int c0, int c1, hipStre
 
 static hipError_t launch_nonlinear(Ctx &c, int rk) { return launch_nonlinear_range(c, rk, 0, c.nfields, c.stream); }
 
+// The mixed form's boundary rows: the right-hand side of the increment
+// d = U_new - U_old on a boundary row is V - (the row applied to U_old), with
+// U_old = plane mean (FP64) + fluctuation (FP32).  Boundary rows are among
+// 0, 1, 2, NY, NY+1 (real_row); one thread per (x,z) column of field
+// f0 + blockIdx.y writes them into the field's R array, after the
+// nonlinear pass (which writes R on its own planes, some of them boundary
+// rows) and before the solve.
+__global__ void __launch_bounds__(256) k_bc_increment(SolveArgs a) {
+  const Geo &g = a.g;
+  const int f = a.f0 + blockIdx.y;
+  const long long col = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= g.pp) return;
+  const int NY = g.ny, nrow = NY + 2;
+  const long long pp = g.pp, ro = (long long)f * nrow;
+  const float *Uo = static_cast<const float *>(a.uold) + (long long)f * a.fs + col;
+  float *R = static_cast<float *>(const_cast<void *>(a.rphys)) + (long long)f * a.fs + col;
+  auto uold = [&](int i) { return a.four[f][(long long)i * g.pf].x + (double)Uo[(long long)i * pp]; };
+  for (int k = 0; k < 5; ++k) {
+    const int i = (k < 3) ? k : NY - 3 + k;
+    if (!a.fac.isbc[ro + i]) continue;
+    const double cl = a.fac.bcl[ro + i], cd = a.fac.bcd[ro + i], cu = a.fac.bcu[ro + i];
+    double s = 0.0;
+    if (cl != 0.0 && i >= 1) s += cl * uold(i - 1);
+    if (cd != 0.0) s += cd * uold(i);
+    if (cu != 0.0 && i + 1 < nrow) s += cu * uold(i + 1);
+    R[(long long)i * pp] = (float)(a.fac.vconst[ro + i] - s);
+  }
+}
+
 // Crank-Nicolson solve of fields f0..f1-1 on stream st.  A velocity
 // field's solution overwrites its own R (other components' nonlinear
 // passes may still read the physical velocity); a scalar's goes to its
@@ -1481,6 +1510,7 @@ static hipError_t launch_solve_range(Ctx &c, int rk, int courant, int f0, int f1
   const int nb = (int)((c.g.pp + cols - 1) / cols);
   const dim3 grid(nb, f1 - f0);
   if (c.mixed) {
+    hipLaunchKernelGGL(k_bc_increment, dim3((unsigned)((c.g.pp + 255) / 256), f1 - f0), dim3(256), 0, st, a);
     if (c.fac.nseg == 4)
       hipLaunchKernelGGL((k_solve_seg<4, float>), grid, dim3(kSegThreads), 0, st, a);
     else
@@ -1589,10 +1619,13 @@ hipError_t launch_substep(Ctx &c, int rk) {
 #define SCHK(expr) \
   if ((e = (expr)) != hipSuccess) return e;
   SCHK(launch_state_to_phys(c));
-  SCHK(launch_nonlinear_range(c, rk, 0, 2, c.stream));
+  // the mixed form's nonlinear blocks fit two to a CU: all components in
+  // one launch; the FP64 blocks take a CU each, so U3 and the scalars run on
+  // the branch beside the solve of U1 and U2
+  SCHK(launch_nonlinear_range(c, rk, 0, c.mixed ? c.nfields : 2, c.stream));
   SCHK(hipEventRecord(c.ev_nl01, c.stream));
   SCHK(hipStreamWaitEvent(c.branch, c.ev_nl01, 0));
-  SCHK(launch_nonlinear_range(c, rk, 2, c.nfields, c.branch));
+  if (!c.mixed) SCHK(launch_nonlinear_range(c, rk, 2, c.nfields, c.branch));
   SCHK(launch_solve_range(c, rk, courant, 2, c.nfields, c.branch));
   SCHK(launch_step_to_four(c, 2, nv, c.branch));
   SCHK(hipEventRecord(c.ev_branch_done, c.branch));
