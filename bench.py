@@ -171,12 +171,51 @@ def ensemble_bench(n, steps, warmup):
         ch.close()
 
 
+def mixed_form(p, gy, ic, steps, warmup):
+    """The FP32/FP64 mixed form on the same workload: its throughput, and the
+    relative L2 difference of each state field from the FP64 path after 10
+    steps from the same state (the FP64 path is itself at the roundoff floor
+    of the oracle)."""
+    import numpy as np
+    from diablo_amd import channel
+
+    ch = channel.Channel(p, gy)
+    ch.initial_condition(ic)
+    ch.set_clock(0.0, p["delta_t"], 0)
+    ch.set_mixed_precision(True)
+    ch.time_step(warmup)
+    ch.synchronize()
+    t0 = time.perf_counter()
+    ch.time_step(steps)
+    ch.synchronize()
+    wall = time.perf_counter() - t0
+    ch.close()
+    pts = p["nx"] * p["ny"] * p["nz"]
+    runs = []
+    for mixed in (False, True):
+        c = channel.Channel(p, gy)
+        c.initial_condition(ic)
+        c.set_clock(0.0, p["delta_t"], 0)
+        c.set_mixed_precision(mixed)
+        c.time_step(10)
+        runs.append(c.download())
+        c.close()
+    err = {}
+    for k in ("cu1", "cu2", "cu3", "cp"):
+        a, b = runs[1][k], runs[0][k]
+        err[k] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    return {"value": round(pts * steps / wall / 1e6, 3), "unit": UNIT, "ms_per_step": round(wall / steps * 1e3, 4),
+            "dtype": "f32 transforms of increments and fluctuations, f64 state, means and solves",
+            "rel_l2_vs_f64_after_10_steps": {k: float(f"{v:.3g}") for k, v in err.items()}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="leave out the FP32/FP64 mixed form's line")
     ap.add_argument("--profile-reps", type=int, default=10)
     ap.add_argument("--ensemble", type=int, default=0, metavar="N",
                     help="only time N uneven shear-layer instances stepped as one graph")
@@ -245,6 +284,8 @@ def main():
         "phases_ms": {k: round(v, 5) for k, v in phases.items()},
         "sim": {"time": t_sim, "dt": dt_now, "time_step": ts},
     }
+    if not args.no_mixed:
+        out["mixed_form"] = mixed_form(p, gy, ic, args.steps, args.warmup)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(p, gy, ic)
     ch.close()

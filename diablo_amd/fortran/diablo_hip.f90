@@ -101,6 +101,11 @@ module diablo_hip
       real(c_double), intent(out) :: out(*)
       real(c_double), intent(out) :: ubulk
     end function dh_save_stats
+    integer(c_int) function dh_set_mixed_precision(ctx, on) bind(C, name='dh_set_mixed_precision')
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: on
+    end function dh_set_mixed_precision
   end interface
 
 contains
