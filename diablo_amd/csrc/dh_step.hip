@@ -359,6 +359,33 @@ __device__ __forceinline__ void nonlinear_pass(const StepArgs &a, C *A) {
 
   C acc[NQ];  // CF of this thread's modes (FP64, or FP32 in the mixed form)
   cf_init_all<NQ>(a, c, J, nthr, acc);
+  constexpr bool DELTA = sizeof(C) != sizeof(double2);  // the mixed form transforms increments
+  if constexpr (DELTA) {
+    // the product terms with a plane-mean factor, in FP64, before the
+    // transformed fluctuation products (few registers are live here)
+    const double2 *CCm = (c < 3) ? a.cu[c] : a.cth + (c - 3) * a.cfs;
+    double mu[4][3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const long long pl = (long long)(J - 1 + j) * g.pf;
+      mu[0][j] = a.cu[0][pl].x;
+      mu[1][j] = a.cu[1][pl].x;
+      mu[2][j] = a.cu[2][pl].x;
+      mu[3][j] = (c >= 3) ? CCm[pl].x : 0.0;
+    }
+    const long long base0 = (long long)J * g.pf;
+#pragma unroll 2
+    for (int q = 0; q < NQ; ++q) {
+      const int m = threadIdx.x + q * nthr;
+      if (m < g.pf) {
+        const int kzp = (int)fdiv((unsigned)m, g.f_nkx1), kx = m - kzp * g.nkx1;
+        const long long o = base0 + m;
+        const double2 L = mean_terms(a, c, J, o, kx == 0 && kzp == 0, g.kx[kx], g.kz[kzp], CCm[o - g.pf], CCm[o],
+                                     CCm[o + g.pf], mu);
+        acc[q] = mkc<C>((Sc<C>)(acc[q].x - L.x), (Sc<C>)(acc[q].y - L.y));
+      }
+    }
+  }
   // the three products, forward transformed and subtracted with their
   // derivative factors: c==0 takes (ikz, ikx, 1), all others (ikx, ikz, 1)
   // (statement order of channel.f:410-571, 669-720)
@@ -367,7 +394,6 @@ __device__ __forceinline__ void nonlinear_pass(const StepArgs &a, C *A) {
   // quotients by per-block constants take their correctly rounded
   // reciprocals (div_by_recip): the same doubles as the divisions
   using S = Sc<C>;
-  constexpr bool DELTA = sizeof(C) != sizeof(double2);  // the mixed form transforms increments
   const S nzd = S(g.dnz), rnz = S(g.rnz);
   // acc -= f (plane / NZ) after the z transform, f = ikz (mode 0), ikx (1)
   // or 1 (2); the wavenumbers are loaded ahead of the LDS reads
@@ -476,18 +502,6 @@ __device__ __forceinline__ void nonlinear_pass(const StepArgs &a, C *A) {
   // 1024-thread variant, whose register budget is half as large)
   constexpr int NB = (REG && sizeof(C) == sizeof(double2)) ? 4 : (sizeof(C) == sizeof(double2) ? 2 : 1);
   const double dyJ = g.dy[J], rdyJ = g.rdy[J];
-  // the mixed form: plane means of U1, U2, U3 and this scalar at J-1..J+1
-  double mu[4][3] = {};
-  if constexpr (DELTA) {
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      const long long pl = (long long)(J - 1 + j) * g.pf;
-      mu[0][j] = a.cu[0][pl].x;
-      mu[1][j] = a.cu[1][pl].x;
-      mu[2][j] = a.cu[2][pl].x;
-      mu[3][j] = (c >= 3) ? CC[pl].x : 0.0;
-    }
-  }
   static_assert(NQ % NB == 0, "mode batches must tile the modes of a thread");
 #pragma unroll
   for (int q0 = 0; q0 < NQ; q0 += NB) {
@@ -547,9 +561,7 @@ __device__ __forceinline__ void nonlinear_pass(const StepArgs &a, C *A) {
           double2 cs = make_double2(div_by_recip(p0[b].x - pm[b].x, dyJ, rdyJ), div_by_recip(p0[b].y - pm[b].y, dyJ, rdyJ));
           cr = make_double2(cr.x - T4 * cs.x, cr.y - T4 * cs.y);
         }
-        double2 cfn = make_double2(acc[q0 + b].x, acc[q0 + b].y);
-        if constexpr (DELTA)
-          cfn = csub(cfn, mean_terms(a, c, J, o, kx == 0 && kzp == 0, g.kx[kx], g.kz[kzp], cm1[b], c0, cp1[b], mu));
+        const double2 cfn = make_double2(acc[q0 + b].x, acc[q0 + b].y);
         CF[o] = cfn;
         cr = make_double2(cr.x + T5 * cfn.x, cr.y + T5 * cfn.y);
         cr = make_double2(cr.x + ev.x, cr.y + ev.y);
