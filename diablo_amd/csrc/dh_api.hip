@@ -15,10 +15,29 @@ struct dh_ctx {
   Ctx c;
 };
 
-// Ensemble over existing contexts (not owned): each member's captured step
-// graph launched on the member's own stream, step by step.
+// Members of one shape: their recorded steps have the same launches (kernel,
+// grid, block, LDS, stream role) and events, so each launch runs once for all
+// of them with the kernel's batched variant over their argument structs.
+struct EnsGroup {
+  std::vector<dh_ctx *> m;
+  // recorded sequences of member 0: the start of the first step (pro), one
+  // step ending where the next begins (unit), the last step (epi)
+  std::vector<RecOp> seq[3];
+  std::vector<void *> dargs[3];  // per launch: device array of the members' argument structs
+  hipStream_t st[3] = {};        // main, branch, side
+  hipEvent_t ev[4] = {};         // ev_nl01, ev_branch_done, ev_fork, ev_join
+  hipEvent_t done = nullptr;
+};
+
+// Ensemble over existing contexts (not owned).
 struct dh_ens {
   std::vector<dh_ctx *> m;
+  std::vector<EnsGroup> g;
+  bool batched = false;   // groups built
+  bool fallback = false;  // grouping failed: each member's own step graph, member by member
+  hipStream_t root = nullptr;
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
+  hipGraphExec_t graph[2] = {};  // 1 and kStepsPerGraph steps
 };
 
 namespace {
@@ -749,6 +768,179 @@ const char *dh_phase_name(int32_t phase) {
   return (phase >= 0 && phase < kNumPhases) ? kPhaseNames[phase] : "";
 }
 
+namespace {
+
+bool same_op(const RecOp &x, const RecOp &y) {
+  return x.kind == y.kind && x.fnb == y.fnb && x.grid.x == y.grid.x && x.grid.y == y.grid.y && x.grid.z == y.grid.z &&
+         x.block.x == y.block.x && x.block.y == y.block.y && x.block.z == y.block.z && x.lds == y.lds &&
+         x.role == y.role && x.ev == y.ev && x.args.size() == y.args.size();
+}
+bool same_seq(const std::vector<RecOp> &x, const std::vector<RecOp> &y) {
+  if (x.size() != y.size()) return false;
+  for (size_t i = 0; i < x.size(); ++i)
+    if (!same_op(x[i], y[i])) return false;
+  return true;
+}
+
+// One step of context c as recorded launches: pro = the start of a first
+// step, unit = three substeps and the boundary into the next step, epi =
+// three substeps and the end of the last step.
+struct StepRec {
+  std::vector<RecOp> q[3];
+};
+
+hipError_t record_step(Ctx &c, StepRec &sr) {
+  std::vector<RecOp> *seq = sr.q;
+  c.recording = true;
+  c.rec.clear();
+  hipError_t e = launch_step_boundary(c, false, true);
+  seq[0] = std::move(c.rec);
+  c.rec.clear();
+  for (int rk = 0; rk < 3 && e == hipSuccess; ++rk) e = launch_substep(c, rk);
+  if (e == hipSuccess) e = launch_step_boundary(c, true, true);
+  seq[1] = std::move(c.rec);
+  c.rec.clear();
+  for (int rk = 0; rk < 3 && e == hipSuccess; ++rk) e = launch_substep(c, rk);
+  if (e == hipSuccess) e = launch_step_boundary(c, true, false);
+  seq[2] = std::move(c.rec);
+  c.rec.clear();
+  c.recording = false;
+  c.tables_pending = false;
+  for (int k = 0; k < 3; ++k)
+    for (const RecOp &o : seq[k])
+      if (o.kind == 0 && o.grid.z != 1) return hipErrorInvalidValue;
+  return e;
+}
+
+void free_groups(dh_ens &e) {
+  for (int k = 0; k < 2; ++k)
+    if (e.graph[k]) (void)hipGraphExecDestroy(e.graph[k]);
+  e.graph[0] = e.graph[1] = nullptr;
+  for (EnsGroup &g : e.g) {
+    for (auto &v : g.dargs)
+      for (void *p : v)
+        if (p) (void)hipFree(p);
+    for (hipStream_t s : g.st)
+      if (s) (void)hipStreamDestroy(s);
+    for (hipEvent_t v : g.ev)
+      if (v) (void)hipEventDestroy(v);
+    if (g.done) (void)hipEventDestroy(g.done);
+  }
+  e.g.clear();
+  if (e.root) (void)hipStreamDestroy(e.root);
+  if (e.ev_in) (void)hipEventDestroy(e.ev_in);
+  if (e.ev_out) (void)hipEventDestroy(e.ev_out);
+  e.root = nullptr;
+  e.ev_in = e.ev_out = nullptr;
+  e.batched = false;
+}
+
+// Groups the members by recorded step and uploads their argument structs.
+hipError_t build_groups(dh_ens &e) {
+  std::vector<StepRec> recs(e.m.size());
+  for (size_t i = 0; i < e.m.size(); ++i) {
+    const hipError_t r = record_step(e.m[i]->c, recs[i]);
+    if (r != hipSuccess) return r;
+  }
+  std::vector<int> gi(e.m.size(), -1);
+  std::vector<size_t> first;  // member whose records define each group
+  for (size_t i = 0; i < e.m.size(); ++i) {
+    for (size_t k = 0; k < first.size() && gi[i] < 0; ++k) {
+      const StepRec &r0 = recs[first[k]];
+      if (same_seq(r0.q[0], recs[i].q[0]) && same_seq(r0.q[1], recs[i].q[1]) && same_seq(r0.q[2], recs[i].q[2]))
+        gi[i] = (int)k;
+    }
+    if (gi[i] < 0) {
+      gi[i] = (int)first.size();
+      first.push_back(i);
+    }
+  }
+  e.g.resize(first.size());
+  hipError_t r;
+#define GCHK(x) \
+  if ((r = (x)) != hipSuccess) return r;
+  GCHK(hipStreamCreateWithFlags(&e.root, hipStreamNonBlocking));
+  GCHK(hipEventCreateWithFlags(&e.ev_in, hipEventDisableTiming));
+  GCHK(hipEventCreateWithFlags(&e.ev_out, hipEventDisableTiming));
+  for (size_t k = 0; k < first.size(); ++k) {
+    EnsGroup &g = e.g[k];
+    std::vector<size_t> idx;
+    for (size_t i = 0; i < e.m.size(); ++i)
+      if (gi[i] == (int)k) {
+        g.m.push_back(e.m[i]);
+        idx.push_back(i);
+      }
+    for (int q = 0; q < 3; ++q) {
+      g.seq[q] = recs[first[k]].q[q];
+      g.dargs[q].assign(g.seq[q].size(), nullptr);
+      for (size_t o = 0; o < g.seq[q].size(); ++o) {
+        if (g.seq[q][o].kind != 0) continue;
+        const size_t sz = g.seq[q][o].args.size();
+        std::vector<unsigned char> host(sz * idx.size());
+        for (size_t j = 0; j < idx.size(); ++j) std::memcpy(&host[j * sz], recs[idx[j]].q[q][o].args.data(), sz);
+        GCHK(hipMalloc(&g.dargs[q][o], host.size()));
+        GCHK(hipMemcpy(g.dargs[q][o], host.data(), host.size(), hipMemcpyHostToDevice));
+      }
+    }
+    for (hipStream_t &st : g.st) GCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    for (hipEvent_t &v : g.ev) GCHK(hipEventCreateWithFlags(&v, hipEventDisableTiming));
+    GCHK(hipEventCreateWithFlags(&g.done, hipEventDisableTiming));
+  }
+#undef GCHK
+  return hipSuccess;
+}
+
+hipError_t replay(EnsGroup &g, int q) {
+  const unsigned n = (unsigned)g.m.size();
+  for (size_t i = 0; i < g.seq[q].size(); ++i) {
+    const RecOp &o = g.seq[q][i];
+    hipStream_t st = g.st[o.role];
+    hipError_t r;
+    if (o.kind == 0) {
+      void *args[] = {&g.dargs[q][i]};
+      r = hipLaunchKernel(o.fnb, dim3(o.grid.x, o.grid.y, n), o.block, args, o.lds, st);
+    } else if (o.kind == 1) {
+      r = hipEventRecord(g.ev[o.ev], st);
+    } else {
+      r = hipStreamWaitEvent(st, g.ev[o.ev], 0);
+    }
+    if (r != hipSuccess) return r;
+  }
+  return hipSuccess;
+}
+
+// nsteps steps of every group, forked from and joined into the root stream.
+hipError_t enqueue_ensemble(dh_ens &e, int nsteps) {
+  hipError_t r;
+  if ((r = hipEventRecord(e.ev_in, e.root)) != hipSuccess) return r;
+  for (EnsGroup &g : e.g) {
+    if ((r = hipStreamWaitEvent(g.st[0], e.ev_in, 0)) != hipSuccess) return r;
+    if ((r = replay(g, 0)) != hipSuccess) return r;
+    for (int s = 0; s + 1 < nsteps; ++s)
+      if ((r = replay(g, 1)) != hipSuccess) return r;
+    if ((r = replay(g, 2)) != hipSuccess) return r;
+    if ((r = hipEventRecord(g.done, g.st[0])) != hipSuccess) return r;
+    if ((r = hipStreamWaitEvent(e.root, g.done, 0)) != hipSuccess) return r;
+  }
+  return hipSuccess;
+}
+
+bool capture_ensemble(dh_ens &e, int nsteps, hipGraphExec_t *exec) {
+  hipGraph_t graph = nullptr;
+  hipError_t r = hipStreamBeginCapture(e.root, hipStreamCaptureModeThreadLocal);
+  if (r == hipSuccess) {
+    const hipError_t q = enqueue_ensemble(e, nsteps);
+    r = hipStreamEndCapture(e.root, &graph);
+    if (q != hipSuccess) r = q;
+  }
+  bool ok = r == hipSuccess && graph && hipGraphInstantiate(exec, graph, nullptr, nullptr, 0) == hipSuccess;
+  if (graph) (void)hipGraphDestroy(graph);
+  if (!ok) (void)hipGetLastError();
+  return ok;
+}
+
+}  // namespace
+
 int dh_ensemble_create(int32_t n, dh_ctx *const *members, dh_ens **out) {
   if (!out) return DH_EARG;
   *out = nullptr;
@@ -765,25 +957,67 @@ int dh_ensemble_create(int32_t n, dh_ctx *const *members, dh_ens **out) {
   return DH_OK;
 }
 
-// Members run on their own streams, so work queued on a member before or
-// after an ensemble call stays ordered with it.  Launching every member's
-// step before the next step of any lets the members' kernels overlap on
-// the device's hardware queues (measured faster than one graph holding all
-// members, whose nodes the runtime runs on few queues).
+// Members of one shape advance together: every launch of their step runs
+// once for all of them (the kernels' batched variants, member blockIdx.z),
+// groups of different shapes on streams of their own, the whole in one
+// captured graph of 1 or kStepsPerGraph steps.  The ensemble's streams wait
+// for work already queued on the members and the members' streams wait for
+// the ensemble, so the members' own entry points stay ordered with it.  The
+// groups are formed at the first call from the members' state then (grid,
+// sizes, precision form); dh_ensemble_time_step falls back to stepping the
+// members one by one if that is not possible.
 int dh_ensemble_time_step(dh_ens *e, int32_t nsteps) {
   if (!e || nsteps < 0) return DH_EARG;
   for (dh_ctx *h : e->m)
     if (!h->c.grid_set) return fail(h->c, DH_ESTATE, "dh_set_grid first");
-  for (int s = 0; s < nsteps; ++s)
-    for (dh_ctx *h : e->m) {
-      const int r = dh_time_step(h, 1);
-      if (r != DH_OK) return r;
+  if (nsteps == 0) return DH_OK;
+  Ctx &c = e->m[0]->c;  // errors are reported on the first member
+  if (!e->batched && !e->fallback) {
+    if (build_groups(*e) == hipSuccess) {
+      e->batched = true;
+    } else {
+      (void)hipGetLastError();
+      free_groups(*e);
+      e->fallback = true;
     }
+  }
+  if (!e->batched) {
+    for (int s = 0; s < nsteps; ++s)
+      for (dh_ctx *h : e->m) {
+        const int r = dh_time_step(h, 1);
+        if (r != DH_OK) return r;
+      }
+    return DH_OK;
+  }
+  for (dh_ctx *h : e->m) {
+    HIPCHK(hipEventRecord(h->c.ev_fork, h->c.stream));
+    HIPCHK(hipStreamWaitEvent(e->root, h->c.ev_fork, 0));
+  }
+  int left = nsteps;
+  if (left >= kStepsPerGraph && !e->graph[1]) (void)capture_ensemble(*e, kStepsPerGraph, &e->graph[1]);
+  if (!e->graph[0]) (void)capture_ensemble(*e, 1, &e->graph[0]);
+  while (e->graph[1] && left >= kStepsPerGraph) {
+    HIPCHK(hipGraphLaunch(e->graph[1], e->root));
+    left -= kStepsPerGraph;
+  }
+  for (; left > 0; --left) {
+    if (e->graph[0])
+      HIPCHK(hipGraphLaunch(e->graph[0], e->root));
+    else
+      HIPCHK(enqueue_ensemble(*e, 1));
+  }
+  HIPCHK(hipEventRecord(e->ev_out, e->root));
+  for (dh_ctx *h : e->m) {
+    HIPCHK(hipStreamWaitEvent(h->c.stream, e->ev_out, 0));
+    h->c.th_phys32 = h->c.mixed;
+  }
   return DH_OK;
 }
 
 int dh_ensemble_synchronize(dh_ens *e) {
   if (!e) return DH_EARG;
+  Ctx &c = e->m[0]->c;
+  if (e->root) HIPCHK(hipStreamSynchronize(e->root));
   for (dh_ctx *h : e->m) {
     const int r = dh_synchronize(h);
     if (r != DH_OK) return r;
@@ -794,6 +1028,7 @@ int dh_ensemble_synchronize(dh_ens *e) {
 int dh_ensemble_destroy(dh_ens *e) {
   if (!e) return DH_EARG;
   (void)dh_ensemble_synchronize(e);
+  free_groups(*e);
   delete e;
   return DH_OK;
 }

@@ -24,11 +24,14 @@ struct PlaneBatch {
   int drop_mean;
 };
 
+struct PlaneArgs {
+  Geo g;
+  PlaneBatch b;
+};
+
 // Fourier -> physical for planes j0 + blockIdx.x of field blockIdx.y.
 template <bool REG, class C>
-__global__ void __launch_bounds__(1024) k_plane_inv(Geo g, PlaneBatch b) {
-  extern __shared__ double2 smem[];
-  C *A = reinterpret_cast<C *>(smem);
+__device__ __forceinline__ void plane_inv_body(const Geo &g, const PlaneBatch &b, C *A) {
   const auto T = load_tables(A, g);
   const int f = blockIdx.y;
   const long long j = b.j0 + blockIdx.x;
@@ -46,9 +49,7 @@ __global__ void __launch_bounds__(1024) k_plane_inv(Geo g, PlaneBatch b) {
 // physical -> Fourier (x r2c with 1/NX and truncation, z c2c with 1/NZ and
 // packing of kz = -NKZ..-1 after 0..NKZ).
 template <bool REG, class C>
-__global__ void __launch_bounds__(1024) k_plane_fwd(Geo g, PlaneBatch b) {
-  extern __shared__ double2 smem[];
-  C *A = reinterpret_cast<C *>(smem);
+__device__ __forceinline__ void plane_fwd_body(const Geo &g, const PlaneBatch &b, C *A) {
   const auto T = load_tables(A, g);
   const int f = blockIdx.y;
   const long long j = b.j0 + blockIdx.x;
@@ -78,6 +79,31 @@ __global__ void __launch_bounds__(1024) k_plane_fwd(Geo g, PlaneBatch b) {
   }
 }
 
+// the kernels: one context (arguments by value), or a batch of contexts of
+// one shape (argument structs in device memory, member blockIdx.z)
+template <bool REG, class C>
+__global__ void __launch_bounds__(1024) k_plane_inv(PlaneArgs pa) {
+  extern __shared__ double2 smem[];
+  plane_inv_body<REG, C>(pa.g, pa.b, reinterpret_cast<C *>(smem));
+}
+template <bool REG, class C>
+__global__ void __launch_bounds__(1024) k_plane_inv_b(const PlaneArgs *av) {
+  extern __shared__ double2 smem[];
+  const PlaneArgs &pa = av[blockIdx.z];
+  plane_inv_body<REG, C>(pa.g, pa.b, reinterpret_cast<C *>(smem));
+}
+template <bool REG, class C>
+__global__ void __launch_bounds__(1024) k_plane_fwd(PlaneArgs pa) {
+  extern __shared__ double2 smem[];
+  plane_fwd_body<REG, C>(pa.g, pa.b, reinterpret_cast<C *>(smem));
+}
+template <bool REG, class C>
+__global__ void __launch_bounds__(1024) k_plane_fwd_b(const PlaneArgs *av) {
+  extern __shared__ double2 smem[];
+  const PlaneArgs &pa = av[blockIdx.z];
+  plane_fwd_body<REG, C>(pa.g, pa.b, reinterpret_cast<C *>(smem));
+}
+
 static hipError_t set_lds(const void *fn, int bytes) {
   return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
 }
@@ -86,21 +112,23 @@ static hipError_t set_lds(const void *fn, int bytes) {
 // plane allows them).
 template <class C>
 static hipError_t plane_inv(Ctx &c, dim3 grid, hipStream_t st, const PlaneBatch &b) {
-  const int lds = lds_bytes_of<C>(c);
+  const PlaneArgs pa{c.g, b};
+  const unsigned lds = lds_bytes_of<C>(c);
   if (c.reg_rows)
-    hipLaunchKernelGGL((k_plane_inv<true, C>), grid, dim3(c.plane_threads), lds, st, c.g, b);
-  else
-    hipLaunchKernelGGL((k_plane_inv<false, C>), grid, dim3(c.plane_threads), lds, st, c.g, b);
-  return hipGetLastError();
+    return klaunch(c, (const void *)k_plane_inv<true, C>, (const void *)k_plane_inv_b<true, C>, grid,
+                   dim3(c.plane_threads), lds, st, pa);
+  return klaunch(c, (const void *)k_plane_inv<false, C>, (const void *)k_plane_inv_b<false, C>, grid,
+                 dim3(c.plane_threads), lds, st, pa);
 }
 template <class C>
 static hipError_t plane_fwd(Ctx &c, dim3 grid, hipStream_t st, const PlaneBatch &b) {
-  const int lds = lds_bytes_of<C>(c);
+  const PlaneArgs pa{c.g, b};
+  const unsigned lds = lds_bytes_of<C>(c);
   if (c.reg_rows)
-    hipLaunchKernelGGL((k_plane_fwd<true, C>), grid, dim3(c.plane_threads), lds, st, c.g, b);
-  else
-    hipLaunchKernelGGL((k_plane_fwd<false, C>), grid, dim3(c.plane_threads), lds, st, c.g, b);
-  return hipGetLastError();
+    return klaunch(c, (const void *)k_plane_fwd<true, C>, (const void *)k_plane_fwd_b<true, C>, grid,
+                   dim3(c.plane_threads), lds, st, pa);
+  return klaunch(c, (const void *)k_plane_fwd<false, C>, (const void *)k_plane_fwd_b<false, C>, grid,
+                 dim3(c.plane_threads), lds, st, pa);
 }
 
 hipError_t launch_fft_to_phys(Ctx &c, const double2 *four, double *phys, int nplanes) {
@@ -187,20 +215,24 @@ hipError_t launch_step_to_four(Ctx &c, int f0, int f1, hipStream_t st) {
   return c.mixed ? plane_fwd<float2>(c, grid, st, b) : plane_fwd<double2>(c, grid, st, b);
 }
 
-hipError_t prepare_fft_kernels(Ctx &c) {
-  const void *fns[] = {(const void *)k_plane_inv<true, double2>, (const void *)k_plane_inv<false, double2>,
-                       (const void *)k_plane_fwd<true, double2>, (const void *)k_plane_fwd<false, double2>};
+template <bool REG, class C>
+static hipError_t set_lds_plane() {
+  const void *fns[] = {(const void *)k_plane_inv<REG, C>, (const void *)k_plane_inv_b<REG, C>,
+                       (const void *)k_plane_fwd<REG, C>, (const void *)k_plane_fwd_b<REG, C>};
   for (const void *fn : fns) {
-    hipError_t e = set_lds(fn, c.lds_bytes);
-    if (e != hipSuccess) return e;
-  }
-  const void *fns32[] = {(const void *)k_plane_inv<true, float2>, (const void *)k_plane_inv<false, float2>,
-                         (const void *)k_plane_fwd<true, float2>, (const void *)k_plane_fwd<false, float2>};
-  for (const void *fn : fns32) {
-    hipError_t e = set_lds(fn, c.lds_bytes32);
+    const hipError_t e = set_lds(fn, kLdsMax);
     if (e != hipSuccess) return e;
   }
   return hipSuccess;
+}
+
+hipError_t prepare_fft_kernels(Ctx &c) {
+  if (c.lds_bytes > kLdsMax) return hipErrorInvalidValue;
+  hipError_t e;
+  if ((e = set_lds_plane<true, double2>()) != hipSuccess) return e;
+  if ((e = set_lds_plane<false, double2>()) != hipSuccess) return e;
+  if ((e = set_lds_plane<true, float2>()) != hipSuccess) return e;
+  return set_lds_plane<false, float2>();
 }
 
 }  // namespace dh

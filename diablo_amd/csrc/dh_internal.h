@@ -82,6 +82,12 @@ constexpr int kRdsRows = 17;
 // Output k = t + 8 k2 of a register row transform (NX = 128) is stored only
 // for k <= NKX = NX/3 = 42, so k2 < kRowK2 covers every stored output.
 constexpr int kRowK2 = 6;
+// Largest dynamic LDS of a workgroup on gfx950 (160 KB per CU).  The
+// dynamic-LDS limit is an attribute of a kernel function, shared by every
+// context in the process, so kernels are given the hardware maximum rather
+// than one context's need (a smaller context created later would otherwise
+// lower it under a larger one).
+constexpr int kLdsMax = 160 * 1024;
 
 // Tables of the y-implicit (Crank-Nicolson) matrices, one set per RK
 // substep and field; with LES off each matrix depends only on j, so every
@@ -94,6 +100,21 @@ struct RealFactors {
   double *bcl, *bcd, *bcu;    // [field][row] coefficients of the boundary rows (isbc = 1)
   int seg_rows;               // rows per register segment of a column
   int nseg;                   // segments per column (4 or 8)
+};
+
+// A launch, event record or stream wait of the step sequence, as the
+// launchers append it in recording mode (Ctx::recording) instead of issuing
+// it: ensembles record one step of each member and replay the launches of
+// members of one shape as single launches of the kernels' batched variants,
+// which read their argument struct from a device array at blockIdx.z.
+struct RecOp {
+  int kind;        // 0: kernel, 1: event record, 2: stream waits on an event
+  const void *fnb; // batched kernel
+  dim3 grid, block;
+  unsigned lds;
+  int role;        // stream: 0 the context's main stream, 1 branch, 2 side
+  int ev;          // event: 0 ev_nl01, 1 ev_branch_done, 2 ev_fork, 3 ev_join
+  std::vector<unsigned char> args;  // the kernel's argument struct
 };
 
 struct Ctx {
@@ -147,7 +168,58 @@ struct Ctx {
   // graphs of one and of kStepsPerGraph full time steps
   hipGraphExec_t step_graph = nullptr, steps_graph = nullptr;
   bool graph_ok = false, steps_graph_ok = false;
+  // recording mode (see RecOp)
+  bool recording = false;
+  std::vector<RecOp> rec;
 };
+
+inline int stream_role(const Ctx &c, hipStream_t st) { return st == c.branch ? 1 : (st == c.side ? 2 : 0); }
+inline hipEvent_t event_of(const Ctx &c, int ev) {
+  return ev == 0 ? c.ev_nl01 : ev == 1 ? c.ev_branch_done : ev == 2 ? c.ev_fork : c.ev_join;
+}
+// Launch of kernel fn (one argument struct A by value) on st, or its record
+// with the batched variant fnb.
+template <class A>
+inline hipError_t klaunch(Ctx &c, const void *fn, const void *fnb, dim3 grid, dim3 block, unsigned lds, hipStream_t st,
+                          const A &a) {
+  if (c.recording) {
+    RecOp o{};
+    o.kind = 0;
+    o.fnb = fnb;
+    o.grid = grid;
+    o.block = block;
+    o.lds = lds;
+    o.role = stream_role(c, st);
+    const unsigned char *b = reinterpret_cast<const unsigned char *>(&a);
+    o.args.assign(b, b + sizeof(A));
+    c.rec.push_back(std::move(o));
+    return hipSuccess;
+  }
+  void *args[] = {const_cast<A *>(&a)};
+  return hipLaunchKernel(fn, grid, block, args, lds, st);
+}
+inline hipError_t krecord(Ctx &c, int ev, hipStream_t st) {
+  if (c.recording) {
+    RecOp o{};
+    o.kind = 1;
+    o.ev = ev;
+    o.role = stream_role(c, st);
+    c.rec.push_back(std::move(o));
+    return hipSuccess;
+  }
+  return hipEventRecord(event_of(c, ev), st);
+}
+inline hipError_t kwait(Ctx &c, hipStream_t st, int ev) {
+  if (c.recording) {
+    RecOp o{};
+    o.kind = 2;
+    o.ev = ev;
+    o.role = stream_role(c, st);
+    c.rec.push_back(std::move(o));
+    return hipSuccess;
+  }
+  return hipStreamWaitEvent(st, event_of(c, ev), 0);
+}
 
 // ---- launchers (dh_fft.hip) ----
 hipError_t prepare_fft_kernels(Ctx &c);

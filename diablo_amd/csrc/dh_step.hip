@@ -594,6 +594,19 @@ k_nonlinear32(StepArgs a) {
   extern __shared__ double2 smem[];
   nonlinear_pass<REG, NQ, float2>(a, reinterpret_cast<float2 *>(smem));
 }
+// batches of contexts of one shape: argument structs in device memory,
+// member blockIdx.z
+template <bool REG, int NQ>
+__global__ void __launch_bounds__(REG ? 512 : 1024) k_nonlinear_b(const StepArgs *av) {
+  extern __shared__ double2 smem[];
+  nonlinear_pass<REG, NQ, double2>(av[blockIdx.z], smem);
+}
+template <bool REG, int NQ>
+__global__ void __launch_bounds__(REG ? 512 : 1024) __attribute__((amdgpu_waves_per_eu(4)))
+k_nonlinear32_b(const StepArgs *av) {
+  extern __shared__ double2 smem[];
+  nonlinear_pass<REG, NQ, float2>(av[blockIdx.z], reinterpret_cast<float2 *>(smem));
+}
 // ---------------------------------------------------------------------------
 // Real Crank-Nicolson systems (matrix build of channel.f:771-1005, BC rows
 // of APPLY_BC_{1,2,3,TH}_{LOWER,UPPER} channel.f:1660-2282).  With LES off
@@ -693,7 +706,7 @@ __host__ __device__ inline void real_row(const FactorIn &in, int f, double temp1
 //   PB_J = prod_{i=J..b-1} -TC_i/TB_i  (b-1: last row of J's segment)
 // The constant right-hand sides of the boundary rows and the rows'
 // coefficients (dt independent) are written by the rk=0 blocks.
-__global__ void __launch_bounds__(256) k_cn_tables(const Clock *clk, FactorIn in, RealFactors fac) {
+__device__ __forceinline__ void cn_tables_body(const Clock *clk, const FactorIn &in, const RealFactors &fac) {
 #pragma clang fp contract(off)
   extern __shared__ double sm[];
   const int nrow = in.ny + 2;
@@ -750,6 +763,16 @@ __global__ void __launch_bounds__(256) k_cn_tables(const Clock *clk, FactorIn in
     fac.pb[o + i] = pb[i];
   }
 }
+struct CnArgs {
+  const Clock *clk;
+  FactorIn in;
+  RealFactors fac;
+};
+__global__ void __launch_bounds__(256) k_cn_tables(CnArgs a) { cn_tables_body(a.clk, a.in, a.fac); }
+__global__ void __launch_bounds__(256) k_cn_tables_b(const CnArgs *av) {
+  const CnArgs &a = av[blockIdx.z];
+  cn_tables_body(a.clk, a.in, a.fac);
+}
 
 // ---------------------------------------------------------------------------
 // Real Crank-Nicolson solves (channel.f:771-1029; the explicit half of the
@@ -788,7 +811,7 @@ struct SolveArgs {
 constexpr int kSegThreads = 256;
 
 template <int NS, class P>
-__global__ void __launch_bounds__(kSegThreads) k_solve_seg(SolveArgs a) {
+__device__ __forceinline__ void solve_seg_body(const SolveArgs &a) {
 #pragma clang fp contract(off)
   constexpr int CW = 64 / NS;  // columns per wave
   constexpr int SR = kSegRows;
@@ -943,10 +966,18 @@ __global__ void __launch_bounds__(kSegThreads) k_solve_seg(SolveArgs a) {
     }
   }
 }
+template <int NS, class P>
+__global__ void __launch_bounds__(kSegThreads) k_solve_seg(SolveArgs a) {
+  solve_seg_body<NS, P>(a);
+}
+template <int NS, class P>
+__global__ void __launch_bounds__(kSegThreads) k_solve_seg_b(const SolveArgs *av) {
+  solve_seg_body<NS, P>(av[blockIdx.z]);
+}
 
 // COURANT (courant.f:45-62) from the CFL minimum of the solve kernel; the
 // solve tables for the new DELTA_T are rebuilt right after by k_cn_tables.
-__global__ void k_courant_dt(Clock *clk, double dt_cap, int variable_dt, int update_dt) {
+__device__ __forceinline__ void courant_dt_body(Clock *clk, double dt_cap, int variable_dt, int update_dt) {
   if (threadIdx.x != 0) return;
   Clock &c = *clk;
   // TEMP4 of this substep was fixed before COURANT (channel.f:80); the
@@ -969,6 +1000,16 @@ __global__ void k_courant_dt(Clock *clk, double dt_cap, int variable_dt, int upd
     c.h_bar[2] = DT * (5.0 / 15.0);
   }
   c.cfl_min_bits = 0x7FF0000000000000ull;
+}
+struct CourantArgs {
+  Clock *clk;
+  double dt_cap;
+  int variable_dt, update_dt;
+};
+__global__ void k_courant_dt(CourantArgs a) { courant_dt_body(a.clk, a.dt_cap, a.variable_dt, a.update_dt); }
+__global__ void k_courant_dt_b(const CourantArgs *av) {
+  const CourantArgs &a = av[blockIdx.z];
+  courant_dt_body(a.clk, a.dt_cap, a.variable_dt, a.update_dt);
 }
 
 // ---------------------------------------------------------------------------
@@ -1137,7 +1178,7 @@ __device__ __forceinline__ void rem_div_sweep_seg(int seg, int s, int lane, int 
 constexpr int kRdtAhead = 4;
 constexpr int kRdtPad = 16;  // >= 4 * kRdtAhead - 1: the sweeps read up to 4 kRdtAhead - 1 rows past their range
 
-__global__ void __launch_bounds__(256) k_rem_div_thomas(RemDivArgs a) {
+__device__ __forceinline__ void rem_div_thomas_body(const RemDivArgs &a) {
 #pragma clang fp contract(off)
   extern __shared__ double st[];
   const Geo &g = a.g;
@@ -1342,6 +1383,8 @@ __global__ void __launch_bounds__(256) k_rem_div_thomas(RemDivArgs a) {
     a.cu[1][pf + m] = a.cin[1][pf + m];
   }
 }
+__global__ void __launch_bounds__(256) k_rem_div_thomas(RemDivArgs a) { rem_div_thomas_body(a); }
+__global__ void __launch_bounds__(256) k_rem_div_thomas_b(const RemDivArgs *av) { rem_div_thomas_body(av[blockIdx.z]); }
 
 // ---------------------------------------------------------------------------
 // launchers
@@ -1420,8 +1463,9 @@ static FactorIn make_factor_in(const Ctx &c) {
 hipError_t launch_cn_tables(Ctx &c, hipStream_t st) {
   FactorIn in = make_factor_in(c);
   const size_t shm = (size_t)8 * (c.g.ny + 2) * sizeof(double);
-  hipLaunchKernelGGL(k_cn_tables, dim3(3 * c.nfields), dim3(256), shm, st, c.clock, in, c.fac);
-  return hipGetLastError();
+  const CnArgs ca{c.clock, in, c.fac};
+  return klaunch(c, (const void *)k_cn_tables, (const void *)k_cn_tables_b, dim3(3 * c.nfields), dim3(256),
+                 (unsigned)shm, st, ca);
 }
 
 // Nonlinear pass of components c0..c1-1 on stream st: one block per plane
@@ -1440,15 +1484,16 @@ static hipError_t launch_nonlinear_range(Ctx &c, int rk, int c0, int c1, hipStre
   const dim3 grid(jhi - jlo + 1, c1 - c0);
   if (c.mixed) {
     if (c.reg_rows_nl)
-      hipLaunchKernelGGL((k_nonlinear32<true, kNlRegQ>), grid, dim3(kNlRegThreads), c.lds_bytes32, st, a);
-    else
-      hipLaunchKernelGGL((k_nonlinear32<false, kMaxQ>), grid, dim3(c.plane_threads), c.lds_bytes32, st, a);
-  } else if (c.reg_rows_nl) {
-    hipLaunchKernelGGL((k_nonlinear<true, kNlRegQ>), grid, dim3(kNlRegThreads), c.lds_bytes, st, a);
-  } else {
-    hipLaunchKernelGGL((k_nonlinear<false, kMaxQ>), grid, dim3(c.plane_threads), c.lds_bytes, st, a);
+      return klaunch(c, (const void *)k_nonlinear32<true, kNlRegQ>, (const void *)k_nonlinear32_b<true, kNlRegQ>, grid,
+                     dim3(kNlRegThreads), c.lds_bytes32, st, a);
+    return klaunch(c, (const void *)k_nonlinear32<false, kMaxQ>, (const void *)k_nonlinear32_b<false, kMaxQ>, grid,
+                   dim3(c.plane_threads), c.lds_bytes32, st, a);
   }
-  return hipGetLastError();
+  if (c.reg_rows_nl)
+    return klaunch(c, (const void *)k_nonlinear<true, kNlRegQ>, (const void *)k_nonlinear_b<true, kNlRegQ>, grid,
+                   dim3(kNlRegThreads), c.lds_bytes, st, a);
+  return klaunch(c, (const void *)k_nonlinear<false, kMaxQ>, (const void *)k_nonlinear_b<false, kMaxQ>, grid,
+                 dim3(c.plane_threads), c.lds_bytes, st, a);
 }
 
 static hipError_t launch_nonlinear(Ctx &c, int rk) { return launch_nonlinear_range(c, rk, 0, c.nfields, c.stream); }
@@ -1460,7 +1505,7 @@ static hipError_t launch_nonlinear(Ctx &c, int rk) { return launch_nonlinear_ran
 // f0 + blockIdx.y writes them into the field's R array, after the
 // nonlinear pass (which writes R on its own planes, some of them boundary
 // rows) and before the solve.
-__global__ void __launch_bounds__(256) k_bc_increment(SolveArgs a) {
+__device__ __forceinline__ void bc_increment_body(const SolveArgs &a) {
   const Geo &g = a.g;
   const int f = a.f0 + blockIdx.y;
   const long long col = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1481,6 +1526,8 @@ __global__ void __launch_bounds__(256) k_bc_increment(SolveArgs a) {
     R[(long long)i * pp] = (float)(a.fac.vconst[ro + i] - s);
   }
 }
+__global__ void __launch_bounds__(256) k_bc_increment(SolveArgs a) { bc_increment_body(a); }
+__global__ void __launch_bounds__(256) k_bc_increment_b(const SolveArgs *av) { bc_increment_body(av[blockIdx.z]); }
 
 // Crank-Nicolson solve of fields f0..f1-1 on stream st.  A velocity
 // field's solution overwrites its own R (other components' nonlinear
@@ -1522,17 +1569,20 @@ static hipError_t launch_solve_range(Ctx &c, int rk, int courant, int f0, int f1
   const int nb = (int)((c.g.pp + cols - 1) / cols);
   const dim3 grid(nb, f1 - f0);
   if (c.mixed) {
-    hipLaunchKernelGGL(k_bc_increment, dim3((unsigned)((c.g.pp + 255) / 256), f1 - f0), dim3(256), 0, st, a);
+    hipError_t e = klaunch(c, (const void *)k_bc_increment, (const void *)k_bc_increment_b,
+                           dim3((unsigned)((c.g.pp + 255) / 256), f1 - f0), dim3(256), 0, st, a);
+    if (e != hipSuccess) return e;
     if (c.fac.nseg == 4)
-      hipLaunchKernelGGL((k_solve_seg<4, float>), grid, dim3(kSegThreads), 0, st, a);
-    else
-      hipLaunchKernelGGL((k_solve_seg<8, float>), grid, dim3(kSegThreads), 0, st, a);
-  } else if (c.fac.nseg == 4) {
-    hipLaunchKernelGGL((k_solve_seg<4, double>), grid, dim3(kSegThreads), 0, st, a);
-  } else {
-    hipLaunchKernelGGL((k_solve_seg<8, double>), grid, dim3(kSegThreads), 0, st, a);
+      return klaunch(c, (const void *)k_solve_seg<4, float>, (const void *)k_solve_seg_b<4, float>, grid,
+                     dim3(kSegThreads), 0, st, a);
+    return klaunch(c, (const void *)k_solve_seg<8, float>, (const void *)k_solve_seg_b<8, float>, grid,
+                   dim3(kSegThreads), 0, st, a);
   }
-  return hipGetLastError();
+  if (c.fac.nseg == 4)
+    return klaunch(c, (const void *)k_solve_seg<4, double>, (const void *)k_solve_seg_b<4, double>, grid,
+                   dim3(kSegThreads), 0, st, a);
+  return klaunch(c, (const void *)k_solve_seg<8, double>, (const void *)k_solve_seg_b<8, double>, grid,
+                 dim3(kSegThreads), 0, st, a);
 }
 
 static hipError_t launch_solve(Ctx &c, int rk, int courant) {
@@ -1574,22 +1624,21 @@ hipError_t launch_rem_div_factors(Ctx &c) {
 hipError_t launch_rem_div(Ctx &c, int rk) {
   RemDivArgs a = make_rem_div_args(c, rk);
   const int nb = (c.g.pf + kRdtModes - 1) / kRdtModes;
-  hipLaunchKernelGGL(k_rem_div_thomas, dim3(nb), dim3(256), rem_div_thomas_lds(c), c.stream, a);
-  return hipGetLastError();
+  return klaunch(c, (const void *)k_rem_div_thomas, (const void *)k_rem_div_thomas_b, dim3(nb), dim3(256),
+                 (unsigned)rem_div_thomas_lds(c), c.stream, a);
 }
 
 // COURANT, then the solve tables for the new DELTA_T on the side stream: the
 // next solve is a substep away, so the rebuild runs beside this substep's
 // transform and pressure solve and is joined at the end of the substep.
 static hipError_t launch_courant(Ctx &c) {
-  hipLaunchKernelGGL(k_courant_dt, dim3(1), dim3(64), 0, c.stream, c.clock, c.dt_cap, c.p.variable_dt,
-                     c.p.update_dt);
-  hipError_t e = hipGetLastError();
+  const CourantArgs ca{c.clock, c.dt_cap, c.p.variable_dt, c.p.update_dt};
+  hipError_t e = klaunch(c, (const void *)k_courant_dt, (const void *)k_courant_dt_b, dim3(1), dim3(64), 0, c.stream, ca);
   if (e != hipSuccess) return e;
-  if ((e = hipEventRecord(c.ev_fork, c.stream)) != hipSuccess) return e;
-  if ((e = hipStreamWaitEvent(c.side, c.ev_fork, 0)) != hipSuccess) return e;
+  if ((e = krecord(c, 2, c.stream)) != hipSuccess) return e;  // ev_fork
+  if ((e = kwait(c, c.side, 2)) != hipSuccess) return e;
   if ((e = launch_cn_tables(c, c.side)) != hipSuccess) return e;
-  if ((e = hipEventRecord(c.ev_join, c.side)) != hipSuccess) return e;
+  if ((e = krecord(c, 3, c.side)) != hipSuccess) return e;  // ev_join
   c.tables_pending = true;
   return hipSuccess;
 }
@@ -1614,7 +1663,7 @@ hipError_t launch_phase(Ctx &c, int rk, int phase) {
 hipError_t join_side(Ctx &c) {
   if (!c.tables_pending) return hipSuccess;
   c.tables_pending = false;
-  return hipStreamWaitEvent(c.stream, c.ev_join, 0);
+  return kwait(c, c.stream, 3);  // ev_join
 }
 
 // One substep.  The nonlinear pass of U1 and U2 fills one wave of blocks
@@ -1635,15 +1684,15 @@ hipError_t launch_substep(Ctx &c, int rk) {
   // one launch; the FP64 blocks take a CU each, so U3 and the scalars run on
   // the branch beside the solve of U1 and U2
   SCHK(launch_nonlinear_range(c, rk, 0, c.mixed ? c.nfields : 2, c.stream));
-  SCHK(hipEventRecord(c.ev_nl01, c.stream));
-  SCHK(hipStreamWaitEvent(c.branch, c.ev_nl01, 0));
+  SCHK(krecord(c, 0, c.stream));  // ev_nl01
+  SCHK(kwait(c, c.branch, 0));
   if (!c.mixed) SCHK(launch_nonlinear_range(c, rk, 2, c.nfields, c.branch));
   SCHK(launch_solve_range(c, rk, courant, 2, c.nfields, c.branch));
   SCHK(launch_step_to_four(c, 2, nv, c.branch));
-  SCHK(hipEventRecord(c.ev_branch_done, c.branch));
+  SCHK(krecord(c, 1, c.branch));  // ev_branch_done
   SCHK(launch_solve_range(c, rk, courant, 0, 2, c.stream));
   SCHK(launch_step_to_four(c, 0, 2, c.stream));
-  SCHK(hipStreamWaitEvent(c.stream, c.ev_branch_done, 0));
+  SCHK(kwait(c, c.stream, 1));
   if (courant) SCHK(launch_courant(c));
   SCHK(launch_step_to_four(c, 3, c.nfields, c.stream));
   SCHK(launch_rem_div(c, rk));
@@ -1654,32 +1703,37 @@ hipError_t launch_substep(Ctx &c, int rk) {
 // Step bookkeeping of diablo.f:66-110 between substeps: TIME = TIME +
 // DELTA_T ending a step (with the DELTA_T COURANT set in it), then
 // TIME_STEP = TIME_STEP + 1 starting the next; one launch serves both.
-__global__ void k_step_boundary(Clock *clk, int end_prev, int begin_next) {
+struct BoundaryArgs {
+  Clock *clk;
+  int end_prev, begin_next;
+};
+__device__ __forceinline__ void step_boundary_body(const BoundaryArgs &a) {
   if (threadIdx.x != 0) return;
-  if (end_prev) clk->time = clk->time + clk->dt;
-  if (begin_next) clk->time_step += 1;
+  Clock *clk = a.clk;
+  if (a.end_prev) clk->time = clk->time + clk->dt;
+  if (a.begin_next) clk->time_step += 1;
 }
+__global__ void k_step_boundary(BoundaryArgs a) { step_boundary_body(a); }
+__global__ void k_step_boundary_b(const BoundaryArgs *av) { step_boundary_body(av[blockIdx.z]); }
 
 hipError_t launch_step_boundary(Ctx &c, bool end_prev, bool begin_next) {
-  hipLaunchKernelGGL(k_step_boundary, dim3(1), dim3(64), 0, c.stream, c.clock, end_prev ? 1 : 0, begin_next ? 1 : 0);
-  return hipGetLastError();
+  const BoundaryArgs ba{c.clock, end_prev ? 1 : 0, begin_next ? 1 : 0};
+  return klaunch(c, (const void *)k_step_boundary, (const void *)k_step_boundary_b, dim3(1), dim3(64), 0, c.stream,
+                 ba);
 }
 
 hipError_t prepare_step_kernels(Ctx &c) {
-  hipError_t e = hipFuncSetAttribute((const void *)k_nonlinear<true, kNlRegQ>,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, c.lds_bytes);
-  if (e != hipSuccess) return e;
-  e = hipFuncSetAttribute((const void *)k_nonlinear<false, kMaxQ>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                          c.lds_bytes);
-  if (e != hipSuccess) return e;
-  e = hipFuncSetAttribute((const void *)k_nonlinear32<true, kNlRegQ>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                          c.lds_bytes32);
-  if (e != hipSuccess) return e;
-  e = hipFuncSetAttribute((const void *)k_nonlinear32<false, kMaxQ>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                          c.lds_bytes32);
-  if (e != hipSuccess) return e;
-  return hipFuncSetAttribute((const void *)k_rem_div_thomas, hipFuncAttributeMaxDynamicSharedMemorySize,
-                             (int)rem_div_thomas_lds(c));
+  if (c.lds_bytes > kLdsMax || (long long)rem_div_thomas_lds(c) > kLdsMax) return hipErrorInvalidValue;
+  const void *fns[] = {(const void *)k_nonlinear<true, kNlRegQ>,    (const void *)k_nonlinear<false, kMaxQ>,
+                       (const void *)k_nonlinear32<true, kNlRegQ>,  (const void *)k_nonlinear32<false, kMaxQ>,
+                       (const void *)k_nonlinear_b<true, kNlRegQ>,  (const void *)k_nonlinear_b<false, kMaxQ>,
+                       (const void *)k_nonlinear32_b<true, kNlRegQ>, (const void *)k_nonlinear32_b<false, kMaxQ>,
+                       (const void *)k_rem_div_thomas,              (const void *)k_rem_div_thomas_b};
+  for (const void *fn : fns) {
+    const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsMax);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 }  // namespace dh

@@ -1,11 +1,15 @@
 """Graph-captured ensembles of uneven shear-layer instances
 (dh_ensemble_*, needs an MI355X).
 
-Three KH instances of different sizes (NX = 64, 96, 128, NY = NX + 1,
-NZ = 4) are stepped together; each must come out bit-identical to the same
-instance stepped alone with dh_time_step, since the ensemble graph runs the
-same kernels in the same order.  The NX = 64 member is also compared with
-the oracle at its roundoff floor (this size runs the LDS row transforms)."""
+Six KH instances (NY = NX + 1, NZ = 4) are stepped together: two of NX = 64
+and two of NX = 128 with different Ri, Re and seeds, one of NX = 96, and one
+more of NX = 64 in the mixed form.  Members of one shape and form run as
+one batch per launch (the kernels' batched variants), so the test covers
+batches of two, batches of one and a group of its own for the mixed form.
+Each member must come out bit-identical to the same instance stepped alone
+with dh_time_step: the batch runs the same kernels with the same arguments
+in the same order.  The first NX = 64 member is also compared with the
+oracle at its roundoff floor (this size runs the LDS row transforms)."""
 import numpy as np
 import pytest
 
@@ -24,21 +28,24 @@ def dor():
 
 def _decks():
     out = []
-    for i, (nx, ri, re) in enumerate([(64, 0.05, 800.0), (96, 0.2, 3000.0), (128, 0.1, 1500.0)]):
+    for i, (nx, ri, re, mixed) in enumerate([(64, 0.05, 800.0, False), (96, 0.2, 3000.0, False),
+                                             (128, 0.1, 1500.0, False), (64, 0.15, 2000.0, False),
+                                             (128, 0.02, 600.0, False), (64, 0.08, 1200.0, True)]):
         p, gy, ic = configs.kh_shear_layer(nx=nx, ny=nx + 1, nz=4, ri=ri, re=re)
-        out.append((p, gy, dict(ic, seed=200 + i)))
+        out.append((p, gy, dict(ic, seed=200 + i), mixed))
     return out
 
 
-def _make(p, gy, ic):
+def _make(p, gy, ic, mixed):
     ch = channel.Channel(p, gy)
     ch.initial_condition(ic)
     ch.set_clock(0.0, p["delta_t"], 0)
+    ch.set_mixed_precision(mixed)
     return ch
 
 
 def test_ensemble_equals_members_alone(dor):
-    print("[ensemble] 3 KH instances, NX = 64, 96, 128", flush=True)
+    print("[ensemble] 6 KH instances, NX = 64 x 2, 96, 128 x 2, 64 mixed", flush=True)
     decks = _decks()
     alone = [_make(*d) for d in decks]
     together = [_make(*d) for d in decks]
@@ -53,7 +60,7 @@ def test_ensemble_equals_members_alone(dor):
             assert np.array_equal(sa[k], sb[k]), (a.nx, k)
         assert a.clock() == b.clock()
     # member 0 (NX = 64) against the oracle: ICs and 3 steps
-    p, gy, ic = decks[0]
+    p, gy, ic, _ = decks[0]
     o = dor.Oracle(p)
     o.set_grid_gy(gy)
     o.create_th(ic["th_ic_type"])
