@@ -138,6 +138,11 @@ def ensemble_bench(n, steps, warmup):
         chs.append(ch)
     pts = sum(p["nx"] * p["ny"] * p["nz"] for p, _, _ in decks)
     ens = channel.Ensemble(chs)
+    # block size per member shape, tuned on the members' own state
+    tuned = ens.autotune(2)
+    block = {}
+    for (p, _, _), t in zip(decks, tuned):
+        block[p["nx"]] = t
     ens.time_step(warmup)
     ens.synchronize()
     t0 = time.perf_counter()
@@ -162,7 +167,7 @@ def ensemble_bench(n, steps, warmup):
         "ms_per_step": round(wall / steps * 1e3, 4), "higher_is_better": True, "dtype": "f64",
         "data": "synthetic (seeded KH tanh shear layer + noise per instance)",
         "config": {"workload": "kh_ensemble", "instances": n, "nx_counts": sizes, "nz": 4,
-                   "points_per_step": pts},
+                   "points_per_step": pts, "block_size_by_nx": block},
         "one_at_a_time": {"value": round(pts * steps / wall_seq / 1e6, 3),
                           "ms_per_step": round(wall_seq / steps * 1e3, 4)},
     }), flush=True)

@@ -70,6 +70,10 @@ EXPORTS = {
     "dh_rk_substep": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "dh_time_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "dh_set_mixed_precision": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "dh_set_plane_threads": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "dh_plane_threads": (ctypes.c_int, [ctypes.c_void_p]),
+    "dh_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]),
+    "dh_ensemble_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "dh_mixed_precision": (ctypes.c_int, [ctypes.c_void_p]),
     "dh_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
     "dh_profile_substep": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, _DP, ctypes.c_int32]),
@@ -277,6 +281,19 @@ class Channel:
     def mixed_precision(self):
         return bool(self.lib.dh_mixed_precision(self.h))
 
+    @property
+    def plane_threads(self):
+        return int(self.lib.dh_plane_threads(self.h))
+
+    def set_plane_threads(self, threads):
+        self._check(self.lib.dh_set_plane_threads(self.h, threads), "dh_set_plane_threads")
+
+    def autotune(self, nsteps=2):
+        """Time the valid block sizes on this context's shape and keep the fastest."""
+        best = ctypes.c_int32(0)
+        self._check(self.lib.dh_autotune(self.h, nsteps, ctypes.byref(best)), "dh_autotune")
+        return best.value
+
     def synchronize(self):
         self._check(self.lib.dh_synchronize(self.h), "dh_synchronize")
 
@@ -339,9 +356,10 @@ def make_channel(name, **kw):
 
 class Ensemble:
     """Independent Channel instances of any sizes stepped together
-    (dh_ensemble_*): each member's step graph on its own stream, all members
-    advanced step by step so their kernels overlap.  The members stay usable
-    on their own between ensemble steps and must outlive it."""
+    (dh_ensemble_*): members of one shape run each launch of their step as
+    one batch, shapes on streams of their own, all in one captured graph.
+    The members stay usable on their own between ensemble steps and must
+    outlive it."""
 
     def __init__(self, members):
         self.lib = load_library()
@@ -357,6 +375,13 @@ class Ensemble:
         rc = self.lib.dh_ensemble_time_step(self.h, n)
         if rc != DH_OK:
             raise DiabloError(f"dh_ensemble_time_step failed ({rc})")
+
+    def autotune(self, nsteps=2):
+        """Tune the block size per member shape (dh_ensemble_autotune)."""
+        rc = self.lib.dh_ensemble_autotune(self.h, nsteps)
+        if rc != DH_OK:
+            raise DiabloError(f"dh_ensemble_autotune failed ({rc})")
+        return [m.plane_threads for m in self.members]
 
     def synchronize(self):
         rc = self.lib.dh_ensemble_synchronize(self.h)

@@ -1,6 +1,7 @@
 // C ABI of libdiablo_hip.so (declarations and reference citations in
 // include/diablo_hip.h): context set-up, host<->device layout conversion,
 // substep/step dispatch and the captured step graph.
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -681,6 +682,122 @@ int dh_set_mixed_precision(dh_ctx *h, int32_t on) {
 
 int dh_mixed_precision(dh_ctx *h) { return h ? (h->c.mixed ? 1 : 0) : DH_EARG; }
 
+static int enqueue_steps(Ctx &c, int nsteps);
+
+// Block sizes of the plane kernels and of the nonlinear pass's LDS variant:
+// any multiple of 64 up to 1024 that leaves the nonlinear pass at most 4
+// modes per thread.  The size only decides which thread computes which
+// butterfly or mode, so results do not depend on it.
+static bool plane_threads_ok(const Ctx &c, int t) {
+  return t >= 64 && t <= 1024 && t % 64 == 0 && (c.reg_rows_nl || t * 4 >= c.g.pf);
+}
+
+int dh_set_plane_threads(dh_ctx *h, int32_t threads) {
+  if (!h) return DH_EARG;
+  Ctx &c = h->c;
+  if (!plane_threads_ok(c, threads)) return fail(c, DH_EARG, "plane threads must be a multiple of 64 in 64..1024 "
+                                                             "with 4 x threads >= the plane's Fourier modes");
+  if (threads == c.plane_threads) return DH_OK;
+  HIPCHK(hipStreamSynchronize(c.stream));
+  c.plane_threads = threads;
+  drop_graphs(c);  // the captured steps hold the old block size
+  return DH_OK;
+}
+
+int dh_plane_threads(dh_ctx *h) { return h ? h->c.plane_threads : DH_EARG; }
+
+// A copy of a context's time-stepped state (Fourier fields, CF, CP, the
+// scalars and the clock) on the device, for trial steps that must leave it
+// unchanged; restore() also rebuilds the CN tables of the restored DELTA_T.
+struct StateCopy {
+  Ctx *c = nullptr;
+  double2 *src[9] = {}, *save[9] = {};
+  size_t cnt[9] = {};
+  Clock *ck = nullptr;
+  hipError_t init(Ctx &cx) {
+    c = &cx;
+    const size_t cf = (size_t)(cx.g.ny + 2) * cx.g.pf, nth = (size_t)(cx.p.n_th > 0 ? cx.p.n_th : 1);
+    double2 *s0[9] = {cx.cu[0], cx.cu[1], cx.cu[2], cx.cp, cx.cf[0], cx.cf[1], cx.cf[2], cx.cth, cx.cfth};
+    const size_t n0[9] = {cf, cf, cf, cf, cf, cf, cf, cf * nth, cf * nth};
+    hipError_t e = hipSuccess;
+    for (int i = 0; i < 9 && e == hipSuccess; ++i) {
+      src[i] = s0[i];
+      cnt[i] = n0[i];
+      e = hipMalloc(&save[i], cnt[i] * sizeof(double2));
+    }
+    if (e == hipSuccess) e = hipMalloc(&ck, sizeof(Clock));
+    if (e == hipSuccess) e = copy(false);
+    return e;
+  }
+  hipError_t copy(bool restore) {
+    hipError_t e = hipSuccess;
+    for (int i = 0; i < 9 && e == hipSuccess; ++i)
+      e = hipMemcpyAsync(restore ? src[i] : save[i], restore ? save[i] : src[i], cnt[i] * sizeof(double2),
+                         hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(restore ? c->clock : ck, restore ? ck : c->clock, sizeof(Clock), hipMemcpyDeviceToDevice,
+                         c->stream);
+    return e;
+  }
+  hipError_t restore() {
+    hipError_t e = copy(true);
+    if (e == hipSuccess) e = launch_cn_tables(*c, c->stream);
+    c->tables_pending = false;
+    return e;
+  }
+  ~StateCopy() {
+    for (double2 *q : save)
+      if (q) (void)hipFree(q);
+    if (ck) (void)hipFree(ck);
+  }
+};
+
+// Times nsteps uncaptured steps (the best of two runs) with each valid block
+// size from the context's current state, restoring the state before each
+// run and at the end, and keeps the fastest.
+int dh_autotune(dh_ctx *h, int32_t nsteps, int32_t *chosen) {
+  if (!h || nsteps < 1) return DH_EARG;
+  Ctx &c = h->c;
+  if (!c.grid_set) return fail(c, DH_ESTATE, "dh_set_grid first");
+  HIPCHK(hipStreamSynchronize(c.stream));
+  StateCopy sc;
+  hipError_t e = sc.init(c);
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  if (e == hipSuccess) e = hipEventCreate(&t0);
+  if (e == hipSuccess) e = hipEventCreate(&t1);
+  const int keep = c.plane_threads;
+  int best = keep;
+  float best_ms = 0.f;
+  for (int t = 64; t <= 1024 && e == hipSuccess; t *= 2) {
+    if (!plane_threads_ok(c, t)) continue;
+    c.plane_threads = t;
+    float run_best = 0.f;
+    for (int rep = 0; rep < 2 && e == hipSuccess; ++rep) {
+      e = sc.restore();
+      if (e == hipSuccess) e = hipEventRecord(t0, c.stream);
+      if (e == hipSuccess && enqueue_steps(c, nsteps) != DH_OK) e = hipErrorLaunchFailure;
+      if (e == hipSuccess) e = hipEventRecord(t1, c.stream);
+      if (e == hipSuccess) e = hipEventSynchronize(t1);
+      float ms = 0.f;
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, t0, t1);
+      if (e == hipSuccess && (rep == 0 || ms < run_best)) run_best = ms;
+    }
+    if (e == hipSuccess && (best_ms == 0.f || run_best < best_ms)) {
+      best_ms = run_best;
+      best = t;
+    }
+  }
+  c.plane_threads = (e == hipSuccess) ? best : keep;
+  if (e == hipSuccess) e = sc.restore();
+  if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+  if (t0) (void)hipEventDestroy(t0);
+  if (t1) (void)hipEventDestroy(t1);
+  drop_graphs(c);
+  if (e != hipSuccess) return fail(c, DH_EHIP, std::string("dh_autotune: ") + hipGetErrorString(e));
+  if (chosen) *chosen = best;
+  return DH_OK;
+}
+
 // nsteps iterations of diablo.f:66-110 without I/O: TIME_STEP += 1, the
 // three substeps, TIME += DELTA_T; the end of one step and the start of the
 // next share one launch.
@@ -1010,6 +1127,86 @@ int dh_ensemble_time_step(dh_ens *e, int32_t nsteps) {
   for (dh_ctx *h : e->m) {
     HIPCHK(hipStreamWaitEvent(h->c.stream, e->ev_out, 0));
     h->c.th_phys32 = h->c.mixed;
+  }
+  return DH_OK;
+}
+
+// Tunes the block size of each shape (sizes, scalars, precision form) as
+// the ensemble runs it: the members of the shape stepped as one batch (a
+// temporary ensemble of them), nsteps steps (best of two) with each
+// power-of-two size, the members' states restored before each run and at
+// the end.  The batches are formed again at the next dh_ensemble_time_step.
+int dh_ensemble_autotune(dh_ens *e, int32_t nsteps) {
+  if (!e || nsteps < 1) return DH_EARG;
+  Ctx &c0 = e->m[0]->c;
+  {
+    Ctx &c = c0;
+    if (e->root) HIPCHK(hipStreamSynchronize(e->root));
+    for (dh_ctx *h : e->m)
+      if (!h->c.grid_set) return fail(h->c, DH_ESTATE, "dh_set_grid first");
+  }
+  free_groups(*e);
+  e->fallback = false;
+  std::vector<bool> done(e->m.size(), false);
+  for (size_t i = 0; i < e->m.size(); ++i) {
+    if (done[i]) continue;
+    const Ctx &ci = e->m[i]->c;
+    std::vector<dh_ctx *> shape;
+    for (size_t j = i; j < e->m.size(); ++j) {
+      const Ctx &cj = e->m[j]->c;
+      if (!done[j] && cj.g.nx == ci.g.nx && cj.g.ny == ci.g.ny && cj.g.nz == ci.g.nz && cj.nfields == ci.nfields &&
+          cj.mixed == ci.mixed && cj.p.variable_dt == ci.p.variable_dt) {
+        shape.push_back(e->m[j]);
+        done[j] = true;
+      }
+    }
+    std::vector<StateCopy> saved(shape.size());
+    hipError_t r = hipSuccess;
+    for (size_t k = 0; k < shape.size() && r == hipSuccess; ++k) {
+      r = hipStreamSynchronize(shape[k]->c.stream);
+      if (r == hipSuccess) r = saved[k].init(shape[k]->c);
+    }
+    if (r != hipSuccess) return fail(c0, DH_EHIP, std::string("dh_ensemble_autotune: ") + hipGetErrorString(r));
+    const int keep = shape[0]->c.plane_threads;
+    int best = keep;
+    double best_s = -1.0;
+    for (int t = 64; t <= 1024; t *= 2) {
+      if (!plane_threads_ok(shape[0]->c, t)) continue;
+      int rc = DH_OK;
+      for (dh_ctx *h : shape)
+        if ((rc = dh_set_plane_threads(h, t)) != DH_OK) return rc;
+      dh_ens tmp;
+      tmp.m = shape;
+      double run_best = -1.0;
+      for (int rep = 0; rep < 3 && rc == DH_OK; ++rep) {
+        for (StateCopy &sc : saved)
+          if ((r = sc.restore()) != hipSuccess) break;
+        for (dh_ctx *h : shape)
+          if (r == hipSuccess) r = hipStreamSynchronize(h->c.stream);
+        if (r != hipSuccess) break;
+        const auto t0 = std::chrono::steady_clock::now();
+        rc = dh_ensemble_time_step(&tmp, nsteps);
+        if (rc == DH_OK) rc = dh_ensemble_synchronize(&tmp);
+        const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (rep > 0 && (run_best < 0 || sec < run_best)) run_best = sec;  // the first run builds and captures
+      }
+      free_groups(tmp);
+      if (r != hipSuccess) return fail(c0, DH_EHIP, std::string("dh_ensemble_autotune: ") + hipGetErrorString(r));
+      if (rc != DH_OK) return rc;
+      if (best_s < 0 || run_best < best_s) {
+        best_s = run_best;
+        best = t;
+      }
+    }
+    for (dh_ctx *h : shape) {
+      const int rc = dh_set_plane_threads(h, best);
+      if (rc != DH_OK) return rc;
+    }
+    for (StateCopy &sc : saved)
+      if ((r = sc.restore()) != hipSuccess) break;
+    for (dh_ctx *h : shape)
+      if (r == hipSuccess) r = hipStreamSynchronize(h->c.stream);
+    if (r != hipSuccess) return fail(c0, DH_EHIP, std::string("dh_ensemble_autotune: ") + hipGetErrorString(r));
   }
   return DH_OK;
 }

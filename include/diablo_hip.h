@@ -141,6 +141,19 @@ int dh_time_step(dh_ctx *ctx, int32_t nsteps);
 int dh_set_mixed_precision(dh_ctx *ctx, int32_t on);
 /* 1 when the mixed form is on, 0 when not. */
 int dh_mixed_precision(dh_ctx *ctx);
+
+/* Launch-shape tuning (no counterpart in the reference; for ensembles of
+ * uneven sizes, where the best block size depends on the plane's shape).
+ * dh_set_plane_threads sets the block size of the plane transform kernels
+ * and of the nonlinear pass's LDS variant: a multiple of 64 in 64..1024
+ * with 4 x threads >= the plane's de-aliased Fourier modes.  Results do not
+ * depend on it.  dh_autotune times nsteps steps (best of two) with each
+ * power-of-two size from the current state, restores the state (Fourier
+ * fields, CF, CP, scalars, clock and CN tables) and keeps the fastest,
+ * returned in *chosen. */
+int dh_set_plane_threads(dh_ctx *ctx, int32_t threads);
+int dh_plane_threads(dh_ctx *ctx);
+int dh_autotune(dh_ctx *ctx, int32_t nsteps, int32_t *chosen);
 /* Per-phase timing of one substep (the reference's only timer is the
  * wall-clock "Seconds per Iteration" of diablo.f:123-127): runs substep
  * rk_step nrep times with HIP events between its kernels on the context
@@ -164,6 +177,9 @@ int dh_synchronize(dh_ctx *ctx);
 int dh_ensemble_create(int32_t n, dh_ctx *const *members, dh_ens **out);
 int dh_ensemble_time_step(dh_ens *ens, int32_t nsteps);
 int dh_ensemble_synchronize(dh_ens *ens);
+/* dh_autotune on the first member of each shape; the other members of that
+ * shape take the same block size, so that they still run as one batch. */
+int dh_ensemble_autotune(dh_ens *ens, int32_t nsteps);
 int dh_ensemble_destroy(dh_ens *ens);
 
 /* REM_DIV_CHAN (channel.f:1103-1214 with APPLY_BC_REM_DIV_MPI
