@@ -48,7 +48,9 @@ __device__ __forceinline__ void plane_inv_body(const Geo &g, const PlaneBatch &b
 
 // physical -> Fourier (x r2c with 1/NX and truncation, z c2c with 1/NZ and
 // packing of kz = -NKZ..-1 after 0..NKZ).
-template <bool REG, class C>
+// COLS: the register column transform when NZ = 128 (left out of the
+// batched kernel, where its registers do not fit beside the batch indexing).
+template <bool REG, class C, bool COLS = REG>
 __device__ __forceinline__ void plane_fwd_body(const Geo &g, const PlaneBatch &b, C *A) {
   const auto T = load_tables(A, g);
   const int f = blockIdx.y;
@@ -67,6 +69,23 @@ __device__ __forceinline__ void plane_fwd_body(const Geo &g, const PlaneBatch &b
   } else {
     plane_load_phys(A, g, T, src);
     plane_rows_fwd(A, g, T);
+  }
+  if (COLS && g.nz == 128) {
+    // z transform in registers, each mode stored where it is computed
+    cols_fwd_reg128(A, g, T, [&](int kx, int zr, C v) {
+      const double nzd = g.dnz, rnz = g.rnz;  // V / NZ as the correctly rounded quotient
+      int kzp = -1;
+      if (zr <= g.nkz)
+        kzp = zr;
+      else if (zr >= g.nz - g.nkz)
+        kzp = zr - (g.nz - 1 - 2 * g.nkz);
+      if (kzp < 0) return;
+      const int m = kzp * g.nkx1 + kx;
+      double2 r = make_double2(div_by_recip((double)v.x, nzd, rnz), div_by_recip((double)v.y, nzd, rnz));
+      if (base) r = make_double2(base[m].x + r.x, base[m].y + r.y);
+      dst[m] = r;
+    });
+    return;
   }
   plane_cols_fwd(A, g, T);
   const double nzd = g.dnz, rnz = g.rnz;  // V / NZ as the correctly rounded quotient
@@ -87,7 +106,7 @@ __global__ void __launch_bounds__(1024) k_plane_inv(PlaneArgs pa) {
   plane_inv_body<REG, C>(pa.g, pa.b, reinterpret_cast<C *>(smem));
 }
 template <bool REG, class C>
-__global__ void __launch_bounds__(1024) k_plane_inv_b(const PlaneArgs *av) {
+__global__ void __launch_bounds__(1024) k_plane_inv_b(const PlaneArgs *__restrict__ av) {
   extern __shared__ double2 smem[];
   const PlaneArgs &pa = av[blockIdx.z];
   plane_inv_body<REG, C>(pa.g, pa.b, reinterpret_cast<C *>(smem));
@@ -98,10 +117,10 @@ __global__ void __launch_bounds__(1024) k_plane_fwd(PlaneArgs pa) {
   plane_fwd_body<REG, C>(pa.g, pa.b, reinterpret_cast<C *>(smem));
 }
 template <bool REG, class C>
-__global__ void __launch_bounds__(1024) k_plane_fwd_b(const PlaneArgs *av) {
+__global__ void __launch_bounds__(1024) k_plane_fwd_b(const PlaneArgs *__restrict__ av) {
   extern __shared__ double2 smem[];
   const PlaneArgs &pa = av[blockIdx.z];
-  plane_fwd_body<REG, C>(pa.g, pa.b, reinterpret_cast<C *>(smem));
+  plane_fwd_body<REG, C, false>(pa.g, pa.b, reinterpret_cast<C *>(smem));
 }
 
 static hipError_t set_lds(const void *fn, int bytes) {

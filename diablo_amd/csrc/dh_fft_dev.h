@@ -603,6 +603,47 @@ __device__ __forceinline__ void rows_fwd_reg64(C *A, const Geo &g, const TablesT
   rows_fwd_reg64<false>(A, g, T, load, [](C v) { return v; });
 }
 
+// Forward z transform of the de-aliased columns kx <= NKX for NZ = 128, in
+// registers: 16 lanes of a wave own a column, lane t holding x_{t+16r},
+// r < 8.  X_{k1+8 k2} = sum_t W16^{t k2} W128^{t k1} sum_r x_{t+16r} W8^{r k1}
+// (k1 < 8, k2 < 16): an 8-point DFT over r in each lane and the twiddle
+// W128^{t k1}, giving Y_t[k1], then a transpose through the column's own
+// LDS slots (only this wave touches them, so no workgroup barrier).  The
+// 16-point sums over t are split by the parity p of k2 = 2m + p:
+// X_{k1+8p+16m} = sum_{s<8} W8^{s m} W16^{s p} (Y_s[k1] + (-1)^p Y_{s+8}[k1]),
+// so lane (k1, p) folds 16 values into 8 as it reads them and ends with an
+// 8-point DFT: 8 live values per lane throughout.  Each result goes to
+// sink(kx, z index, X).  The rows must be complete in LDS (a workgroup
+// barrier before the call).
+template <class C, class Sink>
+__device__ void cols_fwd_reg128(C *A, const Geo &g, const TablesT<C> &T, Sink sink) {
+  const int lane = threadIdx.x & 63, grp = lane >> 4, t = lane & 15;
+  const int k1 = t & 7, p = t >> 3;
+  const int nw = blockDim.x >> 6;
+  const Sc<C> sgn = p ? Sc<C>(-1) : Sc<C>(1);
+  for (int kx = (int)(threadIdx.x >> 6) * 4 + grp; kx < g.nkx1; kx += nw * 4) {
+    C v[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) v[r] = A[(t + 16 * r) * g.ls + kx];
+    dft_small<8, false>(v);
+#pragma unroll
+    for (int j = 1; j < 8; ++j) v[j] = cmul(v[j], T.twc[t * j]);
+    wave_lds_sync();
+#pragma unroll
+    for (int j = 0; j < 8; ++j) A[(16 * j + t) * g.ls + kx] = v[j];
+    wave_lds_sync();
+#pragma unroll
+    for (int s2 = 0; s2 < 8; ++s2) {
+      const C y0 = A[(16 * k1 + s2) * g.ls + kx], y1 = A[(16 * k1 + s2 + 8) * g.ls + kx];
+      const C z = mkc<C>(y0.x + sgn * y1.x, y0.y + sgn * y1.y);
+      v[s2] = s2 ? cmul(z, T.twc[8 * s2 * p]) : z;
+    }
+    dft_small<8, false>(v);
+#pragma unroll
+    for (int m = 0; m < 8; ++m) sink(kx, k1 + 8 * p + 16 * m, v[m]);
+  }
+}
+
 // Inverse x transform of every row (FFT_XZ_TO_PHYSICAL's c2r,
 // mpi.f:1600-1604): X_k (k <= NKX) in natural slots of the LDS rows to the
 // physical pairs, handed to store(z, m, value).

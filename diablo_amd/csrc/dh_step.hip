@@ -597,13 +597,13 @@ k_nonlinear32(StepArgs a) {
 // batches of contexts of one shape: argument structs in device memory,
 // member blockIdx.z
 template <bool REG, int NQ>
-__global__ void __launch_bounds__(REG ? 512 : 1024) k_nonlinear_b(const StepArgs *av) {
+__global__ void __launch_bounds__(REG ? 512 : 1024) k_nonlinear_b(const StepArgs *__restrict__ av) {
   extern __shared__ double2 smem[];
   nonlinear_pass<REG, NQ, double2>(av[blockIdx.z], smem);
 }
 template <bool REG, int NQ>
 __global__ void __launch_bounds__(REG ? 512 : 1024) __attribute__((amdgpu_waves_per_eu(4)))
-k_nonlinear32_b(const StepArgs *av) {
+k_nonlinear32_b(const StepArgs *__restrict__ av) {
   extern __shared__ double2 smem[];
   nonlinear_pass<REG, NQ, float2>(av[blockIdx.z], reinterpret_cast<float2 *>(smem));
 }
@@ -769,7 +769,7 @@ struct CnArgs {
   RealFactors fac;
 };
 __global__ void __launch_bounds__(256) k_cn_tables(CnArgs a) { cn_tables_body(a.clk, a.in, a.fac); }
-__global__ void __launch_bounds__(256) k_cn_tables_b(const CnArgs *av) {
+__global__ void __launch_bounds__(256) k_cn_tables_b(const CnArgs *__restrict__ av) {
   const CnArgs &a = av[blockIdx.z];
   cn_tables_body(a.clk, a.in, a.fac);
 }
@@ -971,7 +971,7 @@ __global__ void __launch_bounds__(kSegThreads) k_solve_seg(SolveArgs a) {
   solve_seg_body<NS, P>(a);
 }
 template <int NS, class P>
-__global__ void __launch_bounds__(kSegThreads) k_solve_seg_b(const SolveArgs *av) {
+__global__ void __launch_bounds__(kSegThreads) k_solve_seg_b(const SolveArgs *__restrict__ av) {
   solve_seg_body<NS, P>(av[blockIdx.z]);
 }
 
@@ -1007,7 +1007,7 @@ struct CourantArgs {
   int variable_dt, update_dt;
 };
 __global__ void k_courant_dt(CourantArgs a) { courant_dt_body(a.clk, a.dt_cap, a.variable_dt, a.update_dt); }
-__global__ void k_courant_dt_b(const CourantArgs *av) {
+__global__ void k_courant_dt_b(const CourantArgs *__restrict__ av) {
   const CourantArgs &a = av[blockIdx.z];
   courant_dt_body(a.clk, a.dt_cap, a.variable_dt, a.update_dt);
 }
@@ -1384,7 +1384,7 @@ __device__ __forceinline__ void rem_div_thomas_body(const RemDivArgs &a) {
   }
 }
 __global__ void __launch_bounds__(256) k_rem_div_thomas(RemDivArgs a) { rem_div_thomas_body(a); }
-__global__ void __launch_bounds__(256) k_rem_div_thomas_b(const RemDivArgs *av) { rem_div_thomas_body(av[blockIdx.z]); }
+__global__ void __launch_bounds__(256) k_rem_div_thomas_b(const RemDivArgs *__restrict__ av) { rem_div_thomas_body(av[blockIdx.z]); }
 
 // ---------------------------------------------------------------------------
 // launchers
@@ -1527,7 +1527,7 @@ __device__ __forceinline__ void bc_increment_body(const SolveArgs &a) {
   }
 }
 __global__ void __launch_bounds__(256) k_bc_increment(SolveArgs a) { bc_increment_body(a); }
-__global__ void __launch_bounds__(256) k_bc_increment_b(const SolveArgs *av) { bc_increment_body(av[blockIdx.z]); }
+__global__ void __launch_bounds__(256) k_bc_increment_b(const SolveArgs *__restrict__ av) { bc_increment_body(av[blockIdx.z]); }
 
 // Crank-Nicolson solve of fields f0..f1-1 on stream st.  A velocity
 // field's solution overwrites its own R (other components' nonlinear
@@ -1714,7 +1714,7 @@ __device__ __forceinline__ void step_boundary_body(const BoundaryArgs &a) {
   if (a.begin_next) clk->time_step += 1;
 }
 __global__ void k_step_boundary(BoundaryArgs a) { step_boundary_body(a); }
-__global__ void k_step_boundary_b(const BoundaryArgs *av) { step_boundary_body(av[blockIdx.z]); }
+__global__ void k_step_boundary_b(const BoundaryArgs *__restrict__ av) { step_boundary_body(av[blockIdx.z]); }
 
 hipError_t launch_step_boundary(Ctx &c, bool end_prev, bool begin_next) {
   const BoundaryArgs ba{c.clock, end_prev ? 1 : 0, begin_next ? 1 : 0};
