@@ -127,6 +127,12 @@ struct dor_state {
   /* REM_DIV work arrays (header:226-228), persistent like COMMON */
   double *MATL_C, *MATD_C, *MATU_C;
   cplx *VEC_C;
+  /* LES (header:164, header_les:5-33): NU_T at GY points (KAPPA_T, set to
+   * 1*NU_T for every scalar at channel.f:313-321, is NU_T itself), the
+   * strain fields CSij1..6 and CTEMP, each EQUIVALENCEd with its physical
+   * view Sij / TEMP */
+  double *NU_T;
+  cplx *CSij[6], *CTEMP;
 };
 
 /* Fortran-layout index helpers */
@@ -181,6 +187,11 @@ dor_state *dor_create(const dor_params *p) {
   s->MATD_C = zalloc(8 * (size_t)(s->NXP + 1) * nyy);
   s->MATU_C = zalloc(8 * (size_t)(s->NXP + 1) * nyy);
   s->VEC_C = zalloc(sizeof(cplx) * (size_t)(s->NXP + 1) * nyy);
+  if (p->les) {
+    s->NU_T = zalloc(8 * s->rfield);
+    for (int q = 0; q < 6; ++q) s->CSij[q] = zalloc(sizeof(cplx) * s->cfield);
+    s->CTEMP = zalloc(sizeof(cplx) * s->cfield);
+  }
   plan_init(&s->px_f, s->NX, -1);
   plan_init(&s->px_b, s->NX, +1);
   plan_init(&s->pz_f, s->NZ, -1);
@@ -197,6 +208,8 @@ void dor_destroy(dor_state *s) {
   free(s->CR2); free(s->CR3); free(s->CF1); free(s->CF2); free(s->CF3);
   free(s->CS1); free(s->CTH); free(s->CFTH); free(s->CRTH); free(s->TH);
   free(s->RTH); free(s->MATL_C); free(s->MATD_C); free(s->MATU_C); free(s->VEC_C);
+  free(s->NU_T); free(s->CTEMP);
+  for (int q = 0; q < 6; ++q) free(s->CSij[q]);
   plan_free(&s->px_f); plan_free(&s->px_b); plan_free(&s->pz_f); plan_free(&s->pz_b);
   free(s);
 }
@@ -232,6 +245,7 @@ double *dor_array(dor_state *s, int which, int64_t *n) {
     case DOR_HBAR: *n = 3; return s->H_BAR;
     case DOR_BETABAR: *n = 3; return s->BETA_BAR;
     case DOR_ZETABAR: *n = 3; return s->ZETA_BAR;
+    case DOR_NUT: *n = s->NU_T ? rf : 0; return s->NU_T;
   }
   *n = 0;
   return NULL;
@@ -503,6 +517,20 @@ void dor_thomas_complex(double *A, double *B, double *C, double *Gd, int INY, in
 
 /* U1/U3/TH share the same row pattern: channel.f:1660-1778 (U1),
  * 1962-2084 (U3), 2088-2241 (TH). */
+/* The eddy viscosity (or diffusivity) part of the implicit rows J=j0..j1 of
+ * column K (channel.f:794-807, 927-941, 991-1005): NU_T at GY points. */
+static void les_rows_gy(dor_state *s, int K, int j0, int j1, double TEMP2, double *L, double *D, double *Uu) {
+  const int NX = s->NX, NXM = s->NXM;
+  const double *DY = s->DY, *DYF = s->DYF, *NU_T = s->NU_T;
+  for (int J = j0; J <= j1; ++J)
+    for (int i = 0; i <= NXM; ++i) {
+      const double nj = NU_T[RIDX(s, i, K, J)], np = NU_T[RIDX(s, i, K, J + 1)];
+      L[i + (size_t)NX * J] = L[i + (size_t)NX * J] - TEMP2 * nj / (DY[J] * DYF[J]);
+      D[i + (size_t)NX * J] = D[i + (size_t)NX * J] + TEMP2 * np / (DY[J + 1] * DYF[J]) + TEMP2 * nj / (DY[J] * DYF[J]);
+      Uu[i + (size_t)NX * J] = Uu[i + (size_t)NX * J] - TEMP2 * np / (DY[J + 1] * DYF[J]);
+    }
+}
+
 static void bc_u13th(int NX, int NY, const double *DY, int bc_min, double c1_min, int bc_max,
                      double c1_max, double *L, double *D, double *U, double *VEC) {
   if (bc_min == 0) {
@@ -564,6 +592,215 @@ static void bc_u2(int NX, int NY, const double *DYF, int bc_min, double c1_min, 
 /* ------------------------------------------------------------------------ */
 /* RK_CHAN_1  channel.f:62-1073, LES=.FALSE., USE_MPI=.TRUE., one rank       */
 /* ------------------------------------------------------------------------ */
+/* Filter widths of les.f:196-217 with -2 C_SMAG^2 absorbed, C_SMAG=0.13,
+ * beta_sgs=1: DYFW(J) = -2 C^2 (DX(1) DYF(J) 2 DZ(1))^(2/3) at GYF and
+ * DYW(J) likewise with DY(J) at GY, J=1..NY+1 (others 0). */
+void dor_les_widths(dor_state *s, double *dyw, double *dyfw) {
+  const double C_SMAG = 0.13, beta_sgs = 1.0;
+  for (int J = 0; J <= s->NY + 1; ++J) dyw[J] = dyfw[J] = 0.0;
+  for (int J = 1; J <= s->NY + 1; ++J) {
+    dyfw[J] = -2.0 * pow(C_SMAG, 2.0) * pow(s->DX1 * beta_sgs * s->DYF[J] * 2.0 * s->DZ1 * beta_sgs, 2.0 / 3.0);
+    dyw[J] = -2.0 * pow(C_SMAG, 2.0) * pow(s->DX1 * beta_sgs * s->DY[J] * 2.0 * s->DZ1 * beta_sgs, 2.0 / 3.0);
+  }
+}
+
+/* APPLY_BC_LES les.f:805-862: with Neumann U or W, the ghost planes copy
+ * the first interior plane so the model sees no wall gradient (Fourier). */
+static void apply_bc_les(dor_state *s) {
+  const int NY = s->NY, NXP = s->NXP, TNKZ = s->TNKZ;
+#define C3(a, i, k, j) a[CIDX(s, i, k, j)]
+  for (int K = 0; K <= TNKZ; ++K)
+    for (int i = 0; i <= NXP - 1; ++i) {
+      if (s->p.u_bc_ymax == 1) { C3(s->CU1, i, K, NY) = C3(s->CU1, i, K, NY - 1); C3(s->CU1, i, K, NY + 1) = C3(s->CU1, i, K, NY); }
+      if (s->p.w_bc_ymax == 1) { C3(s->CU3, i, K, NY) = C3(s->CU3, i, K, NY - 1); C3(s->CU3, i, K, NY + 1) = C3(s->CU3, i, K, NY); }
+      if (s->p.u_bc_ymin == 1) { C3(s->CU1, i, K, 1) = C3(s->CU1, i, K, 2); C3(s->CU1, i, K, 0) = C3(s->CU1, i, K, 1); }
+      if (s->p.w_bc_ymin == 1) { C3(s->CU3, i, K, 1) = C3(s->CU3, i, K, 2); C3(s->CU3, i, K, 0) = C3(s->CU3, i, K, 1); }
+    }
+#undef C3
+}
+
+/* APPLY_BC_VEL_PHYS_MPI mpi.f:1366-1378 -> APPLY_BC_VEL_PHYS_LOWER/UPPER
+ * channel.f:2495-2650 (one rank: both walls), on physical U1..3. */
+static void apply_bc_vel_phys(dor_state *s) {
+  const int NY = s->NY, NZP = s->NZP, NXM = s->NXM;
+  const double *DY = s->DY, *DYF = s->DYF;
+  double *U1 = PHYS(s->CU1), *U2 = PHYS(s->CU2), *U3 = PHYS(s->CU3);
+  const dor_params *p = &s->p;
+#define R3D(a, i, k, j) a[RIDX(s, i, k, j)]
+  for (int K = 0; K <= NZP - 1; ++K)
+    for (int i = 0; i <= NXM; ++i) {
+      if (p->u_bc_ymin == 0) { R3D(U1, i, K, 0) = p->u_bc_ymin_c1; R3D(U1, i, K, 1) = p->u_bc_ymin_c1; }
+      else { R3D(U1, i, K, 1) = R3D(U1, i, K, 2) - DY[2] * p->u_bc_ymin_c1; R3D(U1, i, K, 0) = R3D(U1, i, K, 1) - DY[1] * p->u_bc_ymin_c1; }
+      if (p->w_bc_ymin == 0) { R3D(U3, i, K, 0) = p->w_bc_ymin_c1; R3D(U3, i, K, 1) = p->w_bc_ymin_c1; }
+      else { R3D(U3, i, K, 1) = R3D(U3, i, K, 2) - DY[2] * p->w_bc_ymin_c1; R3D(U3, i, K, 0) = R3D(U3, i, K, 1) - DY[1] * p->w_bc_ymin_c1; }
+      if (p->v_bc_ymin == 0) { R3D(U2, i, K, 1) = 2.0 * p->v_bc_ymin_c1 - R3D(U2, i, K, 2); R3D(U2, i, K, 0) = R3D(U2, i, K, 1); }
+      else { R3D(U2, i, K, 1) = R3D(U2, i, K, 2) - DYF[1] * p->v_bc_ymin_c1; R3D(U2, i, K, 0) = R3D(U2, i, K, 1) - DYF[1] * p->v_bc_ymin_c1; }
+    }
+  for (int K = 0; K <= NZP - 1; ++K)
+    for (int i = 0; i <= NXM; ++i) {
+      if (p->u_bc_ymax == 0) { R3D(U1, i, K, NY) = p->u_bc_ymax_c1; R3D(U1, i, K, NY + 1) = p->u_bc_ymax_c1; }
+      else { R3D(U1, i, K, NY) = R3D(U1, i, K, NY - 1) + DY[NY] * p->u_bc_ymax_c1; R3D(U1, i, K, NY + 1) = R3D(U1, i, K, NY) + DY[NY] * p->u_bc_ymax_c1; }
+      if (p->w_bc_ymax == 0) { R3D(U3, i, K, NY) = p->w_bc_ymax_c1; R3D(U3, i, K, NY + 1) = p->w_bc_ymax_c1; }
+      else { R3D(U3, i, K, NY) = R3D(U3, i, K, NY - 1) + DY[NY] * p->w_bc_ymax_c1; R3D(U3, i, K, NY + 1) = R3D(U3, i, K, NY) + DY[NY] * p->w_bc_ymax_c1; }
+      if (p->v_bc_ymax == 0) R3D(U2, i, K, NY + 1) = 2.0 * p->v_bc_ymax_c1 - R3D(U2, i, K, NY);
+      else R3D(U2, i, K, NY + 1) = R3D(U2, i, K, NY) + DY[NY] * p->v_bc_ymax_c1;
+    }
+#undef R3D
+}
+
+/* les_chan les.f:1-336 (LES_MODEL_TYPE=1) with compute_strain_chan
+ * les.f:526-569 and GHOST_LES_MPI's NPROCY=1 branch mpi.f:493-509. */
+void dor_les_chan(dor_state *s) {
+  const int NY = s->NY, NXP = s->NXP, TNKZ = s->TNKZ, NZP = s->NZP, NXM = s->NXM;
+  const int JSTART = s->JSTART, JEND = s->JEND;
+  const double *DY = s->DY, *DYF = s->DYF, *KX = s->KX, *KZ = s->KZ;
+  cplx *CU1 = s->CU1, *CU2 = s->CU2, *CU3 = s->CU3;
+  cplx *CF1 = s->CF1, *CF2 = s->CF2, *CF3 = s->CF3;
+  cplx *CS11 = s->CSij[0], *CS22 = s->CSij[1], *CS33 = s->CSij[2];
+  cplx *CS4 = s->CSij[3], *CS5 = s->CSij[4], *CS6 = s->CSij[5], *CTEMP = s->CTEMP;
+  double *S11 = PHYS(CS11), *S22 = PHYS(CS22), *S33 = PHYS(CS33), *S4 = PHYS(CS4), *S5 = PHYS(CS5), *S6 = PHYS(CS6);
+  double *TEMP = PHYS(CTEMP), *S1 = PHYS(s->CS1), *NU_T = s->NU_T;
+  double *U1 = PHYS(CU1), *U3 = PHYS(CU3);
+#define C3(a, i, k, j) a[CIDX(s, i, k, j)]
+#define R3D(a, i, k, j) a[RIDX(s, i, k, j)]
+#define CIKX(i) (I * KX[i])
+#define CIKZ(k) (I * KZ[k])
+  /* les.f:45-57, RANKY = 0 = NPROCY-1: the upper-wall branch */
+  const int J1 = JSTART, J2 = NY - 1;
+  /* les.f:62-73 (GHOST_CHAN_MPI is a no-op with NPROCY=1, mpi.f:25) */
+  dor_apply_bc_vel(s);
+  apply_bc_les(s);
+  /* compute_strain_chan les.f:536-564 */
+  for (int J = 1; J <= NY; ++J)
+    for (int K = 0; K <= TNKZ; ++K)
+      for (int i = 0; i <= NXP - 1; ++i) {
+        C3(CS11, i, K, J) = CIKX(i) * C3(CU1, i, K, J);
+        C3(CS22, i, K, J) = (C3(CU2, i, K, J + 1) - C3(CU2, i, K, J)) / DYF[J];
+        C3(CS33, i, K, J) = CIKZ(K) * C3(CU3, i, K, J);
+        C3(CS5, i, K, J) = 0.5 * (CIKZ(K) * C3(CU1, i, K, J) + CIKX(i) * C3(CU3, i, K, J));
+      }
+  for (int J = 1; J <= NY + 1; ++J)
+    for (int K = 0; K <= TNKZ; ++K)
+      for (int i = 0; i <= NXP - 1; ++i) {
+        C3(CS4, i, K, J) = 0.5 * ((C3(CU1, i, K, J) - C3(CU1, i, K, J - 1)) / DY[J] + CIKX(i) * C3(CU2, i, K, J));
+        C3(CS6, i, K, J) = 0.5 * (CIKZ(K) * C3(CU2, i, K, J) + (C3(CU3, i, K, J) - C3(CU3, i, K, J - 1)) / DY[J]);
+      }
+  for (int q = 0; q < 6; ++q) dor_fft_xz_to_physical(s, (double *)s->CSij[q], 0, NY + 1);
+  /* les.f:86-91: the mean velocity profiles */
+  double *U1b = malloc(8 * (size_t)(NY + 2)), *U3b = malloc(8 * (size_t)(NY + 2));
+  for (int J = 0; J <= NY + 1; ++J) {
+    U1b[J] = creal(C3(CU1, 0, 0, J));
+    U3b[J] = creal(C3(CU3, 0, 0, J));
+  }
+  /* les.f:103-105 */
+  dor_fft_xz_to_physical(s, (double *)CU1, 0, NY + 1);
+  dor_fft_xz_to_physical(s, (double *)CU2, 0, NY + 1);
+  dor_fft_xz_to_physical(s, (double *)CU3, 0, NY + 1);
+  /* |S| at GYF, les.f:113-134 */
+  for (int J = JSTART; J <= JEND; ++J)
+    for (int K = 0; K <= NZP - 1; ++K)
+      for (int i = 0; i <= NXM; ++i) {
+        double a4 = 0.5 * (R3D(S4, i, K, J + 1) + R3D(S4, i, K, J) - 0.5 * (U1b[J] - U1b[J - 1]) / DY[J] -
+                           0.5 * (U1b[J + 1] - U1b[J]) / DY[J + 1]);
+        double a6 = 0.5 * (R3D(S6, i, K, J + 1) + R3D(S6, i, K, J) - 0.5 * (U3b[J] - U3b[J - 1]) / DY[J] -
+                           0.5 * (U3b[J + 1] - U3b[J]) / DY[J + 1]);
+        double s11 = R3D(S11, i, K, J), s22 = R3D(S22, i, K, J), s33 = R3D(S33, i, K, J), s5 = R3D(S5, i, K, J);
+        R3D(S1, i, K, J) = sqrt(2.0 * (s11 * s11) + 4.0 * (a4 * a4) + 4.0 * (s5 * s5) + 2.0 * (s22 * s22) +
+                                4.0 * (a6 * a6) + 2.0 * (s33 * s33));
+      }
+  /* |S| at GY, les.f:137-160 */
+  for (int J = 2; J <= NY + 1; ++J)
+    for (int K = 0; K <= NZP - 1; ++K)
+      for (int i = 0; i <= NXM; ++i) {
+#define GYI(A) ((R3D(A, i, K, J) * DYF[J - 1] + R3D(A, i, K, J - 1) * DYF[J]) / (2.0 * DY[J]))
+        double i11 = GYI(S11), i5 = GYI(S5), i22 = GYI(S22), i33 = GYI(S33);
+#undef GYI
+        double a4 = R3D(S4, i, K, J) - 0.5 * (U1b[J] - U1b[J - 1]) / DY[J];
+        double a6 = R3D(S6, i, K, J) - 0.5 * (U3b[J] - U3b[J - 1]) / DY[J];
+        R3D(TEMP, i, K, J) = sqrt(2.0 * (i11 * i11) + 4.0 * (a4 * a4) + 4.0 * (i5 * i5) + 2.0 * (i22 * i22) +
+                                  4.0 * (a6 * a6) + 2.0 * (i33 * i33));
+      }
+  /* |S| S_ij, les.f:164-190 (the dU/dy parts of S4 and S6 are implicit) */
+  for (int J = JSTART; J <= JEND; ++J)
+    for (int K = 0; K <= NZP - 1; ++K)
+      for (int i = 0; i <= NXM; ++i) {
+        R3D(S11, i, K, J) = R3D(S1, i, K, J) * R3D(S11, i, K, J);
+        R3D(S5, i, K, J) = R3D(S1, i, K, J) * R3D(S5, i, K, J);
+        R3D(S22, i, K, J) = 0.0;
+        R3D(S33, i, K, J) = R3D(S1, i, K, J) * R3D(S33, i, K, J);
+      }
+  for (int J = 2; J <= NY + 1; ++J)
+    for (int K = 0; K <= NZP - 1; ++K)
+      for (int i = 0; i <= NXM; ++i) {
+        R3D(S4, i, K, J) = R3D(TEMP, i, K, J) * (R3D(S4, i, K, J) - 0.5 * (R3D(U1, i, K, J) - R3D(U1, i, K, J - 1)) / DY[J]);
+        R3D(S6, i, K, J) = R3D(TEMP, i, K, J) * (R3D(S6, i, K, J) - 0.5 * (R3D(U3, i, K, J) - R3D(U3, i, K, J - 1)) / DY[J]);
+      }
+  /* filter widths with -2 C_SMAG^2 absorbed, les.f:196-217 */
+  double *DYFW = calloc((size_t)(NY + 2), 8), *DYW = calloc((size_t)(NY + 2), 8);
+  dor_les_widths(s, DYW, DYFW);
+  /* NU_T at GY, les.f:221-227, then GHOST_LES_MPI (NPROCY=1) */
+  for (int J = 2; J <= NY; ++J)
+    for (int K = 0; K <= NZP - 1; ++K)
+      for (int i = 0; i <= NXM; ++i) R3D(NU_T, i, K, J) = -0.5 * DYW[J] * R3D(TEMP, i, K, J);
+  for (int K = 0; K <= NZP - 1; ++K)
+    for (int i = 0; i <= NXM; ++i) {
+      R3D(NU_T, i, K, 1) = 0.0;
+      R3D(NU_T, i, K, 2) = 0.0;
+      R3D(NU_T, i, K, NY) = 0.0;
+      R3D(NU_T, i, K, NY + 1) = 0.0;
+    }
+  /* les.f:237-264 */
+  dor_fft_xz_to_fourier(s, S11, 0, NY + 1);
+  dor_fft_xz_to_fourier(s, S33, 0, NY + 1);
+  dor_fft_xz_to_fourier(s, S4, 0, NY + 1);
+  dor_fft_xz_to_fourier(s, S5, 0, NY + 1);
+  dor_fft_xz_to_fourier(s, S6, 0, NY + 1);
+  for (int J = 1; J <= NY + 1; ++J)
+    for (int K = 0; K <= TNKZ; ++K)
+      for (int i = 0; i <= NXP - 1; ++i) {
+        C3(CS11, i, K, J) = DYFW[J] * C3(CS11, i, K, J);
+        C3(CS5, i, K, J) = DYFW[J] * C3(CS5, i, K, J);
+        C3(CS33, i, K, J) = DYFW[J] * C3(CS33, i, K, J);
+      }
+  for (int J = 2; J <= NY + 1; ++J)
+    for (int K = 0; K <= TNKZ; ++K)
+      for (int i = 0; i <= NXP - 1; ++i) {
+        C3(CS4, i, K, J) = DYW[J] * C3(CS4, i, K, J);
+        C3(CS6, i, K, J) = DYW[J] * C3(CS6, i, K, J);
+      }
+  /* SGS divergence into CF, les.f:280-304 */
+  for (int J = J1; J <= J2; ++J)
+    for (int K = 0; K <= TNKZ; ++K)
+      for (int i = 0; i <= NXP - 1; ++i) {
+        C3(CF1, i, K, J) = C3(CF1, i, K, J) - CIKX(i) * C3(CS11, i, K, J) -
+                           (C3(CS4, i, K, J + 1) - C3(CS4, i, K, J)) / DYF[J] - CIKZ(K) * C3(CS5, i, K, J);
+        C3(CF3, i, K, J) = C3(CF3, i, K, J) - CIKX(i) * C3(CS5, i, K, J) -
+                           (C3(CS6, i, K, J + 1) - C3(CS6, i, K, J)) / DYF[J] - CIKZ(K) * C3(CS33, i, K, J);
+      }
+  for (int J = 2; J <= NY; ++J)
+    for (int K = 0; K <= TNKZ; ++K)
+      for (int i = 0; i <= NXP - 1; ++i)
+        C3(CF2, i, K, J) = C3(CF2, i, K, J) - CIKX(i) * C3(CS4, i, K, J) - CIKZ(K) * C3(CS6, i, K, J);
+  /* explicit cross terms, les.f:307-336 */
+  for (int c = 0; c < 2; ++c) {
+    const double *Uc = c ? U3 : U1;
+    for (int K = 0; K <= NZP - 1; ++K)
+      for (int i = 0; i <= NXM; ++i)
+        for (int J = 2; J <= NY; ++J)
+          R3D(TEMP, i, K, J) = R3D(NU_T, i, K, J) * (R3D(Uc, i, K, J) - R3D(Uc, i, K, J - 1)) / DY[J];
+    dor_fft_xz_to_fourier(s, TEMP, 2, NY);
+    for (int K = 0; K <= TNKZ; ++K)
+      for (int i = 0; i <= NXP - 1; ++i)
+        for (int J = 2; J <= NY; ++J)
+          C3(CF2, i, K, J) = C3(CF2, i, K, J) + (c ? CIKZ(K) : CIKX(i)) * C3(CTEMP, i, K, J);
+  }
+  free(U1b); free(U3b); free(DYFW); free(DYW);
+#undef C3
+#undef R3D
+#undef CIKX
+#undef CIKZ
+}
+
 void dor_rk_chan_1(dor_state *s, int RK_STEP) {
   const int NX = s->NX, NY = s->NY, NXP = s->NXP, TNKZ = s->TNKZ, NZP = s->NZP, NXM = s->NXM;
   const int JSTART = s->JSTART, JEND = s->JEND;
@@ -582,7 +819,6 @@ void dor_rk_chan_1(dor_state *s, int RK_STEP) {
   const double TEMP3 = s->ZETA_BAR[rk] * s->H_BAR[rk];
   const double TEMP4 = s->H_BAR[rk];
   const double TEMP5 = s->BETA_BAR[rk] * s->H_BAR[rk];
-  (void)TEMP2;
 #define C3(a, i, k, j) a[CIDX(s, i, k, j)]
 #define R3D(a, i, k, j) a[RIDX(s, i, k, j)]
 #define CIKX(i) (I * KX[i])
@@ -691,10 +927,35 @@ void dor_rk_chan_1(dor_state *s, int RK_STEP) {
   }
   /* USER_RHS_CHAN_FOURIER (channel.f:297): empty in the reference. */
 
-  /* channel.f:386-399: to physical space */
-  dor_fft_xz_to_physical(s, (double *)CU1, 0, NY + 1);
-  dor_fft_xz_to_physical(s, (double *)CU2, 0, NY + 1);
-  dor_fft_xz_to_physical(s, (double *)CU3, 0, NY + 1);
+  /* channel.f:303-391: the subgrid model after a spin-up of 100 steps of a
+   * new flow (it leaves U1..3 physical), else the plain transforms */
+  const int les_now = s->p.les && (!s->p.create_new_flow || s->TIME_STEP > 100);
+  if (les_now) {
+    dor_les_chan(s);
+    apply_bc_vel_phys(s);
+    /* channel.f:313-375: eddy diffusivity KAPPA_T = NU_T of the scalars'
+     * horizontal fluxes, explicit */
+    for (int N = 0; N < s->NTH; ++N) {
+      cplx *CTH = s->CTH + s->cfield * N, *CFTH = s->CFTH + s->cfield * N;
+      for (int d = 0; d < 2; ++d) {
+        for (int J = 1; J <= NY + 1; ++J)
+          for (int K = 0; K <= TNKZ; ++K)
+            for (int i = 0; i <= NXP - 1; ++i) C3(CS1, i, K, J) = (d ? CIKZ(K) : CIKX(i)) * C3(CTH, i, K, J);
+        dor_fft_xz_to_physical(s, (double *)CS1, 0, NY + 1);
+        for (int J = 1; J <= NY + 1; ++J)
+          for (int K = 0; K <= NZP - 1; ++K)
+            for (int i = 0; i <= NXM; ++i) R3D(S1, i, K, J) = R3D(s->NU_T, i, K, J) * R3D(S1, i, K, J);
+        dor_fft_xz_to_fourier(s, S1, 0, NY + 1);
+        for (int J = s->JSTART_TH[N]; J <= s->JEND_TH[N]; ++J)
+          for (int K = 0; K <= TNKZ; ++K)
+            for (int i = 0; i <= NXP - 1; ++i) C3(CFTH, i, K, J) = C3(CFTH, i, K, J) + (d ? CIKZ(K) : CIKX(i)) * C3(CS1, i, K, J);
+      }
+    }
+  } else {
+    dor_fft_xz_to_physical(s, (double *)CU1, 0, NY + 1);
+    dor_fft_xz_to_physical(s, (double *)CU2, 0, NY + 1);
+    dor_fft_xz_to_physical(s, (double *)CU3, 0, NY + 1);
+  }
   for (int N = 0; N < s->NTH; ++N) {
     memcpy(CS1, s->CTH + s->cfield * N, sizeof(cplx) * s->cfield);
     dor_fft_xz_to_physical(s, (double *)CS1, 0, NY + 1);
@@ -814,6 +1075,26 @@ void dor_rk_chan_1(dor_state *s, int RK_STEP) {
         R3D(R2, i, K, J) = R3D(R2, i, K, J) + TEMP1 * NU_V_SCALE *
                                (((R3D(U2, i, K, J + 1) - R3D(U2, i, K, J)) / DYF[J] -
                                  (R3D(U2, i, K, J) - R3D(U2, i, K, J - 1)) / DYF[J - 1]) / DY[J]);
+  /* channel.f:631-659: the eddy viscosity's explicit Crank-Nicolson half
+   * (NU_T at GY; interpolated to GYF for U2) */
+  const double *NU_T = s->NU_T;
+  if (s->p.les) {
+    for (int J = JSTART; J <= JEND; ++J)
+      for (int K = 0; K <= NZP - 1; ++K)
+        for (int i = 0; i <= NXM; ++i) {
+          R3D(R1, i, K, J) = R3D(R1, i, K, J) + TEMP2 * ((R3D(NU_T, i, K, J + 1) * (R3D(U1, i, K, J + 1) - R3D(U1, i, K, J)) / DY[J + 1] -
+                                                          R3D(NU_T, i, K, J) * (R3D(U1, i, K, J) - R3D(U1, i, K, J - 1)) / DY[J]) / DYF[J]);
+          R3D(R3, i, K, J) = R3D(R3, i, K, J) + TEMP2 * ((R3D(NU_T, i, K, J + 1) * (R3D(U3, i, K, J + 1) - R3D(U3, i, K, J)) / DY[J + 1] -
+                                                          R3D(NU_T, i, K, J) * (R3D(U3, i, K, J) - R3D(U3, i, K, J - 1)) / DY[J]) / DYF[J]);
+        }
+    for (int J = 2; J <= NY; ++J)
+      for (int K = 0; K <= NZP - 1; ++K)
+        for (int i = 0; i <= NXM; ++i)
+          R3D(R2, i, K, J) = R3D(R2, i, K, J) +
+                             TEMP2 * ((0.5 * (R3D(NU_T, i, K, J) + R3D(NU_T, i, K, J + 1)) * (R3D(U2, i, K, J + 1) - R3D(U2, i, K, J)) / DYF[J] -
+                                       0.5 * (R3D(NU_T, i, K, J) + R3D(NU_T, i, K, J - 1)) * (R3D(U2, i, K, J) - R3D(U2, i, K, J - 1)) / DYF[J - 1]) /
+                                      DY[J]);
+  }
 
   /* Scalars, channel.f:663-838 */
   for (int N = 0; N < s->NTH; ++N) {
@@ -857,6 +1138,13 @@ void dor_rk_chan_1(dor_state *s, int RK_STEP) {
           R3D(RTH, i, K, J) = R3D(RTH, i, K, J) + (TEMP1 / PR) * NU_V_SCALE *
                                   (((R3D(TH, i, K, J + 1) - R3D(TH, i, K, J)) / DY[J + 1] -
                                     (R3D(TH, i, K, J) - R3D(TH, i, K, J - 1)) / DY[J]) / DYF[J]);
+    /* channel.f:748-760: eddy diffusivity, explicit half */
+    if (s->p.les)
+      for (int J = JS; J <= JE; ++J)
+        for (int K = 0; K <= NZP - 1; ++K)
+          for (int i = 0; i <= NXM; ++i)
+            R3D(RTH, i, K, J) = R3D(RTH, i, K, J) + TEMP2 * ((R3D(NU_T, i, K, J + 1) * (R3D(TH, i, K, J + 1) - R3D(TH, i, K, J)) / DY[J + 1] -
+                                                              R3D(NU_T, i, K, J) * (R3D(TH, i, K, J) - R3D(TH, i, K, J - 1)) / DY[J]) / DYF[J]);
     /* implicit solve, channel.f:771-833 */
 #pragma omp parallel
     {
@@ -872,6 +1160,7 @@ void dor_rk_chan_1(dor_state *s, int RK_STEP) {
             Uu[MI(i, J)] = -(TEMP1 / PR * NU_V_SCALE) / (DY[J + 1] * DYF[J]);
             VEC[MI(i, J)] = R3D(RTH, i, K, J);
           }
+        if (s->p.les) les_rows_gy(s, K, JS, JE, TEMP2, L, D, Uu); /* channel.f:793-807 */
         bc_u13th(NX, NY, DY, s->p.th_bc_ymin[N], s->p.th_bc_ymin_c1[N], s->p.th_bc_ymax[N], s->p.th_bc_ymax_c1[N], L, D, Uu, VEC);
         dor_thomas_real(L, D, Uu, VEC, NY, NX);
         for (int J = JS - 1; J <= JE + 1; ++J)
@@ -900,6 +1189,14 @@ void dor_rk_chan_1(dor_state *s, int RK_STEP) {
               Uu[MI(i, J)] = -TEMP1 * NU_V_SCALE / (DYF[J] * DY[J]);
               VEC[MI(i, J)] = R3D(Rc, i, K, J);
             }
+          if (s->p.les) /* channel.f:862-875 */
+            for (int J = 2; J <= NY; ++J)
+              for (int i = 0; i <= NXM; ++i) {
+                L[MI(i, J)] = L[MI(i, J)] - TEMP2 * 0.5 * (R3D(NU_T, i, K, J) + R3D(NU_T, i, K, J - 1)) / (DYF[J - 1] * DY[J]);
+                D[MI(i, J)] = D[MI(i, J)] + TEMP2 * 0.5 * (R3D(NU_T, i, K, J) + R3D(NU_T, i, K, J + 1)) / (DYF[J] * DY[J]) +
+                              TEMP2 * 0.5 * (R3D(NU_T, i, K, J) + R3D(NU_T, i, K, J - 1)) / (DYF[J - 1] * DY[J]);
+                Uu[MI(i, J)] = Uu[MI(i, J)] - TEMP2 * 0.5 * (R3D(NU_T, i, K, J) + R3D(NU_T, i, K, J + 1)) / (DYF[J] * DY[J]);
+              }
           bc_u2(NX, NY, DYF, s->p.v_bc_ymin, s->p.v_bc_ymin_c1, s->p.v_bc_ymax, s->p.v_bc_ymax_c1, L, D, Uu, VEC);
         } else {
           for (int J = JSTART; J <= JEND; ++J)
@@ -909,6 +1206,7 @@ void dor_rk_chan_1(dor_state *s, int RK_STEP) {
               Uu[MI(i, J)] = -TEMP1 * NU_V_SCALE / (DY[J + 1] * DYF[J]);
               VEC[MI(i, J)] = R3D(Rc, i, K, J);
             }
+          if (s->p.les) les_rows_gy(s, K, JSTART, JEND, TEMP2, L, D, Uu); /* channel.f:927-941, 991-1005 */
           if (comp == 3)
             bc_u13th(NX, NY, DY, s->p.u_bc_ymin, s->p.u_bc_ymin_c1, s->p.u_bc_ymax, s->p.u_bc_ymax_c1, L, D, Uu, VEC);
           else

@@ -46,6 +46,8 @@ typedef struct dor_params {
   double ri[DOR_MAX_TH], pr[DOR_MAX_TH];
   double drhodx[DOR_MAX_TH], drhodz[DOR_MAX_TH];
   int create_new_flow;
+  /* LES (header:168): constant Smagorinsky, LES_MODEL_TYPE=1 (les.f:75-267) */
+  int les;
 } dor_params;
 
 typedef struct dor_state dor_state;
@@ -59,6 +61,7 @@ enum {
   DOR_GY, DOR_GYF, DOR_DY, DOR_DYF,   /* real (0:NY+1) */
   DOR_KX, DOR_KX2, DOR_KZ, DOR_KZ2,   /* real (0:NXP), (0:TNKZ) */
   DOR_HBAR, DOR_BETABAR, DOR_ZETABAR, /* real (3) */
+  DOR_NUT,                            /* real (0:NX+1,0:NZ+1,0:NY+1), LES only */
   DOR_NARRAYS
 };
 
@@ -114,6 +117,14 @@ void dor_create_flow_chan(dor_state *s, int ic_type, double ubulk0,
 /* CREATE_TH_CHAN (set_ics.f:204-273) for ic_type 1..4. */
 void dor_create_th_chan(dor_state *s, int ic_type);
 void dor_apply_bc_vel(dor_state *s);
+/* les_chan (les.f:1-336, one rank, LES_MODEL_TYPE=1): with velocity (and
+ * CF1..3) in Fourier space, sets NU_T and adds the SGS stress divergence to
+ * CF1..3; on return U1..3 are in physical space.  The periodic mean_les
+ * output (les.f:338-517) is left out: it writes a file and changes no state. */
+void dor_les_chan(dor_state *s);
+/* LES filter widths (les.f:196-217), -2 C_SMAG^2 absorbed: dyw at GY,
+ * dyfw at GYF, (0:NY+1), set for J=1..NY+1. */
+void dor_les_widths(dor_state *s, double *dyw, double *dyfw);
 
 /* Plane statistics of SAVE_STATS_CHAN (save_stats.f:136-262) from the
  * current Fourier state, without touching it: out[q*(NY+2) + j] for

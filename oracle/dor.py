@@ -39,11 +39,12 @@ class DorParams(ctypes.Structure):
         ("ri", ctypes.c_double * MAX_TH), ("pr", ctypes.c_double * MAX_TH),
         ("drhodx", ctypes.c_double * MAX_TH), ("drhodz", ctypes.c_double * MAX_TH),
         ("create_new_flow", ctypes.c_int),
+        ("les", ctypes.c_int),
     ]
 
 
 (CU1, CU2, CU3, CP, CR1, CR2, CR3, CF1, CF2, CF3, CS1, CTH, CFTH, CRTH, TH, RTH,
- GY, GYF, DY, DYF, KX, KX2, KZ, KZ2, HBAR, BETABAR, ZETABAR) = range(27)
+ GY, GYF, DY, DYF, KX, KX2, KZ, KZ2, HBAR, BETABAR, ZETABAR, NUT) = range(28)
 
 _lib = None
 
@@ -83,6 +84,7 @@ def load():
         lib.dor_create_flow_chan.argtypes = [vp, ip, ctypes.c_double, ctypes.c_uint64, ctypes.c_double, ctypes.c_double]
         lib.dor_create_th_chan.argtypes = [vp, ip]
         lib.dor_apply_bc_vel.argtypes = [vp]
+        lib.dor_les_chan.argtypes = [vp]
         lib.dor_save_stats.argtypes = [vp, ctypes.c_double, dp, dp]
         lib.dor_gauss.restype = ctypes.c_double
         lib.dor_gauss.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
@@ -231,6 +233,10 @@ class Oracle:
 
     def apply_bc_vel(self):
         self.lib.dor_apply_bc_vel(self.h)
+
+    def les_chan(self):
+        """les_chan: NU_T and the SGS divergence added to CF1..3; U1..3 left physical."""
+        self.lib.dor_les_chan(self.h)
 
     def save_stats(self, ly):
         """SAVE_STATS_CHAN's plane statistics (dict of (NY+2,) arrays) and UBULK."""
