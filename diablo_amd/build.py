@@ -9,7 +9,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libdiablo_hip.so")
-SOURCES = ["dh_api.hip", "dh_fft.hip", "dh_step.hip", "dh_ics.hip", "dh_stats.hip", "dh_poisson.hip", "dh_h5.cpp"]
+SOURCES = ["dh_api.hip", "dh_fft.hip", "dh_step.hip", "dh_ics.hip", "dh_stats.hip", "dh_poisson.hip", "dh_les.hip", "dh_h5.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [

@@ -75,6 +75,8 @@ EXPORTS = {
     "dh_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]),
     "dh_ensemble_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "dh_mixed_precision": (ctypes.c_int, [ctypes.c_void_p]),
+    "dh_set_les": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "dh_les": (ctypes.c_int, [ctypes.c_void_p]),
     "dh_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
     "dh_profile_substep": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, _DP, ctypes.c_int32]),
     "dh_phase_name": (ctypes.c_char_p, [ctypes.c_int32]),
@@ -201,6 +203,8 @@ class Channel:
         self.tnkz = 2 * self.nkz
         if gy is not None:
             self.set_grid(gy)
+        if params.get("les"):
+            self.set_les(True)
 
     def _check(self, rc, what):
         if rc != DH_OK:
@@ -280,6 +284,14 @@ class Channel:
     @property
     def mixed_precision(self):
         return bool(self.lib.dh_mixed_precision(self.h))
+
+    def set_les(self, on):
+        """Constant-Smagorinsky LES (the reference's LES = .TRUE., LES_MODEL_TYPE = 1)."""
+        self._check(self.lib.dh_set_les(self.h, 1 if on else 0), "dh_set_les")
+
+    @property
+    def les(self):
+        return bool(self.lib.dh_les(self.h))
 
     @property
     def plane_threads(self):

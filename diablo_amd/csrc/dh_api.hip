@@ -133,7 +133,7 @@ hipError_t dalloc(T **p, size_t count) {
 void free_all(Ctx &c) {
   void *ptrs[] = {c.tw_row, c.tw_col, c.tw_real, c.posr, c.posrd, c.posz, c.posc, c.kx, c.kz, c.kx2, c.kz2, c.d_dy, c.d_dyf, c.d_gyf, c.d_rdy, c.d_rdyf, c.cu[0], c.cu[1],
                   c.cu[2], c.cp, c.cf[0], c.cf[1], c.cf[2], c.cth, c.cfth, c.phi, c.uphys, c.rphys, c.rd_ta,
-                  c.rd_tb, c.clock, c.fac.pf, c.fac.pb, c.fac.tl, c.fac.tb, c.fac.tc, c.fac.tr, c.fac.isbc, c.rd_tr, c.fac.vconst, c.rd_l, c.rd_u, c.stats, c.cun[0], c.cun[1], c.cun[2], c.uphys32, c.rphys32, c.fac.bcl, c.fac.bcd, c.fac.bcu};
+                  c.rd_tb, c.clock, c.fac.pf, c.fac.pb, c.fac.tl, c.fac.tb, c.fac.tc, c.fac.tr, c.fac.isbc, c.rd_tr, c.fac.vconst, c.rd_l, c.rd_u, c.stats, c.cun[0], c.cun[1], c.cun[2], c.uphys32, c.rphys32, c.fac.bcl, c.fac.bcd, c.fac.bcu, c.nut, c.cul, c.les_four, c.les_phys, c.sgs, c.les_wk, c.d_lesw};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c.step_graph) (void)hipGraphExecDestroy(c.step_graph);
@@ -415,6 +415,8 @@ int dh_destroy(dh_ctx *h) {
 
 const char *dh_last_error(dh_ctx *h) { return h ? h->c.err.c_str() : "null context"; }
 
+static hipError_t upload_les_widths(Ctx &c);
+
 int dh_set_grid(dh_ctx *h, const double *gy, const double *gyf, const double *dy, const double *dyf) {
   if (!h || !gy || !gyf || !dy || !dyf) return DH_EARG;
   Ctx &c = h->c;
@@ -428,6 +430,7 @@ int dh_set_grid(dh_ctx *h, const double *gy, const double *gyf, const double *dy
   HIPCHK(hipMemcpy(c.d_dy, dy, nyy * 8, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c.d_dyf, dyf, nyy * 8, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c.d_gyf, gyf, nyy * 8, hipMemcpyHostToDevice));
+  if (c.d_lesw) HIPCHK(upload_les_widths(c));
   {
     std::vector<double> r(nyy), rf(nyy);
     for (int j = 0; j < nyy; ++j) {
@@ -669,6 +672,7 @@ int dh_set_mixed_precision(dh_ctx *h, int32_t on) {
   if (!h || (on != 0 && on != 1)) return DH_EARG;
   Ctx &c = h->c;
   if ((on != 0) == c.mixed) return DH_OK;
+  if (on && c.les) return fail(c, DH_EARG, "the mixed form does not carry the LES model (FP64 only)");
   HIPCHK(hipStreamSynchronize(c.stream));
   if (on && !c.uphys32) {
     const size_t n = (size_t)(c.g.ny + 2) * (size_t)c.g.pp * (size_t)c.nfields;
@@ -681,6 +685,44 @@ int dh_set_mixed_precision(dh_ctx *h, int32_t on) {
 }
 
 int dh_mixed_precision(dh_ctx *h) { return h ? (h->c.mixed ? 1 : 0) : DH_EARG; }
+
+// LES filter widths (les.f:196-217) with -2 C_SMAG^2 absorbed, C_SMAG =
+// 0.13, beta_sgs = 1: DYW(J) at GY, DYFW(J) at GYF for J = 1..NY+1, from the
+// host's DY, DYF so they match the reference's real powers.
+static hipError_t upload_les_widths(Ctx &c) {
+  const int nyy = c.g.ny + 2;
+  std::vector<double> w(2 * (size_t)nyy, 0.0);
+  const double C_SMAG = 0.13, beta_sgs = 1.0, dx = c.p.lx / c.g.nx, dz = c.p.lz / c.g.nz;
+  for (int J = 1; J <= c.g.ny + 1; ++J) {
+    w[J] = -2.0 * pow(C_SMAG, 2.0) * pow(dx * beta_sgs * c.dy[J] * 2.0 * dz * beta_sgs, 2.0 / 3.0);
+    w[nyy + J] = -2.0 * pow(C_SMAG, 2.0) * pow(dx * beta_sgs * c.dyf[J] * 2.0 * dz * beta_sgs, 2.0 / 3.0);
+  }
+  return hipMemcpy(c.d_lesw, w.data(), w.size() * 8, hipMemcpyHostToDevice);
+}
+
+int dh_set_les(dh_ctx *h, int32_t on) {
+  if (!h || (on != 0 && on != 1)) return DH_EARG;
+  Ctx &c = h->c;
+  if ((on != 0) == c.les) return DH_OK;
+  if (on && c.mixed) return fail(c, DH_EARG, "LES runs in FP64: switch the mixed form off first");
+  HIPCHK(hipStreamSynchronize(c.stream));
+  if (on && !c.nut) {
+    const size_t nyy = (size_t)c.g.ny + 2, pp = (size_t)c.g.pp, pf = (size_t)c.g.pf, nth = (size_t)c.p.n_th;
+    HIPCHK(dalloc(&c.nut, nyy * pp));
+    HIPCHK(dalloc(&c.cul, 3 * nyy * pf));
+    HIPCHK(dalloc(&c.les_four, (kLesFour + 2 * nth) * nyy * pf));
+    HIPCHK(dalloc(&c.les_phys, (kLesPhys + 2 * nth) * nyy * pp));
+    HIPCHK(dalloc(&c.sgs, (3 + nth) * nyy * pf));
+    HIPCHK(dalloc(&c.les_wk, 3 * (size_t)c.nfields * nyy * pp));
+    HIPCHK(dalloc(&c.d_lesw, 2 * nyy));
+  }
+  if (on && c.grid_set) HIPCHK(upload_les_widths(c));
+  c.les = (on != 0);
+  drop_graphs(c);  // the captured steps launch the other path's kernels
+  return DH_OK;
+}
+
+int dh_les(dh_ctx *h) { return h ? (h->c.les ? 1 : 0) : DH_EARG; }
 
 static int enqueue_steps(Ctx &c, int nsteps);
 
@@ -954,6 +996,8 @@ void free_groups(dh_ens &e) {
 
 // Groups the members by recorded step and uploads their argument structs.
 hipError_t build_groups(dh_ens &e) {
+  for (dh_ctx *h : e.m)
+    if (h->c.les) return hipErrorNotSupported;  // the LES kernels have no batched forms
   std::vector<StepRec> recs(e.m.size());
   for (size_t i = 0; i < e.m.size(); ++i) {
     const hipError_t r = record_step(e.m[i]->c, recs[i]);
@@ -1327,6 +1371,7 @@ double *dh_field_dev(dh_ctx *h, int32_t which) {
   if (which >= 100 && which < 100 + c.nfields) return c.uphys + (which - 100) * fs;
   if (which >= 200 && which < 200 + c.nfields && c.uphys32)
     return reinterpret_cast<double *>(c.uphys32 + (which - 200) * fs);
+  if (which == 300) return c.nut;
   return nullptr;
 }
 

@@ -150,6 +150,38 @@ static hipError_t plane_fwd(Ctx &c, dim3 grid, hipStream_t st, const PlaneBatch 
                  dim3(c.plane_threads), lds, st, pa);
 }
 
+hipError_t launch_planes_inv(Ctx &c, const double2 *four, double *phys, int nf, int j0, int nplanes, hipStream_t st) {
+  const long long fs = (long long)(c.g.ny + 2) * c.g.pp, cfs = (long long)(c.g.ny + 2) * c.g.pf;
+  for (int f0 = 0; f0 < nf; f0 += kMaxFields) {
+    const int n = nf - f0 < kMaxFields ? nf - f0 : kMaxFields;
+    PlaneBatch b{};
+    for (int i = 0; i < n; ++i) {
+      b.four[i] = four + (f0 + i) * cfs;
+      b.phys[i] = phys + (f0 + i) * fs;
+    }
+    b.j0 = j0;
+    const hipError_t e = plane_inv<double2>(c, dim3(nplanes, n), st, b);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+hipError_t launch_planes_fwd(Ctx &c, const double *phys, double2 *four, int nf, int j0, int nplanes, hipStream_t st) {
+  const long long fs = (long long)(c.g.ny + 2) * c.g.pp, cfs = (long long)(c.g.ny + 2) * c.g.pf;
+  for (int f0 = 0; f0 < nf; f0 += kMaxFields) {
+    const int n = nf - f0 < kMaxFields ? nf - f0 : kMaxFields;
+    PlaneBatch b{};
+    for (int i = 0; i < n; ++i) {
+      b.four[i] = four + (f0 + i) * cfs;
+      b.phys[i] = const_cast<double *>(phys) + (f0 + i) * fs;
+    }
+    b.j0 = j0;
+    const hipError_t e = plane_fwd<double2>(c, dim3(nplanes, n), st, b);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
 hipError_t launch_fft_to_phys(Ctx &c, const double2 *four, double *phys, int nplanes) {
   PlaneBatch b{};
   b.four[0] = four;
@@ -173,7 +205,8 @@ hipError_t launch_state_to_phys(Ctx &c) {
   const long long fs = (long long)(c.g.ny + 2) * c.g.pp;
   const long long cfs = (long long)(c.g.ny + 2) * c.g.pf;
   for (int f = 0; f < c.nfields; ++f) {
-    b.four[f] = (f < 3) ? c.cu[f] : c.cth + (f - 3) * cfs;
+    // LES: velocity from the copy of CU that carries les_chan's BCs
+    b.four[f] = (f < 3) ? (c.les ? c.cul + f * cfs : c.cu[f]) : c.cth + (f - 3) * cfs;
     b.phys[f] = c.mixed ? (void *)(c.uphys32 + f * fs) : (void *)(c.uphys + f * fs);
   }
   b.j0 = 0;

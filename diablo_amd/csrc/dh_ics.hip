@@ -108,10 +108,8 @@ struct BcArgs {
 };
 
 // APPLY_BC_VEL_LOWER/UPPER in Fourier space, one thread per mode (RANKZ=0).
-__global__ void k_apply_bc_vel(BcArgs a) {
+__device__ void apply_bc_vel_mode(const BcArgs &a, int m) {
   const Geo &g = a.g;
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= g.pf) return;
   const long long pf = g.pf;
   const int NY = g.ny;
   const bool mean = (m == 0);
@@ -192,6 +190,57 @@ __global__ void k_apply_bc_vel(BcArgs a) {
   }
 }
 
+__global__ void k_apply_bc_vel(BcArgs a) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m < a.g.pf) apply_bc_vel_mode(a, m);
+}
+
+// The opening of les_chan (les.f:62-73) when the model runs (channel.f:303:
+// not a new flow, or past step 100): APPLY_BC_VEL_MPI, then APPLY_BC_LES
+// (les.f:805-862), whose Neumann walls copy the first interior plane into
+// the ghost planes of U1 and U3 so the model sees no wall gradient.  They
+// act on a copy of CU (b.cu1..3), from which the substep's physical U and
+// the model are formed: the explicit Fourier terms formed before les_chan
+// (channel.f:88-290) see CU without them, and at the end of the substep CU
+// is made anew from the physical fields.  While the gate is closed the copy
+// is CU itself.
+struct LesBcArgs {
+  BcArgs b;
+  const double2 *cu[3];
+  const Clock *clk;
+  int cnf;
+};
+__global__ void k_les_bc_four(LesBcArgs la) {
+  const BcArgs &a = la.b;
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= a.g.pf) return;
+  const long long pf = a.g.pf;
+  double2 *dst[3] = {a.cu1, a.cu2, a.cu3};
+  for (int f = 0; f < 3; ++f)
+    for (int j = 0; j < a.g.ny + 2; ++j) dst[f][pf * j + m] = la.cu[f][pf * j + m];
+  if (la.cnf && la.clk->time_step <= 100) return;
+  apply_bc_vel_mode(a, m);
+  const int NY = a.g.ny;
+  double2 *U1 = a.cu1 + m, *U3 = a.cu3 + m;
+  auto at = [&](double2 *F, int j) -> double2 & { return F[pf * j]; };
+  if (a.ubmax == 1) {
+    at(U1, NY) = at(U1, NY - 1);
+    at(U1, NY + 1) = at(U1, NY);
+  }
+  if (a.wbmax == 1) {
+    at(U3, NY) = at(U3, NY - 1);
+    at(U3, NY + 1) = at(U3, NY);
+  }
+  if (a.ubmin == 1) {
+    at(U1, 1) = at(U1, 2);
+    at(U1, 0) = at(U1, 1);
+  }
+  if (a.wbmin == 1) {
+    at(U3, 1) = at(U3, 2);
+    at(U3, 0) = at(U3, 1);
+  }
+}
+
 __global__ void k_fill_zero(double2 *p, long long n) {
   const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e < n) p[e] = make_double2(0.0, 0.0);
@@ -214,7 +263,7 @@ static IcArgs ic_args(Ctx &c) {
   return a;
 }
 
-hipError_t launch_apply_bc_vel(Ctx &c) {
+static BcArgs bc_args(Ctx &c) {
   BcArgs b{};
   b.g = c.g;
   b.cu1 = c.cu[0];
@@ -227,8 +276,22 @@ hipError_t launch_apply_bc_vel(Ctx &c) {
   b.ubmax = p.u_bc_ymax; b.vbmax = p.v_bc_ymax; b.wbmax = p.w_bc_ymax;
   b.uc1min = p.u_bc_ymin_c1; b.vc1min = p.v_bc_ymin_c1; b.wc1min = p.w_bc_ymin_c1;
   b.uc1max = p.u_bc_ymax_c1; b.vc1max = p.v_bc_ymax_c1; b.wc1max = p.w_bc_ymax_c1;
+  return b;
+}
+
+hipError_t launch_apply_bc_vel(Ctx &c) {
+  const BcArgs b = bc_args(c);
   hipLaunchKernelGGL(k_apply_bc_vel, dim3((c.g.pf + 63) / 64), dim3(64), 0, c.stream, b);
   return hipGetLastError();
+}
+
+hipError_t launch_les_bc_four(Ctx &c, hipStream_t st) {
+  LesBcArgs la{bc_args(c), {c.cu[0], c.cu[1], c.cu[2]}, c.clock, c.p.create_new_flow};
+  const long long cfs = (long long)(c.g.ny + 2) * c.g.pf;
+  la.b.cu1 = c.cul;
+  la.b.cu2 = c.cul + cfs;
+  la.b.cu3 = c.cul + 2 * cfs;
+  return klaunch(c, (const void *)k_les_bc_four, nullptr, dim3((c.g.pf + 63) / 64), dim3(64), 0, st, la);
 }
 
 hipError_t launch_create_flow(Ctx &c, int ic_type, double ubulk0, unsigned long long seed, double amp, double ymask) {

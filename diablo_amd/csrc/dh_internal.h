@@ -171,7 +171,25 @@ struct Ctx {
   // recording mode (see RecOp)
   bool recording = false;
   std::vector<RecOp> rec;
+  // constant-Smagorinsky LES (les.f:1-336): NU_T at GY points [NY+2][NZ][NX]
+  // (also the scalars' KAPPA_T), the model's Fourier and physical work
+  // fields, the subgrid increments of CF1..3 and CFTH the nonlinear pass
+  // adds, the per-column solve's sweep storage, and the filter widths
+  bool les = false;
+  double *nut = nullptr;
+  double2 *cul = nullptr;       // CU1..3 with les_chan's BCs, [3][NY+2][pf]
+  double2 *les_four = nullptr;  // [kLesFour + 2 N_TH][NY+2][pf]
+  double *les_phys = nullptr;   // [kLesPhys + 2 N_TH][NY+2][NZ][NX]
+  double2 *sgs = nullptr;       // [3 + N_TH][NY+2][pf]
+  double *les_wk = nullptr;     // [2][nfields][NY+2][NZ][NX]
+  double *d_lesw = nullptr;     // DYW (0:NY+1) then DYFW (0:NY+1)
 };
+
+// Work fields of the LES pass: Fourier slots 0..6 (S11, S22, S33, S4, S5,
+// S6 in, then |S| S products and cross terms out), 7 + 2n and 8 + 2n for
+// scalar n; physical slots 0..5 the strain, 7 + 2n, 8 + 2n the scalar
+// gradients, 7 + 2 N_TH + 0..6 the products.
+constexpr int kLesFour = 7, kLesPhys = 14;
 
 inline int stream_role(const Ctx &c, hipStream_t st) { return st == c.branch ? 1 : (st == c.side ? 2 : 0); }
 inline hipEvent_t event_of(const Ctx &c, int ev) {
@@ -237,6 +255,16 @@ hipError_t launch_fields_to_four(Ctx &c, int f0, int f1);
 hipError_t launch_step_to_four(Ctx &c, int f0, int f1, hipStream_t st);  // end of a substep (into CUN / CTH)
 // ---- launchers (dh_ics.hip) ----
 hipError_t launch_apply_bc_vel(Ctx &c);
+// LES (les.f): the gated Fourier BCs that open les_chan, before the state
+// goes to physical space; the model itself (strain, NU_T, subgrid
+// increments, physical BCs) after it; the per-column solve with NU_T
+hipError_t launch_les_bc_four(Ctx &c, hipStream_t st);
+hipError_t launch_les_model(Ctx &c, hipStream_t st);
+hipError_t launch_solve_les_range(Ctx &c, int rk, int courant, int f0, int f1, hipStream_t st);
+// fields f0..f0+nf-1 of strided work arrays (Fourier stride (NY+2) pf,
+// physical (NY+2) NX NZ), planes j0..j0+nplanes-1, in batches of kMaxFields
+hipError_t launch_planes_inv(Ctx &c, const double2 *four, double *phys, int nf, int j0, int nplanes, hipStream_t st);
+hipError_t launch_planes_fwd(Ctx &c, const double *phys, double2 *four, int nf, int j0, int nplanes, hipStream_t st);
 hipError_t launch_create_flow(Ctx &c, int ic_type, double ubulk0, unsigned long long seed, double amp, double ymask);
 hipError_t launch_create_th(Ctx &c, int ic_type);
 // ---- launchers (dh_step.hip) ----

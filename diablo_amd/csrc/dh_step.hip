@@ -38,6 +38,7 @@ struct StepArgs {
   int f_type, n_th;
   double ri[DH_MAX_TH], pr[DH_MAX_TH], drhodx[DH_MAX_TH], drhodz[DH_MAX_TH];
   int jstart_th[DH_MAX_TH], jend_th[DH_MAX_TH];
+  const double2 *sgs;  // LES: subgrid increments of CF1..3, CFTH (null without LES)
 };
 
 __device__ __forceinline__ double kpow(double x, double beta) { return (beta == 1.0) ? x : pow(x, beta); }
@@ -213,16 +214,21 @@ __device__ __forceinline__ double2 cf_init(const StepArgs &a, int c, int J, int 
 // before their arithmetic, so the modes' memory latencies overlap; other
 // configurations take cf_init mode by mode.  Indices beyond the plane are
 // clamped for the loads and their results never stored.
-template <int NQ, class C>
+// SGS: LES, whose subgrid increments (les.f:280-336; channel.f:346-372
+// for the scalars) join the explicit terms before the products, as
+// les_chan adds them to CF ahead of the nonlinear terms.
+template <int NQ, bool SGS, class C>
 __device__ __forceinline__ void cf_init_all(const StepArgs &a, int c, int J, int nthr, C (&acc)[NQ]) {
   const Geo &g = a.g;
-  if (c >= 3 || a.n_th > 0 || a.beta != 1.0) {
+  if (SGS || c >= 3 || a.n_th > 0 || a.beta != 1.0) {
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
       const int m = threadIdx.x + q * nthr;
       if (m < g.pf) {
         const int kzp = (int)fdiv((unsigned)m, g.f_nkx1), kx = m - kzp * g.nkx1;
-        acc[q] = cvt<C>(cf_init(a, c, J, kx, kzp, m));
+        double2 cf = cf_init(a, c, J, kx, kzp, m);
+        if constexpr (SGS) cf = cadd(cf, a.sgs[c * a.cfs + (long long)J * g.pf + m]);
+        acc[q] = cvt<C>(cf);
       }
     }
     return;
@@ -327,7 +333,7 @@ __device__ double2 mean_terms(const StepArgs &a, int c, int J, long long o, bool
 // C: complex type of the plane and the physical arrays (double2, or float2
 // in the mixed form, whose products and transforms run in FP32 while the
 // Fourier state, the accumulation of CF and the RK combination stay FP64).
-template <bool REG, int NQ, class C>
+template <bool REG, int NQ, class C, bool SGS = false>
 __device__ __forceinline__ void nonlinear_pass(const StepArgs &a, C *A) {
   const Geo &g = a.g;
   const auto T = load_tables(A, g);
@@ -358,7 +364,7 @@ __device__ __forceinline__ void nonlinear_pass(const StepArgs &a, C *A) {
   const int nthr = blockDim.x;
 
   C acc[NQ];  // CF of this thread's modes (FP64, or FP32 in the mixed form)
-  cf_init_all<NQ>(a, c, J, nthr, acc);
+  cf_init_all<NQ, SGS>(a, c, J, nthr, acc);
   constexpr bool DELTA = sizeof(C) != sizeof(double2);  // the mixed form transforms increments
   if constexpr (DELTA) {
     // the product terms with a plane-mean factor, in FP64, before the
@@ -564,7 +570,9 @@ __device__ __forceinline__ void nonlinear_pass(const StepArgs &a, C *A) {
         const double2 cfn = make_double2(acc[q0 + b].x, acc[q0 + b].y);
         CF[o] = cfn;
         cr = make_double2(cr.x + T5 * cfn.x, cr.y + T5 * cfn.y);
-        cr = make_double2(cr.x + ev.x, cr.y + ev.y);
+        // LES takes the explicit y-viscous half in physical space, from U with
+        // les_chan's BCs (channel.f:606-626), in its column solve
+        if constexpr (!SGS) cr = make_double2(cr.x + ev.x, cr.y + ev.y);
         if (DELTA) cr = make_double2(cr.x + ev.x, cr.y + ev.y);
         A[inv_row(T, kz_row(g, kzp)) * g.ls + kx] = cvt<C>(cr);
       }
@@ -580,10 +588,10 @@ __device__ __forceinline__ void nonlinear_pass(const StepArgs &a, C *A) {
     plane_store_phys(A, g, T, dst);
   }
 }
-template <bool REG, int NQ>
+template <bool REG, int NQ, bool SGS = false>
 __global__ void __launch_bounds__(REG ? 512 : 1024) k_nonlinear(StepArgs a) {
   extern __shared__ double2 smem[];
-  nonlinear_pass<REG, NQ, double2>(a, smem);
+  nonlinear_pass<REG, NQ, double2, SGS>(a, smem);
 }
 
 // The mixed form: the FP32 plane takes half the LDS and half the registers
@@ -1407,6 +1415,7 @@ static StepArgs make_step_args(Ctx &c, int rk) {
   a.cfs = (long long)(c.g.ny + 2) * c.g.pf;
   a.c0 = 0;
   a.j0 = 1;
+  a.sgs = c.les ? c.sgs : nullptr;
   const dh_params &p = c.p;
   a.nu = p.nu;
   a.nu_v_scale = p.nu_v_scale;
@@ -1488,6 +1497,13 @@ static hipError_t launch_nonlinear_range(Ctx &c, int rk, int c0, int c1, hipStre
                      dim3(kNlRegThreads), c.lds_bytes32, st, a);
     return klaunch(c, (const void *)k_nonlinear32<false, kMaxQ>, (const void *)k_nonlinear32_b<false, kMaxQ>, grid,
                    dim3(c.plane_threads), c.lds_bytes32, st, a);
+  }
+  if (c.les) {  // FP64 only; the LES kernels have no batched forms
+    if (c.reg_rows_nl)
+      return klaunch(c, (const void *)k_nonlinear<true, kNlRegQ, true>, nullptr, grid, dim3(kNlRegThreads),
+                     c.lds_bytes, st, a);
+    return klaunch(c, (const void *)k_nonlinear<false, kMaxQ, true>, nullptr, grid, dim3(c.plane_threads), c.lds_bytes,
+                   st, a);
   }
   if (c.reg_rows_nl)
     return klaunch(c, (const void *)k_nonlinear<true, kNlRegQ>, (const void *)k_nonlinear_b<true, kNlRegQ>, grid,
@@ -1589,6 +1605,157 @@ static hipError_t launch_solve(Ctx &c, int rk, int courant) {
   return launch_solve_range(c, rk, courant, 0, c.nfields, c.stream);
 }
 
+// LES: the implicit solves with the eddy viscosity (channel.f:631-659 and
+// 748-760 explicit, 793-807, 862-875, 927-941, 991-1005 implicit): with
+// NU_T varying over (x, z) every column has its own matrix, so one thread
+// builds and sweeps one column (THOMAS_FORWARD/BACKWARD_REAL_MPI on one
+// rank, mpi.f:630-703, 767-809, with the B = 0 guard), keeping the pivots,
+// right-hand sides and upper diagonal in work arrays between the sweeps.
+// Rows are those of real_row; interior rows (the ones no BC overwrites) add
+// the NU_T parts of the reference's MATL/MATD/MATU, NU_T interpolated to GYF
+// for U2, KAPPA_T = NU_T for the scalars.  Their right-hand sides take the
+// whole explicit half here, in physical space as the reference forms it
+// (the nonlinear pass leaves it out with LES): the molecular part from U
+// after les_chan's physical BCs, then the eddy-viscosity part.
+struct LesSolveArgs {
+  Geo g;
+  Clock *clk;
+  int rk, f0;
+  FactorIn in;
+  const double *rphys;  // R of every field (FP64)
+  const double *uphys;  // U and TH at the start of the substep
+  double *dst[kMaxFields];
+  const double *nut;
+  double *wk;           // [3][nfields][NY+2][NZ][NX]
+  long long fs;
+  int wlo[kMaxFields], whi[kMaxFields];
+  int courant;
+  double cfl, dx, dz;
+};
+
+__global__ void __launch_bounds__(256) k_solve_les(LesSolveArgs a) {
+#pragma clang fp contract(off)
+  __shared__ double red[256 / 64];
+  const Geo &g = a.g;
+  const int f = a.f0 + blockIdx.y;
+  const long long pp = g.pp, col = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = col < pp;
+  const int NY = g.ny, nrow = NY + 2;
+  const double temp1 = a.in.nu * a.clk->h_bar[a.rk] / 2.0;
+  const double temp2 = a.clk->h_bar[a.rk] / 2.0;
+  const double *DY = a.in.dy, *DYF = a.in.dyf;
+  const bool cour = a.courant && f < 3;
+  const double dxf = (f == 0) ? a.dx : a.dz;
+  double dtmin = __longlong_as_double(0x7FF0000000000000ll);
+  if (live) {
+    const double *U = a.uphys + f * a.fs + col, *R = a.rphys + f * a.fs + col, *NT = a.nut + col;
+    const long long ws = (long long)a.in.nfields * nrow * pp;
+    double *Bw = a.wk + (long long)f * nrow * pp + col, *Gw = Bw + ws, *Cw = Gw + ws;
+    auto row = [&](int J, double &L, double &D, double &C, double &V) {
+      int bc;
+      real_row(a.in, f, temp1, J, L, D, C, V, bc);
+      if (bc) return;
+      const double u0 = U[J * pp], um = U[(J - 1) * pp], up = U[(J + 1) * pp];
+      const double n0 = NT[J * pp], nm = NT[(J - 1) * pp], np = NT[(J + 1) * pp];
+      V = R[J * pp];
+      // the explicit Crank-Nicolson half: molecular (channel.f:606-626,
+      // 739-746), then the eddy viscosity or diffusivity
+      if (f == 1)
+        V = V + temp1 * a.in.nu_v_scale * (((up - u0) / DYF[J] - (u0 - um) / DYF[J - 1]) / DY[J]);
+      else if (f < 3)
+        V = V + temp1 * a.in.nu_v_scale * (((up - u0) / DY[J + 1] - (u0 - um) / DY[J]) / DYF[J]);
+      else
+        V = V + (temp1 / a.in.pr[f - 3]) * a.in.nu_v_scale * (((up - u0) / DY[J + 1] - (u0 - um) / DY[J]) / DYF[J]);
+      if (f == 1) {
+        V = V + temp2 * ((0.5 * (n0 + np) * (up - u0) / DYF[J] - 0.5 * (n0 + nm) * (u0 - um) / DYF[J - 1]) / DY[J]);
+        L = L - temp2 * 0.5 * (n0 + nm) / (DYF[J - 1] * DY[J]);
+        D = D + temp2 * 0.5 * (n0 + np) / (DYF[J] * DY[J]) + temp2 * 0.5 * (n0 + nm) / (DYF[J - 1] * DY[J]);
+        C = C - temp2 * 0.5 * (n0 + np) / (DYF[J] * DY[J]);
+      } else {
+        V = V + temp2 * ((np * (up - u0) / DY[J + 1] - n0 * (u0 - um) / DY[J]) / DYF[J]);
+        L = L - temp2 * n0 / (DY[J] * DYF[J]);
+        D = D + temp2 * np / (DY[J + 1] * DYF[J]) + temp2 * n0 / (DY[J] * DYF[J]);
+        C = C - temp2 * np / (DY[J + 1] * DYF[J]);
+      }
+    };
+    double L, D, C, V;
+    row(0, L, D, C, V);
+    double bp = D, gp = V, cp = C;
+    Bw[0] = bp;
+    Gw[0] = gp;
+    Cw[0] = cp;
+    for (int J = 1; J <= NY + 1; ++J) {
+      row(J, L, D, C, V);
+      const double am = -L / bp;
+      D = D + am * cp;
+      V = V + am * gp;
+      Bw[J * pp] = D;
+      Gw[J * pp] = V;
+      Cw[J * pp] = C;
+      bp = D;
+      gp = V;
+      cp = C;
+    }
+    double *Ub = a.dst[f] + col;
+    const int wlo = a.wlo[f], whi = a.whi[f];
+    double xn = gp / bp;  // row NY+1
+    for (int J = NY + 1; J >= 0; --J) {
+      double x;
+      if (J == NY + 1) {
+        x = xn;
+      } else {
+        const double b = Bw[J * pp];
+        x = (b != 0.0) ? (Gw[J * pp] - Cw[J * pp] * xn) / b : 0.0;
+      }
+      xn = x;
+      if (J >= wlo && J <= whi) {
+        Ub[J * pp] = x;
+        if (cour && J >= 1 && J <= NY) {
+          const double dl = (f == 1) ? g.dy[J] : dxf;
+          dtmin = fmin(dtmin, a.cfl * dl / fabs(x));
+        }
+      }
+    }
+  }
+  if (a.courant && f < 3) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int o = 32; o > 0; o >>= 1) dtmin = fmin(dtmin, __shfl_down(dtmin, o));
+    if (lane == 0) red[w] = dtmin;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double m = red[0];
+      for (int q = 1; q < (int)(blockDim.x >> 6); ++q) m = fmin(m, red[q]);
+      if (m >= 0.0) atomicMin(&a.clk->cfl_min_bits, (unsigned long long)__double_as_longlong(m));
+    }
+  }
+}
+
+hipError_t launch_solve_les_range(Ctx &c, int rk, int courant, int f0, int f1, hipStream_t st) {
+  if (f1 <= f0) return hipSuccess;
+  LesSolveArgs a{};
+  a.g = c.g;
+  a.clk = c.clock;
+  a.rk = rk;
+  a.f0 = f0;
+  a.in = make_factor_in(c);
+  a.rphys = c.rphys;
+  a.uphys = c.uphys;
+  a.fs = (long long)(c.g.ny + 2) * c.g.pp;
+  for (int f = 0; f < c.nfields; ++f) {
+    a.dst[f] = (f < 3 ? c.rphys : c.uphys) + f * a.fs;
+    a.wlo[f] = c.wlo[f];
+    a.whi[f] = c.whi[f];
+  }
+  a.nut = c.nut;
+  a.wk = c.les_wk;
+  a.courant = courant;
+  a.cfl = c.p.cfl;
+  a.dx = c.p.lx / c.g.nx;
+  a.dz = c.p.lz / c.g.nz;
+  const dim3 grid((unsigned)((c.g.pp + 255) / 256), f1 - f0);
+  return klaunch(c, (const void *)k_solve_les, nullptr, grid, dim3(256), 0, st, a);
+}
+
 static size_t rem_div_thomas_lds(const Ctx &c) {
   const size_t ntot = c.g.ny + 1 + 2 * kRdtPad;
   return ntot * kRdtModes * 5 * sizeof(double) + ntot * sizeof(double);
@@ -1650,9 +1817,14 @@ const char *const kPhaseNames[kNumPhases] = {"to_physical", "nonlinear", "cn_sol
 hipError_t launch_phase(Ctx &c, int rk, int phase) {
   const int courant = (c.p.variable_dt && rk == 2) ? 1 : 0;
   switch (phase) {
-    case 0: return launch_state_to_phys(c);
+    case 0: {
+      hipError_t e = c.les ? launch_les_bc_four(c, c.stream) : hipSuccess;
+      if (e == hipSuccess) e = launch_state_to_phys(c);
+      if (e == hipSuccess && c.les) e = launch_les_model(c, c.stream);
+      return e;
+    }
     case 1: return launch_nonlinear(c, rk);
-    case 2: return launch_solve(c, rk, courant);
+    case 2: return c.les ? launch_solve_les_range(c, rk, courant, 0, c.nfields, c.stream) : launch_solve(c, rk, courant);
     case 3: return courant ? launch_courant(c) : hipSuccess;
     case 4: return launch_step_to_four(c, 0, c.nfields, c.stream);
     case 5: return launch_rem_div(c, rk);
@@ -1679,7 +1851,9 @@ hipError_t launch_substep(Ctx &c, int rk) {
   hipError_t e;
 #define SCHK(expr) \
   if ((e = (expr)) != hipSuccess) return e;
+  if (c.les) SCHK(launch_les_bc_four(c, c.stream));
   SCHK(launch_state_to_phys(c));
+  if (c.les) SCHK(launch_les_model(c, c.stream));
   // the mixed form's nonlinear blocks fit two to a CU: all components in
   // one launch; the FP64 blocks take a CU each, so U3 and the scalars run on
   // the branch beside the solve of U1 and U2
@@ -1687,10 +1861,11 @@ hipError_t launch_substep(Ctx &c, int rk) {
   SCHK(krecord(c, 0, c.stream));  // ev_nl01
   SCHK(kwait(c, c.branch, 0));
   if (!c.mixed) SCHK(launch_nonlinear_range(c, rk, 2, c.nfields, c.branch));
-  SCHK(launch_solve_range(c, rk, courant, 2, c.nfields, c.branch));
+  SCHK(c.les ? launch_solve_les_range(c, rk, courant, 2, c.nfields, c.branch)
+             : launch_solve_range(c, rk, courant, 2, c.nfields, c.branch));
   SCHK(launch_step_to_four(c, 2, nv, c.branch));
   SCHK(krecord(c, 1, c.branch));  // ev_branch_done
-  SCHK(launch_solve_range(c, rk, courant, 0, 2, c.stream));
+  SCHK(c.les ? launch_solve_les_range(c, rk, courant, 0, 2, c.stream) : launch_solve_range(c, rk, courant, 0, 2, c.stream));
   SCHK(launch_step_to_four(c, 0, 2, c.stream));
   SCHK(kwait(c, c.stream, 1));
   if (courant) SCHK(launch_courant(c));
@@ -1725,6 +1900,7 @@ hipError_t launch_step_boundary(Ctx &c, bool end_prev, bool begin_next) {
 hipError_t prepare_step_kernels(Ctx &c) {
   if (c.lds_bytes > kLdsMax || (long long)rem_div_thomas_lds(c) > kLdsMax) return hipErrorInvalidValue;
   const void *fns[] = {(const void *)k_nonlinear<true, kNlRegQ>,    (const void *)k_nonlinear<false, kMaxQ>,
+                       (const void *)k_nonlinear<true, kNlRegQ, true>, (const void *)k_nonlinear<false, kMaxQ, true>,
                        (const void *)k_nonlinear32<true, kNlRegQ>,  (const void *)k_nonlinear32<false, kMaxQ>,
                        (const void *)k_nonlinear_b<true, kNlRegQ>,  (const void *)k_nonlinear_b<false, kMaxQ>,
                        (const void *)k_nonlinear32_b<true, kNlRegQ>, (const void *)k_nonlinear32_b<false, kMaxQ>,

@@ -142,6 +142,20 @@ int dh_set_mixed_precision(dh_ctx *ctx, int32_t on);
 /* 1 when the mixed form is on, 0 when not. */
 int dh_mixed_precision(dh_ctx *ctx);
 
+/* Constant-Smagorinsky LES, the reference's LES = .TRUE. with
+ * LES_MODEL_TYPE = 1 (header:168, les.f:1-336, channel.f:303-375 and the
+ * NU_T terms of channel.f:631-659, 748-760, 793-807, 862-875, 927-941,
+ * 991-1005): on = 1 runs les_chan in every substep once the spin-up gate of
+ * channel.f:303 is open (CREATE_NEW_FLOW = 0, or TIME_STEP > 100) and the
+ * eddy-viscosity (and, for scalars, KAPPA_T = NU_T) terms in the implicit
+ * solves always.  FP64 only (refused while the mixed form is on).  The first
+ * switch allocates NU_T and the model's work fields, about
+ * (3 (3 + N_TH) + 14 + 2 N_TH) physical and (10 + 3 N_TH) Fourier fields.
+ * The periodic mean_les output (les.f:338-517) is not produced. */
+int dh_set_les(dh_ctx *ctx, int32_t on);
+/* 1 when LES is on, 0 when not. */
+int dh_les(dh_ctx *ctx);
+
 /* Launch-shape tuning (no counterpart in the reference; for ensembles of
  * uneven sizes, where the best block size depends on the plane's shape).
  * dh_set_plane_threads sets the block size of the plane transform kernels
@@ -239,7 +253,8 @@ int dh_fft_xz_to_physical_dev(dh_ctx *ctx, const double *four, double *phys, int
  * field c: for c < 3 the velocity transformed at the start of the last
  * substep (its solve writes over R), for c >= 3 TH as left by its solve;
  * 200+c the same field of the mixed form (float values; null until the
- * mixed form is first switched on). */
+ * mixed form is first switched on); 300 NU_T [NY+2][NZ][NX] at GY points
+ * (null until LES is first switched on). */
 double *dh_field_dev(dh_ctx *ctx, int32_t which);
 /* HIP stream (hipStream_t) the context runs on. */
 void *dh_stream(dh_ctx *ctx);
