@@ -214,6 +214,38 @@ def mixed_form(p, gy, ic, steps, warmup):
             "rel_l2_vs_f64_after_10_steps": {k: float(f"{v:.3g}") for k, v in err.items()}}
 
 
+def les_form(steps, warmup, reps=10):
+    """The constant-Smagorinsky LES channel (Re_tau = 395, 64 x 97 x 64, the
+    model on from the first step): its throughput, the same deck stepped
+    without the model, and the LES substep's phases (the model runs in
+    to_physical, its implicit terms in cn_solve)."""
+    from diablo_amd import channel, configs
+
+    p, gy, ic = configs.les_channel()
+    pts = p["nx"] * p["ny"] * p["nz"]
+    res = {}
+    for les in (True, False):
+        ch = channel.Channel(p, gy)
+        ch.set_les(les)
+        ch.initial_condition(ic)
+        ch.set_clock(0.0, p["delta_t"], 0)
+        ch.time_step(warmup)
+        ch.synchronize()
+        t0 = time.perf_counter()
+        ch.time_step(steps)
+        ch.synchronize()
+        wall = time.perf_counter() - t0
+        res[les] = (pts * steps / wall / 1e6, wall / steps * 1e3)
+        if les:
+            phases = {f"rk{rk}:{k}": round(v, 5) for rk in (1, 2, 3) for k, v in ch.profile_substep(rk, reps).items()}
+        ch.close()
+    return {"workload": "les_channel: constant-Smagorinsky LES, Re_tau=395, 64x97x64, 2pi x 2 x pi, "
+                        "tanh grid CS=1.75, FP64, CFL=0.5 variable dt",
+            "value": round(res[True][0], 3), "unit": UNIT, "ms_per_step": round(res[True][1], 4),
+            "without_model": {"value": round(res[False][0], 3), "ms_per_step": round(res[False][1], 4)},
+            "phases_ms": phases}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -221,6 +253,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="leave out the FP32/FP64 mixed form's line")
+    ap.add_argument("--no-les", action="store_true", help="leave out the LES channel's line")
     ap.add_argument("--profile-reps", type=int, default=10)
     ap.add_argument("--ensemble", type=int, default=0, metavar="N",
                     help="only time N uneven shear-layer instances stepped as one graph")
@@ -291,6 +324,8 @@ def main():
     }
     if not args.no_mixed:
         out["mixed_form"] = mixed_form(p, gy, ic, args.steps, args.warmup)
+    if not args.no_les:
+        out["les_form"] = les_form(args.steps, args.warmup)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(p, gy, ic)
     ch.close()

@@ -210,14 +210,26 @@ struct LesBcArgs {
   const Clock *clk;
   int cnf;
 };
+// Mode m of plane j = blockIdx.y: planes 3..NY-2 are copied by their own
+// threads; the thread of plane 0 copies the planes the BCs read or write
+// (0..2, NY-1..NY+1) and applies the BCs to them, so no other thread
+// writes those planes after it (planes 2 and NY-1, which the BCs only
+// read, are also copied by their own threads, with the same values).
 __global__ void k_les_bc_four(LesBcArgs la) {
   const BcArgs &a = la.b;
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= a.g.pf) return;
   const long long pf = a.g.pf;
+  const int j = blockIdx.y, NYp = a.g.ny;
   double2 *dst[3] = {a.cu1, a.cu2, a.cu3};
+  if (j != 0) {
+    if (j >= 2 && j <= NYp - 1)
+      for (int f = 0; f < 3; ++f) dst[f][pf * j + m] = la.cu[f][pf * j + m];
+    return;
+  }
+  const int edge[6] = {0, 1, 2, NYp - 1, NYp, NYp + 1};
   for (int f = 0; f < 3; ++f)
-    for (int j = 0; j < a.g.ny + 2; ++j) dst[f][pf * j + m] = la.cu[f][pf * j + m];
+    for (int q = 0; q < 6; ++q) dst[f][pf * edge[q] + m] = la.cu[f][pf * edge[q] + m];
   if (la.cnf && la.clk->time_step <= 100) return;
   apply_bc_vel_mode(a, m);
   const int NY = a.g.ny;
@@ -291,7 +303,8 @@ hipError_t launch_les_bc_four(Ctx &c, hipStream_t st) {
   la.b.cu1 = c.cul;
   la.b.cu2 = c.cul + cfs;
   la.b.cu3 = c.cul + 2 * cfs;
-  return klaunch(c, (const void *)k_les_bc_four, nullptr, dim3((c.g.pf + 63) / 64), dim3(64), 0, st, la);
+  return klaunch(c, (const void *)k_les_bc_four, nullptr, dim3((c.g.pf + 255) / 256, c.g.ny + 2), dim3(256), 0, st,
+                 la);
 }
 
 hipError_t launch_create_flow(Ctx &c, int ic_type, double ubulk0, unsigned long long seed, double amp, double ymask) {

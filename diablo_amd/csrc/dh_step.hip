@@ -1651,13 +1651,15 @@ __global__ void __launch_bounds__(256) k_solve_les(LesSolveArgs a) {
     const double *U = a.uphys + f * a.fs + col, *R = a.rphys + f * a.fs + col, *NT = a.nut + col;
     const long long ws = (long long)a.in.nfields * nrow * pp;
     double *Bw = a.wk + (long long)f * nrow * pp + col, *Gw = Bw + ws, *Cw = Gw + ws;
-    auto row = [&](int J, double &L, double &D, double &C, double &V) {
+    // row J from its neighbours' U and NU_T and its R (loads issued by the
+    // caller, eight rows at a time, ahead of the rows' arithmetic: with one
+    // column per thread there are too few waves to hide memory latency)
+    auto row = [&](int J, double um, double u0, double up, double nm, double n0, double np, double r, double &L,
+                   double &D, double &C, double &V) {
       int bc;
       real_row(a.in, f, temp1, J, L, D, C, V, bc);
       if (bc) return;
-      const double u0 = U[J * pp], um = U[(J - 1) * pp], up = U[(J + 1) * pp];
-      const double n0 = NT[J * pp], nm = NT[(J - 1) * pp], np = NT[(J + 1) * pp];
-      V = R[J * pp];
+      V = r;
       // the explicit Crank-Nicolson half: molecular (channel.f:606-626,
       // 739-746), then the eddy viscosity or diffusivity
       if (f == 1)
@@ -1678,42 +1680,72 @@ __global__ void __launch_bounds__(256) k_solve_les(LesSolveArgs a) {
         C = C - temp2 * np / (DY[J + 1] * DYF[J]);
       }
     };
+    constexpr int kB = 8;
     double L, D, C, V;
-    row(0, L, D, C, V);
+    row(0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, L, D, C, V);  // a boundary row of every field
     double bp = D, gp = V, cp = C;
     Bw[0] = bp;
     Gw[0] = gp;
     Cw[0] = cp;
-    for (int J = 1; J <= NY + 1; ++J) {
-      row(J, L, D, C, V);
-      const double am = -L / bp;
-      D = D + am * cp;
-      V = V + am * gp;
-      Bw[J * pp] = D;
-      Gw[J * pp] = V;
-      Cw[J * pp] = C;
-      bp = D;
-      gp = V;
-      cp = C;
+    double um = U[0], u0 = U[pp], nm = NT[0], n0 = NT[pp];
+    for (int J0 = 1; J0 <= NY + 1; J0 += kB) {
+      double rr[kB], uu[kB], nn[kB];
+#pragma unroll
+      for (int q = 0; q < kB; ++q) {
+        const int J = J0 + q;
+        rr[q] = (J <= NY + 1) ? R[J * pp] : 0.0;
+        uu[q] = (J <= NY) ? U[(J + 1) * pp] : 0.0;
+        nn[q] = (J <= NY) ? NT[(J + 1) * pp] : 0.0;
+      }
+#pragma unroll
+      for (int q = 0; q < kB; ++q) {
+        const int J = J0 + q;
+        if (J > NY + 1) break;
+        row(J, um, u0, uu[q], nm, n0, nn[q], rr[q], L, D, C, V);
+        const double am = -L / bp;
+        D = D + am * cp;
+        V = V + am * gp;
+        Bw[J * pp] = D;
+        Gw[J * pp] = V;
+        Cw[J * pp] = C;
+        bp = D;
+        gp = V;
+        cp = C;
+        um = u0;
+        u0 = uu[q];
+        nm = n0;
+        n0 = nn[q];
+      }
     }
     double *Ub = a.dst[f] + col;
     const int wlo = a.wlo[f], whi = a.whi[f];
-    double xn = gp / bp;  // row NY+1
-    for (int J = NY + 1; J >= 0; --J) {
-      double x;
-      if (J == NY + 1) {
-        x = xn;
-      } else {
-        const double b = Bw[J * pp];
-        x = (b != 0.0) ? (Gw[J * pp] - Cw[J * pp] * xn) / b : 0.0;
-      }
-      xn = x;
+    auto put = [&](int J, double x) {
       if (J >= wlo && J <= whi) {
         Ub[J * pp] = x;
         if (cour && J >= 1 && J <= NY) {
           const double dl = (f == 1) ? g.dy[J] : dxf;
           dtmin = fmin(dtmin, a.cfl * dl / fabs(x));
         }
+      }
+    };
+    double xn = gp / bp;  // row NY+1
+    put(NY + 1, xn);
+    for (int J1 = NY; J1 >= 0; J1 -= kB) {
+      double bb[kB], gg[kB], cc[kB];
+#pragma unroll
+      for (int q = 0; q < kB; ++q) {
+        const int J = J1 - q;
+        bb[q] = (J >= 0) ? Bw[J * pp] : 1.0;
+        gg[q] = (J >= 0) ? Gw[J * pp] : 0.0;
+        cc[q] = (J >= 0) ? Cw[J * pp] : 0.0;
+      }
+#pragma unroll
+      for (int q = 0; q < kB; ++q) {
+        const int J = J1 - q;
+        if (J < 0) break;
+        const double x = (bb[q] != 0.0) ? (gg[q] - cc[q] * xn) / bb[q] : 0.0;
+        xn = x;
+        put(J, x);
       }
     }
   }
