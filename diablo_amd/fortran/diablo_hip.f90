@@ -106,6 +106,12 @@ module diablo_hip
       type(c_ptr), value :: ctx
       integer(c_int32_t), value :: on
     end function dh_set_mixed_precision
+    ! LES = .TRUE. with LES_MODEL_TYPE = 1 (header:168, les.f:1-336)
+    integer(c_int) function dh_set_les(ctx, on) bind(C, name='dh_set_les')
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: on
+    end function dh_set_les
   end interface
 
 contains
