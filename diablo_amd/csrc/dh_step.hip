@@ -588,8 +588,11 @@ __device__ __forceinline__ void nonlinear_pass(const StepArgs &a, C *A) {
     plane_store_phys(A, g, T, dst);
   }
 }
-template <bool REG, int NQ, bool SGS = false>
-__global__ void __launch_bounds__(REG ? 512 : 1024) k_nonlinear(StepArgs a) {
+// LB: the block-size bound.  The LDS variant takes any block size up to
+// 1024; its instantiation for blocks of at most 512 threads has twice the
+// register budget (256 VGPRs) and no spills.
+template <bool REG, int NQ, bool SGS = false, int LB = (REG ? 512 : 1024)>
+__global__ void __launch_bounds__(LB) k_nonlinear(StepArgs a) {
   extern __shared__ double2 smem[];
   nonlinear_pass<REG, NQ, double2, SGS>(a, smem);
 }
@@ -604,8 +607,8 @@ k_nonlinear32(StepArgs a) {
 }
 // batches of contexts of one shape: argument structs in device memory,
 // member blockIdx.z
-template <bool REG, int NQ>
-__global__ void __launch_bounds__(REG ? 512 : 1024) k_nonlinear_b(const StepArgs *__restrict__ av) {
+template <bool REG, int NQ, int LB = (REG ? 512 : 1024)>
+__global__ void __launch_bounds__(LB) k_nonlinear_b(const StepArgs *__restrict__ av) {
   extern __shared__ double2 smem[];
   nonlinear_pass<REG, NQ, double2>(av[blockIdx.z], smem);
 }
@@ -1498,18 +1501,24 @@ static hipError_t launch_nonlinear_range(Ctx &c, int rk, int c0, int c1, hipStre
     return klaunch(c, (const void *)k_nonlinear32<false, kMaxQ>, (const void *)k_nonlinear32_b<false, kMaxQ>, grid,
                    dim3(c.plane_threads), c.lds_bytes32, st, a);
   }
+  const bool half = c.plane_threads <= 512;  // the LDS variant's 512-thread instantiation
   if (c.les) {  // FP64 only; the LES kernels have no batched forms
     if (c.reg_rows_nl)
       return klaunch(c, (const void *)k_nonlinear<true, kNlRegQ, true>, nullptr, grid, dim3(kNlRegThreads),
                      c.lds_bytes, st, a);
-    return klaunch(c, (const void *)k_nonlinear<false, kMaxQ, true>, nullptr, grid, dim3(c.plane_threads), c.lds_bytes,
-                   st, a);
+    return klaunch(c,
+                   half ? (const void *)k_nonlinear<false, kMaxQ, true, 512>
+                        : (const void *)k_nonlinear<false, kMaxQ, true, 1024>,
+                   nullptr, grid, dim3(c.plane_threads), c.lds_bytes, st, a);
   }
   if (c.reg_rows_nl)
     return klaunch(c, (const void *)k_nonlinear<true, kNlRegQ>, (const void *)k_nonlinear_b<true, kNlRegQ>, grid,
                    dim3(kNlRegThreads), c.lds_bytes, st, a);
-  return klaunch(c, (const void *)k_nonlinear<false, kMaxQ>, (const void *)k_nonlinear_b<false, kMaxQ>, grid,
-                 dim3(c.plane_threads), c.lds_bytes, st, a);
+  if (half)
+    return klaunch(c, (const void *)k_nonlinear<false, kMaxQ, false, 512>,
+                   (const void *)k_nonlinear_b<false, kMaxQ, 512>, grid, dim3(c.plane_threads), c.lds_bytes, st, a);
+  return klaunch(c, (const void *)k_nonlinear<false, kMaxQ, false, 1024>,
+                 (const void *)k_nonlinear_b<false, kMaxQ, 1024>, grid, dim3(c.plane_threads), c.lds_bytes, st, a);
 }
 
 static hipError_t launch_nonlinear(Ctx &c, int rk) { return launch_nonlinear_range(c, rk, 0, c.nfields, c.stream); }
@@ -1933,6 +1942,9 @@ hipError_t prepare_step_kernels(Ctx &c) {
   if (c.lds_bytes > kLdsMax || (long long)rem_div_thomas_lds(c) > kLdsMax) return hipErrorInvalidValue;
   const void *fns[] = {(const void *)k_nonlinear<true, kNlRegQ>,    (const void *)k_nonlinear<false, kMaxQ>,
                        (const void *)k_nonlinear<true, kNlRegQ, true>, (const void *)k_nonlinear<false, kMaxQ, true>,
+                       (const void *)k_nonlinear<false, kMaxQ, false, 512>,
+                       (const void *)k_nonlinear<false, kMaxQ, true, 512>,
+                       (const void *)k_nonlinear_b<false, kMaxQ, 512>,
                        (const void *)k_nonlinear32<true, kNlRegQ>,  (const void *)k_nonlinear32<false, kMaxQ>,
                        (const void *)k_nonlinear_b<true, kNlRegQ>,  (const void *)k_nonlinear_b<false, kMaxQ>,
                        (const void *)k_nonlinear32_b<true, kNlRegQ>, (const void *)k_nonlinear32_b<false, kMaxQ>,
