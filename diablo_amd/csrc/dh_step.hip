@@ -1642,9 +1642,12 @@ struct LesSolveArgs {
   double cfl, dx, dz;
 };
 
-__global__ void __launch_bounds__(256) k_solve_les(LesSolveArgs a) {
+// One wave per block: the columns of a field spread over as many CUs as
+// there are waves (4096 columns: 64 CUs per field).
+constexpr int kLesSolveThreads = 64;
+__global__ void __launch_bounds__(kLesSolveThreads) k_solve_les(LesSolveArgs a) {
 #pragma clang fp contract(off)
-  __shared__ double red[256 / 64];
+  __shared__ double red[kLesSolveThreads / 64];
   const Geo &g = a.g;
   const int f = a.f0 + blockIdx.y;
   const long long pp = g.pp, col = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1793,8 +1796,8 @@ hipError_t launch_solve_les_range(Ctx &c, int rk, int courant, int f0, int f1, h
   a.cfl = c.p.cfl;
   a.dx = c.p.lx / c.g.nx;
   a.dz = c.p.lz / c.g.nz;
-  const dim3 grid((unsigned)((c.g.pp + 255) / 256), f1 - f0);
-  return klaunch(c, (const void *)k_solve_les, nullptr, grid, dim3(256), 0, st, a);
+  const dim3 grid((unsigned)((c.g.pp + kLesSolveThreads - 1) / kLesSolveThreads), f1 - f0);
+  return klaunch(c, (const void *)k_solve_les, nullptr, grid, dim3(kLesSolveThreads), 0, st, a);
 }
 
 static size_t rem_div_thomas_lds(const Ctx &c) {
