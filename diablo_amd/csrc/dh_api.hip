@@ -749,20 +749,25 @@ int dh_set_plane_threads(dh_ctx *h, int32_t threads) {
 int dh_plane_threads(dh_ctx *h) { return h ? h->c.plane_threads : DH_EARG; }
 
 // A copy of a context's time-stepped state (Fourier fields, CF, CP, the
-// scalars and the clock) on the device, for trial steps that must leave it
-// unchanged; restore() also rebuilds the CN tables of the restored DELTA_T.
+// scalars, NU_T with LES, and the clock) on the device, for trial steps that
+// must leave it unchanged; restore() also rebuilds the CN tables of the
+// restored DELTA_T.
 struct StateCopy {
   Ctx *c = nullptr;
-  double2 *src[9] = {}, *save[9] = {};
-  size_t cnt[9] = {};
+  int n = 0;  // fields saved: the nine of the state, and NU_T with LES
+  double2 *src[10] = {}, *save[10] = {};
+  size_t cnt[10] = {};
   Clock *ck = nullptr;
   hipError_t init(Ctx &cx) {
     c = &cx;
     const size_t cf = (size_t)(cx.g.ny + 2) * cx.g.pf, nth = (size_t)(cx.p.n_th > 0 ? cx.p.n_th : 1);
-    double2 *s0[9] = {cx.cu[0], cx.cu[1], cx.cu[2], cx.cp, cx.cf[0], cx.cf[1], cx.cf[2], cx.cth, cx.cfth};
-    const size_t n0[9] = {cf, cf, cf, cf, cf, cf, cf, cf * nth, cf * nth};
+    double2 *s0[10] = {cx.cu[0], cx.cu[1], cx.cu[2], cx.cp, cx.cf[0], cx.cf[1], cx.cf[2], cx.cth, cx.cfth,
+                       reinterpret_cast<double2 *>(cx.nut)};
+    // NU_T: (NY+2) NX NZ doubles, an even count (NX is even)
+    const size_t n0[10] = {cf, cf, cf, cf, cf, cf, cf, cf * nth, cf * nth, (size_t)(cx.g.ny + 2) * cx.g.pp / 2};
+    n = cx.nut ? 10 : 9;
     hipError_t e = hipSuccess;
-    for (int i = 0; i < 9 && e == hipSuccess; ++i) {
+    for (int i = 0; i < n && e == hipSuccess; ++i) {
       src[i] = s0[i];
       cnt[i] = n0[i];
       e = hipMalloc(&save[i], cnt[i] * sizeof(double2));
@@ -773,7 +778,7 @@ struct StateCopy {
   }
   hipError_t copy(bool restore) {
     hipError_t e = hipSuccess;
-    for (int i = 0; i < 9 && e == hipSuccess; ++i)
+    for (int i = 0; i < n && e == hipSuccess; ++i)
       e = hipMemcpyAsync(restore ? src[i] : save[i], restore ? save[i] : src[i], cnt[i] * sizeof(double2),
                          hipMemcpyDeviceToDevice, c->stream);
     if (e == hipSuccess)

@@ -115,3 +115,66 @@ def test_les_refuses_mixed_form(dor):
     with pytest.raises(channel.DiabloError):
         ch.set_les(True)
     ch.close()
+
+
+def _les_pair(dor, name, step0, **kw):
+    o, p, gy, ic = helpers.make_oracle(dor, name, **kw)
+    st = helpers.oracle_state(dor, o)
+    chs = []
+    for _ in range(2):
+        ch = channel.Channel(p, gy)
+        ch.upload(st)
+        ch.set_clock(0.0, p["delta_t"], step0)
+        chs.append(ch)
+    return chs
+
+
+def test_les_autotune_leaves_state(dor):
+    """dh_autotune's trial steps cross the spin-up gate (steps 99..102 of a
+    new flow) and must leave the state, the clock and NU_T as they were: the
+    tuned context then steps bit-identically to an untuned one at the same
+    block size."""
+    _say("[les] autotune across the gate")
+    a, b = _les_pair(dor, "les_small", 98, create_new_flow=1)
+    t = a.autotune(4)
+    b.set_plane_threads(t)
+    assert np.abs(_nut_gpu(a)).max() == 0.0  # the trial steps left NU_T as it was
+    for x in (a, b):
+        x.time_step(4)
+    sa, sb = a.download(), b.download()
+    for k in ("cu1", "cu2", "cu3", "cp", "cth"):
+        assert np.array_equal(sa[k], sb[k]), k
+    assert np.array_equal(_nut_gpu(a), _nut_gpu(b))
+    assert a.clock() == b.clock()
+    for x in (a, b):
+        x.close()
+
+
+def test_les_member_in_ensemble(dor):
+    """An ensemble with an LES member steps its members one by one (the LES
+    kernels have no batched forms); each equals its solo run bit for bit."""
+    _say("[les] ensemble with an LES member")
+    les = _les_pair(dor, "les_small", 0)
+    plain = []
+    o, p, gy, ic = helpers.make_oracle(dor, "small_case")
+    st = helpers.oracle_state(dor, o)
+    for _ in range(2):
+        ch = channel.Channel(p, gy)
+        ch.upload(st)
+        ch.set_clock(0.0, p["delta_t"], 0)
+        plain.append(ch)
+    solo = [les[0], plain[0]]
+    together = [les[1], plain[1]]
+    for ch in solo:
+        ch.time_step(3)
+    ens = channel.Ensemble(together)
+    ens.time_step(3)
+    ens.synchronize()
+    ens.close()  # before its members
+    for a, b in zip(solo, together):
+        sa, sb = a.download(), b.download()
+        for k in ("cu1", "cu2", "cu3", "cp"):
+            assert np.array_equal(sa[k], sb[k]), k
+    assert np.array_equal(_nut_gpu(solo[0]), _nut_gpu(together[0]))
+    for ch in solo + together:
+        ch.close()
