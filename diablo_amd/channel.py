@@ -77,6 +77,7 @@ EXPORTS = {
     "dh_mixed_precision": (ctypes.c_int, [ctypes.c_void_p]),
     "dh_set_les": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "dh_les": (ctypes.c_int, [ctypes.c_void_p]),
+    "dh_set_les_closure": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32]),
     "dh_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
     "dh_profile_substep": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, _DP, ctypes.c_int32]),
     "dh_phase_name": (ctypes.c_char_p, [ctypes.c_int32]),
@@ -292,6 +293,16 @@ class Channel:
     @property
     def les(self):
         return bool(self.lib.dh_les(self.h))
+
+    def set_les_closure(self, weights=None):
+        """Learned eddy viscosity (MLP 8 -> 32 -> 32 -> 1, BF16): weights as the
+        flat float32 array W1[8][32], b1, W2[32][32], b2, w3, b3; None returns
+        to the Smagorinsky model."""
+        if weights is None:
+            self._check(self.lib.dh_set_les_closure(self.h, None, 0), "dh_set_les_closure")
+            return
+        w = np.ascontiguousarray(weights, dtype=np.float32).ravel()
+        self._check(self.lib.dh_set_les_closure(self.h, w.ctypes.data, w.size), "dh_set_les_closure")
 
     @property
     def plane_threads(self):

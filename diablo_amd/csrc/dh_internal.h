@@ -183,7 +183,15 @@ struct Ctx {
   double2 *sgs = nullptr;       // [3 + N_TH][NY+2][pf]
   double *les_wk = nullptr;     // [2][nfields][NY+2][NZ][NX]
   double *d_lesw = nullptr;     // DYW (0:NY+1) then DYFW (0:NY+1)
+  // learned closure in place of NU_T = C_s^2 Delta^2 |S| (no counterpart in
+  // the reference): BF16 weights in MFMA operand order, FP32 biases, and
+  // the strain features it reads [kLesFeat][NY+2][NZ][NX]
+  bool les_mlp = false;
+  short *d_mlp_w = nullptr;     // layer-1 A operands [2][64][4], layer-2 B operands [2][2][64][4]
+  float *d_mlp_b = nullptr;     // b1[32], b2[32], w3[32], b3
+  double *les_feat = nullptr;
 };
+constexpr int kLesFeat = 8, kMlpHidden = 32;
 
 // Work fields of the LES pass: Fourier slots 0..6 (S11, S22, S33, S4, S5,
 // S6 in, then |S| S products and cross terms out), 7 + 2n and 8 + 2n for

@@ -43,6 +43,11 @@ struct LesArgs {
   int jstart, jend, jstart_th[DH_MAX_TH], jend_th[DH_MAX_TH];
   int ubmin, vbmin, wbmin, ubmax, vbmax, wbmax;
   double uc1min, vc1min, wc1min, uc1max, vc1max, wc1max;
+  // learned closure: features out of k_les_nut, NU_T from k_les_mlp
+  int mlp;
+  double *feat;
+  const short *mlp_w;
+  const float *mlp_b;
 };
 
 __device__ __forceinline__ bool les_gate_open(const LesArgs &a) { return !a.cnf || a.clk->time_step > 100; }
@@ -135,6 +140,23 @@ __global__ void k_les_nut(LesArgs a) {
   const double du1 = u(U1, J) - u(U1, J - 1), du3 = u(U3, J) - u(U3, J - 1);
   P(2) = (J >= 2) ? sy * (S(3, J) - 0.5 * du1 / DY[J]) : 0.0;  // |S| S4 less the implicit dU1/dy
   P(4) = (J >= 2) ? sy * (S(5, J) - 0.5 * du3 / DY[J]) : 0.0;  // |S| S6 less the implicit dU3/dy
+  if (a.mlp) {
+    // the closure's inputs at GY (zero on row 1); NU_T and the terms that
+    // use it follow in k_les_mlp and k_les_post
+    double f[kLesFeat] = {};
+    if (J >= 2) {
+      auto gyi = [&](int s) { return (S(s, J) * DYF[J - 1] + S(s, J - 1) * DYF[J]) / (2.0 * DY[J]); };
+      f[0] = sy;
+      f[1] = S(3, J) - 0.5 * (mean(0, J) - mean(0, J - 1)) / DY[J];
+      f[2] = S(5, J) - 0.5 * (mean(2, J) - mean(2, J - 1)) / DY[J];
+      f[3] = gyi(0);
+      f[4] = gyi(1);
+      f[5] = gyi(2);
+      f[6] = gyi(4);
+    }
+    for (int q = 0; q < kLesFeat; ++q) a.feat[q * fs + (long long)J * pp + col] = f[q];
+    return;
+  }
   const double nut = (J >= 3 && J <= NY - 1) ? -0.5 * a.dyw[J] * sy : 0.0;
   a.nut[(long long)J * pp + col] = nut;
   const bool cross = J >= 2 && J <= NY;
@@ -226,6 +248,110 @@ __global__ void k_les_cf(LesArgs a) {
   }
 }
 
+// The learned closure (no counterpart in the reference): NU_T at the GY
+// point of plane J = -DYW(J)/2 max(0, y), y the output of an MLP
+// 8 -> 32 -> 32 -> 1 (ReLU hidden layers) of the point's strain features
+// (|S| at GY, S4 and S6 less their mean shear, S11, S22, S33 and S5
+// interpolated to GY, one spare zero), on J = 3..NY-1 as the model's own
+// NU_T; a network that returns |S| is the constant-Smagorinsky model.
+// BF16 MFMA (16x16x16, FP32 accumulation): a wave takes 16 points at a time.
+// Layer 1 runs transposed, H1^T = W1^T X^T (A = W1^T from the table, B = the
+// points' features, lane l holding features 4(l/16)..+3 of point l%16), so
+// that its result, hidden units 16t + 4(l/16) + i of point l%16 in lane l,
+// is already the A operand of layer 2 (points x hidden).  Layer 2 leaves
+// hidden unit 16nt + l%16 of points 4(l/16) + i in lane l; layer 3 is a sum
+// over the 16 lanes of each group.
+typedef short mlp_s4 __attribute__((ext_vector_type(4)));
+typedef float mlp_f4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ short to_bf16(float x) {  // round to nearest even
+  unsigned u = __float_as_uint(x);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (short)(u >> 16);
+}
+
+__global__ void __launch_bounds__(256) k_les_mlp(LesArgs a) {
+  if (!les_gate_open(a)) return;
+  const Geo &g = a.g;
+  const int NY = g.ny;
+  const long long pp = g.pp, npts = (long long)(NY + 1) * pp;  // planes 1..NY+1
+  const int lane = threadIdx.x & 63, grp = lane >> 4, col16 = lane & 15;
+  const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
+  const mlp_s4 *W = reinterpret_cast<const mlp_s4 *>(a.mlp_w);
+  const mlp_s4 w1a[2] = {W[lane], W[64 + lane]};
+  const mlp_s4 w2b[2][2] = {{W[128 + lane], W[192 + lane]}, {W[256 + lane], W[320 + lane]}};
+  const float *b1 = a.mlp_b, *b2 = a.mlp_b + kMlpHidden, *w3 = a.mlp_b + 2 * kMlpHidden;
+  const float b3 = a.mlp_b[3 * kMlpHidden];
+  for (long long base = wave * 16; base < npts; base += nwaves * 16) {
+    // B of layer 1: features 4 grp..4 grp+3 of point base + col16 (groups 2, 3: zero)
+    const long long q = base + col16;
+    mlp_s4 x = {0, 0, 0, 0};
+    if (grp < 2 && q < npts) {
+      const double *fp = a.feat + pp + q;  // point q is plane 1 + q / pp
+      x[0] = to_bf16((float)fp[(4 * grp + 0) * a.fs]);
+      x[1] = to_bf16((float)fp[(4 * grp + 1) * a.fs]);
+      x[2] = to_bf16((float)fp[(4 * grp + 2) * a.fs]);
+      x[3] = to_bf16((float)fp[(4 * grp + 3) * a.fs]);
+    }
+    mlp_s4 h1[2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      mlp_f4 d = {0.f, 0.f, 0.f, 0.f};
+      d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w1a[t], x, d, 0, 0, 0);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) h1[t][i] = to_bf16(fmaxf(d[i] + b1[16 * t + 4 * grp + i], 0.f));
+    }
+    float part[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt) {
+      mlp_f4 d = {0.f, 0.f, 0.f, 0.f};
+      d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(h1[0], w2b[0][nt], d, 0, 0, 0);
+      d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(h1[1], w2b[1][nt], d, 0, 0, 0);
+      const float bb = b2[16 * nt + col16], ww = w3[16 * nt + col16];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) part[i] += fmaxf(d[i] + bb, 0.f) * ww;
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      for (int o = 8; o > 0; o >>= 1) part[i] += __shfl_xor(part[i], o, 16);
+    if (col16 == 0) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const long long p = base + 4 * grp + i;
+        if (p < npts) {
+          const int J = 1 + (int)(p / pp);
+          const double y = (double)(part[i] + b3);
+          a.nut[pp + p] = (J >= 3 && J <= NY - 1) ? -0.5 * a.dyw[J] * fmax(y, 0.0) : 0.0;
+        }
+      }
+    }
+  }
+}
+
+// With the closure: the terms k_les_nut leaves to after NU_T, the cross
+// terms NU_T dU1/dy, NU_T dU3/dy and KAPPA_T times the scalar gradients.
+__global__ void k_les_post(LesArgs a) {
+#pragma clang fp contract(off)
+  if (!les_gate_open(a)) return;
+  const Geo &g = a.g;
+  const long long col = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= g.pp) return;
+  const int J = 1 + blockIdx.y, NY = g.ny;
+  const long long pp = g.pp, fs = a.fs, o = (long long)J * pp + col;
+  const int out = kLesPhys - kLesFour + 2 * a.nth;
+  const double *U1 = a.uphys, *U3 = a.uphys + 2 * fs, *DY = g.dy;
+  const double nut = a.nut[o];
+  const bool cross = J >= 2 && J <= NY;
+  a.phys[(out + 5) * fs + o] = cross ? nut * (U1[o] - U1[o - pp]) / DY[J] : 0.0;
+  a.phys[(out + 6) * fs + o] = cross ? nut * (U3[o] - U3[o - pp]) / DY[J] : 0.0;
+  for (int n = 0; n < a.nth; ++n) {
+    double *tx = a.phys + (kLesFour + 2 * n) * fs + o, *tz = tx + fs;
+    *tx = nut * *tx;
+    *tz = nut * *tz;
+  }
+}
+
 static LesArgs les_args(Ctx &c) {
   LesArgs a{};
   a.g = c.g;
@@ -254,6 +380,10 @@ static LesArgs les_args(Ctx &c) {
   a.ubmax = p.u_bc_ymax; a.vbmax = p.v_bc_ymax; a.wbmax = p.w_bc_ymax;
   a.uc1min = p.u_bc_ymin_c1; a.vc1min = p.v_bc_ymin_c1; a.wc1min = p.w_bc_ymin_c1;
   a.uc1max = p.u_bc_ymax_c1; a.vc1max = p.v_bc_ymax_c1; a.wc1max = p.w_bc_ymax_c1;
+  a.mlp = c.les_mlp ? 1 : 0;
+  a.feat = c.les_feat;
+  a.mlp_w = c.d_mlp_w;
+  a.mlp_b = c.d_mlp_b;
   return a;
 }
 
@@ -272,6 +402,12 @@ hipError_t launch_les_model(Ctx &c, hipStream_t st) {
     return e;
   if ((e = klaunch(c, (const void *)k_les_nut, nullptr, dim3(pb, NY + 1), dim3(256), 0, st, a)) != hipSuccess)
     return e;
+  if (c.les_mlp) {
+    // 16 points per wave and pass; a grid of about four waves per SIMD
+    if ((e = klaunch(c, (const void *)k_les_mlp, nullptr, dim3(1024), dim3(256), 0, st, a)) != hipSuccess) return e;
+    if ((e = klaunch(c, (const void *)k_les_post, nullptr, dim3(pb, NY + 1), dim3(256), 0, st, a)) != hipSuccess)
+      return e;
+  }
   if ((e = klaunch(c, (const void *)k_les_bc_phys, nullptr, dim3(pb), dim3(256), 0, st, a)) != hipSuccess) return e;
   // the products (Fourier slots 0..6) and the scalars' fluxes, planes 1..NY
   const int out = kLesPhys - kLesFour + 2 * nth;

@@ -155,6 +155,18 @@ int dh_mixed_precision(dh_ctx *ctx);
 int dh_set_les(dh_ctx *ctx, int32_t on);
 /* 1 when LES is on, 0 when not. */
 int dh_les(dh_ctx *ctx);
+/* A learned closure in place of the Smagorinsky eddy viscosity (no
+ * counterpart in the reference; it replaces les_chan's NU_T = C_s^2 Delta^2
+ * |S|, les.f:196-227, for drivers that train one): NU_T at the GY point of
+ * plane J = -DYW(J)/2 * max(0, y) on J = 3..NY-1, with y an MLP 8 -> 32 ->
+ * 32 -> 1 (ReLU hidden layers) of the point's strain features (|S| at GY,
+ * S4 and S6 less the plane-mean shear, S11, S22, S33, S5 interpolated to GY,
+ * one spare input held at zero), evaluated in BF16 with FP32 accumulation.
+ * w: W1[8][32], b1[32], W2[32][32], b2[32], w3[32], b3 (n = 1377 floats,
+ * W[in][out]); NULL returns to the Smagorinsky model.  A network returning
+ * its first input is that model with |S| rounded to BF16.  The features of
+ * the last substep are dh_field_dev(ctx, 301), [8][NY+2][NZ][NX]. */
+int dh_set_les_closure(dh_ctx *ctx, const float *w, int32_t n);
 
 /* Launch-shape tuning (no counterpart in the reference; for ensembles of
  * uneven sizes, where the best block size depends on the plane's shape).
@@ -254,7 +266,8 @@ int dh_fft_xz_to_physical_dev(dh_ctx *ctx, const double *four, double *phys, int
  * substep (its solve writes over R), for c >= 3 TH as left by its solve;
  * 200+c the same field of the mixed form (float values; null until the
  * mixed form is first switched on); 300 NU_T [NY+2][NZ][NX] at GY points
- * (null until LES is first switched on). */
+ * (null until LES is first switched on); 301 the learned closure's
+ * features (see dh_set_les_closure). */
 double *dh_field_dev(dh_ctx *ctx, int32_t which);
 /* HIP stream (hipStream_t) the context runs on. */
 void *dh_stream(dh_ctx *ctx);
