@@ -217,16 +217,26 @@ def mixed_form(p, gy, ic, steps, warmup):
 def les_form(steps, warmup, reps=10):
     """The constant-Smagorinsky LES channel (Re_tau = 395, 64 x 97 x 64, the
     model on from the first step): its throughput, the same deck stepped
-    without the model, and the LES substep's phases (the model runs in
-    to_physical, its implicit terms in cn_solve)."""
+    without the model and with the learned closure in place of the model's
+    NU_T, and the LES substep's phases (the model runs in to_physical, its
+    implicit terms in cn_solve)."""
     from diablo_amd import channel, configs
+
+    import numpy as np
 
     p, gy, ic = configs.les_channel()
     pts = p["nx"] * p["ny"] * p["nz"]
+    # the learned closure's network that returns |S|: the model itself, in BF16
+    w = np.zeros(8 * 32 + 32 + 32 * 32 + 32 + 32 + 1, np.float32)
+    w[0] = 1.0                      # W1[0][0]
+    w[8 * 32 + 32] = 1.0            # W2[0][0]
+    w[8 * 32 + 32 + 32 * 32 + 32] = 1.0  # w3[0]
     res = {}
-    for les in (True, False):
+    for les in (True, False, "closure"):
         ch = channel.Channel(p, gy)
-        ch.set_les(les)
+        ch.set_les(bool(les))
+        if les == "closure":
+            ch.set_les_closure(w)
         ch.initial_condition(ic)
         ch.set_clock(0.0, p["delta_t"], 0)
         ch.time_step(warmup)
@@ -236,13 +246,15 @@ def les_form(steps, warmup, reps=10):
         ch.synchronize()
         wall = time.perf_counter() - t0
         res[les] = (pts * steps / wall / 1e6, wall / steps * 1e3)
-        if les:
+        if les is True:
             phases = {f"rk{rk}:{k}": round(v, 5) for rk in (1, 2, 3) for k, v in ch.profile_substep(rk, reps).items()}
         ch.close()
     return {"workload": "les_channel: constant-Smagorinsky LES, Re_tau=395, 64x97x64, 2pi x 2 x pi, "
                         "tanh grid CS=1.75, FP64, CFL=0.5 variable dt",
             "value": round(res[True][0], 3), "unit": UNIT, "ms_per_step": round(res[True][1], 4),
             "without_model": {"value": round(res[False][0], 3), "ms_per_step": round(res[False][1], 4)},
+            "learned_closure": {"value": round(res["closure"][0], 3), "ms_per_step": round(res["closure"][1], 4),
+                                "network": "8 -> 32 -> 32 -> 1 BF16 MFMA, weights returning |S| (the model)"},
             "phases_ms": phases}
 
 

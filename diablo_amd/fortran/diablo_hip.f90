@@ -112,6 +112,13 @@ module diablo_hip
       type(c_ptr), value :: ctx
       integer(c_int32_t), value :: on
     end function dh_set_les
+    ! learned closure in place of the Smagorinsky NU_T (no reference counterpart)
+    integer(c_int) function dh_set_les_closure(ctx, w, n) bind(C, name='dh_set_les_closure')
+      import :: c_int, c_int32_t, c_ptr, c_float
+      type(c_ptr), value :: ctx
+      real(c_float), intent(in) :: w(*)
+      integer(c_int32_t), value :: n
+    end function dh_set_les_closure
   end interface
 
 contains
