@@ -1645,9 +1645,18 @@ struct LesSolveArgs {
 // One wave per block: the columns of a field spread over as many CUs as
 // there are waves (4096 columns: 64 CUs per field).
 constexpr int kLesSolveThreads = 64;
+constexpr int kLesRowMax = 8 * kSegRows;  // rows per column (NY+2, as the segmented solve)
 __global__ void __launch_bounds__(kLesSolveThreads) k_solve_les(LesSolveArgs a) {
 #pragma clang fp contract(off)
   __shared__ double red[kLesSolveThreads / 64];
+  // per-row values shared by every column of the field: real_row's entries,
+  // the metric reciprocals of the explicit half, and the eddy-viscosity
+  // coefficients T2/(DY DYF) (halved for U2), so that a column's rows take
+  // multiplications where the reference divides (the same values to
+  // roundoff)
+  __shared__ double tL[kLesRowMax], tD[kLesRowMax], tC[kLesRowMax], tV[kLesRowMax];
+  __shared__ double rUp[kLesRowMax], rDn[kLesRowMax], rC[kLesRowMax], kLo[kLesRowMax], kUp[kLesRowMax];
+  __shared__ int tBc[kLesRowMax];
   const Geo &g = a.g;
   const int f = a.f0 + blockIdx.y;
   const long long pp = g.pp, col = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1656,6 +1665,33 @@ __global__ void __launch_bounds__(kLesSolveThreads) k_solve_les(LesSolveArgs a) 
   const double temp1 = a.in.nu * a.clk->h_bar[a.rk] / 2.0;
   const double temp2 = a.clk->h_bar[a.rk] / 2.0;
   const double *DY = a.in.dy, *DYF = a.in.dyf;
+  const double tv = (f < 3 ? temp1 : temp1 / a.in.pr[f - 3]) * a.in.nu_v_scale;
+  for (int J = threadIdx.x; J < nrow; J += blockDim.x) {
+    double L, D, C, V;
+    int bc;
+    real_row(a.in, f, temp1, J, L, D, C, V, bc);
+    tL[J] = L;
+    tD[J] = D;
+    tC[J] = C;
+    tV[J] = V;
+    tBc[J] = bc;
+    if (!bc) {  // interior rows: 1 <= J <= NY
+      if (f == 1) {
+        rUp[J] = 1.0 / DYF[J];
+        rDn[J] = 1.0 / DYF[J - 1];
+        rC[J] = 1.0 / DY[J];
+        kLo[J] = temp2 * 0.5 / (DYF[J - 1] * DY[J]);
+        kUp[J] = temp2 * 0.5 / (DYF[J] * DY[J]);
+      } else {
+        rUp[J] = 1.0 / DY[J + 1];
+        rDn[J] = 1.0 / DY[J];
+        rC[J] = 1.0 / DYF[J];
+        kLo[J] = temp2 / (DY[J] * DYF[J]);
+        kUp[J] = temp2 / (DY[J + 1] * DYF[J]);
+      }
+    }
+  }
+  __syncthreads();
   const bool cour = a.courant && f < 3;
   const double dxf = (f == 0) ? a.dx : a.dz;
   double dtmin = __longlong_as_double(0x7FF0000000000000ll);
@@ -1665,32 +1701,25 @@ __global__ void __launch_bounds__(kLesSolveThreads) k_solve_les(LesSolveArgs a) 
     double *Bw = a.wk + (long long)f * nrow * pp + col, *Gw = Bw + ws, *Cw = Gw + ws;
     // row J from its neighbours' U and NU_T and its R (loads issued by the
     // caller, eight rows at a time, ahead of the rows' arithmetic: with one
-    // column per thread there are too few waves to hide memory latency)
+    // column per thread there are too few waves to hide memory latency):
+    // the explicit Crank-Nicolson half, molecular (channel.f:606-626,
+    // 739-746) then the eddy viscosity or diffusivity (NU_T interpolated to
+    // GYF for U2), and the NU_T parts of the rows
     auto row = [&](int J, double um, double u0, double up, double nm, double n0, double np, double r, double &L,
                    double &D, double &C, double &V) {
-      int bc;
-      real_row(a.in, f, temp1, J, L, D, C, V, bc);
-      if (bc) return;
-      V = r;
-      // the explicit Crank-Nicolson half: molecular (channel.f:606-626,
-      // 739-746), then the eddy viscosity or diffusivity
-      if (f == 1)
-        V = V + temp1 * a.in.nu_v_scale * (((up - u0) / DYF[J] - (u0 - um) / DYF[J - 1]) / DY[J]);
-      else if (f < 3)
-        V = V + temp1 * a.in.nu_v_scale * (((up - u0) / DY[J + 1] - (u0 - um) / DY[J]) / DYF[J]);
-      else
-        V = V + (temp1 / a.in.pr[f - 3]) * a.in.nu_v_scale * (((up - u0) / DY[J + 1] - (u0 - um) / DY[J]) / DYF[J]);
-      if (f == 1) {
-        V = V + temp2 * ((0.5 * (n0 + np) * (up - u0) / DYF[J] - 0.5 * (n0 + nm) * (u0 - um) / DYF[J - 1]) / DY[J]);
-        L = L - temp2 * 0.5 * (n0 + nm) / (DYF[J - 1] * DY[J]);
-        D = D + temp2 * 0.5 * (n0 + np) / (DYF[J] * DY[J]) + temp2 * 0.5 * (n0 + nm) / (DYF[J - 1] * DY[J]);
-        C = C - temp2 * 0.5 * (n0 + np) / (DYF[J] * DY[J]);
-      } else {
-        V = V + temp2 * ((np * (up - u0) / DY[J + 1] - n0 * (u0 - um) / DY[J]) / DYF[J]);
-        L = L - temp2 * n0 / (DY[J] * DYF[J]);
-        D = D + temp2 * np / (DY[J + 1] * DYF[J]) + temp2 * n0 / (DY[J] * DYF[J]);
-        C = C - temp2 * np / (DY[J + 1] * DYF[J]);
-      }
+      L = tL[J];
+      D = tD[J];
+      C = tC[J];
+      V = tV[J];
+      if (tBc[J]) return;
+      const double gu = (up - u0) * rUp[J], gd = (u0 - um) * rDn[J];
+      V = r + tv * ((gu - gd) * rC[J]);
+      const double lo = (f == 1) ? (n0 + nm) : n0, hi = (f == 1) ? (n0 + np) : np;
+      const double h = (f == 1) ? 0.5 : 1.0;
+      V = V + temp2 * ((h * hi * gu - h * lo * gd) * rC[J]);
+      L = L - kLo[J] * lo;
+      D = D + kUp[J] * hi + kLo[J] * lo;
+      C = C - kUp[J] * hi;
     };
     constexpr int kB = 8;
     double L, D, C, V;
