@@ -214,6 +214,27 @@ def mixed_form(p, gy, ic, steps, warmup):
             "rel_l2_vs_f64_after_10_steps": {k: float(f"{v:.3g}") for k, v in err.items()}}
 
 
+def les_cpu_baseline(budget_s=4.0, max_steps=20):
+    """The oracle's LES channel step (the same deck, model on) timed on the
+    host: a bounded sample of a few steps."""
+    from oracle import dor
+    from tests import helpers
+    o, p, *_ = helpers.make_oracle(dor, "les_channel")
+    o.time_step(1)  # first step outside the sample
+    n, t0 = 0, time.perf_counter()
+    while n < max_steps:
+        o.time_step(1)
+        n += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    pts = p["nx"] * p["ny"] * p["nz"]
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": pts * n / dt / 1e6, "unit": UNIT, "cores": cores, "kind": "port",
+            "sample": f"{n} RK3 steps of the 64x97x64 LES channel (after 1 untimed step), FP64 C restatement "
+                      f"of les_chan and the substep with OpenMP over planes/columns, {dt:.1f} s"}
+
+
 def les_form(steps, warmup, reps=10):
     """The constant-Smagorinsky LES channel (Re_tau = 395, 64 x 97 x 64, the
     model on from the first step): its throughput, the same deck stepped
@@ -338,6 +359,8 @@ def main():
         out["mixed_form"] = mixed_form(p, gy, ic, args.steps, args.warmup)
     if not args.no_les:
         out["les_form"] = les_form(args.steps, args.warmup)
+        if not args.no_cpu_baseline:
+            out["les_form"]["cpu_baseline"] = les_cpu_baseline()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(p, gy, ic)
     ch.close()
