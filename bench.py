@@ -240,7 +240,8 @@ def les_form(steps, warmup, reps=10):
     model on from the first step): its throughput, the same deck stepped
     without the model and with the learned closure in place of the model's
     NU_T, and the LES substep's phases (the model runs in to_physical, its
-    implicit terms in cn_solve)."""
+    implicit terms in cn_solve).  Each context takes the block size
+    dh_autotune finds for it first."""
     from diablo_amd import channel, configs
 
     import numpy as np
@@ -260,19 +261,21 @@ def les_form(steps, warmup, reps=10):
             ch.set_les_closure(w)
         ch.initial_condition(ic)
         ch.set_clock(0.0, p["delta_t"], 0)
+        threads = ch.autotune(2)  # block size of the 64x64-plane kernels, state restored
         ch.time_step(warmup)
         ch.synchronize()
         t0 = time.perf_counter()
         ch.time_step(steps)
         ch.synchronize()
         wall = time.perf_counter() - t0
-        res[les] = (pts * steps / wall / 1e6, wall / steps * 1e3)
+        res[les] = (pts * steps / wall / 1e6, wall / steps * 1e3, threads)
         if les is True:
             phases = {f"rk{rk}:{k}": round(v, 5) for rk in (1, 2, 3) for k, v in ch.profile_substep(rk, reps).items()}
         ch.close()
     return {"workload": "les_channel: constant-Smagorinsky LES, Re_tau=395, 64x97x64, 2pi x 2 x pi, "
                         "tanh grid CS=1.75, FP64, CFL=0.5 variable dt",
             "value": round(res[True][0], 3), "unit": UNIT, "ms_per_step": round(res[True][1], 4),
+            "block_size": res[True][2],
             "without_model": {"value": round(res[False][0], 3), "ms_per_step": round(res[False][1], 4)},
             "learned_closure": {"value": round(res["closure"][0], 3), "ms_per_step": round(res["closure"][1], 4),
                                 "network": "8 -> 32 -> 32 -> 1 BF16 MFMA, weights returning |S| (the model)"},
