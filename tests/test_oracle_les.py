@@ -15,7 +15,9 @@ closed forms and exact equivalences (CPU only).
   with PR = 1 the diffusivity likewise.
 * A mean profile with constant discrete flux NU_T(J) (u(J) - u(J-1))/DY(J)
   is a fixed point of the eddy-viscosity terms alone (NU = 0): every wrong
-  pairing of NU_T(J), NU_T(J+1) with the differences moves it."""
+  pairing of NU_T(J), NU_T(J+1) with the differences moves it.
+* The scalars' horizontal eddy diffusivity (KAPPA_T = NU_T) has a closed
+  form for a scalar varying in z only under the single-x-mode NU_T above."""
 import math
 
 import numpy as np
@@ -222,3 +224,45 @@ def test_spin_up_gate(oracle_mod, cnf, step, runs):
     o.set_clock(0.0, p["delta_t"], step)
     o.rk_chan_1(1)
     assert (np.abs(o.rarr(dor.NUT)).max() > 0) == runs
+
+
+def test_scalar_eddy_diffusivity(oracle_mod):
+    """channel.f:327-375: with u1 = A cos x (Neumann walls, so NU_T = c |sin x|
+    on J = 3..NY-1) and a scalar theta = B cos(kz z), ikx theta = 0 and the
+    eddy-diffusivity part of CFTH is ikz F[NU_T (-B kz sin(kz z))] on
+    JSTART_TH..JEND_TH.  It is the difference of CFTH after a substep with
+    the model on (gate open) and off, from the same state: every other
+    explicit scalar term is the same in both."""
+    dor = oracle_mod
+    A, B = 0.7, 0.4
+    outs = {}
+    for les in (1, 0):
+        o, p = _oracle(dor, nth=1, les=les, cnf=0, u_bc_ymin=1, u_bc_ymax=1, w_bc_ymin=1, w_bc_ymax=1,
+                       th_bc_ymin=[1], th_bc_ymax=[1], th_bc_ymin_c1=[0.0], th_bc_ymax_c1=[0.0], ri=[0.0], pr=[1.0])
+        nx, ny, nz = o.nx, o.ny, o.nz
+        x = np.arange(nx) * 2 * math.pi / nx
+        z = np.arange(nz) * p["lz"] / nz
+        kz1 = 2 * math.pi / p["lz"]
+        _set_phys(o, dor.CU1, A * np.broadcast_to(np.cos(x)[None, None, :], (ny + 2, nz, nx)))
+        _set_phys(o, dor.CTH, B * np.broadcast_to(np.cos(kz1 * z)[None, :, None], (ny + 2, nz, nx)))
+        o.rk_chan_1(1)
+        cf = o.carr(dor.CFTH)
+        outs[les] = (cf[0] if cf.ndim == 4 else cf).copy()
+        if les:
+            nut = o.rarr(dor.NUT)[:, :nz, :nx].copy()
+    dyw, _ = _widths(o, p, dor)
+    for J in range(3, ny):  # the closed form of NU_T the model produced
+        np.testing.assert_allclose(nut[J], -0.5 * dyw[J] * math.sqrt(2.0) * A * np.abs(np.sin(x))[None, :]
+                                   * np.ones((nz, 1)), rtol=1e-12, atol=1e-15)
+    nkx, nkz = nx // 3, nz // 3
+    diff = outs[1] - outs[0]
+    for J in range(1, ny + 1):  # JSTART_TH = 1 (Neumann), JEND_TH = NY
+        flux = nut[J] * (-B * kz1 * np.sin(kz1 * z))[:, None]
+        G = np.fft.fft(np.fft.rfft(flux, axis=1), axis=0) / (nx * nz)
+        want = np.zeros((2 * nkz + 1, nkx + 1), complex)
+        for K in range(2 * nkz + 1):
+            kk = K if K <= nkz else K - 2 * nkz - 1
+            for i in range(nkx + 1):
+                want[K, i] = 1j * kk * kz1 * G[kk % nz, i]
+        got = diff[J, : 2 * nkz + 1, : nkx + 1]
+        np.testing.assert_allclose(got, want, rtol=0, atol=1e-13, err_msg=f"J={J}")
