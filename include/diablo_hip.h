@@ -149,8 +149,9 @@ int dh_mixed_precision(dh_ctx *ctx);
  * channel.f:303 is open (CREATE_NEW_FLOW = 0, or TIME_STEP > 100) and the
  * eddy-viscosity (and, for scalars, KAPPA_T = NU_T) terms in the implicit
  * solves always.  FP64 only (refused while the mixed form is on).  The first
- * switch allocates NU_T and the model's work fields, about
- * (3 (3 + N_TH) + 14 + 2 N_TH) physical and (10 + 3 N_TH) Fourier fields.
+ * switch allocates NU_T, a copy of CU and the model's work fields:
+ * 24 + 5 N_TH fields of (NY+2) NX NZ doubles and 13 + 3 N_TH of (NY+2)
+ * (NKX+1)(2 NKZ+1) complex values (97 MB at 64x97x64, N_TH = 0).
  * The periodic mean_les output (les.f:338-517) is not produced. */
 int dh_set_les(dh_ctx *ctx, int32_t on);
 /* 1 when LES is on, 0 when not. */
