@@ -164,7 +164,8 @@ int64_t read_f64(File &f, const std::string &path, double *out, int64_t cap, int
       n = 1;
       for (int r = 0; r < rank; ++r) {
         const uint64_t d = f.u(dims + 8ull * r, 8);
-        if (d > (1ull << 40) || (uint64_t)n * d > (1ull << 40)) throw H5Error{"dataspace too large"};
+        // n <= 2^40 here, so the product bound is tested without forming it
+        if (d > (1ull << 40) || (d != 0 && (uint64_t)n > (1ull << 40) / d)) throw H5Error{"dataspace too large"};
         n *= (int64_t)d;
       }
     } else if (m.type == 0x03) {  // datatype

@@ -75,12 +75,19 @@ C----*|--.---------.---------.---------.---------.---------.---------.-|-------|
           PRM%DRHODZ(N) = DRHODZ(N)
         END DO
         PRM%CREATE_NEW_FLOW = MERGE(1, 0, CREATE_NEW_FLOW)
-        IF (LES) STOP 'RK_CHAN_1 on the GPU: LES is not supported'
+C The device carries the constant-Smagorinsky model only; les.f:269-272
+C stops on the dynamic models too
+        IF (LES .AND. LES_MODEL_TYPE .NE. 1)
+     &    STOP 'RK_CHAN_1 on the GPU: LES_MODEL_TYPE must be 1'
         TS_FIRST = TIME_STEP
         RC = dh_create(PRM, CTX)
         IF (RC .NE. 0) STOP 'dh_create failed'
         RC = dh_set_grid(CTX, GY, GYF, DY, DYF)
         CALL DH_CHECK(CTX, RC, 'dh_set_grid')
+        IF (LES) THEN
+          RC = dh_set_les(CTX, 1)
+          CALL DH_CHECK(CTX, RC, 'dh_set_les')
+        END IF
         HOST_DIRTY = .TRUE.
       END IF
 

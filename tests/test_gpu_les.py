@@ -3,7 +3,7 @@ terms of the implicit solves) with the oracle's restatement of les.f and
 channel.f (needs an MI355X).
 
 The bar is that of the plain FP64 path: relative L2 <= max(1e-10,
-30 x floor) per field after the steps run, where the floor is the oracle's
+4 x floor) per field after the steps run, where the floor is the oracle's
 own spread under a 1e-16 perturbation of its input velocity; NU_T (a
 nonlinear function of the strain, with |.| and sqrt) is held to the same
 bar against the oracle's NU_T.  Cases: the LES channel of the reference's

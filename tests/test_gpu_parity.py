@@ -1,7 +1,7 @@
 """Parity of the HIP path with the CPU oracle (needs an MI355X).
 
 All work is FP64.  Each field must agree with the oracle to relative L2
-<= max(bar, 30 x floor), where `floor` is the oracle's own spread: the
+<= max(bar, 4 x floor), where `floor` is the oracle's own spread: the
 relative L2 change of its result when its input velocity is perturbed at
 the 1e-16 level (what two correct FP64 implementations with differently
 ordered FFTs differ by).  bar = 1e-10 after 10 RK3 steps (the gate for

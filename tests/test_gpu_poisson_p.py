@@ -4,7 +4,7 @@ restatement of channel.f:1217-1453 (needs an MI355X).
 The state is the deck's initial condition advanced two RK3 steps by the
 oracle, so every term of the right-hand side is non-zero (the small deck
 also has a scalar with buoyancy).  CP must agree to relative L2
-<= max(1e-11, 30 x floor), the floor being the oracle's own change when its
+<= max(1e-11, 4 x floor), the floor being the oracle's own change when its
 input velocity is perturbed at 1e-16: the right-hand side passes through
 nine transforms, and the lowest (kx,kz) Poisson modes are ill-conditioned
 on the stretched grid.  CU and CTH must come back unchanged."""
@@ -45,7 +45,7 @@ def test_poisson_p_matches_oracle(dor, name):
     errs = helpers.field_errors(dor, o, st, keys=keys)
     floor = helpers.field_errors(dor, o, helpers.oracle_state(dor, o2), keys=("cp",))
     print("  err", {k: f"{v:.1e}" for k, v in errs.items()}, " floor cp", f"{floor['cp']:.1e}", flush=True)
-    assert errs["cp"] <= max(1e-11, 30 * floor["cp"]), (errs, floor)
+    assert errs["cp"] <= max(1e-11, 4 * floor["cp"]), (errs, floor)
     for k in ("cu1", "cu2", "cu3"):
         assert errs[k] == 0.0, (k, errs[k])
     for n in range(o.n_th):
