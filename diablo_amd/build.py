@@ -1,34 +1,55 @@
 """Builds libdiablo_hip.so (hipcc, gfx950) in-tree.
 
+Each source is compiled to an object of its own, in parallel, and the
+objects are linked into the library.
+
 Usage: python -m diablo_amd.build [--check]
 """
+import concurrent.futures
 import os
 import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
+OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libdiablo_hip.so")
-SOURCES = ["dh_api.hip", "dh_fft.hip", "dh_step.hip", "dh_ics.hip", "dh_stats.hip", "dh_poisson.hip", "dh_les.hip", "dh_h5.cpp"]
+SOURCES = ["dh_api.hip", "dh_fft.hip", "dh_step.hip", "dh_ics.hip", "dh_stats.hip", "dh_poisson.hip", "dh_les.hip",
+           "dh_h5.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [
-    "-O3", "-std=c++17", "-fPIC", "-shared", "--offload-arch=gfx950",
+    "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
     # packed FP32 instructions stay disabled in every HIP object
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable", "-Wno-unused-result",
 ]
 
 
-def build(verbose=False):
-    srcs = [os.path.join(CSRC, s) for s in SOURCES]
-    cmd = [HIPCC] + FLAGS + ["-o", LIB + ".tmp"] + srcs
+def _compile(src, verbose):
+    obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
+    cmd = [HIPCC] + FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
+    return obj, r
+
+
+def build(verbose=False):
+    os.makedirs(OBJ, exist_ok=True)
+    jobs = min(len(SOURCES), max(1, int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    with concurrent.futures.ThreadPoolExecutor(jobs) as ex:
+        results = list(ex.map(lambda s: _compile(s, verbose), SOURCES))
+    failed = [r for _, r in results if r.returncode != 0]
+    for r in failed:
+        sys.stderr.write(r.stdout + r.stderr)
+    if failed:
+        raise RuntimeError("hipcc failed building libdiablo_hip.so")
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", LIB + ".tmp"] + [o for o, _ in results]
+    r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed building libdiablo_hip.so")
+        raise RuntimeError("hipcc failed linking libdiablo_hip.so")
     os.replace(LIB + ".tmp", LIB)
     return LIB
 

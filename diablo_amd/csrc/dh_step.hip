@@ -1189,6 +1189,11 @@ __device__ __forceinline__ void rem_div_sweep_seg(int seg, int s, int lane, int 
 constexpr int kRdtAhead = 4;
 constexpr int kRdtPad = 16;  // >= 4 * kRdtAhead - 1: the sweeps read up to 4 kRdtAhead - 1 rows past their range
 
+// Rows of the velocity kept in registers between the divergence and the
+// projection: J = 1 + rg + b ngrp for b < kRdtRegRows (all rows while NY <=
+// kRdtRegRows ngrp); rows beyond are read again for the projection.
+constexpr int kRdtRegRows = 5;
+
 __device__ __forceinline__ void rem_div_thomas_body(const RemDivArgs &a) {
 #pragma clang fp contract(off)
   extern __shared__ double st[];
@@ -1203,7 +1208,15 @@ __device__ __forceinline__ void rem_div_thomas_body(const RemDivArgs &a) {
   double *ru = st + 5 * ntot * kRdtModes + kRdtPad;
   const int lane = threadIdx.x % kRdtModes, rg = threadIdx.x / kRdtModes;
   const int ngrp = blockDim.x / kRdtModes;
-  const int m = blockIdx.x * kRdtModes + lane;
+  // mode groups dealt so that XCD x (blocks x, x+8, ...) takes a contiguous
+  // range: neighbouring groups share the cache lines their rows straddle
+  int grp;
+  {
+    const int nb = gridDim.x, lin = blockIdx.x;
+    const int x = lin & 7, q = nb >> 3, r = nb & 7;
+    grp = x * q + min(x, r) + (lin >> 3);
+  }
+  const int m = grp * kRdtModes + lane;
   const bool live = m < g.pf;
   const long long pf = g.pf;
   int kx = 0, kzp = 0;
@@ -1215,44 +1228,63 @@ __device__ __forceinline__ void rem_div_thomas_body(const RemDivArgs &a) {
   const bool mean = (kx == 0 && kzp == 0);
   for (int j = (int)threadIdx.x - kRdtPad; j <= NY + kRdtPad; j += blockDim.x)
     ru[j] = (j >= 0 && j <= NY) ? a.ru[j] : 0.0;
-  // right-hand sides of rows 2..NY-1 (rows 1, NY are BC rows with zero
-  // right-hand side) and the sweep tables of rows 2..NY, in batches of 4
-  // rows per thread
-  for (int J0 = 2 + rg; J0 <= NY; J0 += 4 * ngrp) {
-    double2 c1[4], c2[4], c2p[4], c3[4];
-    double ta[4], tb[4], tr[4];
+  // Each thread's rows J = 1 + rg + b ngrp: the velocity (kept for the
+  // projection), the pressure, CU2 at J+1 and the sweep tables, all loads
+  // issued before any arithmetic.  Right-hand sides of rows 2..NY-1 (rows 1,
+  // NY are BC rows with zero right-hand side) and the tables of rows 2..NY
+  // go to LDS.
+  const bool upd = a.rk >= 0;
+  double2 u1[kRdtRegRows], u2[kRdtRegRows], u3[kRdtRegRows], pc[kRdtRegRows];
+  auto stage_row = [&](int J, double2 c1, double2 c2, double2 c3, double2 c2p, double ta, double tb, double tr) {
+    double2 V = make_double2(0.0, 0.0);
+    if (live && J < NY) {
+      const double2 t1 = cmul_ik(kxv, c1), t3 = cmul_ik(kzv, c3);
+      const double d = g.dyf[J], rd = g.rdyf[J];
+      V = make_double2(t1.x + div_by_recip(c2p.x - c2.x, d, rd) + t3.x, t1.y + div_by_recip(c2p.y - c2.y, d, rd) + t3.y);
+    }
+    Gs[J * kRdtModes + lane] = V;
+    As[J * kRdtModes + lane] = live ? ta : 0.0;
+    Bs[J * kRdtModes + lane] = live ? tb : 1.0;
+    Rs[J * kRdtModes + lane] = live ? tr : 1.0;
+  };
+  {
+    double2 c2p[kRdtRegRows];
+    double ta[kRdtRegRows], tb[kRdtRegRows], tr[kRdtRegRows];
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      const int J = J0 + b * ngrp;
+    for (int b = 0; b < kRdtRegRows; ++b) {
+      const int J = 1 + rg + b * ngrp;
       if (live && J <= NY) {
-        ta[b] = a.ta[pf * J + m];
-        tb[b] = a.tb[pf * J + m];
-        tr[b] = a.tr[pf * J + m];
-        if (J < NY) {
-          c1[b] = a.cin[0][pf * J + m];
-          c3[b] = a.cin[2][pf * J + m];
-          c2[b] = a.cin[1][pf * J + m];
-          c2p[b] = a.cin[1][pf * (J + 1) + m];
+        u1[b] = a.cin[0][pf * J + m];
+        u3[b] = a.cin[2][pf * J + m];
+        u2[b] = a.cin[1][pf * J + m];
+        if (J >= 2) {
+          ta[b] = a.ta[pf * J + m];
+          tb[b] = a.tb[pf * J + m];
+          tr[b] = a.tr[pf * J + m];
+          if (upd) pc[b] = a.cp[pf * J + m];
+          if (J < NY) c2p[b] = a.cin[1][pf * (J + 1) + m];
         }
       }
     }
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      const int J = J0 + b * ngrp;
-      if (J <= NY) {
-        double2 V = make_double2(0.0, 0.0);
-        if (live && J < NY) {
-          const double2 t1 = cmul_ik(kxv, c1[b]), t3 = cmul_ik(kzv, c3[b]);
-          const double d = g.dyf[J], rd = g.rdyf[J];
-          V = make_double2(t1.x + div_by_recip(c2p[b].x - c2[b].x, d, rd) + t3.x,
-                           t1.y + div_by_recip(c2p[b].y - c2[b].y, d, rd) + t3.y);
-        }
-        Gs[J * kRdtModes + lane] = V;
-        As[J * kRdtModes + lane] = live ? ta[b] : 0.0;
-        Bs[J * kRdtModes + lane] = live ? tb[b] : 1.0;
-        Rs[J * kRdtModes + lane] = live ? tr[b] : 1.0;
-      }
+    for (int b = 0; b < kRdtRegRows; ++b) {
+      const int J = 1 + rg + b * ngrp;
+      if (J >= 2 && J <= NY) stage_row(J, u1[b], u2[b], u3[b], c2p[b], ta[b], tb[b], tr[b]);
     }
+  }
+  for (int J = 1 + rg + kRdtRegRows * ngrp; J <= NY; J += ngrp) {  // rows beyond the registers
+    double2 c1 = make_double2(0.0, 0.0), c2 = c1, c3 = c1, c2p = c1;
+    double ta = 0.0, tb = 1.0, tr = 1.0;
+    if (live) {
+      c1 = a.cin[0][pf * J + m];
+      c3 = a.cin[2][pf * J + m];
+      c2 = a.cin[1][pf * J + m];
+      ta = a.ta[pf * J + m];
+      tb = a.tb[pf * J + m];
+      tr = a.tr[pf * J + m];
+      if (J < NY) c2p = a.cin[1][pf * (J + 1) + m];
+    }
+    stage_row(J, c1, c2, c3, c2p, ta, tb, tr);
   }
   lds_barrier();
   if (a.seg > 0) {
@@ -1347,39 +1379,32 @@ __device__ __forceinline__ void rem_div_thomas_body(const RemDivArgs &a) {
   }
   lds_barrier();
   if (!live) return;
-  const double hb = (a.rk < 0) ? 1.0 : (a.cp_old ? a.clk->h_cp : a.clk->h_bar[a.rk]);
+  const double hb = upd ? (a.cp_old ? a.clk->h_cp : a.clk->h_bar[a.rk]) : 1.0;
   const double rhb = 1.0 / hb;
-  for (int J0 = 1 + rg; J0 <= NY; J0 += 4 * ngrp) {
-    double2 u1[4], u2[4], u3[4], pc[4];
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      const int J = J0 + b * ngrp;
-      if (J <= NY) {
-        u1[b] = a.cin[0][pf * J + m];
-        u3[b] = a.cin[2][pf * J + m];
-        u2[b] = a.cin[1][pf * J + m];
-        if (a.rk >= 0) pc[b] = a.cp[pf * J + m];
-      }
+  auto project = [&](int J, double2 c1, double2 c2, double2 c3, double2 p) {
+    const double2 phi = Gs[J * kRdtModes + lane];
+    if (!upd) a.phi[pf * J + m] = phi;  // REM_DIV alone leaves phi (CR1) behind
+    double2 t = cmul_ik(kxv, phi);
+    a.cu[0][pf * J + m] = make_double2(c1.x - t.x, c1.y - t.y);
+    t = cmul_ik(kzv, phi);
+    a.cu[2][pf * J + m] = make_double2(c3.x - t.x, c3.y - t.y);
+    if (J >= 2) {
+      const double2 pm = Gs[(J - 1) * kRdtModes + lane];
+      const double d = g.dy[J], rd = g.rdy[J];
+      a.cu[1][pf * J + m] =
+          make_double2(c2.x - div_by_recip(phi.x - pm.x, d, rd), c2.y - div_by_recip(phi.y - pm.y, d, rd));
+      if (upd) a.cp[pf * J + m] = make_double2(p.x + div_by_recip(phi.x, hb, rhb), p.y + div_by_recip(phi.y, hb, rhb));
     }
+  };
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      const int J = J0 + b * ngrp;
-      if (J > NY) continue;
-      const double2 phi = Gs[J * kRdtModes + lane];
-      a.phi[pf * J + m] = phi;
-      double2 t = cmul_ik(kxv, phi);
-      a.cu[0][pf * J + m] = make_double2(u1[b].x - t.x, u1[b].y - t.y);
-      t = cmul_ik(kzv, phi);
-      a.cu[2][pf * J + m] = make_double2(u3[b].x - t.x, u3[b].y - t.y);
-      if (J >= 2) {
-        const double2 pm = Gs[(J - 1) * kRdtModes + lane];
-        const double d = g.dy[J], rd = g.rdy[J];
-        a.cu[1][pf * J + m] = make_double2(u2[b].x - div_by_recip(phi.x - pm.x, d, rd),
-                                           u2[b].y - div_by_recip(phi.y - pm.y, d, rd));
-        if (a.rk >= 0)
-          a.cp[pf * J + m] = make_double2(pc[b].x + div_by_recip(phi.x, hb, rhb), pc[b].y + div_by_recip(phi.y, hb, rhb));
-      }
-    }
+  for (int b = 0; b < kRdtRegRows; ++b) {
+    const int J = 1 + rg + b * ngrp;
+    if (J <= NY) project(J, u1[b], u2[b], u3[b], pc[b]);
+  }
+  for (int J = 1 + rg + kRdtRegRows * ngrp; J <= NY; J += ngrp) {
+    const double2 c1 = a.cin[0][pf * J + m], c2 = a.cin[1][pf * J + m], c3 = a.cin[2][pf * J + m];
+    const double2 p = upd ? a.cp[pf * J + m] : make_double2(0.0, 0.0);
+    project(J, c1, c2, c3, p);
   }
   // planes the projection leaves alone (the ghost planes 0 and NY+1, and
   // plane 1 of CU2) pass from the transformed velocity unchanged
@@ -1394,8 +1419,13 @@ __device__ __forceinline__ void rem_div_thomas_body(const RemDivArgs &a) {
     a.cu[1][pf + m] = a.cin[1][pf + m];
   }
 }
+// Two blocks per CU (the velocity rows held across the sweep take the
+// registers of a third): 512 resident blocks cover the 457 of a 128 x 128
+// plane's modes at once.
 __global__ void __launch_bounds__(256) k_rem_div_thomas(RemDivArgs a) { rem_div_thomas_body(a); }
-__global__ void __launch_bounds__(256) k_rem_div_thomas_b(const RemDivArgs *__restrict__ av) { rem_div_thomas_body(av[blockIdx.z]); }
+__global__ void __launch_bounds__(256) k_rem_div_thomas_b(const RemDivArgs *__restrict__ av) {
+  rem_div_thomas_body(av[blockIdx.z]);
+}
 
 // ---------------------------------------------------------------------------
 // launchers
