@@ -248,17 +248,24 @@ def les_form(steps, warmup, reps=10):
 
     p, gy, ic = configs.les_channel()
     pts = p["nx"] * p["ny"] * p["nz"]
-    # the learned closure's network that returns |S|: the model itself, in BF16
-    w = np.zeros(8 * 32 + 32 + 32 * 32 + 32 + 32 + 1, np.float32)
-    w[0] = 1.0                      # W1[0][0]
-    w[8 * 32 + 32] = 1.0            # W2[0][0]
-    w[8 * 32 + 32 + 32 * 32 + 32] = 1.0  # w3[0]
+    # the learned closure: the network trained on filtered DNS when its
+    # parameters ship with the package, else one of random weights (the cost
+    # does not depend on the values)
+    from diablo_amd import closure
+    trained = os.path.join(ROOT, "diablo_amd", "data", "closure_re180.npz")
+    if os.path.exists(trained):
+        z = np.load(trained)
+        w, wsrc = z["params"], "trained on filtered Re_tau=180 DNS"
+        scales = dict(ts=float(z["ts"]), ls=float(z["ls"]), nus=float(z["nus"]))
+    else:
+        w, wsrc = closure.init_params(0), "random weights"
+        scales = dict(ts=p["nu"], ls=p["nu"], nus=p["nu"])
     res = {}
     for les in (True, False, "closure"):
         ch = channel.Channel(p, gy)
         ch.set_les(bool(les))
         if les == "closure":
-            ch.set_les_closure(w)
+            ch.set_les_closure(w, **scales)
         ch.initial_condition(ic)
         ch.set_clock(0.0, p["delta_t"], 0)
         threads = ch.autotune(2)  # block size of the 64x64-plane kernels, state restored
@@ -278,8 +285,97 @@ def les_form(steps, warmup, reps=10):
             "block_size": res[True][2],
             "without_model": {"value": round(res[False][0], 3), "ms_per_step": round(res[False][1], 4)},
             "learned_closure": {"value": round(res["closure"][0], 3), "ms_per_step": round(res["closure"][1], 4),
-                                "network": "8 -> 32 -> 32 -> 1 BF16 MFMA, weights returning |S| (the model)"},
+                                "network": f"6 -> 64 -> 64 -> 64 -> 1 GELU/softplus, BF16 MFMA 16x16x32, {wsrc}"},
             "phases_ms": phases}
+
+
+def _event_ms(fn, reps):
+    """Mean milliseconds of fn() over reps launches on the current (null)
+    stream, between HIP events recorded on that stream."""
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    e1.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def batched_microbench(reps=20):
+    """Batched solver and transform microbenchmark: 257 x 512 complex
+    Crank-Nicolson systems of NY = 257 rows on the tanh grid (CS = 1.75),
+    alpha log-uniform in [1e-3, 10] (seed 2), complex Gaussian right-hand
+    sides (seeded on the device), FP64 and FP32 with the residual in FP64;
+    and 257 planes of 512 x 512 through the x/z transforms both ways, with
+    the error of 2 planes against numpy.fft.  Algorithmic bytes: 32 B per
+    row (FP64 solve: right-hand side in, solution out; 16 B in FP32); per
+    transform direction 8 B per real point + 16 B per complex mode."""
+    import numpy as np
+    import torch
+    from diablo_amd import batched
+
+    dev = "cuda:0"
+    ny, nkx, nkz = 257, 257, 512
+    nsys = nkx * nkz
+    sl, su = batched.cn_rows(ny)
+    alpha = np.exp(np.random.default_rng(2).uniform(np.log(1e-3), np.log(10.0), nsys))
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    sl_d, su_d, al_d = d(sl), d(su), d(alpha)
+    g = torch.Generator(device=dev).manual_seed(3)
+    b = torch.complex(torch.randn((ny, nsys), generator=g, device=dev, dtype=torch.float64),
+                      torch.randn((ny, nsys), generator=g, device=dev, dtype=torch.float64))
+    x = torch.empty_like(b)
+    out = {"workload": "batched: 257x512 complex CN systems of NY=257 (tanh grid CS=1.75, alpha log-uniform "
+                       "[1e-3,10]) + 257 planes of 512x512 x/z transforms", "rows": ny * nsys}
+    res = {}
+    for fp32 in (False, True):
+        bb = b.to(torch.complex64) if fp32 else b
+        xx = torch.empty_like(bb)
+        ms = _event_ms(lambda: batched.cn_solve(sl_d, su_d, al_d, bb, xx, fp32=fp32), reps)
+        xr = xx.to(torch.complex128)
+        r = batched.cn_matrix_apply(sl_d, su_d, al_d, xr) - b
+        resid = float(torch.linalg.vector_norm(r) / torch.linalg.vector_norm(b))
+        byts = (16 if fp32 else 32) * ny * nsys
+        gbs = byts / (ms * 1e-3) / 1e9
+        res["fp32" if fp32 else "fp64"] = {"ms": round(ms, 4), "GBs": round(gbs, 1),
+                                           "frac": round(gbs / HBM_PEAK_GBS, 4), "bytes": byts,
+                                           "residual_fp64": float(f"{resid:.3g}")}
+        del xx, xr, r
+    out["cn_solve"] = res
+    del b, x
+    torch.cuda.empty_cache()
+    n = 257
+    rng = np.random.default_rng(4)
+    sample = rng.standard_normal((2, 512, 512))
+    gp = torch.Generator(device=dev).manual_seed(4)
+    fft = {}
+    for fp32 in (False, True):
+        rdt, cdt = (torch.float32, torch.complex64) if fp32 else (torch.float64, torch.complex128)
+        phys = torch.randn((n, 512, 512), generator=gp, device=dev, dtype=torch.float64).to(rdt)
+        phys[:2] = torch.from_numpy(sample).to(dev).to(rdt)
+        four = torch.empty((n, 512, 257), dtype=cdt, device=dev)
+        back = torch.empty_like(phys)
+        ms_f = _event_ms(lambda: batched.fft_to_fourier(phys, four, fp32=fp32), reps)
+        ms_i = _event_ms(lambda: batched.fft_to_physical(four, back, fp32=fp32), reps)
+        ref = np.fft.fft(np.fft.rfft(sample, axis=-1) / 512, axis=-2) / 512
+        f2 = four[:2].cpu().numpy().astype(np.complex128)
+        err = float(np.linalg.norm(f2 - ref) / np.linalg.norm(ref))
+        rt = float(torch.linalg.vector_norm((back - phys).double()) / torch.linalg.vector_norm(phys.double()))
+        esz = 4 if fp32 else 8
+        byts = n * (esz * 512 * 512 + 2 * esz * 512 * 257)
+        fft["fp32" if fp32 else "fp64"] = {
+            "forward_ms": round(ms_f, 4), "inverse_ms": round(ms_i, 4), "bytes_per_direction": byts,
+            "forward_GBs": round(byts / (ms_f * 1e-3) / 1e9, 1), "inverse_GBs": round(byts / (ms_i * 1e-3) / 1e9, 1),
+            "forward_frac": round(byts / (ms_f * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "inverse_frac": round(byts / (ms_i * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "rel_l2_vs_numpy_2_planes": float(f"{err:.3g}"), "round_trip_rel_l2": float(f"{rt:.3g}")}
+        del phys, four, back
+        torch.cuda.empty_cache()
+    out["fft512"] = fft
+    return out
 
 
 def main():
@@ -290,6 +386,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="leave out the FP32/FP64 mixed form's line")
     ap.add_argument("--no-les", action="store_true", help="leave out the LES channel's line")
+    ap.add_argument("--no-batched", action="store_true", help="leave out the batched solver/transform line")
     ap.add_argument("--profile-reps", type=int, default=10)
     ap.add_argument("--ensemble", type=int, default=0, metavar="N",
                     help="only time N uneven shear-layer instances stepped as one graph")
@@ -364,6 +461,8 @@ def main():
         out["les_form"] = les_form(args.steps, args.warmup)
         if not args.no_cpu_baseline:
             out["les_form"]["cpu_baseline"] = les_cpu_baseline()
+    if not args.no_batched:
+        out["batched_microbench"] = batched_microbench()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(p, gy, ic)
     ch.close()

@@ -14,8 +14,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libdiablo_hip.so")
-SOURCES = ["dh_api.hip", "dh_fft.hip", "dh_step.hip", "dh_ics.hip", "dh_stats.hip", "dh_poisson.hip", "dh_les.hip",
-           "dh_h5.cpp"]
+# every HIP and C++ source of csrc/ goes into the library
+SOURCES = sorted(f for f in os.listdir(CSRC) if f.endswith((".hip", ".cpp")))
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [

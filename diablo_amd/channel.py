@@ -98,6 +98,27 @@ EXPORTS = {
     "dh_fft_xz_to_fourier_dev": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32]),
     "dh_fft_xz_to_physical_dev": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32]),
     "dh_field_dev": (ctypes.c_void_p, [ctypes.c_void_p, ctypes.c_int32]),
+    "dh_batched_last_error": (ctypes.c_char_p, []),
+    "dh_les_features": (ctypes.c_int, [ctypes.c_void_p]),
+    "dh_mlp_last_error": (ctypes.c_char_p, []),
+    "dh_mlp_nparams": (ctypes.c_int32, []),
+    "dh_mlp_packed_bytes": (ctypes.c_int64, []),
+    "dh_mlp_work_bytes": (ctypes.c_int64, [ctypes.c_int64]),
+    "dh_mlp_pack_dev": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "dh_mlp_infer_dev": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                        ctypes.c_int64, ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p]),
+    "dh_mlp_grad_dev": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                       ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
+                                       ctypes.c_void_p, ctypes.c_void_p]),
+    "dh_mlp_adam_dev": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                       ctypes.c_float, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p]),
+    "dh_cn_solve_batched_dev": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                               ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_int32,
+                                               ctypes.c_void_p]),
+    "dh_fft512_xz_to_fourier_dev": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32,
+                                                   ctypes.c_void_p]),
+    "dh_fft512_xz_to_physical_dev": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32,
+                                                    ctypes.c_void_p]),
     "dh_stream": (ctypes.c_void_p, [ctypes.c_void_p]),
 }
 
@@ -294,15 +315,20 @@ class Channel:
     def les(self):
         return bool(self.lib.dh_les(self.h))
 
-    def set_les_closure(self, weights=None):
-        """Learned eddy viscosity (MLP 8 -> 32 -> 32 -> 1, BF16): weights as the
-        flat float32 array W1[8][32], b1, W2[32][32], b2, w3, b3; None returns
-        to the Smagorinsky model."""
-        if weights is None:
+    def set_les_closure(self, params=None, ts=1.0, ls=1.0, nus=1.0):
+        """Learned eddy viscosity NU_T = nus softplus(y) (the network of
+        dh_mlp_*: 6 -> 64 -> 64 -> 64 -> 1, GELU, BF16 MFMA): params the flat
+        float32 parameter vector, ts and ls the time and length scales of its
+        inputs; None returns to the Smagorinsky model."""
+        if params is None:
             self._check(self.lib.dh_set_les_closure(self.h, None, 0), "dh_set_les_closure")
             return
-        w = np.ascontiguousarray(weights, dtype=np.float32).ravel()
+        w = np.concatenate([np.asarray(params, dtype=np.float32).ravel(), np.float32([ts, ls, nus])])
         self._check(self.lib.dh_set_les_closure(self.h, w.ctypes.data, w.size), "dh_set_les_closure")
+
+    def les_features(self):
+        """The closure's inputs from the current state (dh_field_dev 301)."""
+        self._check(self.lib.dh_les_features(self.h), "dh_les_features")
 
     @property
     def plane_threads(self):

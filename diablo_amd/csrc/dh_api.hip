@@ -9,6 +9,7 @@
 #include <new>
 
 #include "dh_internal.h"
+#include "dh_mlp_dev.h"
 
 using namespace dh;
 
@@ -133,7 +134,7 @@ hipError_t dalloc(T **p, size_t count) {
 void free_all(Ctx &c) {
   void *ptrs[] = {c.tw_row, c.tw_col, c.tw_real, c.posr, c.posrd, c.posz, c.posc, c.kx, c.kz, c.kx2, c.kz2, c.d_dy, c.d_dyf, c.d_gyf, c.d_rdy, c.d_rdyf, c.cu[0], c.cu[1],
                   c.cu[2], c.cp, c.cf[0], c.cf[1], c.cf[2], c.cth, c.cfth, c.phi, c.uphys, c.rphys, c.rd_ta,
-                  c.rd_tb, c.clock, c.fac.pf, c.fac.pb, c.fac.tl, c.fac.tb, c.fac.tc, c.fac.tr, c.fac.isbc, c.rd_tr, c.fac.vconst, c.rd_l, c.rd_u, c.stats, c.cun[0], c.cun[1], c.cun[2], c.uphys32, c.rphys32, c.fac.bcl, c.fac.bcd, c.fac.bcu, c.nut, c.cul, c.les_four, c.les_phys, c.sgs, c.les_wk, c.d_lesw, c.d_mlp_w, c.d_mlp_b, c.les_feat};
+                  c.rd_tb, c.clock, c.fac.pf, c.fac.pb, c.fac.tl, c.fac.tb, c.fac.tc, c.fac.tr, c.fac.isbc, c.rd_tr, c.fac.vconst, c.rd_l, c.rd_u, c.stats, c.cun[0], c.cun[1], c.cun[2], c.uphys32, c.rphys32, c.fac.bcl, c.fac.bcd, c.fac.bcu, c.nut, c.cul, c.les_four, c.les_phys, c.sgs, c.les_wk, c.d_lesw, c.d_mlp_p, c.d_mlp_pk, c.les_feat};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c.step_graph) (void)hipGraphExecDestroy(c.step_graph);
@@ -724,19 +725,9 @@ int dh_set_les(dh_ctx *h, int32_t on) {
 
 int dh_les(dh_ctx *h) { return h ? (h->c.les ? 1 : 0) : DH_EARG; }
 
-// FP32 -> BF16, round to nearest even (finite inputs).
-static short host_bf16(float x) {
-  unsigned u;
-  memcpy(&u, &x, 4);
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  return (short)(u >> 16);
-}
-
 int dh_set_les_closure(dh_ctx *h, const float *w, int32_t n) {
   if (!h) return DH_EARG;
   Ctx &c = h->c;
-  constexpr int H = kMlpHidden, F = kLesFeat;
-  constexpr int kCount = F * H + H + H * H + H + H + 1;
   if (!w) {
     if (c.les_mlp) {
       HIPCHK(hipStreamSynchronize(c.stream));
@@ -745,41 +736,38 @@ int dh_set_les_closure(dh_ctx *h, const float *w, int32_t n) {
     }
     return DH_OK;
   }
-  if (n != kCount) return fail(c, DH_EARG, "the closure takes 8x32 + 32 + 32x32 + 32 + 32 + 1 = 1377 weights");
+  if (n != kMlpNP + 3)
+    return fail(c, DH_EARG, "the closure takes the network's 10497 parameters then its time, length and NU_T scales");
   for (int i = 0; i < n; ++i)
-    if (!std::isfinite(w[i])) return fail(c, DH_EARG, "closure weights must be finite");
+    if (!std::isfinite(w[i])) return fail(c, DH_EARG, "closure parameters must be finite");
+  if (!(w[kMlpNP] > 0.f && w[kMlpNP + 1] > 0.f && w[kMlpNP + 2] > 0.f))
+    return fail(c, DH_EARG, "the closure's scales must be positive");
   HIPCHK(hipStreamSynchronize(c.stream));
-  if (!c.d_mlp_w) {
-    HIPCHK(dalloc(&c.d_mlp_w, 1536));
-    HIPCHK(dalloc(&c.d_mlp_b, 3 * H + 1));
-    HIPCHK(dalloc(&c.les_feat, (size_t)F * (c.g.ny + 2) * c.g.pp));
+  if (!c.d_mlp_p) {
+    HIPCHK(dalloc(&c.d_mlp_p, (size_t)kMlpNP));
+    HIPCHK(hipMalloc(&c.d_mlp_pk, 16 * (size_t)kPAll));
+    HIPCHK(dalloc(&c.les_feat, (size_t)kLesFeat * (c.g.ny + 2) * c.g.pp));
   }
-  const float *W1 = w, *b1 = W1 + F * H, *W2 = b1 + H, *b2 = W2 + H * H, *w3 = b2 + H, *b3 = w3 + H;
-  std::vector<short> op(1536, 0);
-  for (int t = 0; t < 2; ++t)  // layer 1, A = W1^T: hidden 16t + l%16, features 4(l/16) + j
-    for (int l = 0; l < 64; ++l)
-      for (int j = 0; j < 4; ++j) {
-        const int f = 4 * (l / 16) + j, hh = 16 * t + l % 16;
-        op[(t * 64 + l) * 4 + j] = (f < F) ? host_bf16(W1[f * H + hh]) : 0;
-      }
-  for (int kb = 0; kb < 2; ++kb)  // layer 2, B = W2: inputs 16kb + 4(l/16) + j, outputs 16nt + l%16
-    for (int nt = 0; nt < 2; ++nt)
-      for (int l = 0; l < 64; ++l)
-        for (int j = 0; j < 4; ++j)
-          op[((2 + 2 * kb + nt) * 64 + l) * 4 + j] = host_bf16(W2[(16 * kb + 4 * (l / 16) + j) * H + 16 * nt + l % 16]);
-  std::vector<float> bb(3 * H + 1);
-  for (int i = 0; i < H; ++i) {
-    bb[i] = b1[i];
-    bb[H + i] = b2[i];
-    bb[2 * H + i] = w3[i];
-  }
-  bb[3 * H] = b3[0];
-  HIPCHK(hipMemcpy(c.d_mlp_w, op.data(), op.size() * sizeof(short), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(c.d_mlp_b, bb.data(), bb.size() * sizeof(float), hipMemcpyHostToDevice));
-  if (!c.les_mlp) {
-    c.les_mlp = true;
-    drop_graphs(c);
-  }
+  HIPCHK(hipMemcpy(c.d_mlp_p, w, sizeof(float) * kMlpNP, hipMemcpyHostToDevice));
+  HIPCHK(launch_mlp_pack(c.d_mlp_p, c.d_mlp_pk, c.stream));
+  HIPCHK(hipStreamSynchronize(c.stream));
+  c.mlp_ts = w[kMlpNP];
+  c.mlp_ls = w[kMlpNP + 1];
+  c.mlp_nus = w[kMlpNP + 2];
+  if (!c.les_mlp) c.les_mlp = true;
+  drop_graphs(c);  // the captured steps hold the old scales
+  return DH_OK;
+}
+
+int dh_les_features(dh_ctx *h) {
+  if (!h) return DH_EARG;
+  Ctx &c = h->c;
+  if (!c.les || !c.les_mlp) return fail(c, DH_ESTATE, "the closure's inputs need LES and the closure on");
+  if (!c.grid_set) return fail(c, DH_ESTATE, "no grid");
+  HIPCHK(launch_les_bc_four(c, c.stream));
+  HIPCHK(launch_state_to_phys(c));
+  HIPCHK(launch_les_features(c, c.stream));
+  HIPCHK(hipStreamSynchronize(c.stream));
   return DH_OK;
 }
 

@@ -184,14 +184,18 @@ struct Ctx {
   double *les_wk = nullptr;     // [2][nfields][NY+2][NZ][NX]
   double *d_lesw = nullptr;     // DYW (0:NY+1) then DYFW (0:NY+1)
   // learned closure in place of NU_T = C_s^2 Delta^2 |S| (no counterpart in
-  // the reference): BF16 weights in MFMA operand order, FP32 biases, and
-  // the strain features it reads [kLesFeat][NY+2][NZ][NX]
+  // the reference): the network's FP32 parameters, their BF16 operand
+  // packing (dh_mlp_dev.h), the scales of its inputs and output, and the
+  // inputs and strain it reads and writes [kLesFeat][NY+2][NZ][NX]
   bool les_mlp = false;
-  short *d_mlp_w = nullptr;     // layer-1 A operands [2][64][4], layer-2 B operands [2][2][64][4]
-  float *d_mlp_b = nullptr;     // b1[32], b2[32], w3[32], b3
+  float *d_mlp_p = nullptr;     // parameters (kMlpNP floats)
+  void *d_mlp_pk = nullptr;     // packed operands (kPAll x 16 bytes)
+  double mlp_ts = 1.0, mlp_ls = 1.0, mlp_nus = 1.0;  // time scale, length scale, NU_T scale
   double *les_feat = nullptr;
 };
-constexpr int kLesFeat = 8, kMlpHidden = 32;
+// feature slots of the learned closure: 0..5 the network's inputs, 6..11
+// S11, S22, S33, S12, S13, S23 at GY, 12 |S| at GY
+constexpr int kLesFeat = 13;
 
 // Work fields of the LES pass: Fourier slots 0..6 (S11, S22, S33, S4, S5,
 // S6 in, then |S| S products and cross terms out), 7 + 2n and 8 + 2n for
@@ -268,6 +272,8 @@ hipError_t launch_apply_bc_vel(Ctx &c);
 // increments, physical BCs) after it; the per-column solve with NU_T
 hipError_t launch_les_bc_four(Ctx &c, hipStream_t st);
 hipError_t launch_les_model(Ctx &c, hipStream_t st);
+hipError_t launch_les_features(Ctx &c, hipStream_t st);  // the closure's inputs only (no model step)
+hipError_t launch_mlp_pack(const float *params, void *packed, hipStream_t st);
 hipError_t launch_solve_les_range(Ctx &c, int rk, int courant, int f0, int f1, hipStream_t st);
 // fields f0..f0+nf-1 of strided work arrays (Fourier stride (NY+2) pf,
 // physical (NY+2) NX NZ), planes j0..j0+nplanes-1, in batches of kMaxFields

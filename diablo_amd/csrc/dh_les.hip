@@ -24,6 +24,7 @@
 // reference's implicit NU_T terms, which are not gated, then see.
 #include "dh_fft_dev.h"
 #include "dh_internal.h"
+#include "dh_mlp_dev.h"
 
 namespace dh {
 
@@ -43,11 +44,13 @@ struct LesArgs {
   int jstart, jend, jstart_th[DH_MAX_TH], jend_th[DH_MAX_TH];
   int ubmin, vbmin, wbmin, ubmax, vbmax, wbmax;
   double uc1min, vc1min, wc1min, uc1max, vc1max, wc1max;
-  // learned closure: features out of k_les_nut, NU_T from k_les_mlp
+  // learned closure: inputs out of k_les_nut, NU_T from k_les_mlp
   int mlp;
   double *feat;
-  const short *mlp_w;
-  const float *mlp_b;
+  const bf16x8 *mlp_pk;
+  const float *mlp_p;
+  double mlp_ts, mlp_ls, mlp_nus;
+  const double *gyf;  // GYF (0:NY+1)
 };
 
 __device__ __forceinline__ bool les_gate_open(const LesArgs &a) { return !a.cnf || a.clk->time_step > 100; }
@@ -77,8 +80,10 @@ __global__ void k_les_strain(LesArgs a) {
     F(2) = cmul_ik(KZ, U3);
     const double2 t = cadd(cmul_ik(KZ, U1), cmul_ik(KX, U3));
     F(4) = make_double2(0.5 * t.x, 0.5 * t.y);
+    if (a.mlp) F(6) = cmul_ik(KZ, U1);  // dU1/dz, for the rotation the closure reads
   } else {
     F(0) = F(1) = F(2) = F(4) = zero;
+    if (a.mlp) F(6) = zero;
   }
   const double dy = g.dy[J];
   const double2 ik2x = cmul_ik(KX, U2), ik2z = cmul_ik(KZ, U2);
@@ -89,6 +94,76 @@ __global__ void k_les_strain(LesArgs a) {
     F(kLesFour + 2 * n) = cmul_ik(KX, th);
     F(kLesFour + 2 * n + 1) = cmul_ik(KZ, th);
   }
+}
+
+// The learned closure's inputs at the GY point (x, z) of plane J (zero on
+// row 1): the five invariants of the resolved strain S and rotation O
+// (Pope 1975): tr S^2, tr O^2, tr S^3, tr O^2 S, tr O^2 S^2, in units of
+// the time scale ts, and the wall distance (1 - |y|) in units of the length
+// scale ls, each through asinh; then S11, S22, S33, S12, S13, S23 and |S|
+// (the full strain, with the mean shear) for the training targets.  S11,
+// S22, S33, S13 and dU1/dz live at GYF and are interpolated to GY as
+// les_chan interpolates (les.f:137-160); S12, S23 and the wall-normal
+// derivatives are at GY.  The products the subgrid stress needs are left as
+// the strain (S_ij, with the implicit dU/dy parts taken out of S12 and S23);
+// k_les_post multiplies them by -2 NU_T over the filter widths once NU_T is
+// known.
+__device__ void les_closure_inputs(const LesArgs &a, int J, long long col, double sy) {
+  const Geo &g = a.g;
+  const long long pp = g.pp, fs = a.fs;
+  const int NY = g.ny;
+  const double *DY = g.dy, *DYF = g.dyf;
+  auto S = [&](int s, int j) { return a.phys[s * fs + (long long)j * pp + col]; };
+  const int out = kLesPhys - kLesFour + 2 * a.nth;
+  auto P = [&](int s) -> double & { return a.phys[(out + s) * fs + (long long)J * pp + col]; };
+  const double *U1 = a.uphys, *U3 = a.uphys + 2 * fs;
+  auto u = [&](const double *U, int j) { return U[(long long)j * pp + col]; };
+  const bool gyf_row = J >= a.jstart && J <= a.jend;
+  P(0) = gyf_row ? S(0, J) : 0.0;
+  P(1) = gyf_row ? S(2, J) : 0.0;
+  P(3) = gyf_row ? S(4, J) : 0.0;
+  double f[kLesFeat] = {};
+  if (J >= 2) {
+    const double du1 = (u(U1, J) - u(U1, J - 1)) / DY[J], du3 = (u(U3, J) - u(U3, J - 1)) / DY[J];
+    P(2) = S(3, J) - 0.5 * du1;
+    P(4) = S(5, J) - 0.5 * du3;
+    auto gyi = [&](int s) { return (S(s, J) * DYF[J - 1] + S(s, J - 1) * DYF[J]) / (2.0 * DY[J]); };
+    const double s11 = gyi(0), s22 = gyi(1), s33 = gyi(2), s13 = gyi(4), s12 = S(3, J), s23 = S(5, J);
+    const double o12 = du1 - s12, o23 = s23 - du3, o13 = gyi(6) - s13;  // O_ij = (du_i/dx_j - du_j/dx_i)/2
+    // S and O as 3 x 3 matrices (O antisymmetric)
+    const double Sm[3][3] = {{s11, s12, s13}, {s12, s22, s23}, {s13, s23, s33}};
+    const double Om[3][3] = {{0.0, o12, o13}, {-o12, 0.0, o23}, {-o13, -o23, 0.0}};
+    double S2[3][3], O2[3][3];
+    for (int i = 0; i < 3; ++i)
+      for (int k = 0; k < 3; ++k) {
+        S2[i][k] = Sm[i][0] * Sm[0][k] + Sm[i][1] * Sm[1][k] + Sm[i][2] * Sm[2][k];
+        O2[i][k] = Om[i][0] * Om[0][k] + Om[i][1] * Om[1][k] + Om[i][2] * Om[2][k];
+      }
+    double l1 = 0, l2 = 0, l3 = 0, l4 = 0, l5 = 0;
+    for (int i = 0; i < 3; ++i)
+      for (int k = 0; k < 3; ++k) {
+        l1 += Sm[i][k] * Sm[k][i];
+        l2 += Om[i][k] * Om[k][i];
+        l3 += S2[i][k] * Sm[k][i];
+        l4 += O2[i][k] * Sm[k][i];
+        l5 += O2[i][k] * S2[k][i];
+      }
+    const double t1 = a.mlp_ts, t2 = t1 * t1, t3 = t2 * t1;
+    f[0] = asinh(l1 * t2);
+    f[1] = asinh(l2 * t2);
+    f[2] = asinh(l3 * t3);
+    f[3] = asinh(l4 * t3);
+    f[4] = asinh(l5 * t2 * t2);
+    const double gyJ = a.gyf[J] - 0.5 * DYF[J];  // GY(J)
+    f[5] = asinh(fmax(0.0, 1.0 - fabs(gyJ)) / a.mlp_ls);
+    f[6] = s11; f[7] = s22; f[8] = s33; f[9] = s12; f[10] = s13; f[11] = s23;
+    f[12] = sqrt(2.0 * l1);
+  } else {
+    P(2) = P(4) = 0.0;
+  }
+  for (int q = 0; q < kLesFeat; ++q) a.feat[q * fs + (long long)J * pp + col] = f[q];
+  (void)sy;
+  (void)NY;
 }
 
 // Point (x, z) of plane J = 1 + blockIdx.y: |S| at GYF (les.f:113-134) and
@@ -141,20 +216,7 @@ __global__ void k_les_nut(LesArgs a) {
   P(2) = (J >= 2) ? sy * (S(3, J) - 0.5 * du1 / DY[J]) : 0.0;  // |S| S4 less the implicit dU1/dy
   P(4) = (J >= 2) ? sy * (S(5, J) - 0.5 * du3 / DY[J]) : 0.0;  // |S| S6 less the implicit dU3/dy
   if (a.mlp) {
-    // the closure's inputs at GY (zero on row 1); NU_T and the terms that
-    // use it follow in k_les_mlp and k_les_post
-    double f[kLesFeat] = {};
-    if (J >= 2) {
-      auto gyi = [&](int s) { return (S(s, J) * DYF[J - 1] + S(s, J - 1) * DYF[J]) / (2.0 * DY[J]); };
-      f[0] = sy;
-      f[1] = S(3, J) - 0.5 * (mean(0, J) - mean(0, J - 1)) / DY[J];
-      f[2] = S(5, J) - 0.5 * (mean(2, J) - mean(2, J - 1)) / DY[J];
-      f[3] = gyi(0);
-      f[4] = gyi(1);
-      f[5] = gyi(2);
-      f[6] = gyi(4);
-    }
-    for (int q = 0; q < kLesFeat; ++q) a.feat[q * fs + (long long)J * pp + col] = f[q];
+    les_closure_inputs(a, J, col, sy);
     return;
   }
   const double nut = (J >= 3 && J <= NY - 1) ? -0.5 * a.dyw[J] * sy : 0.0;
@@ -249,88 +311,43 @@ __global__ void k_les_cf(LesArgs a) {
 }
 
 // The learned closure (no counterpart in the reference): NU_T at the GY
-// point of plane J = -DYW(J)/2 max(0, y), y the output of an MLP
-// 8 -> 32 -> 32 -> 1 (ReLU hidden layers) of the point's strain features
-// (|S| at GY, S4 and S6 less their mean shear, S11, S22, S33 and S5
-// interpolated to GY, one spare zero), on J = 3..NY-1 as the model's own
-// NU_T; a network that returns |S| is the constant-Smagorinsky model.
-// BF16 MFMA (16x16x16, FP32 accumulation): a wave takes 16 points at a time.
-// Layer 1 runs transposed, H1^T = W1^T X^T (A = W1^T from the table, B = the
-// points' features, lane l holding features 4(l/16)..+3 of point l%16), so
-// that its result, hidden units 16t + 4(l/16) + i of point l%16 in lane l,
-// is already the A operand of layer 2 (points x hidden).  Layer 2 leaves
-// hidden unit 16nt + l%16 of points 4(l/16) + i in lane l; layer 3 is a sum
-// over the 16 lanes of each group.
-typedef short mlp_s4 __attribute__((ext_vector_type(4)));
-typedef float mlp_f4 __attribute__((ext_vector_type(4)));
-
-__device__ __forceinline__ short to_bf16(float x) {  // round to nearest even
-  unsigned u = __float_as_uint(x);
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  return (short)(u >> 16);
-}
-
+// point of plane J = NUS softplus(y) on J = 3..NY-1, the model's own rows
+// (zero elsewhere, as GHOST_LES_MPI leaves the Smagorinsky NU_T), y the
+// network of dh_mlp_dev.h on the point's inputs (feature slots 0..5), 16
+// points per wave pass on BF16 MFMA.
 __global__ void __launch_bounds__(256) k_les_mlp(LesArgs a) {
   if (!les_gate_open(a)) return;
   const Geo &g = a.g;
   const int NY = g.ny;
   const long long pp = g.pp, npts = (long long)(NY + 1) * pp;  // planes 1..NY+1
-  const int lane = threadIdx.x & 63, grp = lane >> 4, col16 = lane & 15;
+  const int lane = threadIdx.x & 63, c = lane & 15, grp = lane >> 4;
   const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
-  const mlp_s4 *W = reinterpret_cast<const mlp_s4 *>(a.mlp_w);
-  const mlp_s4 w1a[2] = {W[lane], W[64 + lane]};
-  const mlp_s4 w2b[2][2] = {{W[128 + lane], W[192 + lane]}, {W[256 + lane], W[320 + lane]}};
-  const float *b1 = a.mlp_b, *b2 = a.mlp_b + kMlpHidden, *w3 = a.mlp_b + 2 * kMlpHidden;
-  const float b3 = a.mlp_b[3 * kMlpHidden];
+  const float b4 = a.mlp_p[kOB4];
+  const int ldi = (int)a.fs;
   for (long long base = wave * 16; base < npts; base += nwaves * 16) {
-    // B of layer 1: features 4 grp..4 grp+3 of point base + col16 (groups 2, 3: zero)
-    const long long q = base + col16;
-    mlp_s4 x = {0, 0, 0, 0};
-    if (grp < 2 && q < npts) {
+    asm volatile("" ::: "memory");  // operands re-read each pass, not held across it
+    const long long q = base + c;
+    bf16x8 x;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) x[j] = (__bf16)0.0f;
+    if (grp == 0 && q < npts) {
       const double *fp = a.feat + pp + q;  // point q is plane 1 + q / pp
-      x[0] = to_bf16((float)fp[(4 * grp + 0) * a.fs]);
-      x[1] = to_bf16((float)fp[(4 * grp + 1) * a.fs]);
-      x[2] = to_bf16((float)fp[(4 * grp + 2) * a.fs]);
-      x[3] = to_bf16((float)fp[(4 * grp + 3) * a.fs]);
+#pragma unroll
+      for (int j = 0; j < kMlpIn; ++j) x[j] = (__bf16)(float)fp[j * ldi];
     }
-    mlp_s4 h1[2];
-#pragma unroll
-    for (int t = 0; t < 2; ++t) {
-      mlp_f4 d = {0.f, 0.f, 0.f, 0.f};
-      d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w1a[t], x, d, 0, 0, 0);
-#pragma unroll
-      for (int i = 0; i < 4; ++i) h1[t][i] = to_bf16(fmaxf(d[i] + b1[16 * t + 4 * grp + i], 0.f));
-    }
-    float part[4] = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int nt = 0; nt < 2; ++nt) {
-      mlp_f4 d = {0.f, 0.f, 0.f, 0.f};
-      d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(h1[0], w2b[0][nt], d, 0, 0, 0);
-      d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(h1[1], w2b[1][nt], d, 0, 0, 0);
-      const float bb = b2[16 * nt + col16], ww = w3[16 * nt + col16];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) part[i] += fmaxf(d[i] + bb, 0.f) * ww;
-    }
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-      for (int o = 8; o > 0; o >>= 1) part[i] += __shfl_xor(part[i], o, 16);
-    if (col16 == 0) {
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const long long p = base + 4 * grp + i;
-        if (p < npts) {
-          const int J = 1 + (int)(p / pp);
-          const double y = (double)(part[i] + b3);
-          a.nut[pp + p] = (J >= 3 && J <= NY - 1) ? -0.5 * a.dyw[J] * fmax(y, 0.0) : 0.0;
-        }
-      }
+    f32x4 z1[4], z2[4], z3[4];
+    const float y = mlp_forward(a.mlp_pk, b4, x, z1, z2, z3);
+    if (lane < 16 && q < npts) {
+      const int J = 1 + (int)(q / pp);
+      a.nut[pp + q] = (J >= 3 && J <= NY - 1) ? a.mlp_nus * (double)softplus(y) : 0.0;
     }
   }
 }
 
-// With the closure: the terms k_les_nut leaves to after NU_T, the cross
-// terms NU_T dU1/dy, NU_T dU3/dy and KAPPA_T times the scalar gradients.
+// With the closure: the terms k_les_nut leaves to after NU_T: the subgrid
+// stress -2 NU_T S_ij (over the filter width), the cross terms NU_T dU1/dy,
+// NU_T dU3/dy and KAPPA_T times the scalar gradients.
 __global__ void k_les_post(LesArgs a) {
 #pragma clang fp contract(off)
   if (!les_gate_open(a)) return;
@@ -342,6 +359,19 @@ __global__ void k_les_post(LesArgs a) {
   const int out = kLesPhys - kLesFour + 2 * a.nth;
   const double *U1 = a.uphys, *U3 = a.uphys + 2 * fs, *DY = g.dy;
   const double nut = a.nut[o];
+  // the stress -2 NU_T S_ij, over the width k_les_cf multiplies by: NU_T at
+  // GY for S12, S23; at GYF the mean of the two GY points around it
+  if (J >= 2) {
+    const double fy = -2.0 * nut / a.dyw[J];
+    a.phys[(out + 2) * fs + o] *= fy;
+    a.phys[(out + 4) * fs + o] *= fy;
+  }
+  if (J >= a.jstart && J <= a.jend) {
+    const double ff = -2.0 * (0.5 * (nut + a.nut[o + pp])) / a.dyfw[J];
+    a.phys[(out + 0) * fs + o] *= ff;
+    a.phys[(out + 1) * fs + o] *= ff;
+    a.phys[(out + 3) * fs + o] *= ff;
+  }
   const bool cross = J >= 2 && J <= NY;
   a.phys[(out + 5) * fs + o] = cross ? nut * (U1[o] - U1[o - pp]) / DY[J] : 0.0;
   a.phys[(out + 6) * fs + o] = cross ? nut * (U3[o] - U3[o - pp]) / DY[J] : 0.0;
@@ -382,8 +412,12 @@ static LesArgs les_args(Ctx &c) {
   a.uc1max = p.u_bc_ymax_c1; a.vc1max = p.v_bc_ymax_c1; a.wc1max = p.w_bc_ymax_c1;
   a.mlp = c.les_mlp ? 1 : 0;
   a.feat = c.les_feat;
-  a.mlp_w = c.d_mlp_w;
-  a.mlp_b = c.d_mlp_b;
+  a.mlp_pk = static_cast<const bf16x8 *>(c.d_mlp_pk);
+  a.mlp_p = c.d_mlp_p;
+  a.mlp_ts = c.mlp_ts;
+  a.mlp_ls = c.mlp_ls;
+  a.mlp_nus = c.mlp_nus;
+  a.gyf = c.d_gyf;
   return a;
 }
 
@@ -394,9 +428,10 @@ hipError_t launch_les_model(Ctx &c, hipStream_t st) {
   hipError_t e;
   if ((e = klaunch(c, (const void *)k_les_strain, nullptr, dim3(mb, NY + 1), dim3(256), 0, st, a)) != hipSuccess)
     return e;
-  // the strain (slots 0..5) and the scalars' gradients (7 + 2n, 8 + 2n), planes 1..NY+1
+  // the strain (slots 0..5; 6: dU1/dz with the closure) and the scalars'
+  // gradients (7 + 2n, 8 + 2n), planes 1..NY+1
   const long long fs = a.fs, cfs = a.cfs;
-  if ((e = launch_planes_inv(c, c.les_four, c.les_phys, 6, 1, NY + 1, st)) != hipSuccess) return e;
+  if ((e = launch_planes_inv(c, c.les_four, c.les_phys, c.les_mlp ? 7 : 6, 1, NY + 1, st)) != hipSuccess) return e;
   if (nth > 0 && (e = launch_planes_inv(c, c.les_four + kLesFour * cfs, c.les_phys + kLesFour * fs, 2 * nth, 1,
                                         NY + 1, st)) != hipSuccess)
     return e;
@@ -416,6 +451,20 @@ hipError_t launch_les_model(Ctx &c, hipStream_t st) {
                                         st)) != hipSuccess)
     return e;
   return klaunch(c, (const void *)k_les_cf, nullptr, dim3(mb, NY + 2), dim3(256), 0, st, a);
+}
+
+// The closure's inputs from the current physical U and the Fourier copy
+// with les_chan's BCs (both made by the caller), as the model's first
+// kernels form them in a substep; nothing else is written.
+hipError_t launch_les_features(Ctx &c, hipStream_t st) {
+  const LesArgs a = les_args(c);
+  const int NY = c.g.ny;
+  const unsigned mb = (unsigned)((c.g.pf + 255) / 256), pb = (unsigned)((c.g.pp + 255) / 256);
+  hipError_t e;
+  if ((e = klaunch(c, (const void *)k_les_strain, nullptr, dim3(mb, NY + 1), dim3(256), 0, st, a)) != hipSuccess)
+    return e;
+  if ((e = launch_planes_inv(c, c.les_four, c.les_phys, 7, 1, NY + 1, st)) != hipSuccess) return e;
+  return klaunch(c, (const void *)k_les_nut, nullptr, dim3(pb, NY + 1), dim3(256), 0, st, a);
 }
 
 }  // namespace dh

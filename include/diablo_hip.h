@@ -157,17 +157,54 @@ int dh_set_les(dh_ctx *ctx, int32_t on);
 /* 1 when LES is on, 0 when not. */
 int dh_les(dh_ctx *ctx);
 /* A learned closure in place of the Smagorinsky eddy viscosity (no
- * counterpart in the reference; it replaces les_chan's NU_T = C_s^2 Delta^2
- * |S|, les.f:196-227, for drivers that train one): NU_T at the GY point of
- * plane J = -DYW(J)/2 * max(0, y) on J = 3..NY-1, with y an MLP 8 -> 32 ->
- * 32 -> 1 (ReLU hidden layers) of the point's strain features (|S| at GY,
- * S4 and S6 less the plane-mean shear, S11, S22, S33, S5 interpolated to GY,
- * one spare input held at zero), evaluated in BF16 with FP32 accumulation.
- * w: W1[8][32], b1[32], W2[32][32], b2[32], w3[32], b3 (n = 1377 floats,
- * W[in][out]); NULL returns to the Smagorinsky model.  A network returning
- * its first input is that model with |S| rounded to BF16.  The features of
- * the last substep are dh_field_dev(ctx, 301), [8][NY+2][NZ][NX]. */
-int dh_set_les_closure(dh_ctx *ctx, const float *w, int32_t n);
+ * counterpart in the reference; it replaces les_chan's NU_T = C_s^2
+ * Delta^2 |S|, les.f:196-227, for drivers that train one): NU_T at the GY
+ * point of plane J = NUS softplus(y) on J = 3..NY-1 (zero on the rows
+ * GHOST_LES_MPI zeroes), y the network of dh_mlp_* below on the point's
+ * inputs: asinh of the five invariants of the resolved strain S and
+ * rotation O, tr S^2 TS^2, tr O^2 TS^2, tr S^3 TS^3, tr O^2 S TS^3,
+ * tr O^2 S^2 TS^4, and asinh of the wall distance (1 - |y|) / LS.  The
+ * subgrid stress is -2 NU_T S_ij throughout (NU_T at GYF the mean of the two
+ * GY values around it), and the implicit and scalar terms take the same
+ * NU_T.  w: the network's dh_mlp_nparams() parameters, then TS, LS and NUS
+ * (n = dh_mlp_nparams() + 3); NULL returns to the Smagorinsky model.  The
+ * inputs of the last substep, then S11, S22, S33, S12, S13, S23 and |S| at
+ * GY, are dh_field_dev(ctx, 301), [13][NY+2][NZ][NX]. */
+int dh_set_les_closure(dh_ctx *h, const float *w, int32_t n);
+/* The closure's inputs (dh_field_dev(ctx, 301)) from the current state,
+ * as the first kernels of an LES substep form them, without stepping: for
+ * building training samples from filtered fields.  Needs LES and the
+ * closure on and the spin-up gate open. */
+int dh_les_features(dh_ctx *h);
+
+/* The learned closure's network (no counterpart in the reference): 6
+ * inputs -> 64 -> 64 -> 64 -> 1, GELU (tanh form) hidden layers, softplus
+ * output, BF16 operands with FP32 accumulation on gfx950's 16x16x32 BF16
+ * MFMA.  Parameters are dh_mlp_nparams() FP32 values: W1[32][64] (rows 6..31
+ * unused), b1[64], W2[64][64], b2[64], W3[64][64], b3[64], w4[64], b4, with
+ * W[in][out].  Device pointers; asynchronous on `stream`; errors in
+ * dh_mlp_last_error(). */
+const char *dh_mlp_last_error(void);
+int32_t dh_mlp_nparams(void);
+/* Bytes of the packed BF16 operand buffer and of the gradient workspace
+ * for a batch of n samples. */
+int64_t dh_mlp_packed_bytes(void);
+int64_t dh_mlp_work_bytes(int64_t n);
+/* Packs params into the operand order the kernels read. */
+int dh_mlp_pack_dev(const float *params, void *packed, void *stream);
+/* out[i] = scale * softplus(y(x_i)), i < n, x feature-major: x[f * ld + i]. */
+int dh_mlp_infer_dev(const void *packed, const float *params, const float *x, int64_t ld, int64_t n, float scale,
+                     float *out, void *stream);
+/* Loss sum_i (softplus(y_i) - t_i)^2 / n and its gradient (dh_mlp_nparams()
+ * values) for a batch of n samples (a multiple of 32, at most 2^24), x
+ * feature-major [6][n]: the forward and backward pass on MFMA, the weight
+ * gradients as MFMA products over the batch, summed in a fixed order. */
+int dh_mlp_grad_dev(const float *params, const void *packed, const float *x, const float *target, int64_t n,
+                    void *work, int64_t work_bytes, float *grad, float *loss, void *stream);
+/* One Adam step (beta 0.9, 0.999, eps 1e-8; step counts from 1) on params
+ * with moments m, v, then the packing of the new params. */
+int dh_mlp_adam_dev(float *params, float *m, float *v, const float *grad, float lr, int32_t step, void *packed,
+                    void *stream);
 
 /* Launch-shape tuning (no counterpart in the reference; for ensembles of
  * uneven sizes, where the best block size depends on the plane's shape).
@@ -259,6 +296,42 @@ int dh_save_stats(dh_ctx *ctx, double ly, double *out, double *ubulk);
 int dh_fft_xz_to_fourier_dev(dh_ctx *ctx, const double *phys, double *four, int32_t nplanes);
 /* FFT_XZ_TO_PHYSICAL (mpi.f:1547-1606), the inverse direction. */
 int dh_fft_xz_to_physical_dev(dh_ctx *ctx, const double *four, double *phys, int32_t nplanes);
+
+/* Batched kernels on caller-owned device arrays (no context).  Each runs
+ * asynchronously on `stream` (a hipStream_t; NULL for the null stream);
+ * fp32 = 1 selects FP32 storage and arithmetic, 0 FP64.  Errors return
+ * DH_E* with the text in dh_batched_last_error() (per host thread). */
+const char *dh_batched_last_error(void);
+
+/* The work of THOMAS_FORWARD_COMPLEX_MPI + THOMAS_BACKWARD_COMPLEX_MPI
+ * (mpi.f:708-865) on nsys independent complex systems of the
+ * Crank-Nicolson form of channel.f:914-963, whose diagonals are formed on
+ * the device from one coefficient per system and two per row:
+ *   row j:  -alpha_s sl_j x_{j-1} + (1 + alpha_s (sl_j + su_j)) x_j
+ *           - alpha_s su_j x_{j+1} = rhs_j,      j = 0..ny-1,
+ * i.e. MATL/MATD/MATU of the reference with alpha_s = NU T1 / h^2 and sl,
+ * su = h^2/(DY(J) DYF(J)), h^2/(DY(J+1) DYF(J)); sl_j = su_j = 0 makes row j
+ * an identity (Dirichlet) row, and the couplings of row 0 to j = -1 and of
+ * row ny-1 to j = ny are dropped.  sl, su: [ny] doubles; alpha: [nsys]
+ * doubles; rhs, x: [ny][nsys] complex (double2, or float2 when fp32),
+ * systems fastest; x may equal rhs.  2 <= ny <= 288. */
+int dh_cn_solve_batched_dev(const double *sl, const double *su, const double *alpha, const void *rhs, void *x,
+                            int32_t ny, int64_t nsys, int32_t fp32, void *stream);
+
+/* FFT_XZ_TO_FOURIER (mpi.f:1477-1542) on nplanes 512 x 512 planes without
+ * its de-aliasing truncation and packing: the real-to-complex transform
+ * along x divided by NX, then the forward complex transform along z divided
+ * by NZ (FFTW_FORWARD, fft.f:73-74,86-87).  phys: [nplanes][512 z][512 x]
+ * reals; four: [nplanes][512 kz][257 kx] complex, kz in transform order
+ * (0..255, then -256..-1). */
+int dh_fft512_xz_to_fourier_dev(const void *phys, void *four, int64_t nplanes, int32_t fp32, void *stream);
+/* FFT_XZ_TO_PHYSICAL (mpi.f:1547-1606) on the same planes, unnormalised:
+ * the backward complex transform along z, then complex-to-real along x
+ * (the imaginary parts of kx = 0 and 256 ignored, as a c2r ignores them).
+ * four is left unchanged (the reference's transform destroys its input);
+ * the z pass goes through a staging buffer that the library allocates on
+ * the current device at the first call (16 planes, 33.7 MB) and keeps. */
+int dh_fft512_xz_to_physical_dev(const void *four, void *phys, int64_t nplanes, int32_t fp32, void *stream);
 
 /* Device pointer of one state field in the device layout, for tests and
  * benchmarks: which = 0..2 CU1..3, 3 CP, 4..6 CF1..3, 7 phi (CR1 after

@@ -1,0 +1,163 @@
+"""The learned closure's network and its use in the LES (needs an MI355X;
+no counterpart in the reference, whose LES has only the constant-Smagorinsky
+model the closure replaces).
+
+* Inference and the loss gradient of the device kernels (BF16 MFMA, FP32
+  accumulation) against the float64 restatement oracle/mlp_ref.py with the
+  kernels' BF16 roundings (the restatement's own gradient is pinned by
+  central differences, tests/test_oracle_mlp.py).  The remaining differences
+  are FP32 sums in another order and, for a few values in 2^8, an FP32 result
+  on the other side of a BF16 rounding boundary: relative L2 <= 2e-3 over the
+  batch (a wrong lane, unit or sample mapping puts it at O(1)); against the
+  unrounded float64 network <= 5e-2.
+* Adam on the device fits a smooth target (the loss falls tenfold).
+* In the LES: the inputs the substep forms (dh_les_features) agree with
+  their definitions evaluated from the strain the same kernel writes, and a
+  network returning a constant c gives NU_T = NUS c on rows 3..NY-1 and zero
+  on the others; a few substeps with a trained-size random network stay
+  finite."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from diablo_amd import channel, closure, configs
+from oracle import mlp_ref as M
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda:0"
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+def _batch(n, seed):
+    rng = np.random.default_rng(seed)
+    X = (rng.standard_normal((n, M.IN)) * 1.5).astype(np.float32)
+    t = np.abs(rng.standard_normal(n)).astype(np.float32) * 3.0
+    return X, t
+
+
+def test_inference_matches_oracle():
+    _say("[closure] inference vs oracle, n=4096")
+    P = closure.init_params(3)
+    net = closure.Network(P)
+    X, _ = _batch(4096, 1)
+    y = net.predict(torch.from_numpy(np.ascontiguousarray(X.T)).to(DEV)).cpu().numpy()
+    ref = M.predict(P.astype(np.float64), X.astype(np.float64), bf=True)
+    ref64 = M.predict(P.astype(np.float64), X.astype(np.float64), bf=False)
+    e, e64 = _rel(y, ref), _rel(y, ref64)
+    _say(f"  rel L2 vs rounded oracle {e:.2e}, vs float64 {e64:.2e}")
+    assert e <= 2e-3 and e64 <= 5e-2
+
+
+def test_gradient_matches_oracle():
+    n = 2048
+    _say(f"[closure] loss gradient vs oracle, n={n}")
+    P = closure.init_params(5)
+    net = closure.Network(P)
+    X, t = _batch(n, 2)
+    L, g = net.grad(torch.from_numpy(np.ascontiguousarray(X.T)).to(DEV), torch.from_numpy(t).to(DEV))
+    torch.cuda.synchronize()
+    L, g = float(L), g.cpu().numpy().astype(np.float64)
+    Lr, gr = M.loss_grad(P.astype(np.float64), X.astype(np.float64), t.astype(np.float64), bf=True)
+    L64, g64 = M.loss_grad(P.astype(np.float64), X.astype(np.float64), t.astype(np.float64), bf=False)
+    blocks = {"W1": (M.OW1, M.OB1), "b1": (M.OB1, M.OW2), "W2": (M.OW2, M.OB2), "b2": (M.OB2, M.OW3),
+              "W3": (M.OW3, M.OB3), "b3": (M.OB3, M.OW4), "w4": (M.OW4, M.OB4), "b4": (M.OB4, M.NP)}
+    per = {k: _rel(g[a:b], gr[a:b]) for k, (a, b) in blocks.items()}
+    _say(f"  loss {L:.6f} oracle {Lr:.6f}; gradient rel L2 {_rel(g, gr):.2e} (float64 {_rel(g, g64):.2e})")
+    _say("  per block", {k: f"{v:.1e}" for k, v in per.items()})
+    assert abs(L - Lr) <= 1e-4 * Lr
+    assert _rel(g, gr) <= 2e-3 and max(per.values()) <= 1e-2
+    assert _rel(g, g64) <= 5e-2
+
+
+def test_adam_fits_a_smooth_target():
+    n = 8192
+    _say(f"[closure] Adam on a smooth target, n={n}")
+    rng = np.random.default_rng(7)
+    X = rng.uniform(-2, 2, (n, M.IN)).astype(np.float32)
+    t = (np.log1p(np.exp(np.sin(X[:, 0]) + 0.5 * X[:, 1] * X[:, 2] - 0.3 * X[:, 5]))).astype(np.float32)
+    net = closure.Network(seed=1)
+    Xd, td = torch.from_numpy(np.ascontiguousarray(X.T)).to(DEV), torch.from_numpy(t).to(DEV)
+    first = float(net.grad(Xd, td)[0])
+    for it in range(400):
+        L = net.step(Xd, td, 3e-3)
+    last = float(L)
+    _say(f"  loss {first:.4f} -> {last:.4f}")
+    assert last < 0.1 * first
+
+
+def _les_small_channel():
+    p, gy, ic = configs.channel_re180(nx=32, ny=33, nz=32, dt=2e-3)
+    p.update(les=1, create_new_flow=0)
+    return p, gy, ic
+
+
+def _get(ch, which, shape):
+    t = torch.empty(shape, dtype=torch.float64, device=DEV)
+    closure.d2d(t.data_ptr(), ch.field_ptr(which), t.numel() * 8)
+    return t.cpu().numpy()
+
+
+def test_les_inputs_and_constant_network():
+    p, gy, ic = _les_small_channel()
+    _say("[closure] LES inputs and a constant network, 32x33x32")
+    ch = channel.Channel(p, gy)
+    ch.set_les(True)
+    c = 2.5
+    P = np.zeros(channel.load_library().dh_mlp_nparams(), np.float32)
+    P[-1] = math.log(math.expm1(c))  # softplus(b4) = c
+    ts, ls, nus = 0.7, 0.01, p["nu"]
+    ch.set_les_closure(P, ts=ts, ls=ls, nus=nus)
+    ch.initial_condition(ic)
+    ch.set_clock(0.0, p["delta_t"], 0)
+    ch.les_features()
+    ny, nz, nx = p["ny"], p["nz"], p["nx"]
+    f = _get(ch, 301, (13, ny + 2, nz, nx))
+    J = slice(2, ny + 1)
+    S = f[6:12, J]
+    Sm = np.array([[S[0], S[3], S[4]], [S[3], S[1], S[5]], [S[4], S[5], S[2]]])
+    l1 = np.einsum("ij...,ji...->...", Sm, Sm)
+    l3 = np.einsum("ij...,jk...,ki...->...", Sm, Sm, Sm)
+    assert np.allclose(f[0, J], np.arcsinh(l1 * ts ** 2), rtol=1e-12, atol=1e-12)
+    assert np.allclose(f[2, J], np.arcsinh(l3 * ts ** 3), rtol=1e-10, atol=1e-12)
+    assert np.allclose(f[12, J], np.sqrt(2 * l1), rtol=1e-12, atol=1e-14)
+    gyf = np.concatenate([[0.0], 0.5 * (gy[:-1] + gy[1:]), [0.0]])
+    _, _, _, dyf = channel.grid_metrics(gy)
+    gyJ = gyf[2: ny + 1] - 0.5 * dyf[2: ny + 1]
+    assert np.allclose(f[5, J], np.arcsinh(np.maximum(0.0, 1.0 - np.abs(gyJ))[:, None, None] / ls), rtol=1e-12)
+    assert np.all(np.isfinite(f)) and np.abs(f[1, J]).max() > 0.0  # the rotation invariant is not empty
+    ch.time_step(2)
+    nut = _get(ch, 300, (ny + 2, nz, nx))
+    inside = nut[3:ny]
+    assert np.allclose(inside, nus * c, rtol=4e-3)  # BF16 softplus argument
+    assert np.all(nut[:3] == 0.0) and np.all(nut[ny:] == 0.0)
+    st = ch.download()
+    assert all(np.all(np.isfinite(st[k])) for k in ("cu1", "cu2", "cu3", "cp"))
+    ch.close()
+
+
+def test_les_steps_with_a_random_network_stay_finite():
+    p, gy, ic = _les_small_channel()
+    _say("[closure] LES steps with a random network, 32x33x32")
+    ch = channel.Channel(p, gy)
+    ch.set_les(True)
+    P = closure.init_params(11)
+    ch.set_les_closure(P, ts=p["nu"], ls=p["nu"], nus=p["nu"])
+    ch.initial_condition(ic)
+    ch.set_clock(0.0, p["delta_t"], 0)
+    ch.time_step(20)
+    st = ch.download()
+    nut = _get(ch, 300, (p["ny"] + 2, p["nz"], p["nx"]))
+    _say(f"  NU_T/NU max {nut.max() / p['nu']:.3f}")
+    assert all(np.all(np.isfinite(st[k])) for k in ("cu1", "cu2", "cu3", "cp"))
+    assert nut.min() >= 0.0 and nut[3: p["ny"]].max() > 0.0
+    ch.close()
