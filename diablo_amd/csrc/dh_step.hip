@@ -1833,6 +1833,218 @@ __global__ void __launch_bounds__(kLesSolveThreads) k_solve_les(LesSolveArgs a) 
   }
 }
 
+// The same systems solved with each column split over 16 lanes (4 columns
+// per wave, lane = 4 p + s): lane p holds rows [R p, R p + R) of its column
+// (rows beyond NY+1 are identity rows), forms them as k_solve_les does and
+// eliminates them locally into x_r = y_r - v_r X_L - w_r X_R, X_L and X_R the
+// solution just outside the segment (the partition or SPIKE method).  The
+// first and last rows of the 16 segments give a 2 x 2 block-tridiagonal
+// system over the 15 interfaces, solved by one lane per column; every lane
+// then sweeps its rows again with the boundary terms moved to the
+// right-hand side.  Sixteen times the waves of one column per thread; the
+// sums differ from the sequential sweep's by rounding (as the segmented
+// constant-matrix solve's do).
+constexpr int kLesPartThreads = 256;
+template <int R>
+__global__ void __launch_bounds__(kLesPartThreads) k_solve_les_part(LesSolveArgs a) {
+#pragma clang fp contract(off)
+  __shared__ double red[kLesPartThreads / 64];
+  __shared__ double xch[kLesPartThreads / 64][4][16 * 8];  // per wave, column, segment: 8 values
+  __shared__ double tL[kLesRowMax], tD[kLesRowMax], tC[kLesRowMax], tV[kLesRowMax];
+  __shared__ double rUp[kLesRowMax], rDn[kLesRowMax], rC[kLesRowMax], kLo[kLesRowMax], kUp[kLesRowMax];
+  __shared__ int tBc[kLesRowMax];
+  const Geo &g = a.g;
+  const int f = a.f0 + blockIdx.y;
+  const int NY = g.ny, nrow = NY + 2;
+  const double temp1 = a.in.nu * a.clk->h_bar[a.rk] / 2.0;
+  const double temp2 = a.clk->h_bar[a.rk] / 2.0;
+  const double *DY = a.in.dy, *DYF = a.in.dyf;
+  const double tv = (f < 3 ? temp1 : temp1 / a.in.pr[f - 3]) * a.in.nu_v_scale;
+  for (int J = threadIdx.x; J < nrow; J += blockDim.x) {
+    double L, D, C, V;
+    int bc;
+    real_row(a.in, f, temp1, J, L, D, C, V, bc);
+    tL[J] = L;
+    tD[J] = D;
+    tC[J] = C;
+    tV[J] = V;
+    tBc[J] = bc;
+    if (!bc) {
+      if (f == 1) {
+        rUp[J] = 1.0 / DYF[J];
+        rDn[J] = 1.0 / DYF[J - 1];
+        rC[J] = 1.0 / DY[J];
+        kLo[J] = temp2 * 0.5 / (DYF[J - 1] * DY[J]);
+        kUp[J] = temp2 * 0.5 / (DYF[J] * DY[J]);
+      } else {
+        rUp[J] = 1.0 / DY[J + 1];
+        rDn[J] = 1.0 / DY[J];
+        rC[J] = 1.0 / DYF[J];
+        kLo[J] = temp2 / (DY[J] * DYF[J]);
+        kUp[J] = temp2 / (DY[J + 1] * DYF[J]);
+      }
+    }
+  }
+  __syncthreads();
+  const long long pp = g.pp;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, p = lane >> 2, sw = lane & 3;
+  const long long col = ((long long)blockIdx.x * (kLesPartThreads / 64) + w) * 4 + sw;
+  const bool live = col < pp;
+  const long long colc = live ? col : pp - 1;  // idle lanes load in bounds
+  const int r0 = p * R;
+  const double *U = a.uphys + f * a.fs + colc, *Rp = a.rphys + f * a.fs + colc, *NT = a.nut + colc;
+  // the segment's rows and their neighbours' U and NU_T
+  double uu[R + 2], nn[R + 2], rr[R];
+#pragma unroll
+  for (int q = 0; q < R + 2; ++q) {
+    const int J = r0 - 1 + q;
+    const bool in = J >= 0 && J < nrow;
+    uu[q] = in ? U[J * pp] : 0.0;
+    nn[q] = in ? NT[J * pp] : 0.0;
+  }
+#pragma unroll
+  for (int q = 0; q < R; ++q) rr[q] = (r0 + q < nrow) ? Rp[(r0 + q) * pp] : 0.0;
+  double L[R], D[R], C[R], V[R];
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const int J = r0 + q;
+    if (J >= nrow) {
+      L[q] = 0.0; D[q] = 1.0; C[q] = 0.0; V[q] = 0.0;
+      continue;
+    }
+    L[q] = tL[J]; D[q] = tD[J]; C[q] = tC[J]; V[q] = tV[J];
+    if (!tBc[J]) {
+      const double um = uu[q], u0 = uu[q + 1], up = uu[q + 2], nm = nn[q], n0 = nn[q + 1], np = nn[q + 2];
+      const double gu = (up - u0) * rUp[J], gd = (u0 - um) * rDn[J];
+      double v = rr[q] + tv * ((gu - gd) * rC[J]);
+      const double lo = (f == 1) ? (n0 + nm) : n0, hi = (f == 1) ? (n0 + np) : np;
+      const double h = (f == 1) ? 0.5 : 1.0;
+      V[q] = v + temp2 * ((h * hi * gu - h * lo * gd) * rC[J]);
+      L[q] = L[q] - kLo[J] * lo;
+      D[q] = D[q] + kUp[J] * hi + kLo[J] * lo;
+      C[q] = C[q] - kUp[J] * hi;
+    }
+    if (J == 0) L[q] = 0.0;          // no row below 0
+    if (J == nrow - 1) C[q] = 0.0;   // nor above NY+1
+  }
+  // local elimination: cp = C/piv, ip = 1/piv, V -> forward-swept; the left
+  // spike's last forward value
+  double cp[R], ip[R];
+  double vlast = 0.0;
+  {
+    double cprev = 0.0, vprev = 0.0;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const double piv = (q == 0) ? D[q] : D[q] - L[q] * cprev;
+      const double inv = 1.0 / piv;
+      ip[q] = inv;
+      cp[q] = C[q] * inv;
+      if (q > 0) V[q] = V[q] - L[q] * V[q - 1];
+      V[q] = V[q] * inv;
+      vprev = ((q == 0) ? L[q] : -L[q] * vprev) * inv;
+      cprev = cp[q];
+    }
+    vlast = vprev;
+  }
+  // first-row values: y_0 and w_0 by the backward recurrences, v_0 = L_0 /
+  // q_0 with q the pivots of an elimination from the bottom
+  double yf, wf, vf;
+  {
+    double yy = V[R - 1], ww = cp[R - 1];
+#pragma unroll
+    for (int q = R - 2; q >= 0; --q) {
+      yy = V[q] - cp[q] * yy;
+      ww = -cp[q] * ww;
+    }
+    yf = yy;
+    wf = ww;
+    double qq = D[R - 1];
+#pragma unroll
+    for (int q = R - 2; q >= 0; --q) qq = D[q] - C[q] * L[q + 1] / qq;
+    vf = L[0] / qq;
+  }
+  double *rw = xch[w][sw];
+  {
+    double *q8 = rw + p * 8;
+    q8[0] = yf; q8[1] = V[R - 1]; q8[2] = vf; q8[3] = vlast; q8[4] = wf; q8[5] = cp[R - 1];
+  }
+  wave_lds_sync();
+  // interfaces i: P_i = x(last row of segment i), Q_i = x(first row of
+  // segment i+1):  P_i + vl_i P_{i-1} + wl_i Q_i = yl_i,
+  //                vf_{i+1} P_i + Q_i + wf_{i+1} Q_{i+1} = yf_{i+1}
+  if (p == 0) {
+    double pc0 = 0.0, prP = 0.0;
+    // forward: r0 -> [i][1], c0 -> [i][6], r1 -> [i+1][0], c1 -> [i+1][7]
+#pragma unroll 1
+    for (int i = 0; i < 15; ++i) {
+      double *qi = rw + i * 8, *qn = rw + (i + 1) * 8;
+      const double yl = qi[1], vl = qi[3], wl = qi[5], yfn = qn[0], vfn = qn[2], wfn = qn[4];
+      const double m01 = wl - vl * pc0;
+      const double idet = 1.0 / (1.0 - m01 * vfn);
+      const double t0 = yl - vl * prP;
+      const double c0 = -m01 * wfn * idet, c1 = wfn * idet;
+      const double x0 = (t0 - m01 * yfn) * idet, x1 = (yfn - vfn * t0) * idet;
+      qi[1] = x0; qi[6] = c0; qn[0] = x1; qn[7] = c1;
+      pc0 = c0;
+      prP = x0;
+    }
+    double qn = 0.0;
+#pragma unroll 1
+    for (int i = 14; i >= 0; --i) {
+      double *qi = rw + i * 8, *qx = rw + (i + 1) * 8;
+      const double P = qi[1] - qi[6] * qn, Q = qx[0] - qx[7] * qn;
+      qn = Q;
+      qx[0] = P;  // X_L of segment i+1
+      qi[1] = Q;  // X_R of segment i
+    }
+    rw[0] = 0.0;
+    rw[15 * 8 + 1] = 0.0;
+  }
+  wave_lds_sync();
+  const double XL = rw[p * 8 + 0], XR = rw[p * 8 + 1];
+  // the rows with the boundary terms moved right: forward-swept
+  // V - X_L v' - X_R w', then the backward sweep
+  {
+    double vprev = 0.0;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      vprev = ((q == 0) ? L[q] : -L[q] * vprev) * ip[q];
+      V[q] = V[q] - vprev * XL;
+    }
+    V[R - 1] = V[R - 1] - cp[R - 1] * XR;
+  }
+#pragma unroll
+  for (int q = R - 2; q >= 0; --q) V[q] = V[q] - cp[q] * V[q + 1];
+  const bool cour = a.courant && f < 3;
+  const double dxf = (f == 0) ? a.dx : a.dz;
+  double dtmin = __longlong_as_double(0x7FF0000000000000ll);
+  if (live) {
+    double *Ub = a.dst[f] + col;
+    const int wlo = a.wlo[f], whi = a.whi[f];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int J = r0 + q;
+      if (J < nrow && J >= wlo && J <= whi) {
+        Ub[J * pp] = V[q];
+        if (cour && J >= 1 && J <= NY) {
+          const double dl = (f == 1) ? g.dy[J] : dxf;
+          dtmin = fmin(dtmin, a.cfl * dl / fabs(V[q]));
+        }
+      }
+    }
+  }
+  if (a.courant && f < 3) {
+    for (int o = 32; o > 0; o >>= 1) dtmin = fmin(dtmin, __shfl_down(dtmin, o));
+    if (lane == 0) red[w] = dtmin;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double m = red[0];
+      for (int q = 1; q < (int)(blockDim.x >> 6); ++q) m = fmin(m, red[q]);
+      if (m >= 0.0) atomicMin(&a.clk->cfl_min_bits, (unsigned long long)__double_as_longlong(m));
+    }
+  }
+}
+
 hipError_t launch_solve_les_range(Ctx &c, int rk, int courant, int f0, int f1, hipStream_t st) {
   if (f1 <= f0) return hipSuccess;
   LesSolveArgs a{};
@@ -1855,6 +2067,13 @@ hipError_t launch_solve_les_range(Ctx &c, int rk, int courant, int f0, int f1, h
   a.cfl = c.p.cfl;
   a.dx = c.p.lx / c.g.nx;
   a.dz = c.p.lz / c.g.nz;
+  const int nrow = c.g.ny + 2;
+  if (nrow <= 16 * 17) {
+    // columns split over 16 lanes: 16 columns per block of 4 waves
+    const dim3 grid((unsigned)((c.g.pp + 15) / 16), f1 - f0);
+    const void *k = (nrow <= 16 * 9) ? (const void *)k_solve_les_part<9> : (const void *)k_solve_les_part<17>;
+    return klaunch(c, k, nullptr, grid, dim3(kLesPartThreads), 0, st, a);
+  }
   const dim3 grid((unsigned)((c.g.pp + kLesSolveThreads - 1) / kLesSolveThreads), f1 - f0);
   return klaunch(c, (const void *)k_solve_les, nullptr, grid, dim3(kLesSolveThreads), 0, st, a);
 }

@@ -119,7 +119,8 @@ C Will the driver read or rewrite the state before the next substep?
      &      .OR. (TIME_STEP .GE. TS_FIRST - 1 + N_TIME_STEPS)
         INQUIRE(FILE='stop.now', EXIST=STOPFILE)
         CALL WALL_TIME(WT)
-        NEEDED = NEEDED .OR. STOPFILE .OR. (WT-START_TIME .GT. TIME_LIMIT)
+        NEEDED = NEEDED .OR. STOPFILE
+     &      .OR. (WT-START_TIME .GT. TIME_LIMIT)
         IF (NEEDED) THEN
           RC = dh_download_arrays(CTX, CU1, CU2, CU3, CP, CF1, CF2, CF3,
      &                            CTH, CFTH, TH)

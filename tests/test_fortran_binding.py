@@ -33,6 +33,33 @@ def test_module_and_driver_compile_and_link(tmp_path):
     _build(str(tmp_path))
 
 
+@pytest.mark.skipif(not os.path.exists(FC), reason="amdflang not installed")
+def test_drop_in_rk_chan_1_compiles_and_links(tmp_path):
+    """The fixed-form drop-in RK_CHAN_1 compiles against declarations of the
+    reference header's names (tests/fortran/shim_include) and links with the
+    binding module and the library."""
+    tmp = str(tmp_path)
+    libdir = os.path.dirname(channel.LIB_PATH)
+    inc = os.path.join(ROOT, "tests", "fortran", "shim_include")
+    shim = os.path.join(ROOT, "diablo_amd", "fortran", "rk_chan_1.f")
+    main = os.path.join(ROOT, "tests", "fortran", "shim_main.f")
+    exe = os.path.join(tmp, "shim_main")
+    # fixed form ends a statement at column 72: the drop-in must fit
+    for path in (shim, main):
+        for n, line in enumerate(open(path), 1):
+            assert line[:1] in "Cc!*" or len(line.rstrip("\n")) <= 72, f"{path}:{n} beyond column 72"
+    steps = [[FC, "-O1", "-c", MOD, "-o", "diablo_hip.o"],
+             [FC, "-O1", "-ffixed-form", f"-I{inc}", "-c", shim, "-o", "rk_chan_1.o"],
+             [FC, "-O1", "-ffixed-form", f"-I{inc}", "-c", main, "-o", "shim_main.o"],
+             [FC, "shim_main.o", "rk_chan_1.o", "diablo_hip.o", "-o", exe, f"-L{libdir}", "-ldiablo_hip",
+              f"-Wl,-rpath,{libdir}"]]
+    for cmd in steps:
+        r = subprocess.run(cmd, cwd=tmp, capture_output=True, text=True)
+        assert r.returncode == 0, " ".join(cmd) + "\n" + r.stdout + r.stderr
+    r = subprocess.run([exe], cwd=tmp, capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+
+
 @pytest.mark.gpu
 def test_fortran_driver_matches_python(tmp_path):
     from oracle import dor
