@@ -124,10 +124,11 @@ def copy_calibration(mib, reps=5):
                       "best_ms": round(best * 1e3, 4)}), flush=True)
 
 
-def ensemble_bench(n, steps, warmup):
+def ensemble_run(n, steps, warmup, mixed=False, one_at_a_time=True):
     """Uneven shear-layer instances (NX in {64..256}, NY = NX + 1, NZ = 4)
-    stepped as one graph; aggregate sum(NX NY NZ) steps / wall, next to the
-    same instances run one after another (each with its own step graph)."""
+    stepped as one graph (FP64, or the FP32/FP64 mixed form); aggregate
+    sum(NX NY NZ) steps / wall, next to the same instances run one after
+    another (each with its own step graph)."""
     from diablo_amd import channel, configs
     decks = configs.kh_ensemble(n)
     chs = []
@@ -135,6 +136,7 @@ def ensemble_bench(n, steps, warmup):
         ch = channel.Channel(p, gy)
         ch.initial_condition(ic)
         ch.set_clock(0.0, p["delta_t"], 0)
+        ch.set_mixed_precision(mixed)
         chs.append(ch)
     pts = sum(p["nx"] * p["ny"] * p["nz"] for p, _, _ in decks)
     ens = channel.Ensemble(chs)
@@ -149,31 +151,36 @@ def ensemble_bench(n, steps, warmup):
     ens.time_step(steps)
     ens.synchronize()
     wall = time.perf_counter() - t0
-    # the same instances one at a time
-    for ch in chs:
-        ch.time_step(1)
-        ch.synchronize()
-    t1 = time.perf_counter()
-    for ch in chs:
-        ch.time_step(steps)
-        ch.synchronize()
-    wall_seq = time.perf_counter() - t1
     sizes = {}
     for p, _, _ in decks:
         sizes[p["nx"]] = sizes.get(p["nx"], 0) + 1
-    print(json.dumps({
-        "metric": f"Mpts·RK3-steps/s, KH shear-layer ensemble of {n} uneven instances",
-        "value": round(pts * steps / wall / 1e6, 3), "unit": UNIT, "n_gpus": 1, "steps": steps, "warmup": warmup,
-        "ms_per_step": round(wall / steps * 1e3, 4), "higher_is_better": True, "dtype": "f64",
-        "data": "synthetic (seeded KH tanh shear layer + noise per instance)",
-        "config": {"workload": "kh_ensemble", "instances": n, "nx_counts": sizes, "nz": 4,
-                   "points_per_step": pts, "block_size_by_nx": block},
-        "one_at_a_time": {"value": round(pts * steps / wall_seq / 1e6, 3),
-                          "ms_per_step": round(wall_seq / steps * 1e3, 4)},
-    }), flush=True)
+    out = {"value": round(pts * steps / wall / 1e6, 3), "unit": UNIT, "steps": steps, "warmup": warmup,
+           "ms_per_step": round(wall / steps * 1e3, 4),
+           "dtype": "f32/f64 mixed" if mixed else "f64",
+           "config": {"workload": "kh_ensemble", "instances": n, "nx_counts": sizes, "nz": 4,
+                      "points_per_step": pts, "block_size_by_nx": block}}
+    if one_at_a_time:
+        for ch in chs:
+            ch.time_step(1)
+            ch.synchronize()
+        t1 = time.perf_counter()
+        for ch in chs:
+            ch.time_step(steps)
+            ch.synchronize()
+        wall_seq = time.perf_counter() - t1
+        out["one_at_a_time"] = {"value": round(pts * steps / wall_seq / 1e6, 3),
+                                "ms_per_step": round(wall_seq / steps * 1e3, 4)}
     ens.close()
     for ch in chs:
         ch.close()
+    return out
+
+
+def ensemble_bench(n, steps, warmup):
+    out = ensemble_run(n, steps, warmup)
+    out.update({"metric": f"Mpts·RK3-steps/s, KH shear-layer ensemble of {n} uneven instances", "n_gpus": 1,
+                "higher_is_better": True, "data": "synthetic (seeded KH tanh shear layer + noise per instance)"})
+    print(json.dumps(out), flush=True)
 
 
 def mixed_form(p, gy, ic, steps, warmup):
@@ -387,6 +394,9 @@ def main():
     ap.add_argument("--no-mixed", action="store_true", help="leave out the FP32/FP64 mixed form's line")
     ap.add_argument("--no-les", action="store_true", help="leave out the LES channel's line")
     ap.add_argument("--no-batched", action="store_true", help="leave out the batched solver/transform line")
+    ap.add_argument("--ensemble-line", action="store_true",
+                    help="add the 64-instance KH ensemble (FP64 and mixed) to the line; ensembles step member "
+                         "by member while batched ensemble stepping is switched off in the library")
     ap.add_argument("--profile-reps", type=int, default=10)
     ap.add_argument("--ensemble", type=int, default=0, metavar="N",
                     help="only time N uneven shear-layer instances stepped as one graph")
@@ -463,6 +473,10 @@ def main():
             out["les_form"]["cpu_baseline"] = les_cpu_baseline()
     if not args.no_batched:
         out["batched_microbench"] = batched_microbench()
+    if args.ensemble_line:
+        ens = ensemble_run(64, 100, 10)
+        ens["mixed_form"] = ensemble_run(64, 100, 10, mixed=True, one_at_a_time=False)
+        out["kh_ensemble"] = ens
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(p, gy, ic)
     ch.close()

@@ -136,6 +136,13 @@ def load_library(path=LIB_PATH):
         if not os.path.exists(path):
             raise RuntimeError(f"{path} is missing: build it with `python -m diablo_amd.build` "
                                "(the channel step has no CPU fallback)")
+        # torch carries HIP and HSA runtimes of its own.  In a process that
+        # uses both, torch's must be loaded first: loaded after the library
+        # had started the system runtime, torch found no GPU.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         lib = ctypes.CDLL(path)
         for name, (res, args) in EXPORTS.items():
             fn = getattr(lib, name)

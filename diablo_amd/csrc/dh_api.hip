@@ -1148,9 +1148,20 @@ bool capture_ensemble(dh_ens &e, int nsteps, hipGraphExec_t *exec) {
   }
   bool ok = r == hipSuccess && graph && hipGraphInstantiate(exec, graph, nullptr, nullptr, 0) == hipSuccess;
   if (graph) (void)hipGraphDestroy(graph);
-  if (!ok) (void)hipGetLastError();
+  if (!ok) {
+    (void)hipGetLastError();
+    *exec = nullptr;  // callers take a non-null handle as a usable graph
+  }
   return ok;
 }
+
+// Batched stepping (build_groups, capture_ensemble) is switched off: the
+// first batched step of an ensemble of KH instances (NX 64 and 96) ended the
+// process with a host segmentation fault on the MI355X, and no cause has been
+// found in the host logic (run against a stand-in runtime under
+// AddressSanitizer) or in the batched kernels' indexing.  Ensembles step
+// their members one by one, each with its own captured step graph.
+constexpr bool kBatchedEnsembles = false;
 
 }  // namespace
 
@@ -1186,7 +1197,7 @@ int dh_ensemble_time_step(dh_ens *e, int32_t nsteps) {
   if (nsteps == 0) return DH_OK;
   Ctx &c = e->m[0]->c;  // errors are reported on the first member
   if (!e->batched && !e->fallback) {
-    if (build_groups(*e) == hipSuccess) {
+    if (kBatchedEnsembles && build_groups(*e) == hipSuccess) {
       e->batched = true;
     } else {
       (void)hipGetLastError();

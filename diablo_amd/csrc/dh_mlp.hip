@@ -186,7 +186,7 @@ __global__ void __launch_bounds__(64) k_mlp_fwd_bwd(const bf16x8 *pk, const floa
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) sloss += __shfl_xor(sloss, o);
-  if (lane == 0) part[blockIdx.x] = sloss;
+  if (lane == 0) part[blockIdx.x] = sloss * inv_n;  // the loss is the mean over the batch
 }
 
 // Parameter gradients of layer L = blockIdx.y over the samples of slice
@@ -279,7 +279,7 @@ __global__ void __launch_bounds__(64) k_mlp_wgrad(const __bf16 *__restrict__ act
 
 // grad[p] = the sum over slices of the partial sums, in the parameter
 // vector's layout (W1 rows 8..31 are zero: their inputs are zero); loss =
-// the sum over waves.
+// the sum over waves of their parts of the batch mean.
 __global__ void k_mlp_reduce(const float *__restrict__ wpart, int nslice, const float *__restrict__ part, int nwave,
                              float *__restrict__ grad, float *__restrict__ loss) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;

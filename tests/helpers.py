@@ -6,6 +6,13 @@ from diablo_amd import configs
 
 STATE_KEYS = ("cu1", "cu2", "cu3", "cp", "cf1", "cf2", "cf3", "cth", "cfth", "th")
 
+# Batched ensemble stepping is switched off in the library (dh_api.hip,
+# kBatchedEnsembles): its first step ended the process with a host
+# segmentation fault on the MI355X whose cause has not been found.  The tests
+# of that path stay skipped until it is.
+BATCHED_ENSEMBLE_OFF = ("batched ensemble stepping is switched off: its first step ended in a host segmentation "
+                        "fault on the MI355X whose cause is not yet found")
+
 
 def make_oracle(dor, name, **kw):
     p, gy, ic = configs.CONFIGS[name](**kw)

@@ -117,6 +117,7 @@ def test_les_inputs_and_constant_network():
     P[-1] = math.log(math.expm1(c))  # softplus(b4) = c
     ts, ls, nus = 0.7, 0.01, p["nu"]
     ch.set_les_closure(P, ts=ts, ls=ls, nus=nus)
+    ts, ls, nus = (float(np.float32(v)) for v in (ts, ls, nus))  # the scales pass the ABI as float32
     ch.initial_condition(ic)
     ch.set_clock(0.0, p["delta_t"], 0)
     ch.les_features()

@@ -16,7 +16,7 @@ import pytest
 from diablo_amd import channel, configs
 from tests import helpers
 
-pytestmark = pytest.mark.gpu
+pytestmark = [pytest.mark.gpu, pytest.mark.skip(reason=helpers.BATCHED_ENSEMBLE_OFF)]
 
 
 @pytest.fixture(scope="module")

@@ -96,35 +96,48 @@ def test_switching_back_is_the_plain_path(dor):
     ch.close()
 
 
+SPINUP_STEPS = 20000
+
+
 def test_mixed_long_run_statistics():
     """Long-run statistics of the mixed form against the FP64 path on the
-    Re_tau=180 channel (128x129x128, CFL 0.5): 2000 steps from the seeded
-    IC, plane statistics (dh_save_stats) averaged every 10 steps over the
-    last 1000.  Tolerances of the mixed form: mean U+ within 2% and the rms
-    velocities within 5% (relative L2 over the profile); u_tau = 1 here, so
-    ume is U+.  From this IC the noise decays and the mean accelerates
-    towards the laminar profile over these 2000 steps, so this checks the
-    statistics of that transient, not of developed turbulence."""
-    from diablo_amd import configs
-    print("[mixed] long-run statistics, channel_re180, 2000 steps", flush=True)
+    developed turbulent Re_tau=180 channel (128x129x128, CFL 0.5): a seeded
+    transition IC (large-scale modes of finite amplitude over the laminar
+    profile) is run SPINUP_STEPS steps in FP64 to developed turbulence
+    (checked: u_tau from the wall shear within 10 % of 1, the forcing's
+    value, and the rms peak of u beyond 2 u_tau); from that state each form
+    runs 2000 steps, with the plane statistics (dh_save_stats) averaged every
+    10 steps over the last 1000.  Tolerances of the mixed form: mean U+
+    within 2 % and the rms velocities within 5 % (relative L2 over the
+    profile)."""
+    import math
+
+    from diablo_amd import configs, turbulence
+    print(f"[mixed] long-run statistics, channel_re180, {SPINUP_STEPS} steps of spin-up", flush=True)
     p, gy, ic = configs.channel_re180()
+    ch = channel.Channel(p, gy)
+    turbulence.transition_ic(ch, p, gy, seed=1)
+    ch.set_clock(0.0, p["delta_t"], 0)
+    ch.time_step(SPINUP_STEPS)
+    st0 = ch.download()
+    clk = ch.clock()
+    ch.close()
     prof = {}
     for mixed in (False, True):
         ch = channel.Channel(p, gy)
-        ch.initial_condition(ic)
-        ch.set_clock(0.0, p["delta_t"], 0)
+        ch.upload(st0)
+        ch.set_clock(clk[0], clk[1], clk[3])
         ch.set_mixed_precision(mixed)
         ch.time_step(1000)
-        acc = None
-        for _ in range(100):
-            ch.time_step(10)
-            st = ch.save_stats(2.0)
-            v = np.array([st[k] for k in ("ume", "urms", "vrms", "wrms")])
-            acc = v if acc is None else acc + v
-        prof[mixed] = acc / 100
+        stats, _ = turbulence.run_statistics(ch, p, 1000, every=10)
+        prof[mixed] = stats
         ch.close()
-    rel = {k: float(np.linalg.norm(prof[True][i] - prof[False][i]) / np.linalg.norm(prof[False][i]))
-           for i, k in enumerate(("ume", "urms", "vrms", "wrms"))}
+    ref = prof[False]
+    utau = math.sqrt(ref["tau_w"])
+    _say(f"  FP64 u_tau {utau:.3f}, U_b {ref['ubulk']:.2f}, max urms {ref['urms'].max():.2f}")
+    assert abs(utau - 1.0) <= 0.1 and ref["urms"].max() >= 2.0  # developed turbulence
+    rel = {k: float(np.linalg.norm(prof[True][k] - ref[k]) / np.linalg.norm(ref[k]))
+           for k in ("ume", "urms", "vrms", "wrms")}
     _say("  rel L2 mixed vs FP64", _fmt(rel))
     assert rel["ume"] <= 0.02, rel
     assert max(rel["urms"], rel["vrms"], rel["wrms"]) <= 0.05, rel
