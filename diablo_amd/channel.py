@@ -119,6 +119,8 @@ EXPORTS = {
                                                    ctypes.c_void_p]),
     "dh_fft512_xz_to_physical_dev": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32,
                                                     ctypes.c_void_p]),
+    "dh_fft512_xz_to_physical_noalloc_dev": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                                            ctypes.c_int32, ctypes.c_void_p]),
     "dh_stream": (ctypes.c_void_p, [ctypes.c_void_p]),
 }
 

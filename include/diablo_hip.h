@@ -332,6 +332,18 @@ int dh_fft512_xz_to_fourier_dev(const void *phys, void *four, int64_t nplanes, i
  * the z pass goes through a staging buffer that the library allocates on
  * the current device at the first call (16 planes, 33.7 MB) and keeps. */
 int dh_fft512_xz_to_physical_dev(const void *four, void *phys, int64_t nplanes, int32_t fp32, void *stream);
+/* No counterpart in the reference: dh_fft512_xz_to_physical_dev for callers
+ * that cannot allocate device memory (code under stream capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  It calls no allocator:
+ * the z pass goes through a file-scope device array of 8 planes (16.8 MB on
+ * each device the library runs on, held until the process ends) instead of
+ * the library's staging buffer, whose address is looked up once per device
+ * at the first call.  The planes pass through it 8 at a time, so the array
+ * bounds the planes in flight, not the call: any nplanes >= 1 is taken, and
+ * nothing in it needs clearing between calls.  Results are bit-identical to
+ * dh_fft512_xz_to_physical_dev.  Calls that overlap on one GPU (two streams,
+ * two threads) share the array and race. */
+int dh_fft512_xz_to_physical_noalloc_dev(const void *four, void *phys, int64_t nplanes, int32_t fp32, void *stream);
 
 /* Device pointer of one state field in the device layout, for tests and
  * benchmarks: which = 0..2 CU1..3, 3 CP, 4..6 CF1..3, 7 phi (CR1 after
