@@ -296,6 +296,38 @@ def les_form(steps, warmup, reps=10):
             "phases_ms": phases}
 
 
+def learned_les(dns_ms_per_step, steps=1200):
+    """The second half of the headline metric: how much sooner the LES with
+    the trained closure (64 x 65 x 64, Re_tau = 180) reaches statistics than
+    the DNS of config 1, at the statistics the closure pipeline matched.
+    Live: the LES's wall seconds per unit of simulated time with each model,
+    from the filtered DNS state of tests/golden/closure_les_case.npz (its own
+    CFL time steps in developed turbulence, steps / 2 timed), and the DNS's,
+    from this run's ms per step over the DNS time step of developed
+    turbulence recorded by the pipeline.  The profile errors are the
+    pipeline's (python -m diablo_amd.closure, profiles/closure_re180.json)."""
+    import numpy as np
+    from diablo_amd import closure
+    rep = json.load(open(os.path.join(ROOT, "profiles", "closure_re180.json")))
+    z = np.load(os.path.join(ROOT, "tests", "golden", "closure_les_case.npz"))
+    net = np.load(os.path.join(ROOT, "diablo_amd", "data", "closure_re180.npz"))
+    live = {}
+    for model in ("learned", "smagorinsky", "none"):
+        _, info = closure.les_profiles(z, steps, model=model, params=net["params"],
+                                       scales=(float(net["ts"]), float(net["ls"]), float(net["nus"])))
+        live[model] = round(info["wall_s_per_unit_time"], 5)
+    dns_s = dns_ms_per_step * 1e-3 / rep["dns"]["dt_developed"]
+    ap = rep["a_posteriori"]
+    keep = ("U_plus", "urms_plus_vs_filtered", "vrms_plus_vs_filtered", "wrms_plus_vs_filtered", "score")
+    return {"workload": "learned-closure LES, Re_tau=180, 64x65x64 (same domain and grid stretching as config 1), "
+                        "6->64->64->64->1 GELU/softplus on BF16 MFMA trained on the sharp-filtered DNS",
+            "speedup_learned_vs_dns": round(dns_s / live["learned"], 3),
+            "wall_s_per_unit_time": {"dns": round(dns_s, 5), **{f"les_{m}": v for m, v in live.items()}},
+            "dns_dt_developed": rep["dns"]["dt_developed"],
+            "profile_errors": {m: {k: round(ap[m][k], 4) for k in keep} for m in (rep["chosen"], "smagorinsky", "none")},
+            "network": rep["chosen"], "source_of_errors": "profiles/closure_re180.json"}
+
+
 def _event_ms(fn, reps):
     """Mean milliseconds of fn() over reps launches on the current (null)
     stream, between HIP events recorded on that stream."""
@@ -471,6 +503,8 @@ def main():
         out["les_form"] = les_form(args.steps, args.warmup)
         if not args.no_cpu_baseline:
             out["les_form"]["cpu_baseline"] = les_cpu_baseline()
+    if not args.no_les:
+        out["learned_les"] = learned_les(ms_step)
     if not args.no_batched:
         out["batched_microbench"] = batched_microbench()
     if args.ensemble_line:

@@ -13,13 +13,15 @@ Pipeline (``python -m diablo_amd.closure``):
    and interpolated in y, gives the resolved LES velocity (64 x 65 x 64, same
    domain) and the subgrid stress tau_ij = filter(u_i u_j) - u~_i u~_j; the
    LES context forms the closure's inputs and the resolved strain S from the
-   filtered velocity (dh_les_features), and the target is the eddy viscosity
-   that best fits tau's deviatoric part, max(0, -tau^d_ij S_ij / (2 S_kl S_kl)).
+   filtered velocity (dh_les_features), and the targets are eddy viscosities
+   that match the subgrid dissipation -tau^d_ij S_ij with 2 NU_T S_kl S_kl
+   (pointwise or per plane; see VARIANTS).
 3. Adam on the device (dh_mlp_grad_dev, dh_mlp_adam_dev: BF16 MFMA forward
-   and backward passes).
-4. A-priori error on held-out samples, and LES runs with the trained closure
-   and with the Smagorinsky model from the same filtered snapshot, whose
-   mean-velocity and rms profiles are compared with the DNS's.
+   and backward passes), one network per target form.
+4. A-priori errors on the held-out last snapshot, and LES runs from that
+   filtered snapshot with each network, with the Smagorinsky model and with
+   no model, whose mean-velocity and rms profiles are compared with the
+   DNS's; the network with the smallest profile error is kept.
 """
 import argparse
 import ctypes
@@ -183,7 +185,11 @@ class Sampler:
         kz = kz_l * 2 * math.pi / pd["lz"]
         dx, dz = width * pd["lx"] / pl["nx"], width * pd["lz"] / pl["nz"]
         G = np.sinc(kz[:, None] * dz / (2 * math.pi)) * np.sinc(kx[None, :] * dx / (2 * math.pi))  # box filter
-        self.G = torch.from_numpy(G).to(dev).to(torch.complex128)
+        # x-z filters: the box filter of width `width` LES spacings, and the
+        # sharp cutoff at the LES grid's retained modes (the implicit filter of
+        # a pseudo-spectral LES on that grid)
+        self.G = {"box": torch.from_numpy(G).to(dev).to(torch.complex128),
+                  "sharp": torch.ones(G.shape, dtype=torch.complex128, device=dev)}
         # y positions: GYF and GY of both grids (rows 0..NY+1; GY(0) taken one spacing below)
         def pos(gy):
             gyf = turbulence.ch_gyf(gy)
@@ -216,14 +222,33 @@ class Sampler:
         ctx.synchronize()
         return out
 
-    def _filter(self, four, M):
-        """DNS Fourier planes -> box-filtered LES modes, y-interpolated by M."""
-        f = four[:, self.kzmap][:, :, : self.shl[2]] * self.G
+    def _filter(self, four, M, kind):
+        """DNS Fourier planes -> filtered LES modes (filter `kind`),
+        y-interpolated by M."""
+        f = four[:, self.kzmap][:, :, : self.shl[2]] * self.G[kind]
         return self.t.einsum("ij,jkl->ikl", M, f).contiguous()
 
-    def snapshot(self):
-        """(X [6][n] float32, target [n] float32 in units of NU, diagnostics)
-        from the DNS context's current state; rows 3..NY-1 of the LES grid."""
+    def _load(self, cu, kind):
+        ub = [self._filter(cu[0], self.f2f, kind), self._filter(cu[1], self.y2y, kind),
+              self._filter(cu[2], self.f2f, kind)]
+        for c in range(3):
+            d2d(self.les.field_ptr(c), ub[c].data_ptr(), ub[c].numel() * 16)
+
+    def load_resolved(self, kind="sharp"):
+        """The DNS context's current velocity, filtered by `kind`, into the
+        LES context's CU1..3 (U1, U3 at GYF, U2 at GY of the LES grid)."""
+        self._load([self._field(self.dns, c, self.shd) for c in range(3)], kind)
+
+    def snapshot(self, kind="box"):
+        """Samples of the DNS context's current state, filtered by `kind`
+        ("box" or "sharp"), on rows J = 3..NY-1 of
+        the LES grid, flattened row by row (float32 [n], n = rows NZ NX):
+        X [6][n] the closure's inputs; the eddy viscosities (units of NU)
+        that match the subgrid dissipation D = -tau^d_ij S_ij with
+        2 NU_T S_kl S_kl, pointwise (t_point, clipped to [0, 50]; t_signed,
+        clipped to [-50, 50]) and per plane (t_plane, the ratio of the plane
+        means, broadcast); SS = S_kl S_kl and D themselves; and |S| / NU for
+        the Smagorinsky baseline."""
         torch = self.t
         pd, pl = self.pd, self.pl
         cu = [self._field(self.dns, c, self.shd) for c in range(3)]
@@ -234,15 +259,13 @@ class Sampler:
         pbar = []
         for q in prods:
             fq = self._to_four(self.dns, q, pd, self.shd)
-            pbar.append(self._to_phys(self.les, self._filter(fq, self.f2y), pl))
+            pbar.append(self._to_phys(self.les, self._filter(fq, self.f2y, kind), pl))
         # the resolved velocity: into the LES context (U1, U3 at GYF, U2 at GY) ...
-        ub = [self._filter(cu[0], self.f2f), self._filter(cu[1], self.y2y), self._filter(cu[2], self.f2f)]
-        for c in range(3):
-            d2d(self.les.field_ptr(c), ub[c].data_ptr(), ub[c].numel() * 16)
+        self._load(cu, kind)
         # ... and at GY for the stress
-        ug = [self._to_phys(self.les, self._filter(cu[0], self.f2y), pl),
-              self._to_phys(self.les, self._filter(cu[1], self.y2y), pl),
-              self._to_phys(self.les, self._filter(cu[2], self.f2y), pl)]
+        ug = [self._to_phys(self.les, self._filter(cu[0], self.f2y, kind), pl),
+              self._to_phys(self.les, self._filter(cu[1], self.y2y, kind), pl),
+              self._to_phys(self.les, self._filter(cu[2], self.f2y, kind), pl)]
         tau = [pbar[0] - ug[0] * ug[0], pbar[1] - ug[1] * ug[1], pbar[2] - ug[2] * ug[2],
                pbar[3] - ug[0] * ug[1], pbar[4] - ug[0] * ug[2], pbar[5] - ug[1] * ug[2]]
         self.les.les_features()
@@ -254,12 +277,20 @@ class Sampler:
         tdS = (tau[0] - tr) * S[0] + (tau[1] - tr) * S[1] + (tau[2] - tr) * S[2] + 2.0 * (
             tau[3] * S[3] + tau[4] * S[4] + tau[5] * S[5])
         SS = S[0] ** 2 + S[1] ** 2 + S[2] ** 2 + 2.0 * (S[3] ** 2 + S[4] ** 2 + S[5] ** 2)
-        nut = torch.clamp(-tdS / (2.0 * SS + 1e-300), min=0.0) / self.nu
+        D = -tdS
         rows = slice(3, nyl)  # J = 3..NY-1
-        X = feat[:NIN, rows].reshape(NIN, -1).to(torch.float32).contiguous()
-        t = torch.clamp(nut[rows].reshape(-1), max=50.0).to(torch.float32).contiguous()
-        smag = (feat[12, rows].reshape(-1) / self.nu).to(torch.float32)  # |S| / NU, for the baseline
-        return X, t, smag
+        nrow = nyl - 3
+        f32 = lambda a: a.reshape(-1).to(torch.float32).contiguous()
+        ratio = D / (2.0 * SS + 1e-300) / self.nu
+        Dm = D[rows].reshape(nrow, -1).mean(dim=1)
+        SSm = SS[rows].reshape(nrow, -1).mean(dim=1)
+        plane = torch.clamp(Dm / (2.0 * SSm) / self.nu, min=0.0)
+        return {"X": feat[:NIN, rows].reshape(NIN, -1).to(torch.float32).contiguous(),
+                "t_point": f32(torch.clamp(ratio[rows], min=0.0, max=50.0)),
+                "t_signed": f32(torch.clamp(ratio[rows], min=-50.0, max=50.0)),
+                "t_plane": f32(plane[:, None].expand(nrow, pl["nz"] * pl["nx"])),
+                "SS": f32(SS[rows]), "D": f32(D[rows]),
+                "smag": f32(feat[12, rows] / self.nu)}
 
 
 def smagorinsky_scale(pl, cs=0.13):
@@ -271,7 +302,75 @@ def smagorinsky_scale(pl, cs=0.13):
     return cs * cs * (dx * dy * 2.0 * dz) ** (2.0 / 3.0)
 
 
+# ------------------------------------------------------- a-posteriori runs
+def les_profiles(state, steps, model="smagorinsky", params=None, scales=None, every=20, utau=1.0):
+    """LES of the Re_tau = 180 channel on the coarse grid (les_re180) from
+    the resolved velocity in `state` (CU1..3 in the host layout): `model`
+    "none" (no subgrid model), "smagorinsky" (les.f's constant C_s) or
+    "learned" (params and scales = (ts, ls, nus)).  The BCs and the
+    projection are applied first; then 10 steps, half the steps timed (wall
+    seconds per unit of simulated time), and the other half averaged
+    (plane statistics every `every` steps).  Returns the profiles in wall
+    units with the forcing's u_tau, and {u_tau from the wall shear, wall
+    seconds per unit time, simulated time}."""
+    pl, gyl, _ = les_re180()
+    ch = channel.Channel(dict(pl, les=0 if model == "none" else 1), gyl)
+    if model == "learned":
+        ts, ls, nus = scales
+        ch.set_les_closure(params, ts=ts, ls=ls, nus=nus)
+    up = ch.download()
+    for k in ("cu1", "cu2", "cu3"):
+        up[k] = np.asarray(state[k], dtype=np.complex128)
+    ch.upload(up)
+    ch.apply_bc_vel()
+    ch.rem_div()
+    ch.set_clock(0.0, pl["delta_t"], 0)
+    half = steps // 2
+    ch.time_step(10)
+    ch.synchronize()
+    tl0, ts0 = time.time(), ch.clock()[0]
+    ch.time_step(half)
+    ch.synchronize()
+    rate = (time.time() - tl0) / (ch.clock()[0] - ts0)
+    s, _ = turbulence.run_statistics(ch, pl, steps - half, every=every)
+    w = turbulence.wall_units(s, pl["nu"], gyl, utau=utau)
+    out = {"u_tau_wall_shear": math.sqrt(max(s["tau_w"], 0.0)), "wall_s_per_unit_time": rate, "t_sim": ch.clock()[0]}
+    ch.close()
+    return w, out
+
+
+def profile_errors(w, dns, filtered):
+    """Relative RMS errors of LES profiles `w` against the DNS's (`dns`,
+    interpolated to the LES's y+) and against the DNS as the LES grid
+    resolves it (`filtered`, on the same rows); score = the mean's error
+    against the DNS plus the mean of the three rms errors against the
+    filtered DNS."""
+    e = {}
+    rel = lambda a, ref: float(np.sqrt(np.mean((a - ref) ** 2)) / np.sqrt(np.mean(ref ** 2)))
+    for key in PROFILE_KEYS:
+        e[key] = rel(w[key], np.interp(w["y_plus"], dns["y_plus"], dns[key]))
+        e[key + "_vs_filtered"] = rel(w[key], filtered[key])
+    e["score"] = e["U_plus"] + sum(e[k + "_vs_filtered"] for k in PROFILE_KEYS[1:]) / 3.0
+    return e
+
+
 # --------------------------------------------------------------- pipeline
+# Target forms, one network each: (name, x-z filter, target, sampling of the
+# samples).  The filter is the box filter of two LES spacings or the sharp
+# cutoff at the LES grid's modes (Sampler.snapshot).  pointwise: the dissipation-matching NU_T at each point, negative values
+# (backscatter) set to 0; signed: the same with its sign kept (a softplus
+# output cannot go below 0, so the fit settles on the conditional mean where
+# that is positive); dissipation: the signed target with samples drawn with
+# probability (S_kl S_kl)^2, which makes the mean-square loss the least-squares
+# fit of the dissipation D by 2 NU_T S_kl S_kl; plane: the ratio of the plane
+# means of D and 2 S_kl S_kl, the eddy viscosity that gives each plane its
+# mean subgrid dissipation.
+VARIANTS = (("box_pointwise", "box", "t_point", None), ("box_signed", "box", "t_signed", None),
+            ("box_plane", "box", "t_plane", None), ("sharp_signed", "sharp", "t_signed", None),
+            ("sharp_dissipation", "sharp", "t_signed", "SS2"), ("sharp_plane", "sharp", "t_plane", None))
+FILTERS = ("box", "sharp")
+PROFILE_KEYS = ("U_plus", "urms_plus", "vrms_plus", "wrms_plus")
+
 def run_pipeline(out_dir, spin=30000, stat=6000, every=20, nsnap=12, iters=3000, batch=16384, les_steps=12000,
                  log=print):
     torch = _torch()
@@ -295,108 +394,140 @@ def run_pipeline(out_dir, spin=30000, stat=6000, every=20, nsnap=12, iters=3000,
     pl, gyl, _ = les_re180()
     les = channel.Channel(pl, gyl)
     les.set_les(True)
-    net = Network(seed=0)
-    les.set_les_closure(net.params(), ts=pd["nu"], ls=pd["nu"], nus=pd["nu"])
+    les.set_les_closure(init_params(0), ts=pd["nu"], ls=pd["nu"], nus=pd["nu"])  # its inputs need it on
     sampler = Sampler(dns, pd, gyd, les, pl, gyl)
-    Xs, ts_, sm = [], [], []
+    snaps = {kind: [] for kind in FILTERS}
     acc, nacc = None, 0
+    fsamples = []
+
+    def filtered_sample(ts, t, tw, ub):
+        # plane statistics of the DNS velocity as the LES grid resolves it
+        # (sharp cutoff at its modes, y-interpolated to its rows)
+        sampler.load_resolved("sharp")
+        fsamples.append(les.save_stats(2.0))
+
     t0 = time.time()
+    t_stat0 = dns.clock()[0]
     per = max(1, stat // nsnap)
     done = 0
     while done < stat:
         k = min(per, stat - done)
-        s, _ = turbulence.run_statistics(dns, pd, k, every=every)
+        s, _ = turbulence.run_statistics(dns, pd, k, every=every, on_sample=filtered_sample)
         done += k
         acc = s if acc is None else {q: acc[q] + s[q] for q in s if q != "nsamples"}
         nacc += 1
-        X, t, smag = sampler.snapshot()
-        Xs.append(X); ts_.append(t); sm.append(smag)
-        log(f"[samples] snapshot {len(Xs)}: {t.numel()} samples, target mean {float(t.mean()):.3f} NU")
+        for kind in FILTERS:  # the sharp cutoff last: its resolved field stays in the LES context
+            snaps[kind].append(sampler.snapshot(kind))
+            sn = snaps[kind][-1]
+            log(f"[samples] {kind} snapshot {len(snaps[kind])}: {sn['t_point'].numel()} samples, pointwise target "
+                f"mean {float(sn['t_point'].mean()):.3f} NU, plane target mean {float(sn['t_plane'].mean()):.3f} NU")
     dns_wall = time.time() - t0
     dstats = {q: acc[q] / nacc for q in acc}
     # wall units of the forcing: PX0 = -1 on a half height of 1 gives u_tau = 1
     ut = math.sqrt(-pd["px0"])
     wu_d = turbulence.wall_units(dstats, pd["nu"], gyd, utau=ut)
-    rep["dns"] = {"u_tau_wall_shear": math.sqrt(max(dstats["tau_w"], 0.0)), "stat_steps": stat, "stat_wall_s": dns_wall}
-    # training on all but the last snapshot
-    Xtr = torch.cat(Xs[:-1], dim=1).contiguous()
-    ttr = torch.cat(ts_[:-1]).contiguous()
-    Xva, tva, smva = Xs[-1], ts_[-1], sm[-1]
-    n = ttr.numel()
+    fstats = {q: np.mean([f[q] for f in fsamples], axis=0) for q in fsamples[0]}
+    fstats["tau_w"] = dstats["tau_w"]
+    wu_f = turbulence.wall_units(fstats, pl["nu"], gyl, utau=ut)  # the LES grid's rows
+    rep["dns"] = {"u_tau_wall_shear": math.sqrt(max(dstats["tau_w"], 0.0)), "stat_steps": stat, "stat_wall_s": dns_wall,
+                  "dt_developed": (dns.clock()[0] - t_stat0) / stat}
+    # training on all but the last snapshot, one network per target form
+    cat = lambda kind, key: torch.cat([s[key] for s in snaps[kind][:-1]], dim=-1).contiguous()
+    vals = {kind: snaps[kind][-1] for kind in FILTERS}
+    n = vals["box"]["X"].shape[1] * (len(snaps["box"]) - 1)
+    nrow = pl["ny"] - 3
     gen = torch.Generator(device="cuda:0").manual_seed(0)
-    t0 = time.time()
-    losses = []
-    for it in range(iters):
-        lr = 1e-3 * (0.1 ** (it / iters))
-        idx = torch.randint(0, n, (batch,), device="cuda:0", generator=gen)
-        L = net.step(Xtr[:, idx].contiguous(), ttr[idx].contiguous(), lr)
-        if it % 250 == 0 or it == iters - 1:
-            losses.append((it, float(L)))
-            log(f"[train] iter {it} loss {float(L):.4f}")
-    torch.cuda.synchronize()
-    rep["train"] = {"iters": iters, "batch": batch, "samples": n, "wall_s": time.time() - t0, "loss": losses}
-    # a priori on the held-out snapshot
-    pred = net.predict(Xva)
+    nets = {}
+    rep["train"] = {}
+    chunk = 250
+    for name, kind, tkey, weight in VARIANTS:
+        Xtr, ttr = cat(kind, "X"), cat(kind, tkey)
+        w = None
+        if weight == "SS2":
+            w = cat(kind, "SS").double() ** 2
+            w = torch.clamp(w, max=float(torch.quantile(w[::31].float(), 0.999))).float()
+        net = Network(seed=0)
+        t0 = time.time()
+        losses = []
+        for it in range(iters):
+            if it % chunk == 0 and w is not None:
+                draws = torch.multinomial(w, chunk * batch, replacement=True, generator=gen)
+            lr = 1e-3 * (0.1 ** (it / iters))
+            if w is None:
+                idx = torch.randint(0, n, (batch,), device="cuda:0", generator=gen)
+            else:
+                idx = draws[(it % chunk) * batch:(it % chunk + 1) * batch]
+            L = net.step(Xtr[:, idx].contiguous(), ttr[idx].contiguous(), lr)
+            if it % chunk == 0 or it == iters - 1:
+                losses.append((it, float(L)))
+        torch.cuda.synchronize()
+        nets[name] = net
+        rep["train"][name] = {"filter": kind, "target": tkey, "sampling": weight or "uniform", "iters": iters, "batch": batch,
+                              "samples": n, "wall_s": time.time() - t0, "loss": losses}
+        log(f"[train] {name}: loss {losses[0][1]:.4f} -> {losses[-1][1]:.4f}")
+
+    # a priori on the held-out snapshot: the eddy viscosity against the
+    # pointwise target, and the subgrid dissipation 2 NU_T S_kl S_kl against
+    # D pointwise and as plane means
+    def corr(a, b):
+        a0, b0 = a - a.mean(), b - b.mean()
+        return float((a0 * b0).sum() / torch.sqrt((a0 * a0).sum() * (b0 * b0).sum()))
+
+    def apriori(pred, val):
+        Dp = 2.0 * pred * pd["nu"] * val["SS"]
+        pm = Dp.view(nrow, -1).mean(dim=1)
+        tm = val["D"].view(nrow, -1).mean(dim=1)
+        return {"nut_corr": corr(pred, val["t_point"]), "diss_corr": corr(Dp, val["D"]),
+                "plane_diss_rel_l2": float(torch.linalg.vector_norm(pm - tm) / torch.linalg.vector_norm(tm))}
+
     cs2d2 = smagorinsky_scale(pl)
     rows = np.repeat(cs2d2[3: pl["ny"]], pl["nx"] * pl["nz"])
-    smag_pred = smva * torch.from_numpy(rows).to("cuda:0").to(torch.float32)
-    def err(a):
-        return float(torch.linalg.vector_norm(a - tva) / torch.linalg.vector_norm(tva))
-    def corr(a):
-        a0, b0 = a - a.mean(), tva - tva.mean()
-        return float((a0 * b0).sum() / torch.sqrt((a0 * a0).sum() * (b0 * b0).sum()))
-    rep["a_priori"] = {"learned": {"rel_l2": err(pred), "corr": corr(pred)},
-                       "smagorinsky": {"rel_l2": err(smag_pred), "corr": corr(smag_pred)}}
+    rep["a_priori"] = {}
+    for kind in FILTERS:
+        val = vals[kind]
+        smag_pred = val["smag"] * torch.from_numpy(rows).to("cuda:0").to(torch.float32)
+        rep["a_priori"][f"smagorinsky_{kind}"] = apriori(smag_pred, val)
+        rep["a_priori"][f"dns_{kind}_mean_dissipation"] = float(val["D"].mean())
+    for name, kind, _, _ in VARIANTS:
+        rep["a_priori"][name] = apriori(nets[name].predict(vals[kind]["X"]), vals[kind])
     log(f"[a priori] {json.dumps(rep['a_priori'])}")
-    params = net.params()
-    np.savez(os.path.join(out_dir, "closure_trained.npz"), params=params, ts=pd["nu"], ls=pd["nu"], nus=pd["nu"],
-             X_val=Xva[:, :8192].cpu().numpy(), t_val=tva[:8192].cpu().numpy(), smag_val=smag_pred[:8192].cpu().numpy())
-    # a posteriori: LES from the last filtered snapshot, both models
-    res = {}
+
+    # a posteriori: LES from the last (sharp-cutoff) filtered snapshot with
+    # each model, and without one, against the DNS profiles (interpolated in y+)
     ub_state = les.download()
-    for model in ("smagorinsky", "learned"):
-        ch = channel.Channel(pl, gyl)
-        ch.set_les(True)
-        if model == "learned":
-            ch.set_les_closure(params, ts=pd["nu"], ls=pd["nu"], nus=pd["nu"])
-        st0 = dict(ub_state)
-        ch.upload(st0)
-        ch.apply_bc_vel()
-        ch.rem_div()
-        ch.set_clock(0.0, pl["delta_t"], 0)
-        half = les_steps // 2
-        ch.time_step(10)
-        ch.synchronize()
-        tl0, ts0 = time.time(), ch.clock()[0]
-        ch.time_step(half)
-        ch.synchronize()
-        rate = (time.time() - tl0) / (ch.clock()[0] - ts0)
-        s, h = turbulence.run_statistics(ch, pl, les_steps - half, every=every)
-        w = turbulence.wall_units(s, pl["nu"], gyl, utau=ut)
-        res[model] = {"stats": s, "wall": w, "rate": rate, "t_sim": ch.clock()[0],
-                      "u_tau_wall_shear": math.sqrt(max(s["tau_w"], 0.0))}
-        ch.close()
-    # compare with the DNS at the LES rows (interpolation in y+ of the folded profiles)
-    comp = {}
-    for model, r in res.items():
-        w = r["wall"]
-        yp = w["y_plus"]
-        e = {}
-        for key in ("U_plus", "urms_plus", "vrms_plus", "wrms_plus"):
-            ref = np.interp(yp, wu_d["y_plus"], wu_d[key])
-            e[key] = float(np.sqrt(np.mean((w[key] - ref) ** 2)) / np.sqrt(np.mean(ref ** 2)))
-        e["u_tau_wall_shear"] = r["u_tau_wall_shear"]
-        e["wall_s_per_unit_time"] = r["rate"]
-        e["t_sim"] = r["t_sim"]
-        comp[model] = e
+
+    def les_run(model):
+        params = nets[model].params() if model in nets else None
+        w, e = les_profiles(ub_state, les_steps, model="learned" if params is not None else model, params=params,
+                            scales=(pd["nu"], pd["nu"], pd["nu"]), every=every, utau=ut)
+        e.update(profile_errors(w, wu_d, wu_f))
+        log(f"[a posteriori] {model}: {json.dumps(e)}")
+        return e, w
+
+    comp, walls = {}, {}
+    for model in ("none", "smagorinsky") + tuple(nets):
+        comp[model], walls[model] = les_run(model)
     rep["a_posteriori"] = comp
+    chosen = min(nets, key=lambda m: comp[m]["score"])
+    rep["chosen"] = chosen
     # time to statistics: wall seconds per unit of simulated time while stepping
-    rep["time_per_unit_time"] = {"dns_s": dns_rate, "les_learned_s": comp["learned"]["wall_s_per_unit_time"],
+    rep["time_per_unit_time"] = {"dns_s": dns_rate, "les_learned_s": comp[chosen]["wall_s_per_unit_time"],
                                  "les_smagorinsky_s": comp["smagorinsky"]["wall_s_per_unit_time"],
-                                 "speedup_learned_vs_dns": dns_rate / comp["learned"]["wall_s_per_unit_time"]}
+                                 "speedup_learned_vs_dns": dns_rate / comp[chosen]["wall_s_per_unit_time"]}
     rep["profiles"] = {"dns": {k: np.asarray(v).tolist() for k, v in wu_d.items() if isinstance(v, np.ndarray)},
-                       **{m: {k: np.asarray(v).tolist() for k, v in r["wall"].items() if isinstance(v, np.ndarray)}
-                          for m, r in res.items()}}
+                       "dns_filtered": {k: np.asarray(v).tolist() for k, v in wu_f.items()
+                                        if isinstance(v, np.ndarray)},
+                       **{m: {k: np.asarray(v).tolist() for k, v in w.items() if isinstance(v, np.ndarray)}
+                          for m, w in walls.items()}}
+    np.savez(os.path.join(out_dir, "closure_trained.npz"), params=nets[chosen].params(), variant=chosen,
+             ts=pd["nu"], ls=pd["nu"], nus=pd["nu"],
+             **{f"params_{m}": net.params() for m, net in nets.items()})
+    # the a-posteriori case as a fixture: the filtered initial state, and the
+    # DNS profiles in wall units, whole and as the LES grid resolves them
+    np.savez_compressed(os.path.join(out_dir, "les_case.npz"),
+                        **{k: ub_state[k].astype(np.complex64) for k in ("cu1", "cu2", "cu3")},
+                        **{f"dns_{k}": np.asarray(wu_d[k]) for k in ("y_plus",) + PROFILE_KEYS},
+                        **{f"filtered_{k}": np.asarray(wu_f[k]) for k in ("y_plus",) + PROFILE_KEYS})
     rep["total_wall_s"] = time.time() - t_start
     les.close()
     dns.close()
