@@ -51,10 +51,10 @@ def phase_bytes(p, rk):
     b["to_physical"] = nf * nyy * (Fp + Pp)
     cf_old = 3 * (NY - 1) * Fp if rk > 1 else 0
     b["nonlinear"] = 3 * (NY + 1) * Pp + 3 * (NY - 1) * Fp + NY * Fp + cf_old + 3 * (NY - 1) * Fp + 3 * (NY - 1) * Pp
-    b["cn_solve"] = 3 * (NY - 1) * Pp + 3 * nyy * Pp + 3 * (NY + 1) * Pp
+    b["cn_solve"] = 3 * (NY - 1) * Pp + 3 * (NY + 1) * Pp  # R in, the solution written over it
     b["courant"] = 0
     b["to_fourier"] = nf * nyy * (Fp + Pp)
-    b["rem_div"] = 3 * NY * Fp + 3 * NY * Fp + 2 * (NY - 1) * Fp + NY * Fp
+    b["rem_div"] = 3 * NY * Fp + 3 * NY * Fp + 2 * (NY - 1) * Fp  # velocity in and out, CP updated
     return b
 
 
