@@ -325,8 +325,9 @@ __global__ void __launch_bounds__(256) k_les_mlp(LesArgs a) {
   const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
   const float b4 = a.mlp_p[kOB4];
   const int ldi = (int)a.fs;
+  MlpRegs r;
+  mlp_load_regs(a.mlp_pk, r);
   for (long long base = wave * 16; base < npts; base += nwaves * 16) {
-    asm volatile("" ::: "memory");  // operands re-read each pass, not held across it
     const long long q = base + c;
     bf16x8 x;
 #pragma unroll
@@ -336,8 +337,7 @@ __global__ void __launch_bounds__(256) k_les_mlp(LesArgs a) {
 #pragma unroll
       for (int j = 0; j < kMlpIn; ++j) x[j] = (__bf16)(float)fp[j * ldi];
     }
-    f32x4 z1[4], z2[4], z3[4];
-    const float y = mlp_forward(a.mlp_pk, b4, x, z1, z2, z3);
+    const float y = mlp_forward_regs(r, b4, x);
     if (lane < 16 && q < npts) {
       const int J = 1 + (int)(q / pp);
       a.nut[pp + q] = (J >= 3 && J <= NY - 1) ? a.mlp_nus * (double)softplus(y) : 0.0;
@@ -439,7 +439,7 @@ hipError_t launch_les_model(Ctx &c, hipStream_t st) {
     return e;
   if (c.les_mlp) {
     // 16 points per wave and pass; a grid of about four waves per SIMD
-    if ((e = klaunch(c, (const void *)k_les_mlp, nullptr, dim3(1024), dim3(256), 0, st, a)) != hipSuccess) return e;
+    if ((e = klaunch(c, (const void *)k_les_mlp, nullptr, dim3(512), dim3(256), 0, st, a)) != hipSuccess) return e;
     if ((e = klaunch(c, (const void *)k_les_post, nullptr, dim3(pb, NY + 1), dim3(256), 0, st, a)) != hipSuccess)
       return e;
   }

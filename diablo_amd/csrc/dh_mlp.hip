@@ -79,14 +79,13 @@ __global__ void __launch_bounds__(256) k_mlp_infer(const bf16x8 *__restrict__ pk
   const int lane = threadIdx.x & 63;
   const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const long long nw = ((long long)gridDim.x * blockDim.x) >> 6;
+  MlpRegs r;
+  mlp_load_regs(pk, r);
+  const float b4 = P[kOB4];
   for (long long n0 = wave * 16; n0 < N; n0 += nw * 16) {
-    // the operands are read again in every pass (cached) rather than held
-    // in registers across the loop
-    asm volatile("" ::: "memory");
     const long long n = n0 + (lane & 15);
     const bf16x8 xb = load_x(x, ld, n, N, kMlpIn);
-    f32x4 z1[4], z2[4], z3[4];
-    const float y = mlp_forward(pk, P[kOB4], xb, z1, z2, z3);
+    const float y = mlp_forward_regs(r, b4, xb);
     if (lane < 16 && n < N) out[n] = scale * softplus(y);
   }
 }

@@ -52,15 +52,23 @@ constexpr int kPBias = kPTot, kPW4 = kPBias + 12 * 64, kPAll = kPW4 + 4 * 64;
 __host__ __device__ constexpr int mlp_unit(int kb, int g, int j) { return 16 * (2 * kb + j / 4) + 4 * g + (j & 3); }
 
 // GELU in its tanh form (Hendrycks & Gimpel 2016): 0.5 z (1 + tanh(a)),
-// a = sqrt(2/pi) (z + 0.044715 z^3), with tanh(a) = 1 - 2 / (exp(2a) + 1)
-__device__ __forceinline__ float gelu_t(float z) {
-  const float a = 0.7978845608028654f * (z + 0.044715f * z * z * z);
-  return 1.0f - 2.0f / (__expf(2.0f * a) + 1.0f);
+// a = sqrt(2/pi) (z + 0.044715 z^3).  Since 0.5 (1 + tanh(a)) is the
+// logistic function of 2a, GELU = z s with s = 1 / (1 + 2^(-2a log2 e)):
+// one exp2, one add and one reciprocal (v_exp_f32 and v_rcp_f32, each within
+// an FP32 ulp or so; the values go on to BF16), about 7 instructions where a
+// tanh through an IEEE division takes about 22.  Far below zero the exp2
+// overflows to +inf and s to 0; far above it s is 1.
+__device__ __forceinline__ float gelu_s(float z) {
+  // -2 log2(e) sqrt(2/pi) and the same times 0.044715
+  const float c0 = -2.302208198144325f, c1 = -0.10294324f;
+  const float e = __builtin_amdgcn_exp2f(z * fmaf(c1, z * z, c0));
+  return __builtin_amdgcn_rcpf(e + 1.0f);
 }
-__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + gelu_t(z)); }
+__device__ __forceinline__ float gelu(float z) { return z * gelu_s(z); }
+// d GELU / dz = s + z s (1 - s) 2 a'(z), a'(z) = sqrt(2/pi) (1 + 3 0.044715 z^2)
 __device__ __forceinline__ float gelu_grad(float z) {
-  const float t = gelu_t(z);
-  return 0.5f * (1.0f + t) + 0.5f * z * (1.0f - t * t) * 0.7978845608028654f * (1.0f + 0.134145f * z * z);
+  const float s = gelu_s(z);
+  return fmaf(z * s * (1.0f - s), 1.5957691216057308f * fmaf(0.134145f, z * z, 1.0f), s);
 }
 __device__ __forceinline__ float softplus(float y) { return fmaxf(y, 0.0f) + log1pf(__expf(-fabsf(y))); }
 __device__ __forceinline__ float sigmoid(float y) { return 1.0f / (1.0f + __expf(-y)); }
@@ -128,6 +136,53 @@ __device__ __forceinline__ float mlp_forward(const bf16x8 *pk, float b4, const b
   f32x4 d = {0.f, 0.f, 0.f, 0.f};
   d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(pk[kPA4 + lane], pack_b(h, 0), d, 0, 0, 0);
   d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(pk[kPA4 + 64 + lane], pack_b(h, 1), d, 0, 0, 0);
+  return d[0] + b4;
+}
+
+// Inference holds this lane's forward operands in registers for a wave's
+// whole loop (22 A operands and 12 bias tiles, 136 VGPRs): re-read in every
+// 16-sample pass they are 35 KB per wave and pass, more than the vector L1
+// holds, and the pass becomes bound by L2 reads.
+struct MlpRegs {
+  bf16x8 a[22];  // A1[4], A2[4][2], A3[4][2], A4[2]: packed units kPA1 + 64 i
+  f32x4 b[12];   // b1, b2, b3 [mt]
+};
+__device__ __forceinline__ void mlp_load_regs(const bf16x8 *pk, MlpRegs &r) {
+  const int lane = threadIdx.x & 63;
+  const f32x4 *pb = reinterpret_cast<const f32x4 *>(pk) + kPBias;
+#pragma unroll
+  for (int i = 0; i < 22; ++i) r.a[i] = pk[kPA1 + i * 64 + lane];
+#pragma unroll
+  for (int i = 0; i < 12; ++i) r.b[i] = pb[i * 64 + lane];
+}
+
+// mlp_forward's output y (in lanes 0..15) from the operands in registers.
+__device__ __forceinline__ float mlp_forward_regs(const MlpRegs &r, float b4, const bf16x8 &x) {
+  static_assert(kPA2 - kPA1 == 4 * 64 && kPA3 - kPA1 == 12 * 64 && kPA4 - kPA1 == 20 * 64, "A operand order");
+  f32x4 z[4], h[4];
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt) z[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(r.a[mt], x, r.b[mt], 0, 0, 0);
+#pragma unroll
+  for (int layer = 1; layer < 3; ++layer) {
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) h[mt][i] = gelu(z[mt][i]);
+    const bf16x8 b0 = pack_b(h, 0), b1 = pack_b(h, 1);
+    const int a0 = 4 + 8 * (layer - 1);
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+      f32x4 d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(r.a[a0 + 2 * mt], b0, r.b[4 * layer + mt], 0, 0, 0);
+      z[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(r.a[a0 + 2 * mt + 1], b1, d, 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) h[mt][i] = gelu(z[mt][i]);
+  f32x4 d = {0.f, 0.f, 0.f, 0.f};
+  d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(r.a[20], pack_b(h, 0), d, 0, 0, 0);
+  d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(r.a[21], pack_b(h, 1), d, 0, 0, 0);
   return d[0] + b4;
 }
 
