@@ -134,7 +134,7 @@ hipError_t dalloc(T **p, size_t count) {
 void free_all(Ctx &c) {
   void *ptrs[] = {c.tw_row, c.tw_col, c.tw_real, c.posr, c.posrd, c.posz, c.posc, c.kx, c.kz, c.kx2, c.kz2, c.d_dy, c.d_dyf, c.d_gyf, c.d_rdy, c.d_rdyf, c.cu[0], c.cu[1],
                   c.cu[2], c.cp, c.cf[0], c.cf[1], c.cf[2], c.cth, c.cfth, c.phi, c.uphys, c.rphys, c.rd_ta,
-                  c.rd_tb, c.clock, c.fac.pf, c.fac.pb, c.fac.tl, c.fac.tb, c.fac.tc, c.fac.tr, c.fac.isbc, c.rd_tr, c.fac.vconst, c.rd_l, c.rd_u, c.stats, c.cun[0], c.cun[1], c.cun[2], c.uphys32, c.rphys32, c.fac.bcl, c.fac.bcd, c.fac.bcu, c.nut, c.cul, c.les_four, c.les_phys, c.sgs, c.les_wk, c.d_lesw, c.d_mlp_p, c.d_mlp_pk, c.les_feat};
+                  c.rd_tb, c.clock, c.fac.pf, c.fac.pb, c.fac.tl, c.fac.tb, c.fac.tc, c.fac.tr, c.fac.isbc, c.rd_tr, c.fac.vconst, c.rd_l, c.rd_u, c.stats, c.cun[0], c.cun[1], c.cun[2], c.uphys32, c.rphys32, c.fac.bcl, c.fac.bcd, c.fac.bcu, c.nut, c.cul, c.les_four, c.les_phys, c.sgs, c.les_wk, c.d_lesw, c.d_mlp_p, c.d_mlp_pk, c.les_feat, c.gp_scratch};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c.step_graph) (void)hipGraphExecDestroy(c.step_graph);
@@ -234,16 +234,18 @@ int dh_create(const dh_params *p, dh_ctx **out) {
                 (2 * g.lw + 2 * g.nz) * (int)sizeof(int);
   c.lds_bytes32 = g.nz * g.ls * (int)sizeof(float2) + (2 * g.lw + g.nz + 1) * (int)sizeof(float2) +
                   (2 * g.lw + 2 * g.nz) * (int)sizeof(int);
-  if (c.lds_bytes > 160 * 1024) return reject("x-z plane too large for the LDS-resident plane kernels");
+  // planes beyond LDS go through scratch planes in device memory (FP64, no LES)
+  c.gp = c.lds_bytes > kLdsMax;
+  if (gp_table_bytes(c) > 64 * 1024) return reject("x-z plane too large (NX/2 + NZ beyond the transform tables)");
   if (g.ny + 2 > 8 * kSegRows) return reject("NY+2 is beyond the column solver's register segments (8 x 36 rows)");
   int thr = (g.nz * g.lw + 7) / 8;
   thr = ((thr + 63) / 64) * 64;
   if (thr < 64) thr = 64;
   if (thr > 1024) thr = 1024;
   while (thr * 4 < g.pf && thr < 1024) thr += 64;  // k_nonlinear keeps <= 4 modes per thread
-  if (thr * 4 < g.pf) return reject("plane has too many Fourier modes");
+  if (!c.gp && thr * 4 < g.pf) return reject("plane has too many Fourier modes");
   c.plane_threads = thr;
-  c.reg_rows = (g.lw == 64 && g.nz % 8 == 0 && thr % 64 == 0 && g.nkx < 8 * kRowK2);
+  c.reg_rows = !c.gp && (g.lw == 64 && g.nz % 8 == 0 && thr % 64 == 0 && g.nkx < 8 * kRowK2);
   c.reg_rows_nl = c.reg_rows && g.pf <= 512 * 8;  // k_nonlinear's register variant: 512 threads x 8 modes
   {
   }
@@ -317,6 +319,10 @@ int dh_create(const dh_params *p, dh_ctx **out) {
   ACHK(dalloc(&c.cfth, cf_elems * (p->n_th > 0 ? p->n_th : 1)));
   ACHK(dalloc(&c.uphys, ph_elems * c.nfields));
   ACHK(dalloc(&c.rphys, ph_elems * c.nfields));
+  if (c.gp) {
+    c.gp_slot = (long long)g.nz * g.ls + g.pf;
+    ACHK(dalloc(&c.gp_scratch, 2 * (size_t)kGpSlots * (size_t)c.gp_slot));
+  }
   ACHK(dalloc(&c.rd_ta, cf_elems));
   ACHK(dalloc(&c.rd_tb, cf_elems));
   ACHK(dalloc(&c.rd_tr, cf_elems));
@@ -674,6 +680,7 @@ int dh_set_mixed_precision(dh_ctx *h, int32_t on) {
   Ctx &c = h->c;
   if ((on != 0) == c.mixed) return DH_OK;
   if (on && c.les) return fail(c, DH_EARG, "the mixed form does not carry the LES model (FP64 only)");
+  if (on && c.gp) return fail(c, DH_EARG, "planes beyond LDS run in FP64 only");
   HIPCHK(hipStreamSynchronize(c.stream));
   if (on && !c.uphys32) {
     const size_t n = (size_t)(c.g.ny + 2) * (size_t)c.g.pp * (size_t)c.nfields;
@@ -706,6 +713,7 @@ int dh_set_les(dh_ctx *h, int32_t on) {
   Ctx &c = h->c;
   if ((on != 0) == c.les) return DH_OK;
   if (on && c.mixed) return fail(c, DH_EARG, "LES runs in FP64: switch the mixed form off first");
+  if (on && c.gp) return fail(c, DH_EARG, "LES needs planes that fit in LDS");
   HIPCHK(hipStreamSynchronize(c.stream));
   if (on && !c.nut) {
     const size_t nyy = (size_t)c.g.ny + 2, pp = (size_t)c.g.pp, pf = (size_t)c.g.pf, nth = (size_t)c.p.n_th;

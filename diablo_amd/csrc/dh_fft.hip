@@ -29,12 +29,11 @@ struct PlaneArgs {
   PlaneBatch b;
 };
 
-// Fourier -> physical for planes j0 + blockIdx.x of field blockIdx.y.
-template <bool REG, class C>
-__device__ __forceinline__ void plane_inv_body(const Geo &g, const PlaneBatch &b, C *A) {
-  const auto T = load_tables(A, g);
-  const int f = blockIdx.y;
-  const long long j = b.j0 + blockIdx.x;
+// Fourier -> physical for plane j of field f.  A: the plane (LDS, or a
+// scratch plane in device memory), T: the transform tables in LDS.
+template <bool REG, class C, class TT>
+__device__ __forceinline__ void plane_inv_body(const Geo &g, const PlaneBatch &b, C *A, const TT &T, int f,
+                                               long long j) {
   plane_unpack(A, g, T, b.four[f] + j * g.pf, b.drop_mean != 0);
   plane_cols_inv(A, g, T);
   C *dst = reinterpret_cast<C *>(static_cast<Sc<C> *>(b.phys[f]) + j * g.pp);
@@ -50,11 +49,9 @@ __device__ __forceinline__ void plane_inv_body(const Geo &g, const PlaneBatch &b
 // packing of kz = -NKZ..-1 after 0..NKZ).
 // COLS: the register column transform when NZ = 128 (left out of the
 // batched kernel, where its registers do not fit beside the batch indexing).
-template <bool REG, class C, bool COLS = REG>
-__device__ __forceinline__ void plane_fwd_body(const Geo &g, const PlaneBatch &b, C *A) {
-  const auto T = load_tables(A, g);
-  const int f = blockIdx.y;
-  const long long j = b.j0 + blockIdx.x;
+template <bool REG, class C, bool COLS = REG, class TT>
+__device__ __forceinline__ void plane_fwd_body(const Geo &g, const PlaneBatch &b, C *A, const TT &T, int f,
+                                               long long j) {
   const double2 *base = b.base[f] ? b.base[f] + j * g.pf : nullptr;
   double2 *dst = const_cast<double2 *>(b.four[f]) + j * g.pf;
   if (base && (j < b.alo[f] || j > b.ahi[f])) {  // no increment on this plane
@@ -100,27 +97,59 @@ __device__ __forceinline__ void plane_fwd_body(const Geo &g, const PlaneBatch &b
 
 // the kernels: one context (arguments by value), or a batch of contexts of
 // one shape (argument structs in device memory, member blockIdx.z)
+// plane j0 + blockIdx.x of field blockIdx.y
 template <bool REG, class C>
 __global__ void __launch_bounds__(1024) k_plane_inv(PlaneArgs pa) {
   extern __shared__ double2 smem[];
-  plane_inv_body<REG, C>(pa.g, pa.b, reinterpret_cast<C *>(smem));
+  C *A = reinterpret_cast<C *>(smem);
+  plane_inv_body<REG, C>(pa.g, pa.b, A, load_tables(A, pa.g), blockIdx.y, pa.b.j0 + blockIdx.x);
 }
 template <bool REG, class C>
 __global__ void __launch_bounds__(1024) k_plane_inv_b(const PlaneArgs *__restrict__ av) {
   extern __shared__ double2 smem[];
   const PlaneArgs &pa = av[blockIdx.z];
-  plane_inv_body<REG, C>(pa.g, pa.b, reinterpret_cast<C *>(smem));
+  C *A = reinterpret_cast<C *>(smem);
+  plane_inv_body<REG, C>(pa.g, pa.b, A, load_tables(A, pa.g), blockIdx.y, pa.b.j0 + blockIdx.x);
 }
 template <bool REG, class C>
 __global__ void __launch_bounds__(1024) k_plane_fwd(PlaneArgs pa) {
   extern __shared__ double2 smem[];
-  plane_fwd_body<REG, C>(pa.g, pa.b, reinterpret_cast<C *>(smem));
+  C *A = reinterpret_cast<C *>(smem);
+  plane_fwd_body<REG, C>(pa.g, pa.b, A, load_tables(A, pa.g), blockIdx.y, pa.b.j0 + blockIdx.x);
 }
 template <bool REG, class C>
 __global__ void __launch_bounds__(1024) k_plane_fwd_b(const PlaneArgs *__restrict__ av) {
   extern __shared__ double2 smem[];
   const PlaneArgs &pa = av[blockIdx.z];
-  plane_fwd_body<REG, C, false>(pa.g, pa.b, reinterpret_cast<C *>(smem));
+  C *A = reinterpret_cast<C *>(smem);
+  plane_fwd_body<REG, C, false>(pa.g, pa.b, A, load_tables(A, pa.g), blockIdx.y, pa.b.j0 + blockIdx.x);
+}
+
+// Planes too large for LDS (FP64): block b works on its own scratch plane
+// in device memory (`gp`, planes of gp_slot complex values) and takes the
+// (plane, field) items b, b + gridDim.x, ...: nplanes planes of each field.
+// The tables stay in LDS.
+__global__ void __launch_bounds__(kGpThreads) k_plane_inv_gp(PlaneArgs pa, double2 *gp, long long gp_slot,
+                                                             int nplanes, int nitems) {
+  extern __shared__ double2 smem[];
+  const auto T = load_tables_at(smem, pa.g);
+  double2 *A = gp + (long long)blockIdx.x * gp_slot;
+  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const int f = it / nplanes;
+    plane_inv_body<false, double2>(pa.g, pa.b, A, T, f, pa.b.j0 + (it - f * nplanes));
+    __syncthreads();  // the next item's unpack writes over the plane this one stored from
+  }
+}
+__global__ void __launch_bounds__(kGpThreads) k_plane_fwd_gp(PlaneArgs pa, double2 *gp, long long gp_slot,
+                                                             int nplanes, int nitems) {
+  extern __shared__ double2 smem[];
+  const auto T = load_tables_at(smem, pa.g);
+  double2 *A = gp + (long long)blockIdx.x * gp_slot;
+  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const int f = it / nplanes;
+    plane_fwd_body<false, double2, false>(pa.g, pa.b, A, T, f, pa.b.j0 + (it - f * nplanes));
+    __syncthreads();
+  }
 }
 
 static hipError_t set_lds(const void *fn, int bytes) {
@@ -129,8 +158,25 @@ static hipError_t set_lds(const void *fn, int bytes) {
 
 // Launches of the plane kernels in precision C (register rows when the
 // plane allows them).
+// Large planes: the persistent kernels above over grid.x planes of grid.y
+// fields, on the scratch half of the stream (the branch stream has its own).
+template <bool INV>
+static hipError_t plane_gp(Ctx &c, dim3 grid, hipStream_t st, const PlaneBatch &b) {
+  if (c.recording) return hipErrorNotSupported;  // no batched forms
+  PlaneArgs pa{c.g, b};
+  double2 *gp = gp_base(c, st);
+  long long slot = c.gp_slot;
+  int nplanes = (int)grid.x, nitems = (int)(grid.x * grid.y);
+  void *args[] = {&pa, &gp, &slot, &nplanes, &nitems};
+  const unsigned blocks = (unsigned)(nitems < kGpSlots ? nitems : kGpSlots);
+  return hipLaunchKernel(INV ? (const void *)k_plane_inv_gp : (const void *)k_plane_fwd_gp, dim3(blocks),
+                         dim3(kGpThreads), args, (unsigned)gp_table_bytes(c), st);
+}
+
 template <class C>
 static hipError_t plane_inv(Ctx &c, dim3 grid, hipStream_t st, const PlaneBatch &b) {
+  if constexpr (sizeof(C) == sizeof(double2))
+    if (c.gp) return plane_gp<true>(c, grid, st, b);
   const PlaneArgs pa{c.g, b};
   const unsigned lds = lds_bytes_of<C>(c);
   if (c.reg_rows)
@@ -141,6 +187,8 @@ static hipError_t plane_inv(Ctx &c, dim3 grid, hipStream_t st, const PlaneBatch 
 }
 template <class C>
 static hipError_t plane_fwd(Ctx &c, dim3 grid, hipStream_t st, const PlaneBatch &b) {
+  if constexpr (sizeof(C) == sizeof(double2))
+    if (c.gp) return plane_gp<false>(c, grid, st, b);
   const PlaneArgs pa{c.g, b};
   const unsigned lds = lds_bytes_of<C>(c);
   if (c.reg_rows)
@@ -279,7 +327,7 @@ static hipError_t set_lds_plane() {
 }
 
 hipError_t prepare_fft_kernels(Ctx &c) {
-  if (c.lds_bytes > kLdsMax) return hipErrorInvalidValue;
+  if (c.lds_bytes > kLdsMax && !c.gp) return hipErrorInvalidValue;
   hipError_t e;
   if ((e = set_lds_plane<true, double2>()) != hipSuccess) return e;
   if ((e = set_lds_plane<false, double2>()) != hipSuccess) return e;

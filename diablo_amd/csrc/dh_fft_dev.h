@@ -97,6 +97,22 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// Barrier after a stage of a plane transform.  The plane is in LDS for
+// planes that fit (lds_barrier), or, for larger planes, in a scratch plane in
+// device memory that only this block uses, whose stores the barrier must
+// also order (__syncthreads: workgroup-scope fence on all memory).  The
+// plane pointer's address space is known once the kernel is inlined, so the
+// test folds away.
+template <class P>
+__device__ __forceinline__ void plane_barrier(const P *A) {
+#ifdef __HIP_DEVICE_COMPILE__  // the builtin exists for the device pass only
+  if (__builtin_amdgcn_is_shared((const void *)A))
+    lds_barrier();
+  else
+    __syncthreads();
+#endif
+}
+
 // n / b for a divisor b whose correctly rounded reciprocal r = 1/b is known:
 // q0 = n r, e = n - b q0 (exact by fma), q = q0 + e r.  With r correctly
 // rounded this is the correctly rounded quotient (Markstein's theorem) away
@@ -207,11 +223,10 @@ struct TablesT {
 };
 using Tables = TablesT<double2>;
 
-// Copies the tables into LDS; returns pointers into `smem` after the plane.
+// Copies the tables into LDS at p; returns pointers into it.
 template <class C>
-__device__ TablesT<C> load_tables(C *smem, const Geo &g) {
+__device__ TablesT<C> load_tables_at(C *p, const Geo &g) {
   TablesT<C> t;
-  C *p = smem + g.nz * g.ls;
   t.twr = p;
   t.twc = p + g.lw;
   t.twx = p + g.lw + g.nz;
@@ -233,6 +248,11 @@ __device__ TablesT<C> load_tables(C *smem, const Geo &g) {
   for (int i = threadIdx.x; i <= g.lw; i += blockDim.x) t.twx[i] = cvt<C>(g.tw_real[i]);
   lds_barrier();
   return t;
+}
+// The same behind an LDS plane that starts at smem.
+template <class C>
+__device__ TablesT<C> load_tables(C *smem, const Geo &g) {
+  return load_tables_at(smem + g.nz * g.ls, g);
 }
 
 // One in-place stage of radix R over `ntrans` transforms of length N.
@@ -283,7 +303,7 @@ __device__ __forceinline__ void fft_stage(C *A, const FftStage &st, int N, int n
 #pragma unroll
     for (int q = 0; q < R; ++q) base[q * es] = v[q];
   }
-  lds_barrier();
+  plane_barrier(A);
 }
 
 template <bool INV, bool COLS, bool DIT, class C>
@@ -362,7 +382,7 @@ __device__ void plane_rows_fwd(C *A, const Geo &g, const TablesT<C> &T) {
       row[k2] = mkc<C>(div_by_recip(X.x, nxd, rnx), div_by_recip(X.y, nxd, rnx));
     }
   }
-  lds_barrier();
+  plane_barrier(A);
 }
 
 // Inverse x transform of every row (FFT_XZ_TO_PHYSICAL's c2r, mpi.f:1600-1604).
@@ -395,7 +415,7 @@ __device__ void plane_rows_inv(C *A, const Geo &g, const TablesT<C> &T) {
       row[k2] = mkc<C>(a.x - wb.y, a.y + wb.x);
     }
   }
-  lds_barrier();
+  plane_barrier(A);
   lds_fft<true, false, false>(A, g.prow, g.nz, g.f_nz, g.ls, T.twr);
 }
 
@@ -466,7 +486,7 @@ __device__ void plane_unpack(C *A, const Geo &g, const TablesT<C> &T, const doub
     for (int b = 0; b < B; ++b)
       if (slot[b] >= 0) A[slot[b]] = v[b];
   }
-  lds_barrier();
+  plane_barrier(A);
 }
 
 // Physical plane (natural order in HBM) into the digit-reversed input slots.
@@ -495,7 +515,7 @@ __device__ void plane_load_phys(C *A, const Geo &g, const TablesT<C> &T, const C
     for (int b = 0; b < B; ++b)
       if (sl[b] >= 0) A[sl[b]] = v[b];
   }
-  lds_barrier();
+  plane_barrier(A);
 }
 
 // ---------------------------------------------------------------------------

@@ -88,6 +88,10 @@ constexpr int kRowK2 = 6;
 // than one context's need (a smaller context created later would otherwise
 // lower it under a larger one).
 constexpr int kLdsMax = 160 * 1024;
+// Planes too large for LDS (dh_create sets Ctx::gp) go through scratch
+// planes in device memory: kGpSlots per stream half, one per block of the
+// persistent plane kernels (kGpThreads threads).
+constexpr int kGpSlots = 512, kGpThreads = 512;
 
 // Tables of the y-implicit (Crank-Nicolson) matrices, one set per RK
 // substep and field; with LES off each matrix depends only on j, so every
@@ -192,7 +196,21 @@ struct Ctx {
   void *d_mlp_pk = nullptr;     // packed operands (kPAll x 16 bytes)
   double mlp_ts = 1.0, mlp_ls = 1.0, mlp_nus = 1.0;  // time scale, length scale, NU_T scale
   double *les_feat = nullptr;
+  // planes too large for LDS: kGpSlots scratch slots for the main stream's
+  // kernels and kGpSlots for the branch's, gp_slot complex values each (a
+  // plane of NZ rows of stride LS, then the nonlinear pass's accumulators
+  // of the PF modes)
+  bool gp = false;
+  double2 *gp_scratch = nullptr;
+  long long gp_slot = 0;
 };
+inline double2 *gp_base(const Ctx &c, hipStream_t st) {
+  return c.gp_scratch + (st == c.branch ? (long long)kGpSlots * c.gp_slot : 0LL);
+}
+// LDS of the transform tables alone (load_tables_at), for the large-plane kernels
+inline int gp_table_bytes(const Ctx &c) {
+  return (2 * c.g.lw + c.g.nz + 1) * (int)sizeof(double2) + (2 * c.g.lw + 2 * c.g.nz) * (int)sizeof(int);
+}
 // feature slots of the learned closure: 0..5 the network's inputs, 6..11
 // S11, S22, S33, S12, S13, S23 at GY, 12 |S| at GY
 constexpr int kLesFeat = 13;

@@ -588,6 +588,136 @@ __device__ __forceinline__ void nonlinear_pass(const StepArgs &a, C *A) {
     plane_store_phys(A, g, T, dst);
   }
 }
+// The nonlinear pass for planes too large for LDS (FP64, no LES): the same
+// terms in the same order as nonlinear_pass's LDS variant, with the plane in
+// a scratch plane A in device memory and the CF accumulators of the PF modes
+// in ACC (device memory after the plane), modes taken m = threadIdx.x,
+// + blockDim.x, ... in every loop.
+__device__ void nonlinear_pass_gp(const StepArgs &a, int J, int c, double2 *A, double2 *ACC, const Tables &T) {
+  const Geo &g = a.g;
+  int jlo = 2, jhi = g.ny;
+  if (c >= 3) {
+    jlo = a.jstart_th[c - 3];
+    jhi = a.jend_th[c - 3];
+  }
+  if (J < jlo || J > jhi) return;
+  const int rk = a.rk;
+  const double hb = a.clk->h_bar[rk];
+  const double T3 = a.clk->zeta_bar[rk] * hb;
+  const double T4 = hb;
+  const double T5 = a.clk->beta_bar[rk] * hb;
+  const double time = a.clk->time;
+  const int nthr = blockDim.x;
+  for (int m = threadIdx.x; m < g.pf; m += nthr) {
+    const int kzp = (int)fdiv((unsigned)m, g.f_nkx1), kx = m - kzp * g.nkx1;
+    ACC[m] = cf_init(a, c, J, kx, kzp, m);
+  }
+  // the three products, forward transformed and subtracted with their
+  // derivative factors: c==0 takes (ikz, ikx, 1), all others (ikx, ikz, 1)
+  const int pz = (c == 0) ? 0 : 1;
+  const LoadTile lt(g);
+  const int nw = nthr >> 6;
+  const double nzd = g.dnz, rnz = g.rnz;
+  for (int p = 0; p < 3; ++p) {
+    const Prod<double2> pr = make_prod<double2>(a, c, p, J);
+    for (int t0 = threadIdx.x >> 6; t0 < lt.ntiles; t0 += nw) {
+      int z, xp;
+      if (lt.at(g, t0, z, xp)) A[phys_slot(T, g, z, xp)] = eval_prod(pr, (long long)z * g.nx + 2 * xp);
+    }
+    __syncthreads();
+    plane_rows_fwd(A, g, T);
+    plane_cols_fwd(A, g, T);
+    const int mode = (p == 2) ? 2 : (p == pz ? 0 : 1);
+    for (int m = threadIdx.x; m < g.pf; m += nthr) {
+      const int kzp = (int)fdiv((unsigned)m, g.f_nkx1), kx = m - kzp * g.nkx1;
+      const double2 s0 = A[fwd_slot(g, T, kx, kzp)];
+      const double2 sv = make_double2(div_by_recip(s0.x, nzd, rnz), div_by_recip(s0.y, nzd, rnz));
+      ACC[m] = csub(ACC[m], (mode == 2) ? sv : cmul_ik((mode == 0) ? g.kz[kzp] : g.kx[kx], sv));
+    }
+    __syncthreads();  // the next product (or CR) is written over the plane
+  }
+  // RK combination (channel.f:88-167, 581-595; scalars 254-270, 723-729),
+  // new CF stored, CR scattered into the plane for the inverse transform
+  for (int e = threadIdx.x; e < g.nz * g.nkx1; e += nthr) {
+    const int zr = (int)fdiv((unsigned)e, g.f_nkx1), kx = e - zr * g.nkx1;
+    if (zr > g.nkz && zr < g.nz - g.nkz) A[inv_row(T, zr) * g.ls + kx] = make_double2(0.0, 0.0);
+  }
+  double2 *CF = (c < 3) ? a.cf[c] : a.cfth + (c - 3) * a.cfs;
+  const double2 *CC = (c < 3) ? a.cu[c] : a.cth + (c - 3) * a.cfs;
+  const long long base = (long long)J * g.pf;
+  // explicit half of the Crank-Nicolson y-viscous term (channel.f:606-626,
+  // 739-747) in Fourier space, as in nonlinear_pass
+  const double coefv = (c < 3) ? (a.nu * hb / 2.0) * a.nu_v_scale : ((a.nu * hb / 2.0) / a.pr[c - 3]) * a.nu_v_scale;
+  double ey0, ey1, ey2, rey0, rey1, rey2;
+  if (c == 1) {
+    ey0 = g.dyf[J];
+    ey1 = g.dyf[J - 1];
+    ey2 = g.dy[J];
+    rey0 = g.rdyf[J];
+    rey1 = g.rdyf[J - 1];
+    rey2 = g.rdy[J];
+  } else {
+    ey0 = g.dy[J + 1];
+    ey1 = g.dy[J];
+    ey2 = g.dyf[J];
+    rey0 = g.rdy[J + 1];
+    rey1 = g.rdy[J];
+    rey2 = g.rdyf[J];
+  }
+  const double dyJ = g.dy[J], rdyJ = g.rdy[J];
+  for (int m = threadIdx.x; m < g.pf; m += nthr) {
+    const int kzp = (int)fdiv((unsigned)m, g.f_nkx1), kx = m - kzp * g.nkx1;
+    const long long o = base + m;
+    const double2 c0 = CC[o], cm1 = CC[o - g.pf], cp1 = CC[o + g.pf];
+    double2 cr = c0;
+    const double2 ev = make_double2(
+        coefv * div_by_recip(div_by_recip(cp1.x - c0.x, ey0, rey0) - div_by_recip(c0.x - cm1.x, ey1, rey1), ey2, rey2),
+        coefv * div_by_recip(div_by_recip(cp1.y - c0.y, ey0, rey0) - div_by_recip(c0.y - cm1.y, ey1, rey1), ey2, rey2));
+    if (rk > 0) {
+      const double2 old = CF[o];
+      cr = make_double2(cr.x + T3 * old.x, cr.y + T3 * old.y);
+    }
+    if (c == 0 || c == 2) {
+      const double2 ikp = cmul_ik((c == 0) ? g.kx[kx] : g.kz[kzp], a.cp[o]);
+      cr = make_double2(cr.x - T4 * ikp.x, cr.y - T4 * ikp.y);
+      if (c == 0 && kx == 0 && kzp == 0) {
+        if (a.f_type == 1)
+          cr.x = cr.x - T4 * a.px0;
+        else if (a.f_type == 2)
+          cr.x = cr.x - T4 * (a.px0 + a.amp_omega0 * cos(a.omega0 * time));
+      }
+    } else if (c == 1) {
+      const double2 p0 = a.cp[o], pm = a.cp[o - g.pf];
+      const double2 cs = make_double2(div_by_recip(p0.x - pm.x, dyJ, rdyJ), div_by_recip(p0.y - pm.y, dyJ, rdyJ));
+      cr = make_double2(cr.x - T4 * cs.x, cr.y - T4 * cs.y);
+    }
+    const double2 cfn = ACC[m];
+    CF[o] = cfn;
+    cr = make_double2(cr.x + T5 * cfn.x, cr.y + T5 * cfn.y);
+    cr = make_double2(cr.x + ev.x, cr.y + ev.y);
+    A[inv_row(T, kz_row(g, kzp)) * g.ls + kx] = cr;
+  }
+  __syncthreads();
+  plane_cols_inv(A, g, T);
+  plane_rows_inv(A, g, T);
+  plane_store_phys(A, g, T, reinterpret_cast<double2 *>(static_cast<double *>(a.rphys) + (long long)c * a.fs + (long long)J * g.pp));
+}
+
+// Persistent over the (plane, component) items of a launch: block b on
+// its own scratch slot (plane, then accumulators) at gp + b gp_slot.
+__global__ void __launch_bounds__(kGpThreads) k_nonlinear_gp(StepArgs a, double2 *gp, long long gp_slot, int ncomp,
+                                                             int nitems) {
+  extern __shared__ double2 smem[];
+  const Tables T = load_tables_at(smem, a.g);
+  double2 *A = gp + (long long)blockIdx.x * gp_slot;
+  double2 *ACC = A + (long long)a.g.nz * a.g.ls;
+  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const int jr = it / ncomp;
+    nonlinear_pass_gp(a, a.j0 + jr, a.c0 + (it - jr * ncomp), A, ACC, T);
+    __syncthreads();  // the next item writes over the plane this one stored from
+  }
+}
+
 // LB: the block-size bound.  The LDS variant takes any block size up to
 // 1024; its instantiation for blocks of at most 512 threads has twice the
 // register budget (256 VGPRs) and no spills.
@@ -1524,6 +1654,15 @@ static hipError_t launch_nonlinear_range(Ctx &c, int rk, int c0, int c1, hipStre
   a.c0 = c0;
   a.j0 = jlo;
   const dim3 grid(jhi - jlo + 1, c1 - c0);
+  if (c.gp) {  // planes beyond LDS (FP64, no LES)
+    if (c.recording) return hipErrorNotSupported;
+    double2 *gp = gp_base(c, st);
+    long long slot = c.gp_slot;
+    int ncomp = c1 - c0, nitems = (int)(grid.x * grid.y);
+    void *args[] = {&a, &gp, &slot, &ncomp, &nitems};
+    return hipLaunchKernel((const void *)k_nonlinear_gp, dim3(nitems < kGpSlots ? nitems : kGpSlots),
+                           dim3(kGpThreads), args, (unsigned)gp_table_bytes(c), st);
+  }
   if (c.mixed) {
     if (c.reg_rows_nl)
       return klaunch(c, (const void *)k_nonlinear32<true, kNlRegQ>, (const void *)k_nonlinear32_b<true, kNlRegQ>, grid,
@@ -2220,7 +2359,7 @@ hipError_t launch_step_boundary(Ctx &c, bool end_prev, bool begin_next) {
 }
 
 hipError_t prepare_step_kernels(Ctx &c) {
-  if (c.lds_bytes > kLdsMax || (long long)rem_div_thomas_lds(c) > kLdsMax) return hipErrorInvalidValue;
+  if ((c.lds_bytes > kLdsMax && !c.gp) || (long long)rem_div_thomas_lds(c) > kLdsMax) return hipErrorInvalidValue;
   const void *fns[] = {(const void *)k_nonlinear<true, kNlRegQ>,    (const void *)k_nonlinear<false, kMaxQ>,
                        (const void *)k_nonlinear<true, kNlRegQ, true>, (const void *)k_nonlinear<false, kMaxQ, true>,
                        (const void *)k_nonlinear<false, kMaxQ, false, 512>,
