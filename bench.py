@@ -297,35 +297,57 @@ def les_form(steps, warmup, reps=10):
 
 
 def learned_les(dns_ms_per_step, steps=1200):
-    """The second half of the headline metric: how much sooner the LES with
-    the trained closure (64 x 65 x 64, Re_tau = 180) reaches statistics than
-    the DNS of config 1, at the statistics the closure pipeline matched.
-    Live: the LES's wall seconds per unit of simulated time with each model,
-    from the filtered DNS state of tests/golden/closure_les_case.npz (its own
-    CFL time steps in developed turbulence, steps / 2 timed), and the DNS's,
-    from this run's ms per step over the DNS time step of developed
-    turbulence recorded by the pipeline.  The profile errors are the
-    pipeline's (python -m diablo_amd.closure, profiles/closure_re180.json)."""
+    """The second half of the headline metric: how much sooner an LES with
+    the trained closure reaches statistics than the DNS, at the statistics
+    the closure pipeline matched (python -m diablo_amd.closure; profile
+    errors quoted from profiles/closure_re*.json).  Live: each LES's wall
+    seconds per unit of simulated time, from the filtered DNS state of its
+    case (tests/golden/closure_les_case*.npz), with its own CFL time steps in
+    developed turbulence (steps / 2 timed), for the trained network, the
+    Smagorinsky model and no model; and the DNS's, as its ms per step
+    (timed here) over the DNS time step of developed turbulence the pipeline
+    recorded.  Cases: config 3 (Re_tau = 395, DNS 256 x 193 x 256 timed live
+    for 10 steps, LES 64 x 97 x 64) and the Re_tau = 180 channel (DNS = this
+    bench's config-1 step, LES 64 x 65 x 64)."""
     import numpy as np
-    from diablo_amd import closure
-    rep = json.load(open(os.path.join(ROOT, "profiles", "closure_re180.json")))
-    z = np.load(os.path.join(ROOT, "tests", "golden", "closure_les_case.npz"))
-    net = np.load(os.path.join(ROOT, "diablo_amd", "data", "closure_re180.npz"))
-    live = {}
-    for model in ("learned", "smagorinsky", "none"):
-        _, info = closure.les_profiles(z, steps, model=model, params=net["params"],
-                                       scales=(float(net["ts"]), float(net["ls"]), float(net["nus"])))
-        live[model] = round(info["wall_s_per_unit_time"], 5)
-    dns_s = dns_ms_per_step * 1e-3 / rep["dns"]["dt_developed"]
-    ap = rep["a_posteriori"]
-    keep = ("U_plus", "urms_plus_vs_filtered", "vrms_plus_vs_filtered", "wrms_plus_vs_filtered", "score")
-    return {"workload": "learned-closure LES, Re_tau=180, 64x65x64 (same domain and grid stretching as config 1), "
-                        "6->64->64->64->1 GELU/softplus on BF16 MFMA trained on the sharp-filtered DNS",
-            "speedup_learned_vs_dns": round(dns_s / live["learned"], 3),
-            "wall_s_per_unit_time": {"dns": round(dns_s, 5), **{f"les_{m}": v for m, v in live.items()}},
-            "dns_dt_developed": rep["dns"]["dt_developed"],
-            "profile_errors": {m: {k: round(ap[m][k], 4) for k in keep} for m in (rep["chosen"], "smagorinsky", "none")},
-            "network": rep["chosen"], "source_of_errors": "profiles/closure_re180.json"}
+    from diablo_amd import channel, closure, turbulence
+    out = {}
+    for case, deck, case_file, net_file in (("re395", closure.les_re395, "closure_les_case_re395.npz", "closure_re395.npz"),
+                                            ("re180", closure.les_re180, "closure_les_case.npz", "closure_re180.npz")):
+        rep = json.load(open(os.path.join(ROOT, "profiles", f"closure_{case}.json")))
+        z = np.load(os.path.join(ROOT, "tests", "golden", case_file))
+        net = np.load(os.path.join(ROOT, "diablo_amd", "data", net_file))
+        live = {}
+        for model in ("learned", "smagorinsky", "none"):
+            _, info = closure.les_profiles(z, steps, model=model, params=net["params"], deck=deck,
+                                           scales=(float(net["ts"]), float(net["ls"]), float(net["nus"])))
+            live[model] = round(info["wall_s_per_unit_time"], 5)
+        if case == "re395":
+            pd, gyd, _ = closure.dns_re395()
+            dns = channel.Channel(pd, gyd)
+            turbulence.transition_ic(dns, pd, gyd, seed=1, ubulk=17.5)
+            dns.set_clock(0.0, 1e-5, 0)
+            dns.time_step(3)
+            dns.synchronize()
+            t0 = time.perf_counter()
+            dns.time_step(10)
+            dns.synchronize()
+            ms = (time.perf_counter() - t0) / 10 * 1e3
+            dns.close()
+        else:
+            ms = dns_ms_per_step
+        dns_s = ms * 1e-3 / rep["dns"]["dt_developed"]
+        ap = rep["a_posteriori"]
+        keep = ("U_plus", "urms_plus_vs_filtered", "vrms_plus_vs_filtered", "wrms_plus_vs_filtered",
+                "urms_plus", "vrms_plus", "wrms_plus", "score")
+        out[case] = {"workload": rep["case"],
+                     "speedup_learned_vs_dns": round(dns_s / live["learned"], 3),
+                     "dns_ms_per_step": round(ms, 3), "dns_dt_developed": rep["dns"]["dt_developed"],
+                     "wall_s_per_unit_time": {"dns": round(dns_s, 5), **{f"les_{m}": v for m, v in live.items()}},
+                     "profile_errors": {m: {k: round(ap[m][k], 4) for k in keep} for m in (rep["chosen"], "smagorinsky", "none")},
+                     "network": f"6->64->64->64->1 GELU/softplus on BF16 MFMA, target form {rep['chosen']}",
+                     "source_of_errors": f"profiles/closure_{case}.json"}
+    return out
 
 
 def _event_ms(fn, reps):

@@ -153,6 +153,53 @@ def les_re180(nx=64, ny=65, nz=64, dt=2e-3):
     return p, configs.make_gy(ny, 2.0, 1.75, 1), ic
 
 
+def dns_re395(nx=256, ny=193, nz=256):
+    """The DNS behind the Re_tau = 395 LES (configs.les_channel's domain,
+    2 pi x 2 x pi, tanh grid CS = 1.75, constant mean pressure gradient) on
+    the 256 x 193 x 256 grid, no model."""
+    p, _, ic = configs.les_channel(nx=nx, ny=ny, nz=nz)
+    p.update(les=0)
+    return p, configs.make_gy(ny, 2.0, 1.75, 1), ic
+
+
+def les_re395():
+    """The Re_tau = 395 LES: configs.les_channel (64 x 97 x 64), the model
+    on from the first step."""
+    return configs.les_channel()
+
+
+def _y_positions(gy):
+    """GYF (0:NY+1) and GY with GY(0) taken one spacing below GY(1)."""
+    return turbulence.ch_gyf(gy), np.concatenate([[2 * gy[0] - gy[1]], gy])
+
+
+def refine_state(src, ps, gys, dst, pdst, gyd):
+    """The velocity of context src, on a coarser grid of the same domain,
+    into context dst: the Fourier modes both grids keep carried over, linear
+    interpolation in y (U1 and U3 at GYF, U2 at GY), then APPLY_BC_VEL and
+    REM_DIV on dst."""
+    torch = _torch()
+    dev = "cuda:0"
+    nkxs, nkzs = ps["nx"] // 3, ps["nz"] // 3
+    nkxd, tnkzd = pdst["nx"] // 3, 2 * (pdst["nz"] // 3)
+    shs = (ps["ny"] + 2, 2 * nkzs + 1, nkxs + 1)
+    shd = (pdst["ny"] + 2, tnkzd + 1, nkxd + 1)
+    kz_s = np.array([k if k <= nkzs else k - 2 * nkzs - 1 for k in range(2 * nkzs + 1)])
+    kzmap = torch.from_numpy(np.where(kz_s >= 0, kz_s, tnkzd + 1 + kz_s)).to(dev)
+    (fs, ys), (fd, yd) = _y_positions(gys), _y_positions(gyd)
+    mat = lambda a, b: torch.from_numpy(_interp_matrix(a, b)).to(dev).to(torch.complex128)
+    Mf, My = mat(fs, fd), mat(ys, yd)
+    for c in range(3):
+        f = torch.empty(shs, dtype=torch.complex128, device=dev)
+        d2d(f.data_ptr(), src.field_ptr(c), f.numel() * 16)
+        t = torch.einsum("ij,jkl->ikl", My if c == 1 else Mf, f)
+        out = torch.zeros(shd, dtype=torch.complex128, device=dev)
+        out[:, kzmap, : nkxs + 1] = t
+        d2d(dst.field_ptr(c), out.data_ptr(), out.numel() * 16)
+    dst.apply_bc_vel()
+    dst.rem_div()
+
+
 def _interp_matrix(src, dst):
     """Linear interpolation in y: rows of dst positions from src positions
     (both increasing), extrapolating at the ends."""
@@ -190,13 +237,9 @@ class Sampler:
         # a pseudo-spectral LES on that grid)
         self.G = {"box": torch.from_numpy(G).to(dev).to(torch.complex128),
                   "sharp": torch.ones(G.shape, dtype=torch.complex128, device=dev)}
-        # y positions: GYF and GY of both grids (rows 0..NY+1; GY(0) taken one spacing below)
-        def pos(gy):
-            gyf = turbulence.ch_gyf(gy)
-            g = np.concatenate([[2 * gy[0] - gy[1]], gy])
-            return gyf, g
-        fd, yd = pos(gyd)
-        fl, yl = pos(gyl)
+        # y positions: GYF and GY of both grids (rows 0..NY+1)
+        fd, yd = _y_positions(gyd)
+        fl, yl = _y_positions(gyl)
         m = lambda s, d: torch.from_numpy(_interp_matrix(s, d)).to(dev).to(torch.complex128)
         self.f2f, self.y2y, self.f2y = m(fd, fl), m(yd, yl), m(fd, yl)
         self.nu = pd["nu"]
@@ -239,10 +282,11 @@ class Sampler:
         LES context's CU1..3 (U1, U3 at GYF, U2 at GY of the LES grid)."""
         self._load([self._field(self.dns, c, self.shd) for c in range(3)], kind)
 
-    def snapshot(self, kind="box"):
-        """Samples of the DNS context's current state, filtered by `kind`
-        ("box" or "sharp"), on rows J = 3..NY-1 of
-        the LES grid, flattened row by row (float32 [n], n = rows NZ NX):
+    def snapshot(self, kinds=("box", "sharp")):
+        """{kind: samples} for each filter kind ("box", "sharp"; the LES
+        context is left holding the last kind's resolved velocity): samples of
+        the DNS context's current state on rows J = 3..NY-1 of the LES grid,
+        flattened row by row (float32 [n], n = rows NZ NX):
         X [6][n] the closure's inputs; the eddy viscosities (units of NU)
         that match the subgrid dissipation D = -tau^d_ij S_ij with
         2 NU_T S_kl S_kl, pointwise (t_point, clipped to [0, 50]; t_signed,
@@ -256,10 +300,16 @@ class Sampler:
         u2f = u[1].clone()
         u2f[1:-1] = 0.5 * (u[1][1:-1] + u[1][2:])  # V at GYF
         prods = [u[0] * u[0], u2f * u2f, u[2] * u[2], u[0] * u2f, u[0] * u[2], u2f * u[2]]
-        pbar = []
-        for q in prods:
-            fq = self._to_four(self.dns, q, pd, self.shd)
-            pbar.append(self._to_phys(self.les, self._filter(fq, self.f2y, kind), pl))
+        fqs = [self._to_four(self.dns, q, pd, self.shd) for q in prods]
+        out = {}
+        for kind in kinds:
+            out[kind] = self._samples(cu, fqs, kind)
+        return out
+
+    def _samples(self, cu, fqs, kind):
+        torch = self.t
+        pd, pl = self.pd, self.pl
+        pbar = [self._to_phys(self.les, self._filter(fq, self.f2y, kind), pl) for fq in fqs]
         # the resolved velocity: into the LES context (U1, U3 at GYF, U2 at GY) ...
         self._load(cu, kind)
         # ... and at GY for the stress
@@ -303,7 +353,7 @@ def smagorinsky_scale(pl, cs=0.13):
 
 
 # ------------------------------------------------------- a-posteriori runs
-def les_profiles(state, steps, model="smagorinsky", params=None, scales=None, every=20, utau=1.0):
+def les_profiles(state, steps, model="smagorinsky", params=None, scales=None, every=20, utau=1.0, deck=None):
     """LES of the Re_tau = 180 channel on the coarse grid (les_re180) from
     the resolved velocity in `state` (CU1..3 in the host layout): `model`
     "none" (no subgrid model), "smagorinsky" (les.f's constant C_s) or
@@ -312,8 +362,9 @@ def les_profiles(state, steps, model="smagorinsky", params=None, scales=None, ev
     seconds per unit of simulated time), and the other half averaged
     (plane statistics every `every` steps).  Returns the profiles in wall
     units with the forcing's u_tau, and {u_tau from the wall shear, wall
-    seconds per unit time, simulated time}."""
-    pl, gyl, _ = les_re180()
+    seconds per unit time, simulated time}.  deck: the LES deck (les_re180
+    unless given)."""
+    pl, gyl, _ = (deck or les_re180)()
     ch = channel.Channel(dict(pl, les=0 if model == "none" else 1), gyl)
     if model == "learned":
         ts, ls, nus = scales
@@ -371,27 +422,67 @@ VARIANTS = (("box_pointwise", "box", "t_point", None), ("box_signed", "box", "t_
 FILTERS = ("box", "sharp")
 PROFILE_KEYS = ("U_plus", "urms_plus", "vrms_plus", "wrms_plus")
 
-def run_pipeline(out_dir, spin=30000, stat=6000, every=20, nsnap=12, iters=3000, batch=16384, les_steps=12000,
-                 log=print):
+# Cases of the pipeline: the Re_tau = 180 channel (DNS 128 x 129 x 128,
+# LES 64 x 65 x 64), and config 3's Re_tau = 395 channel (DNS 256 x 193 x
+# 256, spun up on 128 x 129 x 128 and refined; LES 64 x 97 x 64; 200
+# snapshots).
+CASES = {
+    "re180": dict(dns=configs.channel_re180, coarse=None, les=les_re180, ubulk=15.7, spin=30000, adjust=0,
+                  stat=6000, nsnap=12, iters=3000),
+    "re395": dict(dns=dns_re395, coarse=lambda: dns_re395(128, 129, 128), les=les_re395, ubulk=17.5, spin=20000,
+                  adjust=4000, stat=5000, nsnap=200, iters=6000),
+}
+
+
+def run_pipeline(out_dir, case="re180", spin=None, stat=None, every=20, nsnap=None, iters=None, batch=16384,
+                 les_steps=12000, adjust=None, log=print):
     torch = _torch()
+    C = CASES[case]
+    spin = C["spin"] if spin is None else spin
+    stat = C["stat"] if stat is None else stat
+    nsnap = C["nsnap"] if nsnap is None else nsnap
+    iters = C["iters"] if iters is None else iters
+    adjust = C["adjust"] if adjust is None else adjust
     t_start = time.time()
-    rep = {}
-    pd, gyd, icd = configs.channel_re180()
-    dns = channel.Channel(pd, gyd)
-    turbulence.transition_ic(dns, pd, gyd, seed=1)
-    dns.set_clock(0.0, pd["delta_t"], 0)
+    rep = {"case": case}
+    pd, gyd, icd = C["dns"]()
+    fmt = lambda tag: (lambda ts, t, tw, ub: log(f"[{tag}] step {ts} t {t:.2f} u_tau {math.sqrt(max(tw, 0)):.3f} "
+                                                 f"U_b {ub:.2f}"))
+    hist_row = lambda h: (round(h[0], 3), round(math.sqrt(max(h[1], 0)), 4), round(h[2], 3))
     t0 = time.time()
-    dns.time_step(10)
-    dns.synchronize()
-    t_sim0, t_rate0 = dns.clock()[0], time.time()
-    st, hist = turbulence.run_statistics(dns, pd, spin, every=500,
-                                         on_sample=lambda ts, t, tw, ub: log(f"[dns] step {ts} t {t:.2f} "
-                                                                             f"u_tau {math.sqrt(max(tw, 0)):.3f} "
-                                                                             f"U_b {ub:.2f}"))
-    dns_rate = (time.time() - t_rate0) / (dns.clock()[0] - t_sim0)  # wall s per unit time, stepping
-    rep["dns_spinup"] = {"steps": spin, "wall_s": time.time() - t0, "u_tau_last": math.sqrt(max(hist[-1][1], 0)),
-                         "history": [(round(h[0], 3), round(math.sqrt(max(h[1], 0)), 4), round(h[2], 3)) for h in hist]}
-    pl, gyl, _ = les_re180()
+    if C["coarse"] is None:
+        dns = channel.Channel(pd, gyd)
+        turbulence.transition_ic(dns, pd, gyd, seed=1, ubulk=C["ubulk"])
+        dns.set_clock(0.0, pd["delta_t"], 0)
+        dns.time_step(10)
+        dns.synchronize()
+        t_sim0, t_rate0 = dns.clock()[0], time.time()
+        st, hist = turbulence.run_statistics(dns, pd, spin, every=500, on_sample=fmt("dns"))
+        dns_rate = (time.time() - t_rate0) / (dns.clock()[0] - t_sim0)  # wall s per unit time, stepping
+        rep["dns_spinup"] = {"steps": spin, "wall_s": time.time() - t0, "u_tau_last": math.sqrt(max(hist[-1][1], 0)),
+                             "history": [hist_row(h) for h in hist]}
+    else:
+        # spin-up on the coarse grid, then the state refined onto the DNS grid
+        # and settled there (the DNS's own rate is measured while it settles)
+        pc, gyc, _ = C["coarse"]()
+        src = channel.Channel(pc, gyc)
+        turbulence.transition_ic(src, pc, gyc, seed=1, ubulk=C["ubulk"])
+        src.set_clock(0.0, pc["delta_t"], 0)
+        _, hist = turbulence.run_statistics(src, pc, spin, every=500, on_sample=fmt("coarse"))
+        dns = channel.Channel(pd, gyd)
+        refine_state(src, pc, gyc, dns, pd, gyd)
+        t_src, dt_src, _, _ = src.clock()
+        src.close()
+        dns.set_clock(t_src, dt_src / 3.0, 0)  # COURANT sets the fine grid's step from the first step on
+        dns.time_step(10)
+        dns.synchronize()
+        t_sim0, t_rate0 = dns.clock()[0], time.time()
+        _, hist2 = turbulence.run_statistics(dns, pd, adjust, every=250, on_sample=fmt("dns"))
+        dns_rate = (time.time() - t_rate0) / (dns.clock()[0] - t_sim0)
+        rep["dns_spinup"] = {"coarse_grid": [pc["nx"], pc["ny"], pc["nz"]], "coarse_steps": spin, "fine_steps": adjust,
+                             "wall_s": time.time() - t0, "u_tau_last": math.sqrt(max(hist2[-1][1], 0)),
+                             "history": [hist_row(h) for h in hist], "history_fine": [hist_row(h) for h in hist2]}
+    pl, gyl, _ = C["les"]()
     les = channel.Channel(pl, gyl)
     les.set_les(True)
     les.set_les_closure(init_params(0), ts=pd["nu"], ls=pd["nu"], nus=pd["nu"])  # its inputs need it on
@@ -416,8 +507,9 @@ def run_pipeline(out_dir, spin=30000, stat=6000, every=20, nsnap=12, iters=3000,
         done += k
         acc = s if acc is None else {q: acc[q] + s[q] for q in s if q != "nsamples"}
         nacc += 1
-        for kind in FILTERS:  # the sharp cutoff last: its resolved field stays in the LES context
-            snaps[kind].append(sampler.snapshot(kind))
+        sn_all = sampler.snapshot(FILTERS)  # the sharp cutoff last: its resolved field stays in the LES context
+        for kind in FILTERS:
+            snaps[kind].append(sn_all[kind])
             sn = snaps[kind][-1]
             log(f"[samples] {kind} snapshot {len(snaps[kind])}: {sn['t_point'].numel()} samples, pointwise target "
                 f"mean {float(sn['t_point'].mean()):.3f} NU, plane target mean {float(sn['t_plane'].mean()):.3f} NU")
@@ -449,9 +541,12 @@ def run_pipeline(out_dir, spin=30000, stat=6000, every=20, nsnap=12, iters=3000,
         net = Network(seed=0)
         t0 = time.time()
         losses = []
+        if w is not None:  # inverse-CDF draws (more samples than torch.multinomial takes categories)
+            cdf = torch.cumsum(w.double(), 0)
         for it in range(iters):
             if it % chunk == 0 and w is not None:
-                draws = torch.multinomial(w, chunk * batch, replacement=True, generator=gen)
+                u = torch.rand(chunk * batch, dtype=torch.float64, device="cuda:0", generator=gen) * cdf[-1]
+                draws = torch.clamp(torch.searchsorted(cdf, u), max=n - 1)
             lr = 1e-3 * (0.1 ** (it / iters))
             if w is None:
                 idx = torch.randint(0, n, (batch,), device="cuda:0", generator=gen)
@@ -499,7 +594,7 @@ def run_pipeline(out_dir, spin=30000, stat=6000, every=20, nsnap=12, iters=3000,
     def les_run(model):
         params = nets[model].params() if model in nets else None
         w, e = les_profiles(ub_state, les_steps, model="learned" if params is not None else model, params=params,
-                            scales=(pd["nu"], pd["nu"], pd["nu"]), every=every, utau=ut)
+                            scales=(pd["nu"], pd["nu"], pd["nu"]), every=every, utau=ut, deck=C["les"])
         e.update(profile_errors(w, wu_d, wu_f))
         log(f"[a posteriori] {model}: {json.dumps(e)}")
         return e, w
@@ -537,14 +632,17 @@ def run_pipeline(out_dir, spin=30000, stat=6000, every=20, nsnap=12, iters=3000,
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default="gpurun_out")
-    ap.add_argument("--spin", type=int, default=30000)
-    ap.add_argument("--stat", type=int, default=6000)
-    ap.add_argument("--iters", type=int, default=3000)
+    ap.add_argument("--case", default="re180", choices=sorted(CASES))
+    ap.add_argument("--spin", type=int, default=None)
+    ap.add_argument("--adjust", type=int, default=None)
+    ap.add_argument("--stat", type=int, default=None)
+    ap.add_argument("--nsnap", type=int, default=None)
+    ap.add_argument("--iters", type=int, default=None)
     ap.add_argument("--les-steps", type=int, default=12000)
     a = ap.parse_args()
     os.makedirs(a.out, exist_ok=True)
-    rep = run_pipeline(a.out, spin=a.spin, stat=a.stat, iters=a.iters, les_steps=a.les_steps,
-                       log=lambda s: print(s, flush=True))
+    rep = run_pipeline(a.out, case=a.case, spin=a.spin, stat=a.stat, nsnap=a.nsnap, iters=a.iters,
+                       les_steps=a.les_steps, adjust=a.adjust, log=lambda s: print(s, flush=True))
     with open(os.path.join(a.out, "closure_report.json"), "w") as f:
         json.dump(rep, f)
     print(json.dumps({k: v for k, v in rep.items() if k != "profiles"}), flush=True)
