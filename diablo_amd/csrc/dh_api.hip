@@ -1,6 +1,7 @@
 // C ABI of libdiablo_hip.so (declarations and reference citations in
 // include/diablo_hip.h): context set-up, host<->device layout conversion,
 // substep/step dispatch and the captured step graph.
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -236,7 +237,8 @@ int dh_create(const dh_params *p, dh_ctx **out) {
                   (2 * g.lw + 2 * g.nz) * (int)sizeof(int);
   // planes beyond LDS go through scratch planes in device memory (FP64, no LES)
   c.gp = c.lds_bytes > kLdsMax;
-  if (gp_table_bytes(c) > 64 * 1024) return reject("x-z plane too large (NX/2 + NZ beyond the transform tables)");
+  if (c.gp && gp_stage_elems(g) < std::max(g.ls, 2 * g.nz))
+    return reject("x-z plane too large (a row or a column of it does not fit the staging buffer)");
   if (g.ny + 2 > 8 * kSegRows) return reject("NY+2 is beyond the column solver's register segments (8 x 36 rows)");
   int thr = (g.nz * g.lw + 7) / 8;
   thr = ((thr + 63) / 64) * 64;
@@ -320,7 +322,7 @@ int dh_create(const dh_params *p, dh_ctx **out) {
   ACHK(dalloc(&c.uphys, ph_elems * c.nfields));
   ACHK(dalloc(&c.rphys, ph_elems * c.nfields));
   if (c.gp) {
-    c.gp_slot = (long long)g.nz * g.ls + g.pf;
+    c.gp_slot = (long long)g.nz * g.nkx1 + g.pf;  // the plane between passes, then the accumulators
     ACHK(dalloc(&c.gp_scratch, 2 * (size_t)kGpSlots * (size_t)c.gp_slot));
   }
   ACHK(dalloc(&c.rd_ta, cf_elems));

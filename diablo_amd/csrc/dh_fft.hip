@@ -125,29 +125,74 @@ __global__ void __launch_bounds__(1024) k_plane_fwd_b(const PlaneArgs *__restric
   plane_fwd_body<REG, C, false>(pa.g, pa.b, A, load_tables(A, pa.g), blockIdx.y, pa.b.j0 + blockIdx.x);
 }
 
-// Planes too large for LDS (FP64): block b works on its own scratch plane
-// in device memory (`gp`, planes of gp_slot complex values) and takes the
-// (plane, field) items b, b + gridDim.x, ...: nplanes planes of each field.
-// The tables stay in LDS.
-__global__ void __launch_bounds__(kGpThreads) k_plane_inv_gp(PlaneArgs pa, double2 *gp, long long gp_slot,
-                                                             int nplanes, int nitems) {
+// Planes too large for LDS (FP64): block b keeps the plane between its
+// row and column pass in its own scratch slot Q = gp + b gp_slot (device
+// memory) and takes the (plane, field) items b, b + gridDim.x, ...: nplanes
+// planes of each field.  The tables and the staging buffer are in LDS.
+// Fourier -> physical as plane_inv_body; physical -> Fourier as
+// plane_fwd_body.
+__device__ __forceinline__ void plane_inv_gp_body(const Geo &g, const PlaneBatch &b, double2 *Q, const Tables &T,
+                                                  double2 *buf, int stage, int f, long long j) {
+  const double2 *src = b.four[f] ? b.four[f] + j * g.pf : nullptr;
+  const bool drop_mean = b.drop_mean != 0;
+  gp_cols_inv(Q, g, T, buf, stage, [&](int kx, int zr) {
+    int kzp = -1;  // unpack of mpi.f:1562-1574
+    if (zr <= g.nkz)
+      kzp = zr;
+    else if (zr >= g.nz - g.nkz)
+      kzp = zr - (g.nz - 1 - g.nkz) + g.nkz;
+    return (kzp >= 0 && src && !(drop_mean && kzp == 0 && kx == 0)) ? src[kzp * g.nkx1 + kx] : make_double2(0.0, 0.0);
+  });
+  __syncthreads();  // Q complete before the row pass reads it
+  double2 *dst = reinterpret_cast<double2 *>(static_cast<double *>(b.phys[f]) + j * g.pp);
+  gp_rows_inv(Q, g, T, buf, stage, [&](int z, int m, double2 v) { dst[z * g.lw + m] = v; });
+}
+
+__device__ __forceinline__ void plane_fwd_gp_body(const Geo &g, const PlaneBatch &b, double2 *Q, const Tables &T,
+                                                  double2 *buf, int stage, int f, long long j) {
+  const double2 *base = b.base[f] ? b.base[f] + j * g.pf : nullptr;
+  double2 *dst = const_cast<double2 *>(b.four[f]) + j * g.pf;
+  if (base && (j < b.alo[f] || j > b.ahi[f])) {  // no increment on this plane
+    if (base != dst)
+      for (int m = threadIdx.x; m < g.pf; m += blockDim.x) dst[m] = base[m];
+    return;
+  }
+  const void *ph = (b.alt[f] && j >= b.alo[f] && j <= b.ahi[f]) ? b.alt[f] : b.phys[f];
+  const double2 *src = reinterpret_cast<const double2 *>(static_cast<const double *>(ph) + j * g.pp);
+  gp_rows_fwd(Q, g, T, buf, stage, [&](int z, int m) { return src[z * g.lw + m]; });
+  __syncthreads();
+  const double nzd = g.dnz, rnz = g.rnz;  // V / NZ as the correctly rounded quotient
+  gp_cols_fwd(Q, g, T, buf, stage, [&](int kx, int kzp, double2 v) {
+    const int m = kzp * g.nkx1 + kx;
+    double2 r = make_double2(div_by_recip(v.x, nzd, rnz), div_by_recip(v.y, nzd, rnz));
+    if (base) r = make_double2(base[m].x + r.x, base[m].y + r.y);
+    dst[m] = r;
+  });
+}
+
+__global__ void __launch_bounds__(kGpThreads, 4) k_plane_inv_gp(PlaneArgs pa, double2 *gp, long long gp_slot,
+                                                                int nplanes, int nitems) {
   extern __shared__ double2 smem[];
-  const auto T = load_tables_at(smem, pa.g);
-  double2 *A = gp + (long long)blockIdx.x * gp_slot;
+  const Tables T = load_tables_at(smem, pa.g);
+  double2 *buf = smem + gp_table_elems(pa.g);
+  const int stage = gp_stage_elems(pa.g);
+  double2 *Q = gp + (long long)blockIdx.x * gp_slot;
   for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
     const int f = it / nplanes;
-    plane_inv_body<false, double2>(pa.g, pa.b, A, T, f, pa.b.j0 + (it - f * nplanes));
-    __syncthreads();  // the next item's unpack writes over the plane this one stored from
+    plane_inv_gp_body(pa.g, pa.b, Q, T, buf, stage, f, pa.b.j0 + (it - f * nplanes));
+    __syncthreads();  // the next item writes over Q
   }
 }
-__global__ void __launch_bounds__(kGpThreads) k_plane_fwd_gp(PlaneArgs pa, double2 *gp, long long gp_slot,
-                                                             int nplanes, int nitems) {
+__global__ void __launch_bounds__(kGpThreads, 4) k_plane_fwd_gp(PlaneArgs pa, double2 *gp, long long gp_slot,
+                                                                int nplanes, int nitems) {
   extern __shared__ double2 smem[];
-  const auto T = load_tables_at(smem, pa.g);
-  double2 *A = gp + (long long)blockIdx.x * gp_slot;
+  const Tables T = load_tables_at(smem, pa.g);
+  double2 *buf = smem + gp_table_elems(pa.g);
+  const int stage = gp_stage_elems(pa.g);
+  double2 *Q = gp + (long long)blockIdx.x * gp_slot;
   for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
     const int f = it / nplanes;
-    plane_fwd_body<false, double2, false>(pa.g, pa.b, A, T, f, pa.b.j0 + (it - f * nplanes));
+    plane_fwd_gp_body(pa.g, pa.b, Q, T, buf, stage, f, pa.b.j0 + (it - f * nplanes));
     __syncthreads();
   }
 }
@@ -170,7 +215,7 @@ static hipError_t plane_gp(Ctx &c, dim3 grid, hipStream_t st, const PlaneBatch &
   void *args[] = {&pa, &gp, &slot, &nplanes, &nitems};
   const unsigned blocks = (unsigned)(nitems < kGpSlots ? nitems : kGpSlots);
   return hipLaunchKernel(INV ? (const void *)k_plane_inv_gp : (const void *)k_plane_fwd_gp, dim3(blocks),
-                         dim3(kGpThreads), args, (unsigned)gp_table_bytes(c), st);
+                         dim3(kGpThreads), args, (unsigned)kGpLds, st);
 }
 
 template <class C>
@@ -329,6 +374,8 @@ static hipError_t set_lds_plane() {
 hipError_t prepare_fft_kernels(Ctx &c) {
   if (c.lds_bytes > kLdsMax && !c.gp) return hipErrorInvalidValue;
   hipError_t e;
+  if ((e = set_lds((const void *)k_plane_inv_gp, kGpLds)) != hipSuccess) return e;
+  if ((e = set_lds((const void *)k_plane_fwd_gp, kGpLds)) != hipSuccess) return e;
   if ((e = set_lds_plane<true, double2>()) != hipSuccess) return e;
   if ((e = set_lds_plane<false, double2>()) != hipSuccess) return e;
   if ((e = set_lds_plane<true, float2>()) != hipSuccess) return e;

@@ -352,17 +352,19 @@ struct LoadTile {
 // Forward x transform of every row (FFT_XZ_TO_FOURIER's r2c, mpi.f:1490-1506).
 // On entry row z holds (x_{2m}, x_{2m+1}) in slot posr[m]; on exit slot k
 // holds X_k / NX for k <= NKX (natural order).
+// nz rows (fnz: the reciprocal for fdiv) instead of the plane's NZ: a chunk
+// of a plane's rows.
 template <class C>
-__device__ void plane_rows_fwd(C *A, const Geo &g, const TablesT<C> &T) {
+__device__ void plane_rows_fwd(C *A, const Geo &g, const TablesT<C> &T, int nz, unsigned long long fnz) {
   using S = Sc<C>;
   const int h = g.lw;  // NX/2
-  lds_fft<false, false, true>(A, g.prow, g.nz, g.f_nz, g.ls, T.twr);
+  lds_fft<false, false, true>(A, g.prow, nz, fnz, g.ls, T.twr);
   // even/odd split: X_k = (Z_k + conj(Z_{h-k}))/2 - i w^-k (Z_k - conj(Z_{h-k}))/2
   const int npair = h / 2 + 1;  // k = 0..h/2 paired with h-k
-  const int total = g.nz * npair;
+  const int total = nz * npair;
   const S nxd = S(g.dnx), rnx = S(g.rnx);  // X / NX as the correctly rounded quotient
   for (int e = threadIdx.x; e < total; e += blockDim.x) {
-    const int k = (int)fdiv((unsigned)e, g.f_nz), z = e - k * g.nz;
+    const int k = (int)fdiv((unsigned)e, fnz), z = e - k * nz;
     C *row = A + z * g.ls;
     const int k2 = (k == 0) ? 0 : h - k;
     const C zk = row[k];
@@ -390,12 +392,12 @@ __device__ void plane_rows_fwd(C *A, const Geo &g, const TablesT<C> &T) {
 // Nyquist term are zero; the imaginary part of the k=0 term is ignored); on
 // exit slot posrd[m] holds (x_{2m}, x_{2m+1}).
 template <class C>
-__device__ void plane_rows_inv(C *A, const Geo &g, const TablesT<C> &T) {
+__device__ void plane_rows_inv(C *A, const Geo &g, const TablesT<C> &T, int nz, unsigned long long fnz) {
   const int h = g.lw;
   const int npair = h / 2 + 1;
-  const int total = g.nz * npair;
+  const int total = nz * npair;
   for (int e = threadIdx.x; e < total; e += blockDim.x) {
-    const int k = (int)fdiv((unsigned)e, g.f_nz), z = e - k * g.nz;
+    const int k = (int)fdiv((unsigned)e, fnz), z = e - k * nz;
     C *row = A + z * g.ls;
     const int k2 = h - k;  // partner (k=0 pairs with the Nyquist term)
     C xk = (k <= g.nkx) ? row[k] : mkc<C>(0.0, 0.0);
@@ -416,7 +418,15 @@ __device__ void plane_rows_inv(C *A, const Geo &g, const TablesT<C> &T) {
     }
   }
   plane_barrier(A);
-  lds_fft<true, false, false>(A, g.prow, g.nz, g.f_nz, g.ls, T.twr);
+  lds_fft<true, false, false>(A, g.prow, nz, fnz, g.ls, T.twr);
+}
+template <class C>
+__device__ __forceinline__ void plane_rows_fwd(C *A, const Geo &g, const TablesT<C> &T) {
+  plane_rows_fwd(A, g, T, g.nz, g.f_nz);
+}
+template <class C>
+__device__ __forceinline__ void plane_rows_inv(C *A, const Geo &g, const TablesT<C> &T) {
+  plane_rows_inv(A, g, T, g.nz, g.f_nz);
 }
 
 // z transforms over the de-aliased columns kx = 0..NKX: forward by DIF
@@ -717,6 +727,148 @@ __device__ void plane_store_phys(const C *A, const Geo &g, const TablesT<C> &T, 
   for (int t = threadIdx.x >> 6; t < lt.ntiles; t += nw) {
     int z, m;
     if (lt.at(g, t, z, m)) dst[z * g.lw + m] = A[inv_slot(g, T, z, m)];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Planes too large for LDS.  A transform runs as a row pass and a column
+// pass, each staged through an LDS buffer `buf` of `stage` complex values:
+// a chunk of whole rows (row stride LS, as in an LDS plane) or of whole
+// columns (stride K + 1) is filled, transformed by the routines above over
+// the chunk's rows or columns, and drained.  Between the
+// passes the plane lives in device memory as Q: NZ rows of the NKX + 1 kept
+// x modes (row stride NKX + 1), written once and read once per transform.
+// Rows and columns are transformed independently of each other, so the
+// results are those of the whole-plane routines, bit for bit.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ unsigned long long recip_dev(unsigned d) { return ((1ull << 32) + d - 1) / d; }
+
+// For e = threadIdx.x, + blockDim.x, ... < n: store(e, load(e)), the loads of
+// B iterations issued before their stores so that their latencies overlap.
+template <int B, class C, class Load, class Store>
+__device__ __forceinline__ void gp_move(int n, Load load, Store store) {
+  for (int base = threadIdx.x; base < n; base += B * blockDim.x) {
+    C v[B];
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+      const int e = base + b * blockDim.x;
+      if (e < n) v[b] = load(e);
+    }
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+      const int e = base + b * blockDim.x;
+      if (e < n) store(e, v[b]);
+    }
+  }
+}
+
+// Forward x transform of every row: phys(z, m) gives physical pair m
+// (x = 2m, 2m+1) of row z; X_k / NX for k <= NKX goes to Q[z (NKX+1) + k].
+// B: the loads issued together (phys may read several values per pair).
+template <int B = 4, class C, class Phys>
+__device__ void gp_rows_fwd(C *Q, const Geo &g, const TablesT<C> &T, C *buf, int stage, Phys phys) {
+  const int R = stage / g.ls;  // rows per chunk
+  for (int r0 = 0; r0 < g.nz; r0 += R) {
+    const int nr = min(R, g.nz - r0);
+    gp_move<B, C>(
+        nr * g.lw,
+        [&](int e) {
+          const int z = (int)fdiv((unsigned)e, g.f_lw);
+          return phys(r0 + z, e - z * g.lw);
+        },
+        [&](int e, C v) {
+          const int z = (int)fdiv((unsigned)e, g.f_lw);
+          buf[z * g.ls + T.posr[e - z * g.lw]] = v;
+        });
+    lds_barrier();
+    plane_rows_fwd(buf, g, T, nr, recip_dev((unsigned)nr));  // ends on a barrier
+    C *q = Q + (long long)r0 * g.nkx1;
+    for (int e = threadIdx.x; e < nr * g.nkx1; e += blockDim.x) {
+      const int z = (int)fdiv((unsigned)e, g.f_nkx1);
+      q[e] = buf[z * g.ls + (e - z * g.nkx1)];
+    }
+    lds_barrier();  // buf is refilled
+  }
+}
+
+// Forward z transform of Q's columns: out(kx, kzp, V) takes each kept mode
+// (packed kz index kzp) before the division by NZ.
+template <class C, class Out>
+__device__ void gp_cols_fwd(const C *Q, const Geo &g, const TablesT<C> &T, C *buf, int stage, Out out) {
+  const int K = stage / g.nz - 1, lc = K + 1;  // columns per chunk, their stride in buf
+  for (int k0 = 0; k0 < g.nkx1; k0 += K) {
+    const int nk = min(K, g.nkx1 - k0);
+    const unsigned long long fk = recip_dev((unsigned)nk);
+    gp_move<4, C>(
+        g.nz * nk,
+        [&](int e) {
+          const int z = (int)fdiv((unsigned)e, fk);
+          return Q[z * g.nkx1 + k0 + (e - z * nk)];
+        },
+        [&](int e, C v) {
+          const int z = (int)fdiv((unsigned)e, fk);
+          buf[z * lc + (e - z * nk)] = v;
+        });
+    lds_barrier();
+    lds_fft<false, true, false>(buf, g.pcol, nk, fk, lc, T.twc);  // plane_cols_fwd over the chunk
+    for (int e = threadIdx.x; e < (g.tnkz + 1) * nk; e += blockDim.x) {
+      const int kzp = (int)fdiv((unsigned)e, fk), j = e - kzp * nk;
+      out(k0 + j, kzp, buf[T.posz[kz_row(g, kzp)] * lc + j]);
+    }
+    lds_barrier();
+  }
+}
+
+// Inverse z transform of the columns kx <= NKX: spec(kx, zr) gives row zr
+// (natural order, zeros in the de-aliased gap) of the unpacked z spectrum;
+// the natural rows go to Q.
+template <class C, class Spec>
+__device__ void gp_cols_inv(C *Q, const Geo &g, const TablesT<C> &T, C *buf, int stage, Spec spec) {
+  const int K = stage / g.nz - 1, lc = K + 1;
+  for (int k0 = 0; k0 < g.nkx1; k0 += K) {
+    const int nk = min(K, g.nkx1 - k0);
+    const unsigned long long fk = recip_dev((unsigned)nk);
+    gp_move<4, C>(
+        g.nz * nk,
+        [&](int e) {
+          const int zr = (int)fdiv((unsigned)e, fk);
+          return spec(k0 + (e - zr * nk), zr);
+        },
+        [&](int e, C v) {
+          const int zr = (int)fdiv((unsigned)e, fk);
+          buf[inv_row(T, zr) * lc + (e - zr * nk)] = v;
+        });
+    lds_barrier();
+    lds_fft<true, true, true>(buf, g.pcol, nk, fk, lc, T.twc);  // plane_cols_inv over the chunk
+    for (int e = threadIdx.x; e < g.nz * nk; e += blockDim.x) {
+      const int z = (int)fdiv((unsigned)e, fk), j = e - z * nk;
+      Q[z * g.nkx1 + k0 + j] = buf[z * lc + j];
+    }
+    lds_barrier();
+  }
+}
+
+// Inverse x transform of Q's rows: store(z, m, v) takes physical pair m of
+// row z.
+template <class C, class Store>
+__device__ void gp_rows_inv(const C *Q, const Geo &g, const TablesT<C> &T, C *buf, int stage, Store store) {
+  const int R = stage / g.ls;
+  for (int r0 = 0; r0 < g.nz; r0 += R) {
+    const int nr = min(R, g.nz - r0);
+    const C *q = Q + (long long)r0 * g.nkx1;
+    gp_move<4, C>(
+        nr * g.nkx1, [&](int e) { return q[e]; },
+        [&](int e, C v) {
+          const int z = (int)fdiv((unsigned)e, g.f_nkx1);
+          buf[z * g.ls + (e - z * g.nkx1)] = v;
+        });
+    lds_barrier();
+    plane_rows_inv(buf, g, T, nr, recip_dev((unsigned)nr));  // reads modes k <= NKX only; ends on a barrier
+    for (int e = threadIdx.x; e < nr * g.lw; e += blockDim.x) {
+      const int z = (int)fdiv((unsigned)e, g.f_lw), m = e - z * g.lw;
+      store(r0 + z, m, buf[z * g.ls + T.posrd[m]]);
+    }
+    lds_barrier();
   }
 }
 

@@ -198,8 +198,8 @@ struct Ctx {
   double *les_feat = nullptr;
   // planes too large for LDS: kGpSlots scratch slots for the main stream's
   // kernels and kGpSlots for the branch's, gp_slot complex values each (a
-  // plane of NZ rows of stride LS, then the nonlinear pass's accumulators
-  // of the PF modes)
+  // plane between the row and the column pass, NZ rows of the NKX+1 kept x
+  // modes, then the nonlinear pass's accumulators of the PF modes)
   bool gp = false;
   double2 *gp_scratch = nullptr;
   long long gp_slot = 0;
@@ -207,10 +207,15 @@ struct Ctx {
 inline double2 *gp_base(const Ctx &c, hipStream_t st) {
   return c.gp_scratch + (st == c.branch ? (long long)kGpSlots * c.gp_slot : 0LL);
 }
-// LDS of the transform tables alone (load_tables_at), for the large-plane kernels
-inline int gp_table_bytes(const Ctx &c) {
-  return (2 * c.g.lw + c.g.nz + 1) * (int)sizeof(double2) + (2 * c.g.lw + 2 * c.g.nz) * (int)sizeof(int);
+// The large-plane kernels' LDS (kGpLds bytes: two blocks per CU, which their
+// launch bounds of 4 waves per SIMD also leave room for in registers): the
+// transform tables (load_tables_at), in double2 units rounded up, then the
+// staging buffer of the row and column passes.
+constexpr int kGpLds = 80 * 1024;
+__host__ __device__ inline int gp_table_elems(const Geo &g) {
+  return (2 * g.lw + g.nz + 1) + ((2 * g.lw + 2 * g.nz) * (int)sizeof(int) + 15) / 16;
 }
+__host__ __device__ inline int gp_stage_elems(const Geo &g) { return kGpLds / 16 - gp_table_elems(g); }
 // feature slots of the learned closure: 0..5 the network's inputs, 6..11
 // S11, S22, S33, S12, S13, S23 at GY, 12 |S| at GY
 constexpr int kLesFeat = 13;

@@ -589,11 +589,13 @@ __device__ __forceinline__ void nonlinear_pass(const StepArgs &a, C *A) {
   }
 }
 // The nonlinear pass for planes too large for LDS (FP64, no LES): the same
-// terms in the same order as nonlinear_pass's LDS variant, with the plane in
-// a scratch plane A in device memory and the CF accumulators of the PF modes
-// in ACC (device memory after the plane), modes taken m = threadIdx.x,
+// terms in the same order as nonlinear_pass's LDS variant, with the
+// transforms staged through the LDS buffer `buf` (gp_rows_fwd and its
+// kin), the plane between their passes in Q and the CF accumulators of the
+// PF modes in ACC (device memory), modes taken m = threadIdx.x,
 // + blockDim.x, ... in every loop.
-__device__ void nonlinear_pass_gp(const StepArgs &a, int J, int c, double2 *A, double2 *ACC, const Tables &T) {
+__device__ void nonlinear_pass_gp(const StepArgs &a, int J, int c, double2 *Q, double2 *ACC, const Tables &T,
+                                  double2 *buf, int stage) {
   const Geo &g = a.g;
   int jlo = 2, jhi = g.ny;
   if (c >= 3) {
@@ -615,33 +617,23 @@ __device__ void nonlinear_pass_gp(const StepArgs &a, int J, int c, double2 *A, d
   // the three products, forward transformed and subtracted with their
   // derivative factors: c==0 takes (ikz, ikx, 1), all others (ikx, ikz, 1)
   const int pz = (c == 0) ? 0 : 1;
-  const LoadTile lt(g);
-  const int nw = nthr >> 6;
   const double nzd = g.dnz, rnz = g.rnz;
+  __syncthreads();  // ACC initialised before the first product's modes are taken from it
+#pragma unroll 1
   for (int p = 0; p < 3; ++p) {
     const Prod<double2> pr = make_prod<double2>(a, c, p, J);
-    for (int t0 = threadIdx.x >> 6; t0 < lt.ntiles; t0 += nw) {
-      int z, xp;
-      if (lt.at(g, t0, z, xp)) A[phys_slot(T, g, z, xp)] = eval_prod(pr, (long long)z * g.nx + 2 * xp);
-    }
-    __syncthreads();
-    plane_rows_fwd(A, g, T);
-    plane_cols_fwd(A, g, T);
+    gp_rows_fwd<1>(Q, g, T, buf, stage, [&](int z, int xp) { return eval_prod(pr, (long long)z * g.nx + 2 * xp); });
+    __syncthreads();  // Q complete before the column pass
     const int mode = (p == 2) ? 2 : (p == pz ? 0 : 1);
-    for (int m = threadIdx.x; m < g.pf; m += nthr) {
-      const int kzp = (int)fdiv((unsigned)m, g.f_nkx1), kx = m - kzp * g.nkx1;
-      const double2 s0 = A[fwd_slot(g, T, kx, kzp)];
+    gp_cols_fwd(Q, g, T, buf, stage, [&](int kx, int kzp, double2 s0) {
+      const int m = kzp * g.nkx1 + kx;
       const double2 sv = make_double2(div_by_recip(s0.x, nzd, rnz), div_by_recip(s0.y, nzd, rnz));
       ACC[m] = csub(ACC[m], (mode == 2) ? sv : cmul_ik((mode == 0) ? g.kz[kzp] : g.kx[kx], sv));
-    }
-    __syncthreads();  // the next product (or CR) is written over the plane
+    });
+    __syncthreads();  // the next product is written over Q; ACC's modes change threads
   }
-  // RK combination (channel.f:88-167, 581-595; scalars 254-270, 723-729),
-  // new CF stored, CR scattered into the plane for the inverse transform
-  for (int e = threadIdx.x; e < g.nz * g.nkx1; e += nthr) {
-    const int zr = (int)fdiv((unsigned)e, g.f_nkx1), kx = e - zr * g.nkx1;
-    if (zr > g.nkz && zr < g.nz - g.nkz) A[inv_row(T, zr) * g.ls + kx] = make_double2(0.0, 0.0);
-  }
+  // RK combination (channel.f:88-167, 581-595; scalars 254-270, 723-729):
+  // new CF stored, CR written over ACC for the inverse transform
   double2 *CF = (c < 3) ? a.cf[c] : a.cfth + (c - 3) * a.cfs;
   const double2 *CC = (c < 3) ? a.cu[c] : a.cth + (c - 3) * a.cfs;
   const long long base = (long long)J * g.pf;
@@ -695,26 +687,36 @@ __device__ void nonlinear_pass_gp(const StepArgs &a, int J, int c, double2 *A, d
     CF[o] = cfn;
     cr = make_double2(cr.x + T5 * cfn.x, cr.y + T5 * cfn.y);
     cr = make_double2(cr.x + ev.x, cr.y + ev.y);
-    A[inv_row(T, kz_row(g, kzp)) * g.ls + kx] = cr;
+    ACC[m] = cr;
   }
   __syncthreads();
-  plane_cols_inv(A, g, T);
-  plane_rows_inv(A, g, T);
-  plane_store_phys(A, g, T, reinterpret_cast<double2 *>(static_cast<double *>(a.rphys) + (long long)c * a.fs + (long long)J * g.pp));
+  gp_cols_inv(Q, g, T, buf, stage, [&](int kx, int zr) {
+    int kzp = -1;
+    if (zr <= g.nkz)
+      kzp = zr;
+    else if (zr >= g.nz - g.nkz)
+      kzp = zr - (g.nz - 1 - g.nkz) + g.nkz;
+    return kzp >= 0 ? ACC[kzp * g.nkx1 + kx] : make_double2(0.0, 0.0);
+  });
+  __syncthreads();
+  double2 *dst = reinterpret_cast<double2 *>(static_cast<double *>(a.rphys) + (long long)c * a.fs + (long long)J * g.pp);
+  gp_rows_inv(Q, g, T, buf, stage, [&](int z, int m, double2 v) { dst[z * g.lw + m] = v; });
 }
 
 // Persistent over the (plane, component) items of a launch: block b on
-// its own scratch slot (plane, then accumulators) at gp + b gp_slot.
-__global__ void __launch_bounds__(kGpThreads) k_nonlinear_gp(StepArgs a, double2 *gp, long long gp_slot, int ncomp,
-                                                             int nitems) {
+// its own scratch slot (Q, then the accumulators) at gp + b gp_slot.
+__global__ void __launch_bounds__(kGpThreads, 4) k_nonlinear_gp(StepArgs a, double2 *gp, long long gp_slot, int ncomp,
+                                                                int nitems) {
   extern __shared__ double2 smem[];
   const Tables T = load_tables_at(smem, a.g);
-  double2 *A = gp + (long long)blockIdx.x * gp_slot;
-  double2 *ACC = A + (long long)a.g.nz * a.g.ls;
+  double2 *buf = smem + gp_table_elems(a.g);
+  const int stage = gp_stage_elems(a.g);
+  double2 *Q = gp + (long long)blockIdx.x * gp_slot;
+  double2 *ACC = Q + (long long)a.g.nz * a.g.nkx1;
   for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
     const int jr = it / ncomp;
-    nonlinear_pass_gp(a, a.j0 + jr, a.c0 + (it - jr * ncomp), A, ACC, T);
-    __syncthreads();  // the next item writes over the plane this one stored from
+    nonlinear_pass_gp(a, a.j0 + jr, a.c0 + (it - jr * ncomp), Q, ACC, T, buf, stage);
+    __syncthreads();  // the next item writes over Q and ACC
   }
 }
 
@@ -1661,7 +1663,7 @@ static hipError_t launch_nonlinear_range(Ctx &c, int rk, int c0, int c1, hipStre
     int ncomp = c1 - c0, nitems = (int)(grid.x * grid.y);
     void *args[] = {&a, &gp, &slot, &ncomp, &nitems};
     return hipLaunchKernel((const void *)k_nonlinear_gp, dim3(nitems < kGpSlots ? nitems : kGpSlots),
-                           dim3(kGpThreads), args, (unsigned)gp_table_bytes(c), st);
+                           dim3(kGpThreads), args, (unsigned)kGpLds, st);
   }
   if (c.mixed) {
     if (c.reg_rows_nl)
@@ -2373,7 +2375,7 @@ hipError_t prepare_step_kernels(Ctx &c) {
     const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsMax);
     if (e != hipSuccess) return e;
   }
-  return hipSuccess;
+  return hipFuncSetAttribute((const void *)k_nonlinear_gp, hipFuncAttributeMaxDynamicSharedMemorySize, kGpLds);
 }
 
 }  // namespace dh
