@@ -296,6 +296,25 @@ def les_form(steps, warmup, reps=10):
             "phases_ms": phases}
 
 
+def large_plane_dns_ms(steps, warmup=3):
+    """Milliseconds per step of config 3's DNS (Re_tau = 395, 256 x 193 x 256,
+    FP64), whose 528 KB x-z planes go through the large-plane kernels: from
+    the seeded transitional IC, `warmup` steps untimed, then `steps` timed."""
+    from diablo_amd import channel, closure, turbulence
+    pd, gyd, _ = closure.dns_re395()
+    dns = channel.Channel(pd, gyd)
+    turbulence.transition_ic(dns, pd, gyd, seed=1, ubulk=17.5)
+    dns.set_clock(0.0, 1e-5, 0)
+    dns.time_step(warmup)
+    dns.synchronize()
+    t0 = time.perf_counter()
+    dns.time_step(steps)
+    dns.synchronize()
+    ms = (time.perf_counter() - t0) / steps * 1e3
+    dns.close()
+    return ms
+
+
 def learned_les(dns_ms_per_step, steps=1200):
     """The second half of the headline metric: how much sooner an LES with
     the trained closure reaches statistics than the DNS, at the statistics
@@ -310,7 +329,7 @@ def learned_les(dns_ms_per_step, steps=1200):
     for 10 steps, LES 64 x 97 x 64) and the Re_tau = 180 channel (DNS = this
     bench's config-1 step, LES 64 x 65 x 64)."""
     import numpy as np
-    from diablo_amd import channel, closure, turbulence
+    from diablo_amd import closure
     out = {}
     for case, deck, case_file, net_file in (("re395", closure.les_re395, "closure_les_case_re395.npz", "closure_re395.npz"),
                                             ("re180", closure.les_re180, "closure_les_case.npz", "closure_re180.npz")):
@@ -323,17 +342,7 @@ def learned_les(dns_ms_per_step, steps=1200):
                                            scales=(float(net["ts"]), float(net["ls"]), float(net["nus"])))
             live[model] = round(info["wall_s_per_unit_time"], 5)
         if case == "re395":
-            pd, gyd, _ = closure.dns_re395()
-            dns = channel.Channel(pd, gyd)
-            turbulence.transition_ic(dns, pd, gyd, seed=1, ubulk=17.5)
-            dns.set_clock(0.0, 1e-5, 0)
-            dns.time_step(3)
-            dns.synchronize()
-            t0 = time.perf_counter()
-            dns.time_step(10)
-            dns.synchronize()
-            ms = (time.perf_counter() - t0) / 10 * 1e3
-            dns.close()
+            ms = large_plane_dns_ms(10)
         else:
             ms = dns_ms_per_step
         dns_s = ms * 1e-3 / rep["dns"]["dt_developed"]
@@ -454,6 +463,8 @@ def main():
     ap.add_argument("--profile-reps", type=int, default=10)
     ap.add_argument("--ensemble", type=int, default=0, metavar="N",
                     help="only time N uneven shear-layer instances stepped as one graph")
+    ap.add_argument("--large-plane", type=int, default=0, metavar="N",
+                    help="only time N steps of the 256 x 193 x 256 DNS (large-plane kernels)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -466,6 +477,12 @@ def main():
         return
     if args.ensemble:
         ensemble_bench(args.ensemble, args.steps, args.warmup)
+        return
+    if args.large_plane:
+        ms = large_plane_dns_ms(args.large_plane)
+        print(json.dumps({"workload": "DNS Re_tau=395, 256x193x256, FP64 (large-plane kernels)",
+                          "steps": args.large_plane, "ms_per_step": round(ms, 3),
+                          "mpts_steps_per_s": round(256 * 193 * 256 / (ms * 1e-3) / 1e6, 1)}), flush=True)
         return
 
     from diablo_amd import channel, configs

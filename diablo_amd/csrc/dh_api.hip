@@ -235,7 +235,7 @@ int dh_create(const dh_params *p, dh_ctx **out) {
                 (2 * g.lw + 2 * g.nz) * (int)sizeof(int);
   c.lds_bytes32 = g.nz * g.ls * (int)sizeof(float2) + (2 * g.lw + g.nz + 1) * (int)sizeof(float2) +
                   (2 * g.lw + 2 * g.nz) * (int)sizeof(int);
-  // planes beyond LDS go through scratch planes in device memory (FP64, no LES)
+  // planes beyond LDS go through the staged large-plane kernels (FP64, no LES)
   c.gp = c.lds_bytes > kLdsMax;
   if (c.gp && gp_stage_elems(g) < std::max(g.ls, 2 * g.nz))
     return reject("x-z plane too large (a row or a column of it does not fit the staging buffer)");

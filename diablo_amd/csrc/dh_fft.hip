@@ -29,8 +29,8 @@ struct PlaneArgs {
   PlaneBatch b;
 };
 
-// Fourier -> physical for plane j of field f.  A: the plane (LDS, or a
-// scratch plane in device memory), T: the transform tables in LDS.
+// Fourier -> physical for plane j of field f.  A: the plane in LDS, T: the
+// transform tables in LDS.
 template <bool REG, class C, class TT>
 __device__ __forceinline__ void plane_inv_body(const Geo &g, const PlaneBatch &b, C *A, const TT &T, int f,
                                                long long j) {

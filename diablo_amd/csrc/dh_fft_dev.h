@@ -97,22 +97,6 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-// Barrier after a stage of a plane transform.  The plane is in LDS for
-// planes that fit (lds_barrier), or, for larger planes, in a scratch plane in
-// device memory that only this block uses, whose stores the barrier must
-// also order (__syncthreads: workgroup-scope fence on all memory).  The
-// plane pointer's address space is known once the kernel is inlined, so the
-// test folds away.
-template <class P>
-__device__ __forceinline__ void plane_barrier(const P *A) {
-#ifdef __HIP_DEVICE_COMPILE__  // the builtin exists for the device pass only
-  if (__builtin_amdgcn_is_shared((const void *)A))
-    lds_barrier();
-  else
-    __syncthreads();
-#endif
-}
-
 // n / b for a divisor b whose correctly rounded reciprocal r = 1/b is known:
 // q0 = n r, e = n - b q0 (exact by fma), q = q0 + e r.  With r correctly
 // rounded this is the correctly rounded quotient (Markstein's theorem) away
@@ -303,7 +287,7 @@ __device__ __forceinline__ void fft_stage(C *A, const FftStage &st, int N, int n
 #pragma unroll
     for (int q = 0; q < R; ++q) base[q * es] = v[q];
   }
-  plane_barrier(A);
+  lds_barrier();
 }
 
 template <bool INV, bool COLS, bool DIT, class C>
@@ -384,7 +368,7 @@ __device__ void plane_rows_fwd(C *A, const Geo &g, const TablesT<C> &T, int nz, 
       row[k2] = mkc<C>(div_by_recip(X.x, nxd, rnx), div_by_recip(X.y, nxd, rnx));
     }
   }
-  plane_barrier(A);
+  lds_barrier();
 }
 
 // Inverse x transform of every row (FFT_XZ_TO_PHYSICAL's c2r, mpi.f:1600-1604).
@@ -417,7 +401,7 @@ __device__ void plane_rows_inv(C *A, const Geo &g, const TablesT<C> &T, int nz, 
       row[k2] = mkc<C>(a.x - wb.y, a.y + wb.x);
     }
   }
-  plane_barrier(A);
+  lds_barrier();
   lds_fft<true, false, false>(A, g.prow, nz, fnz, g.ls, T.twr);
 }
 template <class C>
@@ -496,7 +480,7 @@ __device__ void plane_unpack(C *A, const Geo &g, const TablesT<C> &T, const doub
     for (int b = 0; b < B; ++b)
       if (slot[b] >= 0) A[slot[b]] = v[b];
   }
-  plane_barrier(A);
+  lds_barrier();
 }
 
 // Physical plane (natural order in HBM) into the digit-reversed input slots.
@@ -525,7 +509,7 @@ __device__ void plane_load_phys(C *A, const Geo &g, const TablesT<C> &T, const C
     for (int b = 0; b < B; ++b)
       if (sl[b] >= 0) A[sl[b]] = v[b];
   }
-  plane_barrier(A);
+  lds_barrier();
 }
 
 // ---------------------------------------------------------------------------

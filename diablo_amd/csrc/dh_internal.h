@@ -88,10 +88,11 @@ constexpr int kRowK2 = 6;
 // than one context's need (a smaller context created later would otherwise
 // lower it under a larger one).
 constexpr int kLdsMax = 160 * 1024;
-// Planes too large for LDS (dh_create sets Ctx::gp) go through scratch
-// planes in device memory: kGpSlots per stream half, one per block of the
-// persistent plane kernels (kGpThreads threads).
-constexpr int kGpSlots = 512, kGpThreads = 512;
+// Planes too large for LDS (dh_create sets Ctx::gp) are transformed in
+// chunks staged through LDS, the plane between the row and the column pass
+// held in a scratch slot in device memory: kGpSlots slots per stream half,
+// one per block of the persistent large-plane kernels (kGpThreads threads).
+constexpr int kGpSlots = 256, kGpThreads = 1024;
 
 // Tables of the y-implicit (Crank-Nicolson) matrices, one set per RK
 // substep and field; with LES off each matrix depends only on j, so every
@@ -207,11 +208,13 @@ struct Ctx {
 inline double2 *gp_base(const Ctx &c, hipStream_t st) {
   return c.gp_scratch + (st == c.branch ? (long long)kGpSlots * c.gp_slot : 0LL);
 }
-// The large-plane kernels' LDS (kGpLds bytes: two blocks per CU, which their
-// launch bounds of 4 waves per SIMD also leave room for in registers): the
+// The large-plane kernels' LDS (all of a CU's: one block of 16 waves per
+// CU, 4 per SIMD, which their launch bounds keep to 128 VGPRs; against two
+// blocks of half the size, the larger chunks halve the barriers per plane
+// and the step at 256 x 193 x 256 takes 7.4 instead of 7.9 ms): the
 // transform tables (load_tables_at), in double2 units rounded up, then the
 // staging buffer of the row and column passes.
-constexpr int kGpLds = 80 * 1024;
+constexpr int kGpLds = kLdsMax;
 __host__ __device__ inline int gp_table_elems(const Geo &g) {
   return (2 * g.lw + g.nz + 1) + ((2 * g.lw + 2 * g.nz) * (int)sizeof(int) + 15) / 16;
 }
