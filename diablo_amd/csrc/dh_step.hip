@@ -872,7 +872,9 @@ __device__ __forceinline__ void cn_tables_body(const Clock *clk, const FactorIn 
     }
   }
   __syncthreads();
-  // the forward elimination is a recurrence: one thread, operands in LDS
+  // the forward elimination is a recurrence: one thread, operands in LDS.
+  // Each row's operands are read one row ahead, so that their LDS latency
+  // overlaps the previous row's division instead of adding to it.
   double *tl = sm + 3 * nrow, *tb = sm + 4 * nrow, *tr = sm + 5 * nrow, *pf = sm + 6 * nrow, *pb = sm + 7 * nrow;
   const int sr = in.seg_rows;
   if (threadIdx.x == 0) {
@@ -880,17 +882,33 @@ __device__ __forceinline__ void cn_tables_body(const Clock *clk, const FactorIn 
     tl[0] = 0.0;
     tb[0] = bp;
     pf[0] = 0.0;
+    double an = a[1], bn = b[1], cn = c[0];
+    int js = (sr == 1) ? 0 : 1;  // J % sr
     for (int J = 1; J < nrow; ++J) {
-      const double m = -a[J] / bp;
-      bp = b[J] + m * c[J - 1];
+      const double aj = an, bj = bn, cj = cn;
+      if (J + 1 < nrow) {
+        an = a[J + 1];
+        bn = b[J + 1];
+        cn = c[J];
+      }
+      const double m = -aj / bp;
+      bp = bj + m * cj;
       tl[J] = m;
       tb[J] = bp;
-      pfv = (J % sr == 0) ? m : pfv * m;
+      pfv = (js == 0) ? m : pfv * m;
       pf[J] = pfv;
+      js = (js + 1 == sr) ? 0 : js + 1;
     }
+  }
+  __syncthreads();
+  // the backward factors' quotients do not depend on one another: every
+  // thread takes some, and one thread chains the products
+  for (int J = threadIdx.x; J < nrow; J += blockDim.x) tr[J] = (tb[J] != 0.0) ? -c[J] / tb[J] : 0.0;
+  __syncthreads();
+  if (threadIdx.x == 0) {
     double pbv = 0.0;
     for (int J = nrow - 1; J >= 0; --J) {
-      const double fj = (tb[J] != 0.0) ? -c[J] / tb[J] : 0.0;
+      const double fj = tr[J];
       pbv = (J % sr == sr - 1 || J == nrow - 1) ? fj : fj * pbv;
       pb[J] = pbv;
     }
