@@ -13,10 +13,10 @@
 //   inverse transforms of those fields
 //   k_les_nut      |S| at GYF and GY, NU_T (with its zero ghost rows),
 //                  |S| S_ij, the cross terms NU_T dU/dy and KAPPA_T dTH/dx
-//   k_les_bc_phys  the physical BCs of U1..3
 //   forward transforms of the products, planes 1..NY
 //   k_les_cf       the subgrid divergence as increments of CF1..3 and CFTH,
-//                  which the nonlinear pass adds to its explicit terms
+//                  which the nonlinear pass adds to its explicit terms, and
+//                  in its last row of blocks the physical BCs of U1..3
 // Every kernel checks the spin-up gate of channel.f:303 on the device clock
 // (the model waits for step 101 of a new flow), so one captured graph
 // serves both sides of it; while the gate is closed the increments are zero
@@ -233,8 +233,10 @@ __global__ void k_les_nut(LesArgs a) {
 }
 
 // APPLY_BC_VEL_PHYS_LOWER/UPPER (channel.f:2495-2650) on the physical
-// U1..3 after the model has read them.
-__global__ void k_les_bc_phys(LesArgs a) {
+// U1..3 after the model has read them: the last row of k_les_cf's blocks
+// (nothing between the model's last read of U and the nonlinear pass reads
+// it, and k_les_cf works on Fourier fields only).
+__device__ __forceinline__ void les_bc_phys_body(const LesArgs &a) {
   if (!les_gate_open(a)) return;
   const Geo &g = a.g;
   const long long col = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -266,6 +268,10 @@ __global__ void k_les_bc_phys(LesArgs a) {
 __global__ void k_les_cf(LesArgs a) {
 #pragma clang fp contract(off)
   const Geo &g = a.g;
+  if (blockIdx.y == (unsigned)g.ny + 2) {  // the physical BCs of U1..3
+    les_bc_phys_body(a);
+    return;
+  }
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= g.pf) return;
   const int J = blockIdx.y, NY = g.ny;
@@ -443,14 +449,15 @@ hipError_t launch_les_model(Ctx &c, hipStream_t st) {
     if ((e = klaunch(c, (const void *)k_les_post, nullptr, dim3(pb, NY + 1), dim3(256), 0, st, a)) != hipSuccess)
       return e;
   }
-  if ((e = klaunch(c, (const void *)k_les_bc_phys, nullptr, dim3(pb), dim3(256), 0, st, a)) != hipSuccess) return e;
   // the products (Fourier slots 0..6) and the scalars' fluxes, planes 1..NY
   const int out = kLesPhys - kLesFour + 2 * nth;
   if ((e = launch_planes_fwd(c, c.les_phys + out * fs, c.les_four, 7, 1, NY, st)) != hipSuccess) return e;
   if (nth > 0 && (e = launch_planes_fwd(c, c.les_phys + kLesFour * fs, c.les_four + kLesFour * cfs, 2 * nth, 1, NY,
                                         st)) != hipSuccess)
     return e;
-  return klaunch(c, (const void *)k_les_cf, nullptr, dim3(mb, NY + 2), dim3(256), 0, st, a);
+  // rows 0..NY+1: the increments of mode blocks 0..mb-1; row NY+2: the
+  // physical BCs of column blocks 0..pb-1
+  return klaunch(c, (const void *)k_les_cf, nullptr, dim3(mb > pb ? mb : pb, NY + 3), dim3(256), 0, st, a);
 }
 
 // The closure's inputs from the current physical U and the Fourier copy
