@@ -360,10 +360,11 @@ def les_profiles(state, steps, model="smagorinsky", params=None, scales=None, ev
     "learned" (params and scales = (ts, ls, nus)).  The BCs and the
     projection are applied first; then 10 steps, half the steps timed (wall
     seconds per unit of simulated time), and the other half averaged
-    (plane statistics every `every` steps).  Returns the profiles in wall
-    units with the forcing's u_tau, and {u_tau from the wall shear, wall
-    seconds per unit time, simulated time}.  deck: the LES deck (les_re180
-    unless given)."""
+    (plane statistics every `every` steps).  The plane kernels' block size
+    is tuned first (dh_autotune: the state is restored and the results do
+    not depend on it).  Returns the profiles in wall units with the
+    forcing's u_tau, and {u_tau from the wall shear, wall seconds per unit
+    time, simulated time}.  deck: the LES deck (les_re180 unless given)."""
     pl, gyl, _ = (deck or les_re180)()
     ch = channel.Channel(dict(pl, les=0 if model == "none" else 1), gyl)
     if model == "learned":
@@ -376,6 +377,7 @@ def les_profiles(state, steps, model="smagorinsky", params=None, scales=None, ev
     ch.apply_bc_vel()
     ch.rem_div()
     ch.set_clock(0.0, pl["delta_t"], 0)
+    ch.autotune(2)
     half = steps // 2
     ch.time_step(10)
     ch.synchronize()
