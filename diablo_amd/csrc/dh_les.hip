@@ -333,16 +333,27 @@ __global__ void __launch_bounds__(256) k_les_mlp(LesArgs a) {
   const int ldi = (int)a.fs;
   MlpRegs r;
   mlp_load_regs(a.mlp_pk, r);
+  // the features of the wave's next 16 points are requested before the
+  // current 16 go through the network, so their latency overlaps it
+  double f[kMlpIn];
+  auto fetch = [&](long long q) {
+#pragma unroll
+    for (int j = 0; j < kMlpIn; ++j) f[j] = 0.0;
+    if (grp == 0 && q < npts) {
+      const double *fp = a.feat + pp + q;  // point q is plane 1 + q / pp
+#pragma unroll
+      for (int j = 0; j < kMlpIn; ++j) f[j] = fp[j * ldi];
+    }
+  };
+  fetch(wave * 16 + c);
   for (long long base = wave * 16; base < npts; base += nwaves * 16) {
     const long long q = base + c;
     bf16x8 x;
 #pragma unroll
     for (int j = 0; j < 8; ++j) x[j] = (__bf16)0.0f;
-    if (grp == 0 && q < npts) {
-      const double *fp = a.feat + pp + q;  // point q is plane 1 + q / pp
 #pragma unroll
-      for (int j = 0; j < kMlpIn; ++j) x[j] = (__bf16)(float)fp[j * ldi];
-    }
+    for (int j = 0; j < kMlpIn; ++j) x[j] = (__bf16)(float)f[j];
+    fetch(base + nwaves * 16 + c);
     const float y = mlp_forward_regs(r, b4, x);
     if (lane < 16 && q < npts) {
       const int J = 1 + (int)(q / pp);
@@ -444,7 +455,8 @@ hipError_t launch_les_model(Ctx &c, hipStream_t st) {
   if ((e = klaunch(c, (const void *)k_les_nut, nullptr, dim3(pb, NY + 1), dim3(256), 0, st, a)) != hipSuccess)
     return e;
   if (c.les_mlp) {
-    // 16 points per wave and pass; a grid of about four waves per SIMD
+    // 16 points per wave and pass; a grid of two waves per SIMD (512 blocks
+    // of 4 waves over 256 CUs of 4 SIMDs), as many as its ~150 VGPRs allow
     if ((e = klaunch(c, (const void *)k_les_mlp, nullptr, dim3(512), dim3(256), 0, st, a)) != hipSuccess) return e;
     if ((e = klaunch(c, (const void *)k_les_post, nullptr, dim3(pb, NY + 1), dim3(256), 0, st, a)) != hipSuccess)
       return e;
