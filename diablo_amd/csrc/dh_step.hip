@@ -2336,12 +2336,22 @@ hipError_t launch_substep(Ctx &c, int rk) {
   SCHK(launch_state_to_phys(c));
   if (c.les) SCHK(launch_les_model(c, c.stream));
   // the mixed form's nonlinear blocks fit two to a CU: all components in
-  // one launch; the FP64 blocks take a CU each, so U3 and the scalars run on
-  // the branch beside the solve of U1 and U2
-  SCHK(launch_nonlinear_range(c, rk, 0, c.mixed ? c.nfields : 2, c.stream));
-  SCHK(krecord(c, 0, c.stream));  // ev_nl01
-  SCHK(kwait(c, c.branch, 0));
-  if (!c.mixed) SCHK(launch_nonlinear_range(c, rk, 2, c.nfields, c.branch));
+  // one launch, and the branch forks after it.  Otherwise U3 and the scalars
+  // run on the branch, which forks as soon as the fields are in physical
+  // space (all its nonlinear pass reads): where the planes are small enough
+  // for several blocks per CU (the LES decks) the two passes run together,
+  // and where a block takes a CU (the 128 x 128 planes) the branch's blocks
+  // take the CUs that U1's and U2's free.
+  if (c.mixed) {
+    SCHK(launch_nonlinear_range(c, rk, 0, c.nfields, c.stream));
+    SCHK(krecord(c, 0, c.stream));  // ev_nl01
+    SCHK(kwait(c, c.branch, 0));
+  } else {
+    SCHK(krecord(c, 0, c.stream));  // ev_nl01
+    SCHK(kwait(c, c.branch, 0));
+    SCHK(launch_nonlinear_range(c, rk, 0, 2, c.stream));
+    SCHK(launch_nonlinear_range(c, rk, 2, c.nfields, c.branch));
+  }
   SCHK(c.les ? launch_solve_les_range(c, rk, courant, 2, c.nfields, c.branch)
              : launch_solve_range(c, rk, courant, 2, c.nfields, c.branch));
   SCHK(launch_step_to_four(c, 2, nv, c.branch));
