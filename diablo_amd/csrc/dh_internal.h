@@ -297,7 +297,9 @@ hipError_t launch_apply_bc_vel(Ctx &c);
 // goes to physical space; the model itself (strain, NU_T, subgrid
 // increments, physical BCs) after it; the per-column solve with NU_T
 hipError_t launch_les_bc_four(Ctx &c, hipStream_t st);
-hipError_t launch_les_model(Ctx &c, hipStream_t st);
+hipError_t launch_les_model(Ctx &c, hipStream_t st);       // launch_les_strain, then launch_les_model_rest
+hipError_t launch_les_strain(Ctx &c, hipStream_t st);      // the strain and its inverse transforms
+hipError_t launch_les_model_rest(Ctx &c, hipStream_t st);  // NU_T, the products' transforms, the increments
 hipError_t launch_les_features(Ctx &c, hipStream_t st);  // the closure's inputs only (no model step)
 hipError_t launch_mlp_pack(const float *params, void *packed, hipStream_t st);
 hipError_t launch_solve_les_range(Ctx &c, int rk, int courant, int f0, int f1, hipStream_t st);

@@ -438,10 +438,14 @@ static LesArgs les_args(Ctx &c) {
   return a;
 }
 
-hipError_t launch_les_model(Ctx &c, hipStream_t st) {
+// The model's first part: the strain and the scalars' gradients in Fourier
+// space and their inverse transforms.  It reads only the Fourier copy with
+// les_chan's BCs, as the substep's inverse transform of U does, so the two
+// can run side by side.
+hipError_t launch_les_strain(Ctx &c, hipStream_t st) {
   const LesArgs a = les_args(c);
   const int NY = c.g.ny, nth = c.p.n_th;
-  const unsigned mb = (unsigned)((c.g.pf + 255) / 256), pb = (unsigned)((c.g.pp + 255) / 256);
+  const unsigned mb = (unsigned)((c.g.pf + 255) / 256);
   hipError_t e;
   if ((e = klaunch(c, (const void *)k_les_strain, nullptr, dim3(mb, NY + 1), dim3(256), 0, st, a)) != hipSuccess)
     return e;
@@ -452,6 +456,17 @@ hipError_t launch_les_model(Ctx &c, hipStream_t st) {
   if (nth > 0 && (e = launch_planes_inv(c, c.les_four + kLesFour * cfs, c.les_phys + kLesFour * fs, 2 * nth, 1,
                                         NY + 1, st)) != hipSuccess)
     return e;
+  return hipSuccess;
+}
+
+// The rest, once the strain and the physical U are both there: NU_T (or
+// the closure's), the products' forward transforms and the increments.
+hipError_t launch_les_model_rest(Ctx &c, hipStream_t st) {
+  const LesArgs a = les_args(c);
+  const int NY = c.g.ny, nth = c.p.n_th;
+  const unsigned mb = (unsigned)((c.g.pf + 255) / 256), pb = (unsigned)((c.g.pp + 255) / 256);
+  const long long fs = a.fs, cfs = a.cfs;
+  hipError_t e;
   if ((e = klaunch(c, (const void *)k_les_nut, nullptr, dim3(pb, NY + 1), dim3(256), 0, st, a)) != hipSuccess)
     return e;
   if (c.les_mlp) {
@@ -470,6 +485,11 @@ hipError_t launch_les_model(Ctx &c, hipStream_t st) {
   // rows 0..NY+1: the increments of mode blocks 0..mb-1; row NY+2: the
   // physical BCs of column blocks 0..pb-1
   return klaunch(c, (const void *)k_les_cf, nullptr, dim3(mb > pb ? mb : pb, NY + 3), dim3(256), 0, st, a);
+}
+
+hipError_t launch_les_model(Ctx &c, hipStream_t st) {
+  const hipError_t e = launch_les_strain(c, st);
+  return e == hipSuccess ? launch_les_model_rest(c, st) : e;
 }
 
 // The closure's inputs from the current physical U and the Fourier copy

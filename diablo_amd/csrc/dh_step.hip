@@ -2332,9 +2332,20 @@ hipError_t launch_substep(Ctx &c, int rk) {
   hipError_t e;
 #define SCHK(expr) \
   if ((e = (expr)) != hipSuccess) return e;
-  if (c.les) SCHK(launch_les_bc_four(c, c.stream));
-  SCHK(launch_state_to_phys(c));
-  if (c.les) SCHK(launch_les_model(c, c.stream));
+  if (c.les) {
+    // the model's strain and its inverse transforms on the side stream,
+    // beside the inverse transform of U; both read the BC'd Fourier copy
+    SCHK(launch_les_bc_four(c, c.stream));
+    SCHK(krecord(c, 2, c.stream));  // ev_fork
+    SCHK(kwait(c, c.side, 2));
+    SCHK(launch_les_strain(c, c.side));
+    SCHK(krecord(c, 3, c.side));  // ev_join
+    SCHK(launch_state_to_phys(c));
+    SCHK(kwait(c, c.stream, 3));
+    SCHK(launch_les_model_rest(c, c.stream));
+  } else {
+    SCHK(launch_state_to_phys(c));
+  }
   // the mixed form's nonlinear blocks fit two to a CU: all components in
   // one launch, and the branch forks after it.  Otherwise U3 and the scalars
   // run on the branch, which forks as soon as the fields are in physical
