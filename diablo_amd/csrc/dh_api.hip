@@ -247,8 +247,11 @@ int dh_create(const dh_params *p, dh_ctx **out) {
   while (thr * 4 < g.pf && thr < 1024) thr += 64;  // k_nonlinear keeps <= 4 modes per thread
   if (!c.gp && thr * 4 < g.pf) return reject("plane has too many Fourier modes");
   c.plane_threads = thr;
-  c.reg_rows = !c.gp && (g.lw == 64 && g.nz % 8 == 0 && thr % 64 == 0 && g.nkx < 8 * kRowK2);
-  c.reg_rows_nl = c.reg_rows && g.pf <= 512 * 8;  // k_nonlinear's register variant: 512 threads x 8 modes
+  // register rows: NX = 128 (slabs of 8 rows per wave) or NX = 64 (slabs of 16)
+  c.reg_rows = !c.gp && thr % 64 == 0 &&
+               ((g.lw == 64 && g.nz % 8 == 0 && g.nkx < 8 * kRowK2) || (g.lw == 32 && g.nz % 16 == 0));
+  // k_nonlinear's register variant (NX = 128): 512 threads x 8 modes
+  c.reg_rows_nl = c.reg_rows && g.lw == 64 && g.pf <= 512 * 8;
   {
   }
   g.f_npair = recip(g.lw / 2 + 1);

@@ -38,7 +38,10 @@ __device__ __forceinline__ void plane_inv_body(const Geo &g, const PlaneBatch &b
   plane_cols_inv(A, g, T);
   C *dst = reinterpret_cast<C *>(static_cast<Sc<C> *>(b.phys[f]) + j * g.pp);
   if constexpr (REG) {
-    rows_inv_reg64(A, g, T, [&](int z, int m, C v) { dst[z * 64 + m] = v; });
+    if (g.lw == 64)
+      rows_inv_reg64(A, g, T, [&](int z, int m, C v) { dst[z * 64 + m] = v; });
+    else
+      rows_inv_reg32(A, g, T, [&](int z, int m, C v) { dst[z * 32 + m] = v; });
   } else {
     plane_rows_inv(A, g, T);
     plane_store_phys(A, g, T, dst);
@@ -62,7 +65,10 @@ __device__ __forceinline__ void plane_fwd_body(const Geo &g, const PlaneBatch &b
   const void *ph = (b.alt[f] && j >= b.alo[f] && j <= b.ahi[f]) ? b.alt[f] : b.phys[f];
   const C *src = reinterpret_cast<const C *>(static_cast<const Sc<C> *>(ph) + j * g.pp);
   if constexpr (REG) {
-    rows_fwd_reg64(A, g, T, [&](int z, int m) { return src[z * 64 + m]; });
+    if (g.lw == 64)
+      rows_fwd_reg64(A, g, T, [&](int z, int m) { return src[z * 64 + m]; });
+    else
+      rows_fwd_reg32(A, g, T, [&](int z, int m) { return src[z * 32 + m]; });
   } else {
     plane_load_phys(A, g, T, src);
     plane_rows_fwd(A, g, T);

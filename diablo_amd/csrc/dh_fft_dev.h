@@ -703,6 +703,129 @@ __device__ void rows_inv_reg64(C *A, const Geo &g, const TablesT<C> &T, Store st
   }
 }
 
+// ---------------------------------------------------------------------------
+// Register-resident row transforms for NX = 64 (32 complex pairs per row).
+// A wave owns a slab of 16 rows; lane = 4 * row + t (t < 4), eight values
+// per lane.  The 32-point complex DFT is split 32 = 4 x 8 with an 8-point
+// DFT in each lane's registers, a twiddle, a 4 x 8 transpose through the
+// row's own LDS slots (lane t then takes the two columns 2t and 2t + 1),
+// and 4-point DFTs.  As for NX = 128, only lanes of one wave touch a slab,
+// so the exchanges need no workgroup barrier.
+// ---------------------------------------------------------------------------
+
+// Forward x transform of every row, physical pair m of row z from load(z, m),
+// to X_k / NX for k <= NKX in natural slots of the LDS rows
+// (FFT_XZ_TO_FOURIER's r2c, mpi.f:1490-1506).  With m = t + 4r (r < 8) and
+// k = k1 + 8 k2 (k1 < 8, k2 < 4):
+//   Z_{k1+8k2} = sum_t W4^{t k2} W32^{t k1} sum_r W8^{r k1} x_{t+4r}.
+template <class C, class Load>
+__device__ void rows_fwd_reg32(C *A, const Geo &g, const TablesT<C> &T, Load load) {
+  using S = Sc<C>;
+  const int lane = threadIdx.x & 63, rho = lane >> 2, t = lane & 3;
+  const int nw = blockDim.x >> 6;
+  const S nxd = S(g.dnx), rnx = S(g.rnx);  // X / NX as the correctly rounded quotient
+  for (int slab = threadIdx.x >> 6; slab * 16 < g.nz; slab += nw) {
+    const int z = slab * 16 + rho;  // < NZ: these rows require NZ % 16 == 0
+    C *row = A + z * g.ls;
+    C v[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) v[r] = load(z, t + 4 * r);
+    dft_small<8, false>(v);  // v[k1] = sum_r W8^{r k1} x_{t+4r}
+#pragma unroll
+    for (int k1 = 1; k1 < 8; ++k1) v[k1] = cmul(v[k1], T.twr[t * k1]);  // W32^{t k1}
+#pragma unroll
+    for (int k1 = 0; k1 < 8; ++k1) row[4 * k1 + t] = v[k1];
+    wave_lds_sync();
+    C w[8];  // w[4h + k2] = Z_{k1 + 8 k2}, k1 = 2t + h
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      C q[4];
+#pragma unroll
+      for (int tt = 0; tt < 4; ++tt) q[tt] = row[4 * (2 * t + h) + tt];
+      dft_small<4, false>(q);
+#pragma unroll
+      for (int k2 = 0; k2 < 4; ++k2) w[4 * h + k2] = q[k2];
+    }
+    wave_lds_sync();
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int k2 = 0; k2 < 4; ++k2) row[2 * t + h + 8 * k2] = w[4 * h + k2];
+    wave_lds_sync();
+    C zm[8];  // Z_{32-k}
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int k2 = 0; k2 < 4; ++k2) zm[4 * h + k2] = row[(32 - (2 * t + h) - 8 * k2) & 31];
+    wave_lds_sync();
+    // X_k = (Z_k + conj Z_{32-k})/2 - i w^k (Z_k - conj Z_{32-k})/2, w = e^{-2 pi i/NX}
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int k2 = 0; k2 < 4; ++k2) {
+        const int k = 2 * t + h + 8 * k2;
+        const C zk = w[4 * h + k2], zmk = zm[4 * h + k2];
+        C a = mkc<C>(S(0.5) * (zk.x + zmk.x), S(0.5) * (zk.y - zmk.y));
+        C b = mkc<C>(S(0.5) * (zk.x - zmk.x), S(0.5) * (zk.y + zmk.y));
+        C wb = cmul(T.twx[k], b);
+        if (k <= g.nkx) row[k] = mkc<C>(div_by_recip(a.x + wb.y, nxd, rnx), div_by_recip(a.y - wb.x, nxd, rnx));
+      }
+  }
+  lds_barrier();
+}
+
+// Inverse x transform of every row for NX = 64 (FFT_XZ_TO_PHYSICAL's c2r,
+// mpi.f:1600-1604): X_k (k <= NKX) in natural slots of the LDS rows to the
+// physical pairs, handed to store(z, m, value).  Lane t forms Z_k for
+// k = t + 4 k2 (k2 < 8); with m = m1 + 8 m2 (m1 < 8, m2 < 4):
+//   c_{m1+8m2} = sum_{k1} W4^{-m2 k1} W32^{-m1 k1} sum_{k2} W8^{-m1 k2} Z_{k1+4k2}.
+template <class C, class Store>
+__device__ void rows_inv_reg32(C *A, const Geo &g, const TablesT<C> &T, Store store) {
+  const int lane = threadIdx.x & 63, rho = lane >> 2, t = lane & 3;
+  const int nw = blockDim.x >> 6;
+  for (int slab = threadIdx.x >> 6; slab * 16 < g.nz; slab += nw) {
+    const int z = slab * 16 + rho;
+    C *row = A + z * g.ls;
+    // Z_k = (X_k + conj X_{32-k}) + i w^k (X_k - conj X_{32-k}), w = e^{+2 pi i/NX}
+    // (X_k = 0 above NKX; Im X_0 ignored)
+    C v[8];
+#pragma unroll
+    for (int k2 = 0; k2 < 8; ++k2) {
+      const int k = t + 4 * k2, km = (32 - k) & 31;
+      C xk = mkc<C>(0.0, 0.0), xm = mkc<C>(0.0, 0.0);
+      if (k <= g.nkx) xk = row[k];
+      if (k != 0 && km <= g.nkx) xm = row[km];
+      if (k == 0) xk.y = 0.0;
+      C a = mkc<C>(xk.x + xm.x, xk.y - xm.y);
+      C b = mkc<C>(xk.x - xm.x, xk.y + xm.y);
+      C wb = cmul(cconj(T.twx[k]), b);
+      v[k2] = mkc<C>(a.x - wb.y, a.y + wb.x);
+    }
+    wave_lds_sync();
+    dft_small<8, true>(v);  // v[m1] = sum_k2 W8^{-m1 k2} Z_{t+4k2}
+#pragma unroll
+    for (int m1 = 1; m1 < 8; ++m1) v[m1] = cmul(v[m1], cconj(T.twr[m1 * t]));  // W32^{-m1 t}
+#pragma unroll
+    for (int m1 = 0; m1 < 8; ++m1) row[4 * m1 + t] = v[m1];
+    wave_lds_sync();
+    C w[8];  // w[4h + m2] = c_{m1 + 8 m2}, m1 = 2t + h
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      C q[4];
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk) q[kk] = row[4 * (2 * t + h) + kk];
+      dft_small<4, true>(q);
+#pragma unroll
+      for (int m2 = 0; m2 < 4; ++m2) w[4 * h + m2] = q[m2];
+    }
+    wave_lds_sync();
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int m2 = 0; m2 < 4; ++m2) store(z, 2 * t + h + 8 * m2, w[4 * h + m2]);
+  }
+}
+
 // Physical plane from the inverse-transform slots to natural order in HBM.
 template <class C>
 __device__ void plane_store_phys(const C *A, const Geo &g, const TablesT<C> &T, C *dst) {

@@ -131,7 +131,7 @@ struct Ctx {
   int lds_bytes;     // dynamic LDS of the FP64 plane kernels
   int lds_bytes32;   // the same for the FP32 transforms of the mixed form
   int plane_threads;
-  bool reg_rows = false;     // plane kernels: row transforms in registers (NX = 128, NZ a multiple of 8)
+  bool reg_rows = false;     // plane kernels: row transforms in registers (NX = 128, NZ a multiple of 8; NX = 64, NZ of 16)
   bool reg_rows_nl = false;  // the same in k_nonlinear
   hipStream_t stream = nullptr;
   hipStream_t side = nullptr;          // table rebuild after COURANT, overlapped with the substep's tail

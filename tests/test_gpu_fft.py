@@ -1,6 +1,8 @@
 """FFT_XZ_TO_FOURIER / FFT_XZ_TO_PHYSICAL of the HIP library against the
 oracle's restatement of mpi.f:1477-1606 on random planes (FP64, relative
-L2 <= 1e-13 per batch)."""
+L2 <= 1e-13 per batch).  The decks cover the LDS row transforms (16 x 8
+and 128 x 4 planes), the register rows for NX = 128 (channel_re180) and
+for NX = 64 (les_channel, 64 x 64 planes)."""
 import numpy as np
 import pytest
 
@@ -9,7 +11,7 @@ from tests import devmem
 
 pytestmark = pytest.mark.gpu
 
-GRIDS = {"small_case": {}, "kh_shear_layer": {}, "channel_re180": {}}
+GRIDS = {"small_case": {}, "kh_shear_layer": {}, "channel_re180": {}, "les_channel": {}}
 
 
 @pytest.fixture(scope="module")

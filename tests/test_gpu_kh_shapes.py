@@ -5,7 +5,7 @@ NX = 192 and 256 (NY = NX + 1, NZ = 4), the two largest sizes of the
 bar: max(1e-10, 4 x floor) per field), and an NX = 256 instance in the
 mixed form within 1e-5 of the FP64 oracle after 3 steps (the mixed form's
 bar in test_gpu_mixed).  Both sizes run the LDS row transforms (the
-register rows serve NX = 128 only)."""
+register rows serve NX = 64 and 128 only)."""
 import pytest
 
 from diablo_amd import channel, configs
