@@ -252,6 +252,9 @@ int dh_create(const dh_params *p, dh_ctx **out) {
                ((g.lw == 64 && g.nz % 8 == 0 && g.nkx < 8 * kRowK2) || (g.lw == 32 && g.nz % 16 == 0));
   // k_nonlinear's register variant (NX = 128): 512 threads x 8 modes
   c.reg_rows_nl = c.reg_rows && g.lw == 64 && g.pf <= 512 * 8;
+  // NX = 64 with at most 2048 modes: the nonlinear pass's register rows are
+  // in its <= 512-thread instantiation, and every offered size runs them
+  if (c.reg_rows && g.lw == 32 && g.pf <= 4 * 512 && c.plane_threads > 512) c.plane_threads = 512;
   {
   }
   g.f_npair = recip(g.lw / 2 + 1);
@@ -786,19 +789,29 @@ int dh_les_features(dh_ctx *h) {
 
 static int enqueue_steps(Ctx &c, int nsteps);
 
+// Whether the nonlinear pass's LDS variant of <= 512 threads serves the
+// plane with register x rows (NX = 64): then only blocks of up to 512
+// threads are offered, so that every valid size runs the same rows.
+static bool rows32_caps_threads(const Ctx &c) {
+  return c.reg_rows && c.g.lw == 32 && c.g.pf <= 4 * 512;
+}
+
 // Block sizes of the plane kernels and of the nonlinear pass's LDS variant:
-// any multiple of 64 up to 1024 that leaves the nonlinear pass at most 4
-// modes per thread.  The size only decides which thread computes which
-// butterfly or mode, so results do not depend on it.
+// any multiple of 64 up to 1024 (512 where rows32_caps_threads) that leaves
+// the nonlinear pass at most 4 modes per thread.  The size only decides
+// which thread computes which butterfly or mode, so results do not depend
+// on it.
 static bool plane_threads_ok(const Ctx &c, int t) {
-  return t >= 64 && t <= 1024 && t % 64 == 0 && (c.reg_rows_nl || t * 4 >= c.g.pf);
+  return t >= 64 && t <= (rows32_caps_threads(c) ? 512 : 1024) && t % 64 == 0 &&
+         (c.reg_rows_nl || t * 4 >= c.g.pf);
 }
 
 int dh_set_plane_threads(dh_ctx *h, int32_t threads) {
   if (!h) return DH_EARG;
   Ctx &c = h->c;
-  if (!plane_threads_ok(c, threads)) return fail(c, DH_EARG, "plane threads must be a multiple of 64 in 64..1024 "
-                                                             "with 4 x threads >= the plane's Fourier modes");
+  if (!plane_threads_ok(c, threads))
+    return fail(c, DH_EARG, "plane threads must be a multiple of 64 in 64..1024 (64..512 for NX = 64 planes "
+                            "of at most 2048 Fourier modes) with 4 x threads >= the plane's Fourier modes");
   if (threads == c.plane_threads) return DH_OK;
   HIPCHK(hipStreamSynchronize(c.stream));
   c.plane_threads = threads;

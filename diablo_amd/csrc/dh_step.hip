@@ -34,6 +34,7 @@ struct StepArgs {
   long long fs;   // physical field stride (NY+2)*NX*NZ
   long long cfs;  // Fourier field stride (NY+2)*pf
   int c0, j0;     // component of blockIdx.y = 0 and plane of the first block row
+  int rows32;     // FP64 LDS variant of <= 512 threads: x rows of NX = 64 in registers (NZ % 16 == 0)
   double nu, nu_v_scale, beta, i_ro, grav_x, grav_y, grav_z, px0, omega0, amp_omega0;
   int f_type, n_th;
   double ri[DH_MAX_TH], pr[DH_MAX_TH], drhodx[DH_MAX_TH], drhodz[DH_MAX_TH];
@@ -333,7 +334,9 @@ __device__ double2 mean_terms(const StepArgs &a, int c, int J, long long o, bool
 // C: complex type of the plane and the physical arrays (double2, or float2
 // in the mixed form, whose products and transforms run in FP32 while the
 // Fourier state, the accumulation of CF and the RK combination stay FP64).
-template <bool REG, int NQ, class C, bool SGS = false>
+// R32: the LDS variant may take the x rows of NX = 64 in registers (when
+// a.rows32 is set); compiled only where the register budget holds them.
+template <bool REG, int NQ, class C, bool SGS = false, bool R32 = false>
 __device__ __forceinline__ void nonlinear_pass(const StepArgs &a, C *A) {
   const Geo &g = a.g;
   const auto T = load_tables(A, g);
@@ -462,12 +465,16 @@ __device__ __forceinline__ void nonlinear_pass(const StepArgs &a, C *A) {
   } else {
     for (int p = 0; p < 3; ++p) {
       const Prod<C> pr = make_prod<C>(a, c, p, J);
-      for (int t0 = threadIdx.x >> 6; t0 < lt.ntiles; t0 += nw) {
-        int z, xp;
-        if (lt.at(g, t0, z, xp)) A[phys_slot(T, g, z, xp)] = eval_prod(pr, (long long)z * g.nx + 2 * xp);
+      if (R32 && a.rows32) {
+        rows_fwd_reg32(A, g, T, [&](int z, int xp) { return eval_prod(pr, (long long)z * g.nx + 2 * xp); });
+      } else {
+        for (int t0 = threadIdx.x >> 6; t0 < lt.ntiles; t0 += nw) {
+          int z, xp;
+          if (lt.at(g, t0, z, xp)) A[phys_slot(T, g, z, xp)] = eval_prod(pr, (long long)z * g.nx + 2 * xp);
+        }
+        lds_barrier();
+        plane_rows_fwd(A, g, T);
       }
-      lds_barrier();
-      plane_rows_fwd(A, g, T);
       plane_cols_fwd(A, g, T);
       accumulate(p == 2 ? 2 : (p == pz ? 0 : 1));
     }
@@ -583,6 +590,8 @@ __device__ __forceinline__ void nonlinear_pass(const StepArgs &a, C *A) {
   C *dst = reinterpret_cast<C *>(static_cast<Sc<C> *>(a.rphys) + (long long)c * a.fs + (long long)J * g.pp);
   if constexpr (REG) {
     rows_inv_reg64(A, g, T, [&](int z, int m, C v) { dst[z * 64 + m] = v; });
+  } else if (R32 && a.rows32) {
+    rows_inv_reg32(A, g, T, [&](int z, int m, C v) { dst[z * 32 + m] = v; });
   } else {
     plane_rows_inv(A, g, T);
     plane_store_phys(A, g, T, dst);
@@ -722,11 +731,12 @@ __global__ void __launch_bounds__(kGpThreads, 4) k_nonlinear_gp(StepArgs a, doub
 
 // LB: the block-size bound.  The LDS variant takes any block size up to
 // 1024; its instantiation for blocks of at most 512 threads has twice the
-// register budget (256 VGPRs) and no spills.
+// register budget (256 VGPRs) and no spills, and takes the NX = 64 rows in
+// registers.
 template <bool REG, int NQ, bool SGS = false, int LB = (REG ? 512 : 1024)>
 __global__ void __launch_bounds__(LB) k_nonlinear(StepArgs a) {
   extern __shared__ double2 smem[];
-  nonlinear_pass<REG, NQ, double2, SGS>(a, smem);
+  nonlinear_pass<REG, NQ, double2, SGS, !REG && LB <= 512>(a, smem);
 }
 
 // The mixed form: the FP32 plane takes half the LDS and half the registers
@@ -742,7 +752,7 @@ k_nonlinear32(StepArgs a) {
 template <bool REG, int NQ, int LB = (REG ? 512 : 1024)>
 __global__ void __launch_bounds__(LB) k_nonlinear_b(const StepArgs *__restrict__ av) {
   extern __shared__ double2 smem[];
-  nonlinear_pass<REG, NQ, double2>(av[blockIdx.z], smem);
+  nonlinear_pass<REG, NQ, double2, false, !REG && LB <= 512>(av[blockIdx.z], smem);
 }
 template <bool REG, int NQ>
 __global__ void __launch_bounds__(REG ? 512 : 1024) __attribute__((amdgpu_waves_per_eu(4)))
@@ -1599,6 +1609,7 @@ static StepArgs make_step_args(Ctx &c, int rk) {
   a.c0 = 0;
   a.j0 = 1;
   a.sgs = c.les ? c.sgs : nullptr;
+  a.rows32 = (c.reg_rows && c.g.lw == 32 && !c.mixed && c.plane_threads <= 512) ? 1 : 0;
   const dh_params &p = c.p;
   a.nu = p.nu;
   a.nu_v_scale = p.nu_v_scale;
@@ -1662,14 +1673,53 @@ hipError_t launch_cn_tables(Ctx &c, hipStream_t st) {
 
 // Nonlinear pass of components c0..c1-1 on stream st: one block per plane
 // J of the components' ranges (JSTART..JEND, channel.f:77-626) and component.
-static hipError_t launch_nonlinear_range(Ctx &c, int rk, int c0, int c1, hipStream_t st) {
-  if (c1 <= c0) return hipSuccess;
-  StepArgs a = make_step_args(c, rk);
-  int jlo = c.g.ny, jhi = 1;
+// The nonlinear pass's kernel for the context's precision, model, row
+// variant and block size (planes in LDS), with its block and LDS sizes.
+struct NlKernel {
+  const void *fn, *fnb;  // the plain and the batched form (nullptr: none)
+  int threads;
+  unsigned lds;
+};
+static NlKernel nonlinear_kernel(const Ctx &c) {
+  if (c.mixed) {
+    if (c.reg_rows_nl)
+      return {(const void *)k_nonlinear32<true, kNlRegQ>, (const void *)k_nonlinear32_b<true, kNlRegQ>, kNlRegThreads,
+              (unsigned)c.lds_bytes32};
+    return {(const void *)k_nonlinear32<false, kMaxQ>, (const void *)k_nonlinear32_b<false, kMaxQ>, c.plane_threads,
+            (unsigned)c.lds_bytes32};
+  }
+  const bool half = c.plane_threads <= 512;  // the LDS variant's 512-thread instantiation
+  const unsigned lds = (unsigned)c.lds_bytes;
+  if (c.les) {  // FP64 only; the LES kernels have no batched forms
+    if (c.reg_rows_nl) return {(const void *)k_nonlinear<true, kNlRegQ, true>, nullptr, kNlRegThreads, lds};
+    return {half ? (const void *)k_nonlinear<false, kMaxQ, true, 512> : (const void *)k_nonlinear<false, kMaxQ, true, 1024>,
+            nullptr, c.plane_threads, lds};
+  }
+  if (c.reg_rows_nl)
+    return {(const void *)k_nonlinear<true, kNlRegQ>, (const void *)k_nonlinear_b<true, kNlRegQ>, kNlRegThreads, lds};
+  if (half)
+    return {(const void *)k_nonlinear<false, kMaxQ, false, 512>, (const void *)k_nonlinear_b<false, kMaxQ, 512>,
+            c.plane_threads, lds};
+  return {(const void *)k_nonlinear<false, kMaxQ, false, 1024>, (const void *)k_nonlinear_b<false, kMaxQ, 1024>,
+          c.plane_threads, lds};
+}
+
+// Planes of the nonlinear pass over fields c0..c1-1: rows jlo..jhi (empty
+// when jhi < jlo).
+static void nonlinear_rows(const Ctx &c, int c0, int c1, int &jlo, int &jhi) {
+  jlo = c.g.ny;
+  jhi = 1;
   for (int k = c0; k < c1; ++k) {
     jlo = min(jlo, k < 3 ? 2 : c.jstart_th[k - 3]);
     jhi = max(jhi, k < 3 ? c.g.ny : c.jend_th[k - 3]);
   }
+}
+
+static hipError_t launch_nonlinear_range(Ctx &c, int rk, int c0, int c1, hipStream_t st) {
+  if (c1 <= c0) return hipSuccess;
+  StepArgs a = make_step_args(c, rk);
+  int jlo, jhi;
+  nonlinear_rows(c, c0, c1, jlo, jhi);
   if (jhi < jlo) return hipSuccess;
   a.c0 = c0;
   a.j0 = jlo;
@@ -1683,31 +1733,25 @@ static hipError_t launch_nonlinear_range(Ctx &c, int rk, int c0, int c1, hipStre
     return hipLaunchKernel((const void *)k_nonlinear_gp, dim3(nitems < kGpSlots ? nitems : kGpSlots),
                            dim3(kGpThreads), args, (unsigned)kGpLds, st);
   }
-  if (c.mixed) {
-    if (c.reg_rows_nl)
-      return klaunch(c, (const void *)k_nonlinear32<true, kNlRegQ>, (const void *)k_nonlinear32_b<true, kNlRegQ>, grid,
-                     dim3(kNlRegThreads), c.lds_bytes32, st, a);
-    return klaunch(c, (const void *)k_nonlinear32<false, kMaxQ>, (const void *)k_nonlinear32_b<false, kMaxQ>, grid,
-                   dim3(c.plane_threads), c.lds_bytes32, st, a);
-  }
-  const bool half = c.plane_threads <= 512;  // the LDS variant's 512-thread instantiation
-  if (c.les) {  // FP64 only; the LES kernels have no batched forms
-    if (c.reg_rows_nl)
-      return klaunch(c, (const void *)k_nonlinear<true, kNlRegQ, true>, nullptr, grid, dim3(kNlRegThreads),
-                     c.lds_bytes, st, a);
-    return klaunch(c,
-                   half ? (const void *)k_nonlinear<false, kMaxQ, true, 512>
-                        : (const void *)k_nonlinear<false, kMaxQ, true, 1024>,
-                   nullptr, grid, dim3(c.plane_threads), c.lds_bytes, st, a);
-  }
-  if (c.reg_rows_nl)
-    return klaunch(c, (const void *)k_nonlinear<true, kNlRegQ>, (const void *)k_nonlinear_b<true, kNlRegQ>, grid,
-                   dim3(kNlRegThreads), c.lds_bytes, st, a);
-  if (half)
-    return klaunch(c, (const void *)k_nonlinear<false, kMaxQ, false, 512>,
-                   (const void *)k_nonlinear_b<false, kMaxQ, 512>, grid, dim3(c.plane_threads), c.lds_bytes, st, a);
-  return klaunch(c, (const void *)k_nonlinear<false, kMaxQ, false, 1024>,
-                 (const void *)k_nonlinear_b<false, kMaxQ, 1024>, grid, dim3(c.plane_threads), c.lds_bytes, st, a);
+  const NlKernel k = nonlinear_kernel(c);
+  return klaunch(c, k.fn, k.fnb, grid, dim3(k.threads), k.lds, st, a);
+}
+
+// Whether every block of the all-field nonlinear launch is resident on the
+// GPU at once (small planes, several blocks per CU).  Splitting the pass
+// over two streams then frees no compute units for the solve and costs a
+// fork and a join.
+static bool nonlinear_fits_at_once(const Ctx &c) {
+  if (c.gp) return false;
+  int jlo, jhi;
+  nonlinear_rows(c, 0, c.nfields, jlo, jhi);
+  if (jhi < jlo) return true;
+  const NlKernel k = nonlinear_kernel(c);
+  int per_cu = 0, dev = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k.fn, k.threads, k.lds) != hipSuccess) return false;
+  if (hipGetDevice(&dev) != hipSuccess) return false;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return false;
+  return (long long)(jhi - jlo + 1) * c.nfields <= (long long)per_cu * cus;
 }
 
 static hipError_t launch_nonlinear(Ctx &c, int rk) { return launch_nonlinear_range(c, rk, 0, c.nfields, c.stream); }
@@ -2319,12 +2363,14 @@ hipError_t join_side(Ctx &c) {
   return kwait(c, c.stream, 3);  // ev_join
 }
 
-// One substep.  The nonlinear pass of U1 and U2 fills one wave of blocks
-// (one per plane and component); the last velocity component's nonlinear
-// pass, solve and transform back (then the scalars' nonlinear pass and
-// solve) run as a branch beside the solve and transform of U1 and U2, which
-// use the compute units that pass leaves idle.  The scalars' transform back,
-// COURANT and REM_DIV follow on the context stream once the branch has
+// One substep.  Where every block of the nonlinear pass is resident at once
+// (small planes), each phase is one launch over all fields on the context
+// stream.  Otherwise the nonlinear pass of U1 and U2 fills one wave of
+// blocks (one per plane and component); the last velocity component's
+// nonlinear pass, solve and transform back (then the scalars' nonlinear pass
+// and solve) run as a branch beside the solve and transform of U1 and U2,
+// which use the compute units that pass leaves idle.  The scalars' transform
+// back, COURANT and REM_DIV follow on the context stream once the branch has
 // joined.  The kernels and their arithmetic are those of launch_phase.
 hipError_t launch_substep(Ctx &c, int rk) {
   const int courant = (c.p.variable_dt && rk == 2) ? 1 : 0;
@@ -2353,6 +2399,17 @@ hipError_t launch_substep(Ctx &c, int rk) {
   // for several blocks per CU (the LES decks) the two passes run together,
   // and where a block takes a CU (the 128 x 128 planes) the branch's blocks
   // take the CUs that U1's and U2's free.
+  if (nonlinear_fits_at_once(c)) {
+    // one launch per phase, in launch_phase's order; the solve tables for
+    // a new DELTA_T build beside the transform back and REM_DIV
+    SCHK(launch_nonlinear_range(c, rk, 0, c.nfields, c.stream));
+    SCHK(c.les ? launch_solve_les_range(c, rk, courant, 0, c.nfields, c.stream)
+               : launch_solve_range(c, rk, courant, 0, c.nfields, c.stream));
+    if (courant) SCHK(launch_courant(c));
+    SCHK(launch_step_to_four(c, 0, c.nfields, c.stream));
+    SCHK(launch_rem_div(c, rk));
+    return join_side(c);
+  }
   if (c.mixed) {
     SCHK(launch_nonlinear_range(c, rk, 0, c.nfields, c.stream));
     SCHK(krecord(c, 0, c.stream));  // ev_nl01

@@ -210,8 +210,9 @@ int dh_mlp_adam_dev(float *params, float *m, float *v, const float *grad, float 
  * uneven sizes, where the best block size depends on the plane's shape).
  * dh_set_plane_threads sets the block size of the plane transform kernels
  * and of the nonlinear pass's LDS variant: a multiple of 64 in 64..1024
- * with 4 x threads >= the plane's de-aliased Fourier modes.  Results do not
- * depend on it.  dh_autotune times nsteps steps (best of two) with each
+ * (64..512 for NX = 64 planes of at most 2048 modes, whose nonlinear pass
+ * takes its x rows in registers) with 4 x threads >= the plane's
+ * de-aliased Fourier modes.  Results do not depend on it.  dh_autotune times nsteps steps (best of two) with each
  * power-of-two size from the current state, restores the state (Fourier
  * fields, CF, CP, scalars, clock and CN tables) and keeps the fastest,
  * returned in *chosen. */

@@ -4,7 +4,9 @@ dh_ensemble_autotune; needs an MI355X).
 The block size of the plane transforms and of the nonlinear pass's LDS
 variant only decides which thread computes which butterfly or mode, so
 results must be bit-identical for every valid size: on the LDS path
-(small_case, 16x17x8) and on the register path (the 128x129x128 channel).
+(small_case, 16x17x8), on the register path (the 128x129x128 channel) and
+on the NX = 64 register rows with the LES model (64x97x64, 256 and 512
+threads; such planes offer no larger size).
 dh_autotune must leave the state and clock exactly as it found them, and
 an autotuned ensemble must still equal its members stepped alone."""
 import numpy as np
@@ -37,7 +39,7 @@ def _pair(dor, name):
     return chs
 
 
-@pytest.mark.parametrize("name,other", [("small_case", 256), ("channel_re180", 512)])
+@pytest.mark.parametrize("name,other", [("small_case", 256), ("channel_re180", 512), ("les_channel", 512)])
 def test_block_size_does_not_change_results(dor, name, other):
     print(f"[autotune] block size {name}", flush=True)
     a, b = _pair(dor, name)
@@ -51,6 +53,9 @@ def test_block_size_does_not_change_results(dor, name, other):
         assert np.array_equal(sa[k], sb[k]), k
     with pytest.raises(channel.DiabloError):
         b.set_plane_threads(96 + 1)
+    if name == "les_channel":
+        with pytest.raises(channel.DiabloError):
+            b.set_plane_threads(1024)
     a.close()
     b.close()
 
