@@ -7,19 +7,25 @@
 
 namespace dh {
 
+// Fields per launch: the state (U1..3, TH) and the LES model's strain
+// slots fit one launch.
+constexpr int kBatchFields = 16;
+
 // Physical planes are double (FP64 transforms, C = double2) or float (the
 // mixed form, C = float2); the Fourier side is always FP64.
 struct PlaneBatch {
-  const double2 *four[kMaxFields];  // Fourier source/destination per field
-  void *phys[kMaxFields];           // physical destination/source per field (Sc<C>)
-  int j0;                           // first plane
+  const double2 *four[kBatchFields];  // Fourier source/destination per field
+  void *phys[kBatchFields];           // physical destination/source per field (Sc<C>)
+  int j0;                             // first plane
   // forward only: planes alo..ahi of field f come from alt[f] when set
-  const void *alt[kMaxFields];
-  int alo[kMaxFields], ahi[kMaxFields];
+  const void *alt[kBatchFields];
+  int alo[kBatchFields], ahi[kBatchFields];
   // forward only, the mixed form: when base[f] is set, planes alo..ahi of
   // alt[f] hold increments and the destination becomes base + transform;
   // the other planes are base itself (base may be the destination)
-  const double2 *base[kMaxFields];
+  const double2 *base[kBatchFields];
+  // inverse only: planes of field f below jlo[f] are left alone
+  int jlo[kBatchFields];
   // inverse only, the mixed form: transform the fluctuations (no plane mean)
   int drop_mean;
 };
@@ -34,6 +40,7 @@ struct PlaneArgs {
 template <bool REG, class C, class TT>
 __device__ __forceinline__ void plane_inv_body(const Geo &g, const PlaneBatch &b, C *A, const TT &T, int f,
                                                long long j) {
+  if (j < b.jlo[f]) return;
   plane_unpack(A, g, T, b.four[f] + j * g.pf, b.drop_mean != 0);
   plane_cols_inv(A, g, T);
   C *dst = reinterpret_cast<C *>(static_cast<Sc<C> *>(b.phys[f]) + j * g.pp);
@@ -139,6 +146,7 @@ __global__ void __launch_bounds__(1024) k_plane_fwd_b(const PlaneArgs *__restric
 // plane_fwd_body.
 __device__ __forceinline__ void plane_inv_gp_body(const Geo &g, const PlaneBatch &b, double2 *Q, const Tables &T,
                                                   double2 *buf, int stage, int f, long long j) {
+  if (j < b.jlo[f]) return;
   const double2 *src = b.four[f] ? b.four[f] + j * g.pf : nullptr;
   const bool drop_mean = b.drop_mean != 0;
   gp_cols_inv(Q, g, T, buf, stage, [&](int kx, int zr) {
@@ -251,8 +259,8 @@ static hipError_t plane_fwd(Ctx &c, dim3 grid, hipStream_t st, const PlaneBatch 
 
 hipError_t launch_planes_inv(Ctx &c, const double2 *four, double *phys, int nf, int j0, int nplanes, hipStream_t st) {
   const long long fs = (long long)(c.g.ny + 2) * c.g.pp, cfs = (long long)(c.g.ny + 2) * c.g.pf;
-  for (int f0 = 0; f0 < nf; f0 += kMaxFields) {
-    const int n = nf - f0 < kMaxFields ? nf - f0 : kMaxFields;
+  for (int f0 = 0; f0 < nf; f0 += kBatchFields) {
+    const int n = nf - f0 < kBatchFields ? nf - f0 : kBatchFields;
     PlaneBatch b{};
     for (int i = 0; i < n; ++i) {
       b.four[i] = four + (f0 + i) * cfs;
@@ -267,8 +275,8 @@ hipError_t launch_planes_inv(Ctx &c, const double2 *four, double *phys, int nf, 
 
 hipError_t launch_planes_fwd(Ctx &c, const double *phys, double2 *four, int nf, int j0, int nplanes, hipStream_t st) {
   const long long fs = (long long)(c.g.ny + 2) * c.g.pp, cfs = (long long)(c.g.ny + 2) * c.g.pf;
-  for (int f0 = 0; f0 < nf; f0 += kMaxFields) {
-    const int n = nf - f0 < kMaxFields ? nf - f0 : kMaxFields;
+  for (int f0 = 0; f0 < nf; f0 += kBatchFields) {
+    const int n = nf - f0 < kBatchFields ? nf - f0 : kBatchFields;
     PlaneBatch b{};
     for (int i = 0; i < n; ++i) {
       b.four[i] = four + (f0 + i) * cfs;
@@ -312,6 +320,33 @@ hipError_t launch_state_to_phys(Ctx &c) {
   b.drop_mean = c.mixed;  // the mixed form keeps the plane means in FP64
   const dim3 grid(c.g.ny + 2, c.nfields);
   return c.mixed ? plane_inv<float2>(c, grid, c.stream, b) : plane_inv<double2>(c, grid, c.stream, b);
+}
+
+// The LES substep's inverse transforms in one launch: the state fields as
+// launch_state_to_phys (FP64, U from the copy with les_chan's BCs), planes
+// 0..NY+1, and n more fields four[i] -> phys[i] (the model's strain and
+// gradients), planes 1..NY+1.  Two launches where they exceed a batch.
+hipError_t launch_state_and_fields_to_phys(Ctx &c, const double2 *const *four, double *const *phys, int n,
+                                           hipStream_t st) {
+  if (c.mixed || !c.les || c.nfields + n > kBatchFields) {
+    hipError_t e = launch_state_to_phys(c);
+    for (int i = 0; i < n && e == hipSuccess; ++i) e = launch_planes_inv(c, four[i], phys[i], 1, 1, c.g.ny + 1, st);
+    return e;
+  }
+  PlaneBatch b{};
+  const long long fs = (long long)(c.g.ny + 2) * c.g.pp;
+  const long long cfs = (long long)(c.g.ny + 2) * c.g.pf;
+  for (int f = 0; f < c.nfields; ++f) {
+    b.four[f] = (f < 3) ? c.cul + f * cfs : c.cth + (f - 3) * cfs;
+    b.phys[f] = c.uphys + f * fs;
+  }
+  for (int i = 0; i < n; ++i) {
+    b.four[c.nfields + i] = four[i];
+    b.phys[c.nfields + i] = phys[i];
+    b.jlo[c.nfields + i] = 1;
+  }
+  b.j0 = 0;
+  return plane_inv<double2>(c, dim3(c.g.ny + 2, c.nfields + n), st, b);
 }
 
 // FP64 fields f0..f1-1 (U1,U2,U3,TH...) back to Fourier space, planes

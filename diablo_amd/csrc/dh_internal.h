@@ -288,6 +288,9 @@ hipError_t launch_fft_to_phys(Ctx &c, const double2 *four, double *phys, int npl
 hipError_t launch_stats(Ctx &c, double *out_dev);  // profiles of dh_save_stats into out_dev
 hipError_t launch_fft_to_four(Ctx &c, const double *phys, double2 *four, int nplanes);
 hipError_t launch_state_to_phys(Ctx &c);
+// LES: the state (planes 0..NY+1) and n strain fields (planes 1..NY+1) in one launch
+hipError_t launch_state_and_fields_to_phys(Ctx &c, const double2 *const *four, double *const *phys, int n,
+                                           hipStream_t st);
 hipError_t launch_state_to_four(Ctx &c);
 hipError_t launch_fields_to_four(Ctx &c, int f0, int f1);
 hipError_t launch_step_to_four(Ctx &c, int f0, int f1, hipStream_t st);  // end of a substep (into CUN / CTH)
@@ -297,9 +300,9 @@ hipError_t launch_apply_bc_vel(Ctx &c);
 // goes to physical space; the model itself (strain, NU_T, subgrid
 // increments, physical BCs) after it; the per-column solve with NU_T
 hipError_t launch_les_bc_four(Ctx &c, hipStream_t st);
-hipError_t launch_les_model(Ctx &c, hipStream_t st);       // launch_les_strain, then launch_les_model_rest
-hipError_t launch_les_strain(Ctx &c, hipStream_t st);      // the strain and its inverse transforms
 hipError_t launch_les_model_rest(Ctx &c, hipStream_t st);  // NU_T, the products' transforms, the increments
+// k_les_strain, then the state's and the strain's inverse transforms in one launch
+hipError_t launch_les_strain_and_state(Ctx &c, hipStream_t st);
 hipError_t launch_les_features(Ctx &c, hipStream_t st);  // the closure's inputs only (no model step)
 hipError_t launch_mlp_pack(const float *params, void *packed, hipStream_t st);
 hipError_t launch_solve_les_range(Ctx &c, int rk, int courant, int f0, int f1, hipStream_t st);

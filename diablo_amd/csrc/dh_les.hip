@@ -438,27 +438,6 @@ static LesArgs les_args(Ctx &c) {
   return a;
 }
 
-// The model's first part: the strain and the scalars' gradients in Fourier
-// space and their inverse transforms.  It reads only the Fourier copy with
-// les_chan's BCs, as the substep's inverse transform of U does, so the two
-// can run side by side.
-hipError_t launch_les_strain(Ctx &c, hipStream_t st) {
-  const LesArgs a = les_args(c);
-  const int NY = c.g.ny, nth = c.p.n_th;
-  const unsigned mb = (unsigned)((c.g.pf + 255) / 256);
-  hipError_t e;
-  if ((e = klaunch(c, (const void *)k_les_strain, nullptr, dim3(mb, NY + 1), dim3(256), 0, st, a)) != hipSuccess)
-    return e;
-  // the strain (slots 0..5; 6: dU1/dz with the closure) and the scalars'
-  // gradients (7 + 2n, 8 + 2n), planes 1..NY+1
-  const long long fs = a.fs, cfs = a.cfs;
-  if ((e = launch_planes_inv(c, c.les_four, c.les_phys, c.les_mlp ? 7 : 6, 1, NY + 1, st)) != hipSuccess) return e;
-  if (nth > 0 && (e = launch_planes_inv(c, c.les_four + kLesFour * cfs, c.les_phys + kLesFour * fs, 2 * nth, 1,
-                                        NY + 1, st)) != hipSuccess)
-    return e;
-  return hipSuccess;
-}
-
 // The rest, once the strain and the physical U are both there: NU_T (or
 // the closure's), the products' forward transforms and the increments.
 hipError_t launch_les_model_rest(Ctx &c, hipStream_t st) {
@@ -487,9 +466,31 @@ hipError_t launch_les_model_rest(Ctx &c, hipStream_t st) {
   return klaunch(c, (const void *)k_les_cf, nullptr, dim3(mb > pb ? mb : pb, NY + 3), dim3(256), 0, st, a);
 }
 
-hipError_t launch_les_model(Ctx &c, hipStream_t st) {
-  const hipError_t e = launch_les_strain(c, st);
-  return e == hipSuccess ? launch_les_model_rest(c, st) : e;
+// The model's first part with the substep's inverse transform of the state:
+// the strain and the scalars' gradients in Fourier space (k_les_strain, from
+// the Fourier copy with les_chan's BCs), then one inverse transform launch
+// of the state fields (planes 0..NY+1, U from that copy), the strain's slots
+// (0..5; 6: dU1/dz with the closure) and the gradients (7 + 2n, 8 + 2n),
+// planes 1..NY+1 of the last two groups.
+hipError_t launch_les_strain_and_state(Ctx &c, hipStream_t st) {
+  const LesArgs a = les_args(c);
+  const int NY = c.g.ny, nth = c.p.n_th;
+  const unsigned mb = (unsigned)((c.g.pf + 255) / 256);
+  hipError_t e;
+  if ((e = klaunch(c, (const void *)k_les_strain, nullptr, dim3(mb, NY + 1), dim3(256), 0, st, a)) != hipSuccess)
+    return e;
+  const double2 *four[kLesFour + 2 * DH_MAX_TH];
+  double *phys[kLesFour + 2 * DH_MAX_TH];
+  int n = 0;
+  for (int s = 0; s < (c.les_mlp ? 7 : 6); ++s, ++n) {
+    four[n] = c.les_four + s * a.cfs;
+    phys[n] = c.les_phys + s * a.fs;
+  }
+  for (int s = 0; s < 2 * nth; ++s, ++n) {
+    four[n] = c.les_four + (kLesFour + s) * a.cfs;
+    phys[n] = c.les_phys + (kLesFour + s) * a.fs;
+  }
+  return launch_state_and_fields_to_phys(c, four, phys, n, st);
 }
 
 // The closure's inputs from the current physical U and the Fourier copy

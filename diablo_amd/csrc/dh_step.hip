@@ -2343,9 +2343,10 @@ hipError_t launch_phase(Ctx &c, int rk, int phase) {
   const int courant = (c.p.variable_dt && rk == 2) ? 1 : 0;
   switch (phase) {
     case 0: {
-      hipError_t e = c.les ? launch_les_bc_four(c, c.stream) : hipSuccess;
-      if (e == hipSuccess) e = launch_state_to_phys(c);
-      if (e == hipSuccess && c.les) e = launch_les_model(c, c.stream);
+      if (!c.les) return launch_state_to_phys(c);
+      hipError_t e = launch_les_bc_four(c, c.stream);
+      if (e == hipSuccess) e = launch_les_strain_and_state(c, c.stream);
+      if (e == hipSuccess) e = launch_les_model_rest(c, c.stream);
       return e;
     }
     case 1: return launch_nonlinear(c, rk);
@@ -2379,15 +2380,10 @@ hipError_t launch_substep(Ctx &c, int rk) {
 #define SCHK(expr) \
   if ((e = (expr)) != hipSuccess) return e;
   if (c.les) {
-    // the model's strain and its inverse transforms on the side stream,
-    // beside the inverse transform of U; both read the BC'd Fourier copy
+    // the model's strain, then its inverse transforms and U's in one
+    // launch (both read the BC'd Fourier copy)
     SCHK(launch_les_bc_four(c, c.stream));
-    SCHK(krecord(c, 2, c.stream));  // ev_fork
-    SCHK(kwait(c, c.side, 2));
-    SCHK(launch_les_strain(c, c.side));
-    SCHK(krecord(c, 3, c.side));  // ev_join
-    SCHK(launch_state_to_phys(c));
-    SCHK(kwait(c, c.stream, 3));
+    SCHK(launch_les_strain_and_state(c, c.stream));
     SCHK(launch_les_model_rest(c, c.stream));
   } else {
     SCHK(launch_state_to_phys(c));
