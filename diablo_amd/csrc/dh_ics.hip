@@ -107,87 +107,96 @@ struct BcArgs {
   double uc1min, vc1min, wc1min, uc1max, vc1max, wc1max;
 };
 
-// APPLY_BC_VEL_LOWER/UPPER in Fourier space, one thread per mode (RANKZ=0).
-__device__ void apply_bc_vel_mode(const BcArgs &a, int m) {
-  const Geo &g = a.g;
-  const long long pf = g.pf;
-  const int NY = g.ny;
-  const bool mean = (m == 0);
+// APPLY_BC_VEL_LOWER/UPPER in Fourier space for one mode (mean: the (0,0)
+// mode), on its edge planes through at(f, e): f = 0..2 for U1..U3, e = 0..5
+// for planes 0, 1, 2, NY-1, NY, NY+1 (distinct: NY >= 4), in the
+// reference's statement order.
+template <class At>
+__device__ __forceinline__ void bc_vel_edges(const BcArgs &a, bool mean, At at) {
+  const int NY = a.g.ny;
   const double2 zero = make_double2(0.0, 0.0);
-  double2 *U1 = a.cu1 + m, *U2 = a.cu2 + m, *U3 = a.cu3 + m;
-  auto at = [&](double2 *F, int j) -> double2 & { return F[pf * j]; };
   // lower wall
   if (a.ubmin == 0) {
-    at(U1, 0) = zero;
-    at(U1, 1) = zero;
-    if (mean) at(U1, 1) = at(U1, 0) = make_double2(a.uc1min, 0.0);
+    at(0, 0) = zero;
+    at(0, 1) = zero;
+    if (mean) at(0, 1) = at(0, 0) = make_double2(a.uc1min, 0.0);
   } else {
-    at(U1, 1) = at(U1, 2);
-    at(U1, 0) = at(U1, 1);
+    at(0, 1) = at(0, 2);
+    at(0, 0) = at(0, 1);
     if (mean) {
-      at(U1, 1) = make_double2(at(U1, 2).x - a.dy[2] * a.uc1min, at(U1, 2).y);
-      at(U1, 0) = make_double2(at(U1, 1).x - a.dy[1] * a.uc1min, at(U1, 1).y);
+      at(0, 1) = make_double2(at(0, 2).x - a.dy[2] * a.uc1min, at(0, 2).y);
+      at(0, 0) = make_double2(at(0, 1).x - a.dy[1] * a.uc1min, at(0, 1).y);
     }
   }
   if (a.wbmin == 0) {
-    at(U3, 0) = zero;
-    at(U3, 1) = zero;
-    if (mean) at(U3, 1) = at(U3, 0) = make_double2(a.wc1min, 0.0);
+    at(2, 0) = zero;
+    at(2, 1) = zero;
+    if (mean) at(2, 1) = at(2, 0) = make_double2(a.wc1min, 0.0);
   } else {
-    at(U3, 1) = at(U3, 2);
-    at(U3, 0) = at(U3, 1);
+    at(2, 1) = at(2, 2);
+    at(2, 0) = at(2, 1);
     if (mean) {
-      at(U3, 1) = make_double2(at(U3, 2).x - a.dy[2] * a.wc1min, at(U3, 2).y);
-      at(U3, 0) = make_double2(at(U3, 1).x - a.dy[1] * a.wc1min, at(U3, 1).y);
+      at(2, 1) = make_double2(at(2, 2).x - a.dy[2] * a.wc1min, at(2, 2).y);
+      at(2, 0) = make_double2(at(2, 1).x - a.dy[1] * a.wc1min, at(2, 1).y);
     }
   }
   if (a.vbmin == 0) {
-    const double2 v2 = at(U2, 2);
-    at(U2, 1) = make_double2(2.0 * a.vc1min - v2.x, -v2.y);
-    at(U2, 0) = at(U2, 1);
+    const double2 v2 = at(1, 2);
+    at(1, 1) = make_double2(2.0 * a.vc1min - v2.x, -v2.y);
+    at(1, 0) = at(1, 1);
   } else {
-    at(U2, 1) = at(U2, 2);
-    at(U2, 0) = at(U2, 1);
+    at(1, 1) = at(1, 2);
+    at(1, 0) = at(1, 1);
     if (mean) {
-      at(U2, 1) = make_double2(at(U2, 2).x - a.dyf[1] * a.vc1min, at(U2, 2).y);
-      at(U2, 0) = make_double2(at(U2, 1).x - a.dyf[1] * a.vc1min, at(U2, 1).y);
+      at(1, 1) = make_double2(at(1, 2).x - a.dyf[1] * a.vc1min, at(1, 2).y);
+      at(1, 0) = make_double2(at(1, 1).x - a.dyf[1] * a.vc1min, at(1, 1).y);
     }
   }
   // upper wall
   if (a.ubmax == 0) {
-    at(U1, NY) = zero;
-    at(U1, NY + 1) = zero;
-    if (mean) at(U1, NY) = at(U1, NY + 1) = make_double2(a.uc1max, 0.0);
+    at(0, 4) = zero;
+    at(0, 5) = zero;
+    if (mean) at(0, 4) = at(0, 5) = make_double2(a.uc1max, 0.0);
   } else {
-    at(U1, NY) = at(U1, NY - 1);
-    at(U1, NY + 1) = at(U1, NY);
+    at(0, 4) = at(0, 3);
+    at(0, 5) = at(0, 4);
     if (mean) {
-      at(U1, NY) = make_double2(at(U1, NY - 1).x + a.dy[NY] * a.uc1max, at(U1, NY - 1).y);
-      at(U1, NY + 1) = make_double2(at(U1, NY).x + a.dy[NY] * a.uc1max, at(U1, NY).y);
+      at(0, 4) = make_double2(at(0, 3).x + a.dy[NY] * a.uc1max, at(0, 3).y);
+      at(0, 5) = make_double2(at(0, 4).x + a.dy[NY] * a.uc1max, at(0, 4).y);
     }
   }
   if (a.wbmax == 0) {
-    at(U3, NY) = zero;
-    at(U3, NY + 1) = zero;
+    at(2, 4) = zero;
+    at(2, 5) = zero;
     if (mean) {
-      at(U3, NY) = make_double2(a.wc1max, 0.0);
-      at(U3, NY + 1) = zero;  // channel.f:2445 overrides the ghost value
+      at(2, 4) = make_double2(a.wc1max, 0.0);
+      at(2, 5) = zero;  // channel.f:2445 overrides the ghost value
     }
   } else {
-    at(U3, NY) = at(U3, NY - 1);
-    at(U3, NY + 1) = at(U3, NY);
+    at(2, 4) = at(2, 3);
+    at(2, 5) = at(2, 4);
     if (mean) {
-      at(U3, NY) = make_double2(at(U3, NY - 1).x + a.dy[NY] * a.wc1max, at(U3, NY - 1).y);
-      at(U3, NY + 1) = make_double2(at(U3, NY).x + a.dy[NY] * a.wc1max, at(U3, NY).y);
+      at(2, 4) = make_double2(at(2, 3).x + a.dy[NY] * a.wc1max, at(2, 3).y);
+      at(2, 5) = make_double2(at(2, 4).x + a.dy[NY] * a.wc1max, at(2, 4).y);
     }
   }
   if (a.vbmax == 0) {
-    if (mean) at(U2, NY + 1) = make_double2(2.0 * a.vc1max - at(U2, NY).x, -at(U2, NY).y);
+    if (mean) at(1, 5) = make_double2(2.0 * a.vc1max - at(1, 4).x, -at(1, 4).y);
   } else {
-    at(U2, NY) = at(U2, NY - 1);
-    at(U2, NY + 1) = at(U2, NY);
-    if (mean) at(U2, NY + 1) = make_double2(at(U2, NY).x + a.dy[NY] * a.vc1max, at(U2, NY).y);
+    at(1, 4) = at(1, 3);
+    at(1, 5) = at(1, 4);
+    if (mean) at(1, 5) = make_double2(at(1, 4).x + a.dy[NY] * a.vc1max, at(1, 4).y);
   }
+}
+
+// edge index e of bc_vel_edges -> plane
+__device__ __forceinline__ int edge_plane(int NY, int e) { return e < 3 ? e : NY - 4 + e; }
+
+// APPLY_BC_VEL_LOWER/UPPER in Fourier space, one thread per mode (RANKZ=0).
+__device__ void apply_bc_vel_mode(const BcArgs &a, int m) {
+  const long long pf = a.g.pf;
+  double2 *F[3] = {a.cu1 + m, a.cu2 + m, a.cu3 + m};
+  bc_vel_edges(a, m == 0, [&](int f, int e) -> double2 & { return F[f][pf * edge_plane(a.g.ny, e)]; });
 }
 
 __global__ void k_apply_bc_vel(BcArgs a) {
@@ -211,10 +220,11 @@ struct LesBcArgs {
   int cnf;
 };
 // Mode m of plane j = blockIdx.y: planes 3..NY-2 are copied by their own
-// threads; the thread of plane 0 copies the planes the BCs read or write
-// (0..2, NY-1..NY+1) and applies the BCs to them, so no other thread
-// writes those planes after it (planes 2 and NY-1, which the BCs only
-// read, are also copied by their own threads, with the same values).
+// threads; the thread of plane 0 loads the planes the BCs read or write
+// (0..2, NY-1..NY+1), applies the BCs to them in registers and stores
+// them, so no other thread writes those planes after it (planes 2 and
+// NY-1, which the BCs only read, are also copied by their own threads, with
+// the same values).
 __global__ void k_les_bc_four(LesBcArgs la) {
   const BcArgs &a = la.b;
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
@@ -227,30 +237,36 @@ __global__ void k_les_bc_four(LesBcArgs la) {
       for (int f = 0; f < 3; ++f) dst[f][pf * j + m] = la.cu[f][pf * j + m];
     return;
   }
-  const int edge[6] = {0, 1, 2, NYp - 1, NYp, NYp + 1};
+  double2 r[3][6];  // edge planes 0, 1, 2, NY-1, NY, NY+1 of U1..U3
+#pragma unroll
   for (int f = 0; f < 3; ++f)
-    for (int q = 0; q < 6; ++q) dst[f][pf * edge[q] + m] = la.cu[f][pf * edge[q] + m];
-  if (la.cnf && la.clk->time_step <= 100) return;
-  apply_bc_vel_mode(a, m);
-  const int NY = a.g.ny;
-  double2 *U1 = a.cu1 + m, *U3 = a.cu3 + m;
-  auto at = [&](double2 *F, int j) -> double2 & { return F[pf * j]; };
-  if (a.ubmax == 1) {
-    at(U1, NY) = at(U1, NY - 1);
-    at(U1, NY + 1) = at(U1, NY);
+#pragma unroll
+    for (int e = 0; e < 6; ++e) r[f][e] = la.cu[f][pf * edge_plane(NYp, e) + m];
+  if (!(la.cnf && la.clk->time_step <= 100)) {
+    auto at = [&](int f, int e) -> double2 & { return r[f][e]; };
+    bc_vel_edges(a, m == 0, at);
+    // APPLY_BC_LES: Neumann walls copy the first interior plane outwards
+    if (a.ubmax == 1) {
+      at(0, 4) = at(0, 3);
+      at(0, 5) = at(0, 4);
+    }
+    if (a.wbmax == 1) {
+      at(2, 4) = at(2, 3);
+      at(2, 5) = at(2, 4);
+    }
+    if (a.ubmin == 1) {
+      at(0, 1) = at(0, 2);
+      at(0, 0) = at(0, 1);
+    }
+    if (a.wbmin == 1) {
+      at(2, 1) = at(2, 2);
+      at(2, 0) = at(2, 1);
+    }
   }
-  if (a.wbmax == 1) {
-    at(U3, NY) = at(U3, NY - 1);
-    at(U3, NY + 1) = at(U3, NY);
-  }
-  if (a.ubmin == 1) {
-    at(U1, 1) = at(U1, 2);
-    at(U1, 0) = at(U1, 1);
-  }
-  if (a.wbmin == 1) {
-    at(U3, 1) = at(U3, 2);
-    at(U3, 0) = at(U3, 1);
-  }
+#pragma unroll
+  for (int f = 0; f < 3; ++f)
+#pragma unroll
+    for (int e = 0; e < 6; ++e) dst[f][pf * edge_plane(NYp, e) + m] = r[f][e];
 }
 
 __global__ void k_fill_zero(double2 *p, long long n) {
