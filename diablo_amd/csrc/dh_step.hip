@@ -859,12 +859,12 @@ __host__ __device__ inline void real_row(const FactorIn &in, int f, double temp1
 //   PB_J = prod_{i=J..b-1} -TC_i/TB_i  (b-1: last row of J's segment)
 // The constant right-hand sides of the boundary rows and the rows'
 // coefficients (dt independent) are written by the rk=0 blocks.
-__device__ __forceinline__ void cn_tables_body(const Clock *clk, const FactorIn &in, const RealFactors &fac) {
+__device__ __forceinline__ void cn_tables_body(const Clock *clk, const FactorIn &in, const RealFactors &fac, int blk) {
 #pragma clang fp contract(off)
   extern __shared__ double sm[];
   const int nrow = in.ny + 2;
   double *a = sm, *b = sm + nrow, *c = sm + 2 * nrow;
-  const int rk = blockIdx.x / in.nfields, f = blockIdx.x - rk * in.nfields;
+  const int rk = blk / in.nfields, f = blk - rk * in.nfields;
   const double temp1 = in.nu * clk->h_bar[rk] / 2.0;
   for (int i = threadIdx.x; i < nrow; i += blockDim.x) {
     double L, D, U, V;
@@ -917,10 +917,12 @@ __device__ __forceinline__ void cn_tables_body(const Clock *clk, const FactorIn 
   __syncthreads();
   if (threadIdx.x == 0) {
     double pbv = 0.0;
+    int js = (nrow - 1) % sr;  // J % sr, counted down with J
     for (int J = nrow - 1; J >= 0; --J) {
       const double fj = tr[J];
-      pbv = (J % sr == sr - 1 || J == nrow - 1) ? fj : fj * pbv;
+      pbv = (js == sr - 1 || J == nrow - 1) ? fj : fj * pbv;
       pb[J] = pbv;
+      js = (js == 0) ? sr - 1 : js - 1;
     }
   }
   __syncthreads();
@@ -939,10 +941,10 @@ struct CnArgs {
   FactorIn in;
   RealFactors fac;
 };
-__global__ void __launch_bounds__(256) k_cn_tables(CnArgs a) { cn_tables_body(a.clk, a.in, a.fac); }
+__global__ void __launch_bounds__(256) k_cn_tables(CnArgs a) { cn_tables_body(a.clk, a.in, a.fac, blockIdx.x); }
 __global__ void __launch_bounds__(256) k_cn_tables_b(const CnArgs *__restrict__ av) {
   const CnArgs &a = av[blockIdx.z];
-  cn_tables_body(a.clk, a.in, a.fac);
+  cn_tables_body(a.clk, a.in, a.fac, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1354,7 +1356,8 @@ constexpr int kRdtPad = 16;  // >= 4 * kRdtAhead - 1: the sweeps read up to 4 kR
 // kRdtRegRows ngrp); rows beyond are read again for the projection.
 constexpr int kRdtRegRows = 5;
 
-__device__ __forceinline__ void rem_div_thomas_body(const RemDivArgs &a) {
+// nb blocks of modes, this one lin
+__device__ __forceinline__ void rem_div_thomas_body(const RemDivArgs &a, int nb, int lin) {
 #pragma clang fp contract(off)
   extern __shared__ double st[];
   const Geo &g = a.g;
@@ -1372,7 +1375,6 @@ __device__ __forceinline__ void rem_div_thomas_body(const RemDivArgs &a) {
   // range: neighbouring groups share the cache lines their rows straddle
   int grp;
   {
-    const int nb = gridDim.x, lin = blockIdx.x;
     const int x = lin & 7, q = nb >> 3, r = nb & 7;
     grp = x * q + min(x, r) + (lin >> 3);
   }
@@ -1582,9 +1584,20 @@ __device__ __forceinline__ void rem_div_thomas_body(const RemDivArgs &a) {
 // Two blocks per CU (the velocity rows held across the sweep take the
 // registers of a third): 512 resident blocks cover the 457 of a 128 x 128
 // plane's modes at once.
-__global__ void __launch_bounds__(256) k_rem_div_thomas(RemDivArgs a) { rem_div_thomas_body(a); }
+__global__ void __launch_bounds__(256) k_rem_div_thomas(RemDivArgs a) { rem_div_thomas_body(a, gridDim.x, blockIdx.x); }
 __global__ void __launch_bounds__(256) k_rem_div_thomas_b(const RemDivArgs *__restrict__ av) {
-  rem_div_thomas_body(av[blockIdx.z]);
+  rem_div_thomas_body(av[blockIdx.z], gridDim.x, blockIdx.x);
+}
+// REM_DIV with the CN tables of a new DELTA_T in the grid's last ncn blocks
+// (the substep after COURANT: the tables serve the next substep's solve,
+// and REM_DIV reads none of what they write).
+__global__ void __launch_bounds__(256) k_rem_div_thomas_cn(RemDivArgs a, CnArgs cn, int ncn) {
+  const int nrd = (int)gridDim.x - ncn;
+  if ((int)blockIdx.x >= nrd) {
+    cn_tables_body(cn.clk, cn.in, cn.fac, (int)blockIdx.x - nrd);
+    return;
+  }
+  rem_div_thomas_body(a, nrd, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -2320,12 +2333,28 @@ hipError_t launch_rem_div(Ctx &c, int rk) {
                  (unsigned)rem_div_thomas_lds(c), c.stream, a);
 }
 
+// REM_DIV and the CN tables for the DELTA_T COURANT just set, in one launch
+// (not in recording mode, which has no batched form of it).
+static hipError_t launch_rem_div_cn(Ctx &c, int rk) {
+  if (c.recording) return hipErrorNotSupported;
+  RemDivArgs a = make_rem_div_args(c, rk);
+  const CnArgs cn{c.clock, make_factor_in(c), c.fac};
+  const int nb = (c.g.pf + kRdtModes - 1) / kRdtModes, ncn = 3 * c.nfields;
+  const size_t lrd = rem_div_thomas_lds(c), lcn = (size_t)8 * (c.g.ny + 2) * sizeof(double);
+  const size_t lds = lrd > lcn ? lrd : lcn;
+  hipLaunchKernelGGL(k_rem_div_thomas_cn, dim3(nb + ncn), dim3(256), (unsigned)lds, c.stream, a, cn, ncn);
+  return hipGetLastError();
+}
+
 // COURANT, then the solve tables for the new DELTA_T on the side stream: the
 // next solve is a substep away, so the rebuild runs beside this substep's
 // transform and pressure solve and is joined at the end of the substep.
-static hipError_t launch_courant(Ctx &c) {
+static hipError_t launch_courant_dt(Ctx &c) {
   const CourantArgs ca{c.clock, c.dt_cap, c.p.variable_dt, c.p.update_dt};
-  hipError_t e = klaunch(c, (const void *)k_courant_dt, (const void *)k_courant_dt_b, dim3(1), dim3(64), 0, c.stream, ca);
+  return klaunch(c, (const void *)k_courant_dt, (const void *)k_courant_dt_b, dim3(1), dim3(64), 0, c.stream, ca);
+}
+static hipError_t launch_courant(Ctx &c) {
+  hipError_t e = launch_courant_dt(c);
   if (e != hipSuccess) return e;
   if ((e = krecord(c, 2, c.stream)) != hipSuccess) return e;  // ev_fork
   if ((e = kwait(c, c.side, 2)) != hipSuccess) return e;
@@ -2375,6 +2404,10 @@ hipError_t join_side(Ctx &c) {
 // joined.  The kernels and their arithmetic are those of launch_phase.
 hipError_t launch_substep(Ctx &c, int rk) {
   const int courant = (c.p.variable_dt && rk == 2) ? 1 : 0;
+  // after COURANT the CN tables of the new DELTA_T are built in REM_DIV's
+  // launch (a fork to a side stream and its join cost more on the device
+  // than the tables take)
+  const bool fuse_tables = courant && !c.recording;
   const int nv = c.nfields < 3 ? c.nfields : 3;
   hipError_t e;
 #define SCHK(expr) \
@@ -2401,9 +2434,9 @@ hipError_t launch_substep(Ctx &c, int rk) {
     SCHK(launch_nonlinear_range(c, rk, 0, c.nfields, c.stream));
     SCHK(c.les ? launch_solve_les_range(c, rk, courant, 0, c.nfields, c.stream)
                : launch_solve_range(c, rk, courant, 0, c.nfields, c.stream));
-    if (courant) SCHK(launch_courant(c));
+    if (courant) SCHK(fuse_tables ? launch_courant_dt(c) : launch_courant(c));
     SCHK(launch_step_to_four(c, 0, c.nfields, c.stream));
-    SCHK(launch_rem_div(c, rk));
+    SCHK(fuse_tables ? launch_rem_div_cn(c, rk) : launch_rem_div(c, rk));
     return join_side(c);
   }
   if (c.mixed) {
@@ -2423,9 +2456,9 @@ hipError_t launch_substep(Ctx &c, int rk) {
   SCHK(c.les ? launch_solve_les_range(c, rk, courant, 0, 2, c.stream) : launch_solve_range(c, rk, courant, 0, 2, c.stream));
   SCHK(launch_step_to_four(c, 0, 2, c.stream));
   SCHK(kwait(c, c.stream, 1));
-  if (courant) SCHK(launch_courant(c));
+  if (courant) SCHK(fuse_tables ? launch_courant_dt(c) : launch_courant(c));
   SCHK(launch_step_to_four(c, 3, c.nfields, c.stream));
-  SCHK(launch_rem_div(c, rk));
+  SCHK(fuse_tables ? launch_rem_div_cn(c, rk) : launch_rem_div(c, rk));
 #undef SCHK
   return join_side(c);
 }
@@ -2462,7 +2495,8 @@ hipError_t prepare_step_kernels(Ctx &c) {
                        (const void *)k_nonlinear32<true, kNlRegQ>,  (const void *)k_nonlinear32<false, kMaxQ>,
                        (const void *)k_nonlinear_b<true, kNlRegQ>,  (const void *)k_nonlinear_b<false, kMaxQ>,
                        (const void *)k_nonlinear32_b<true, kNlRegQ>, (const void *)k_nonlinear32_b<false, kMaxQ>,
-                       (const void *)k_rem_div_thomas,              (const void *)k_rem_div_thomas_b};
+                       (const void *)k_rem_div_thomas,              (const void *)k_rem_div_thomas_b,
+                       (const void *)k_rem_div_thomas_cn};
   for (const void *fn : fns) {
     const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsMax);
     if (e != hipSuccess) return e;
