@@ -1750,22 +1750,15 @@ static hipError_t launch_nonlinear_range(Ctx &c, int rk, int c0, int c1, hipStre
   return klaunch(c, k.fn, k.fnb, grid, dim3(k.threads), k.lds, st, a);
 }
 
-// Whether every block of the all-field nonlinear launch is resident on the
-// GPU at once (small planes, several blocks per CU).  Splitting the pass
-// over two streams then frees no compute units for the solve and costs a
-// fork and a join.
-static bool nonlinear_fits_at_once(const Ctx &c) {
-  if (c.gp) return false;
-  int jlo, jhi;
-  nonlinear_rows(c, 0, c.nfields, jlo, jhi);
-  if (jhi < jlo) return true;
-  const NlKernel k = nonlinear_kernel(c);
-  int per_cu = 0, dev = 0, cus = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k.fn, k.threads, k.lds) != hipSuccess) return false;
-  if (hipGetDevice(&dev) != hipSuccess) return false;
-  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return false;
-  return (long long)(jhi - jlo + 1) * c.nfields <= (long long)per_cu * cus;
-}
+// Whether the substep runs flat: each phase one launch over all fields on
+// the context stream.  Planes in LDS do: with U3's nonlinear pass, solve
+// and transform on a branch beside U1's and U2's, the branch's solve and
+// transform wait for its pass, and the fork and join cost 6-10 us each
+// (config 1: 0.638 ms per step flat against 0.657 with the branch).  The
+// large-plane kernels keep the branch (256 x 193 x 256: 7.2 ms per step
+// against 7.55 flat).
+static bool flat_substep(const Ctx &c) { return !c.gp; }
+
 
 static hipError_t launch_nonlinear(Ctx &c, int rk) { return launch_nonlinear_range(c, rk, 0, c.nfields, c.stream); }
 
@@ -2393,15 +2386,14 @@ hipError_t join_side(Ctx &c) {
   return kwait(c, c.stream, 3);  // ev_join
 }
 
-// One substep.  Where every block of the nonlinear pass is resident at once
-// (small planes), each phase is one launch over all fields on the context
-// stream.  Otherwise the nonlinear pass of U1 and U2 fills one wave of
-// blocks (one per plane and component); the last velocity component's
-// nonlinear pass, solve and transform back (then the scalars' nonlinear pass
-// and solve) run as a branch beside the solve and transform of U1 and U2,
-// which use the compute units that pass leaves idle.  The scalars' transform
-// back, COURANT and REM_DIV follow on the context stream once the branch has
-// joined.  The kernels and their arithmetic are those of launch_phase.
+// One substep.  With planes in LDS each phase is one launch over all fields
+// on the context stream (flat_substep).  On the large-plane path the
+// nonlinear pass of U1 and U2 runs on the context stream and the last
+// velocity component's nonlinear pass, solve and transform back (then the
+// scalars' nonlinear pass and solve) as a branch beside it and U1's and
+// U2's solve and transform; the scalars' transform back, COURANT and
+// REM_DIV follow on the context stream once the branch has joined.  The
+// kernels and their arithmetic are those of launch_phase.
 hipError_t launch_substep(Ctx &c, int rk) {
   const int courant = (c.p.variable_dt && rk == 2) ? 1 : 0;
   // after COURANT the CN tables of the new DELTA_T are built in REM_DIV's
@@ -2421,16 +2413,9 @@ hipError_t launch_substep(Ctx &c, int rk) {
   } else {
     SCHK(launch_state_to_phys(c));
   }
-  // the mixed form's nonlinear blocks fit two to a CU: all components in
-  // one launch, and the branch forks after it.  Otherwise U3 and the scalars
-  // run on the branch, which forks as soon as the fields are in physical
-  // space (all its nonlinear pass reads): where the planes are small enough
-  // for several blocks per CU (the LES decks) the two passes run together,
-  // and where a block takes a CU (the 128 x 128 planes) the branch's blocks
-  // take the CUs that U1's and U2's free.
-  if (nonlinear_fits_at_once(c)) {
-    // one launch per phase, in launch_phase's order; the solve tables for
-    // a new DELTA_T build beside the transform back and REM_DIV
+  if (flat_substep(c)) {
+    // one launch per phase, in launch_phase's order (the solve tables of a
+    // new DELTA_T in REM_DIV's launch)
     SCHK(launch_nonlinear_range(c, rk, 0, c.nfields, c.stream));
     SCHK(c.les ? launch_solve_les_range(c, rk, courant, 0, c.nfields, c.stream)
                : launch_solve_range(c, rk, courant, 0, c.nfields, c.stream));
@@ -2439,21 +2424,17 @@ hipError_t launch_substep(Ctx &c, int rk) {
     SCHK(fuse_tables ? launch_rem_div_cn(c, rk) : launch_rem_div(c, rk));
     return join_side(c);
   }
-  if (c.mixed) {
-    SCHK(launch_nonlinear_range(c, rk, 0, c.nfields, c.stream));
-    SCHK(krecord(c, 0, c.stream));  // ev_nl01
-    SCHK(kwait(c, c.branch, 0));
-  } else {
-    SCHK(krecord(c, 0, c.stream));  // ev_nl01
-    SCHK(kwait(c, c.branch, 0));
-    SCHK(launch_nonlinear_range(c, rk, 0, 2, c.stream));
-    SCHK(launch_nonlinear_range(c, rk, 2, c.nfields, c.branch));
-  }
-  SCHK(c.les ? launch_solve_les_range(c, rk, courant, 2, c.nfields, c.branch)
-             : launch_solve_range(c, rk, courant, 2, c.nfields, c.branch));
+  // planes beyond LDS (FP64, no LES model): U3 and the scalars run on the
+  // branch, which forks as soon as the fields are in physical space (all
+  // its nonlinear pass reads), each pass on its own scratch half
+  SCHK(krecord(c, 0, c.stream));  // ev_nl01
+  SCHK(kwait(c, c.branch, 0));
+  SCHK(launch_nonlinear_range(c, rk, 0, 2, c.stream));
+  SCHK(launch_nonlinear_range(c, rk, 2, c.nfields, c.branch));
+  SCHK(launch_solve_range(c, rk, courant, 2, c.nfields, c.branch));
   SCHK(launch_step_to_four(c, 2, nv, c.branch));
   SCHK(krecord(c, 1, c.branch));  // ev_branch_done
-  SCHK(c.les ? launch_solve_les_range(c, rk, courant, 0, 2, c.stream) : launch_solve_range(c, rk, courant, 0, 2, c.stream));
+  SCHK(launch_solve_range(c, rk, courant, 0, 2, c.stream));
   SCHK(launch_step_to_four(c, 0, 2, c.stream));
   SCHK(kwait(c, c.stream, 1));
   if (courant) SCHK(fuse_tables ? launch_courant_dt(c) : launch_courant(c));
