@@ -924,8 +924,10 @@ int dh_autotune(dh_ctx *h, int32_t nsteps, int32_t *chosen) {
 static int enqueue_steps(Ctx &c, int nsteps) {
   HIPCHK(launch_step_boundary(c, false, true));
   for (int s = 0; s < nsteps; ++s) {
-    for (int rk = 0; rk < 3; ++rk) HIPCHK(launch_substep(c, rk));
-    HIPCHK(launch_step_boundary(c, true, s + 1 < nsteps));
+    // the boundary into the next step (or the end of the last) rides in the
+    // last substep's REM_DIV launch
+    for (int rk = 0; rk < 2; ++rk) HIPCHK(launch_substep(c, rk));
+    HIPCHK(launch_substep(c, 2, 1, s + 1 < nsteps ? 1 : 0));
   }
   return DH_OK;
 }

@@ -314,11 +314,13 @@ hipError_t launch_create_flow(Ctx &c, int ic_type, double ubulk0, unsigned long 
 hipError_t launch_create_th(Ctx &c, int ic_type);
 // ---- launchers (dh_step.hip) ----
 hipError_t prepare_step_kernels(Ctx &c);
-hipError_t launch_substep(Ctx &c, int rk);            // rk 0..2
+// rk 0..2; bnd_end/bnd_begin: the step boundary after it (TIME += DELTA_T,
+// TIME_STEP += 1), done within REM_DIV's launch
+hipError_t launch_substep(Ctx &c, int rk, int bnd_end = 0, int bnd_begin = 0);
 constexpr int kNumPhases = 6;
 extern const char *const kPhaseNames[kNumPhases];
 hipError_t launch_phase(Ctx &c, int rk, int phase);
-hipError_t launch_rem_div(Ctx &c, int rk_for_cp);     // rk_for_cp<0: no CP update
+hipError_t launch_rem_div(Ctx &c, int rk_for_cp, int bnd_end = 0, int bnd_begin = 0);  // rk_for_cp<0: no CP update
 hipError_t launch_step_boundary(Ctx &c, bool end_prev, bool begin_next);  // TIME += DELTA_T, TIME_STEP += 1
 hipError_t launch_cn_tables(Ctx &c, hipStream_t st);
 hipError_t join_side(Ctx &c);  // main stream waits for a pending table rebuild

@@ -1201,6 +1201,10 @@ struct RemDivArgs {
   const double *rl, *ru;  // 1/(DY(J)DYF(J)), 1/(DY(J+1)DYF(J))
   double *ta, *tb, *tr;   // sweep multipliers, pivots, 1/pivots [J][mode]
   int seg;                // rows per sweep segment; 0: one sequential sweep
+  // the step boundary after a step's last substep (diablo.f:66-110): TIME
+  // += DELTA_T (bnd_end), TIME_STEP += 1 (bnd_begin), by one thread; no
+  // thread of the launch reads TIME or TIME_STEP
+  int bnd_end, bnd_begin;
 };
 
 // Pivots of the pressure-Poisson systems (channel.f:1112-1165 with the BC
@@ -1360,6 +1364,10 @@ constexpr int kRdtRegRows = 5;
 __device__ __forceinline__ void rem_div_thomas_body(const RemDivArgs &a, int nb, int lin) {
 #pragma clang fp contract(off)
   extern __shared__ double st[];
+  if (lin == 0 && threadIdx.x == 0) {
+    if (a.bnd_end) a.clk->time = a.clk->time + a.clk->dt;
+    if (a.bnd_begin) a.clk->time_step += 1;
+  }
   const Geo &g = a.g;
   const int NY = g.ny, ntot = NY + 1 + 2 * kRdtPad;
   // LDS, row J at index J + kRdtPad: G [row][mode] complex, A, B and 1/B
@@ -2319,8 +2327,10 @@ hipError_t launch_rem_div_factors(Ctx &c) {
   return hipGetLastError();
 }
 
-hipError_t launch_rem_div(Ctx &c, int rk) {
+hipError_t launch_rem_div(Ctx &c, int rk, int bnd_end, int bnd_begin) {
   RemDivArgs a = make_rem_div_args(c, rk);
+  a.bnd_end = bnd_end;
+  a.bnd_begin = bnd_begin;
   const int nb = (c.g.pf + kRdtModes - 1) / kRdtModes;
   return klaunch(c, (const void *)k_rem_div_thomas, (const void *)k_rem_div_thomas_b, dim3(nb), dim3(256),
                  (unsigned)rem_div_thomas_lds(c), c.stream, a);
@@ -2328,9 +2338,11 @@ hipError_t launch_rem_div(Ctx &c, int rk) {
 
 // REM_DIV and the CN tables for the DELTA_T COURANT just set, in one launch
 // (not in recording mode, which has no batched form of it).
-static hipError_t launch_rem_div_cn(Ctx &c, int rk) {
+static hipError_t launch_rem_div_cn(Ctx &c, int rk, int bnd_end, int bnd_begin) {
   if (c.recording) return hipErrorNotSupported;
   RemDivArgs a = make_rem_div_args(c, rk);
+  a.bnd_end = bnd_end;
+  a.bnd_begin = bnd_begin;
   const CnArgs cn{c.clock, make_factor_in(c), c.fac};
   const int nb = (c.g.pf + kRdtModes - 1) / kRdtModes, ncn = 3 * c.nfields;
   const size_t lrd = rem_div_thomas_lds(c), lcn = (size_t)8 * (c.g.ny + 2) * sizeof(double);
@@ -2394,7 +2406,7 @@ hipError_t join_side(Ctx &c) {
 // U2's solve and transform; the scalars' transform back, COURANT and
 // REM_DIV follow on the context stream once the branch has joined.  The
 // kernels and their arithmetic are those of launch_phase.
-hipError_t launch_substep(Ctx &c, int rk) {
+hipError_t launch_substep(Ctx &c, int rk, int bnd_end, int bnd_begin) {
   const int courant = (c.p.variable_dt && rk == 2) ? 1 : 0;
   // after COURANT the CN tables of the new DELTA_T are built in REM_DIV's
   // launch (a fork to a side stream and its join cost more on the device
@@ -2421,7 +2433,7 @@ hipError_t launch_substep(Ctx &c, int rk) {
                : launch_solve_range(c, rk, courant, 0, c.nfields, c.stream));
     if (courant) SCHK(fuse_tables ? launch_courant_dt(c) : launch_courant(c));
     SCHK(launch_step_to_four(c, 0, c.nfields, c.stream));
-    SCHK(fuse_tables ? launch_rem_div_cn(c, rk) : launch_rem_div(c, rk));
+    SCHK(fuse_tables ? launch_rem_div_cn(c, rk, bnd_end, bnd_begin) : launch_rem_div(c, rk, bnd_end, bnd_begin));
     return join_side(c);
   }
   // planes beyond LDS (FP64, no LES model): U3 and the scalars run on the
@@ -2439,7 +2451,7 @@ hipError_t launch_substep(Ctx &c, int rk) {
   SCHK(kwait(c, c.stream, 1));
   if (courant) SCHK(fuse_tables ? launch_courant_dt(c) : launch_courant(c));
   SCHK(launch_step_to_four(c, 3, c.nfields, c.stream));
-  SCHK(fuse_tables ? launch_rem_div_cn(c, rk) : launch_rem_div(c, rk));
+  SCHK(fuse_tables ? launch_rem_div_cn(c, rk, bnd_end, bnd_begin) : launch_rem_div(c, rk, bnd_end, bnd_begin));
 #undef SCHK
   return join_side(c);
 }
