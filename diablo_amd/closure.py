@@ -1,6 +1,6 @@
 """The learned eddy-viscosity closure: its network on the device, training
 samples from filtered DNS of this solver, training, and the LES runs that
-judge it against the constant-Smagorinsky model (no counterpart in the
+compare it with the constant-Smagorinsky model (no counterpart in the
 reference; the network stands in for NU_T = C_s^2 Delta^2 |S| of
 ``les.f:196-227``, and the strain it reads is ``compute_strain_chan``'s,
 ``les.f:526-569``).
@@ -633,7 +633,7 @@ def run_pipeline(out_dir, case="re180", spin=None, stat=None, every=20, nsnap=No
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--out", default="gpurun_out")
+    ap.add_argument("--out", default="closure_out")
     ap.add_argument("--case", default="re180", choices=sorted(CASES))
     ap.add_argument("--spin", type=int, default=None)
     ap.add_argument("--adjust", type=int, default=None)

@@ -2,8 +2,8 @@ C----*|--.---------.---------.---------.---------.---------.---------.-|-------|
 C Drop-in RK_CHAN_1 (replaces channel.f:62-1073) running the substep on the
 C GPU through libdiablo_hip.so.  Compiled in the reference's build tree with
 C its header; link this object ahead of channel.o (whose RK_CHAN_1 symbol is
-C weakened, see INTEGRATION.md) together with diablo_hip.o and
-C -ldiablo_hip.  The device keeps the state between substeps; it is copied
+C weakened: objcopy --weaken-symbol=rk_chan_1_) together with diablo_hip.o
+C and -ldiablo_hip.  The device keeps the state between substeps; it is copied
 C back after RK_STEP=3 whenever the driver is about to read or rewrite it
 C (SAVE_STATS / SAVE_FLOW / the last step / END_RUN, diablo.f:87-119), and
 C copied in again before the next RK_STEP=1.
