@@ -9,7 +9,10 @@ substeps of diablo.f:70-84 plus TIME+=DELTA_T.
 Prints one JSON line: Mpts*RK3-steps/s = NX*NY*NZ*steps/wall/1e6, the HBM
 roofline of the dominant kernel (algorithmic bytes / mean HIP-event
 duration), the roofline of the whole step against the algorithmic bytes
-per step, and the CPU baseline (the oracle, OpenMP on the host cores).
+per step, and the CPU baseline (the oracle, OpenMP on the host cores);
+beside it the mixed form, the LES deck, the learned-LES time to
+statistics, the batched solver and 512^2 transforms, and the 64-instance
+shear-layer ensemble (each can be left out with its --no-* flag).
 """
 import argparse
 import json
@@ -457,9 +460,10 @@ def main():
     ap.add_argument("--no-mixed", action="store_true", help="leave out the FP32/FP64 mixed form's line")
     ap.add_argument("--no-les", action="store_true", help="leave out the LES channel's line")
     ap.add_argument("--no-batched", action="store_true", help="leave out the batched solver/transform line")
-    ap.add_argument("--ensemble-line", action="store_true",
-                    help="add the 64-instance KH ensemble (FP64 and mixed) to the line; ensembles step member "
-                         "by member while batched ensemble stepping is switched off in the library")
+    ap.add_argument("--no-ensemble", action="store_true",
+                    help="leave out the 64-instance KH ensemble (FP64 and mixed); ensembles step member by "
+                         "member, each member's step graph on its own stream, while batched ensemble stepping "
+                         "is switched off in the library")
     ap.add_argument("--profile-reps", type=int, default=10)
     ap.add_argument("--ensemble", type=int, default=0, metavar="N",
                     help="only time N uneven shear-layer instances stepped as one graph")
@@ -546,7 +550,7 @@ def main():
         out["learned_les"] = learned_les(ms_step)
     if not args.no_batched:
         out["batched_microbench"] = batched_microbench()
-    if args.ensemble_line:
+    if not args.no_ensemble:
         ens = ensemble_run(64, 100, 10)
         ens["mixed_form"] = ensemble_run(64, 100, 10, mixed=True, one_at_a_time=False)
         out["kh_ensemble"] = ens
