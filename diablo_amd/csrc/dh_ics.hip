@@ -219,6 +219,28 @@ struct LesBcArgs {
   const Clock *clk;
   int cnf;
 };
+// APPLY_BC_LES (les.f:805-862) on the edge planes of bc_vel_edges: Neumann
+// walls copy the first interior plane outwards.
+template <class At>
+__device__ __forceinline__ void bc_les_edges(const BcArgs &a, At at) {
+  if (a.ubmax == 1) {
+    at(0, 4) = at(0, 3);
+    at(0, 5) = at(0, 4);
+  }
+  if (a.wbmax == 1) {
+    at(2, 4) = at(2, 3);
+    at(2, 5) = at(2, 4);
+  }
+  if (a.ubmin == 1) {
+    at(0, 1) = at(0, 2);
+    at(0, 0) = at(0, 1);
+  }
+  if (a.wbmin == 1) {
+    at(2, 1) = at(2, 2);
+    at(2, 0) = at(2, 1);
+  }
+}
+
 // Mode m of plane j = blockIdx.y: planes 3..NY-2 are copied by their own
 // threads; the thread of plane 0 loads the planes the BCs read or write
 // (0..2, NY-1..NY+1), applies the BCs to them in registers and stores
@@ -245,28 +267,102 @@ __global__ void k_les_bc_four(LesBcArgs la) {
   if (!(la.cnf && la.clk->time_step <= 100)) {
     auto at = [&](int f, int e) -> double2 & { return r[f][e]; };
     bc_vel_edges(a, m == 0, at);
-    // APPLY_BC_LES: Neumann walls copy the first interior plane outwards
-    if (a.ubmax == 1) {
-      at(0, 4) = at(0, 3);
-      at(0, 5) = at(0, 4);
-    }
-    if (a.wbmax == 1) {
-      at(2, 4) = at(2, 3);
-      at(2, 5) = at(2, 4);
-    }
-    if (a.ubmin == 1) {
-      at(0, 1) = at(0, 2);
-      at(0, 0) = at(0, 1);
-    }
-    if (a.wbmin == 1) {
-      at(2, 1) = at(2, 2);
-      at(2, 0) = at(2, 1);
-    }
+    bc_les_edges(a, at);
   }
 #pragma unroll
   for (int f = 0; f < 3; ++f)
 #pragma unroll
     for (int e = 0; e < 6; ++e) dst[f][pf * edge_plane(NYp, e) + m] = r[f][e];
+}
+
+// compute_strain_chan (les.f:536-556) of mode m at plane J from U1..U3 at
+// J-1, J, J+1 (u[f][d], plane J-1+d) with les_chan's BCs: S11, S22, S33,
+// S5 on J <= NY (zero at NY+1), S4, S6 on J = 1..NY+1, dU1/dz with the
+// closure; ikx TH and ikz TH per scalar (channel.f:328-358).
+__device__ __forceinline__ void les_strain_mode(const Geo &g, const LesStrainOut &so, int m, int J,
+                                                const double2 (&u)[3][3]) {
+#pragma clang fp contract(off)
+  const int kzp = m / g.nkx1, kx = m - kzp * g.nkx1;
+  const double KX = g.kx[kx], KZ = g.kz[kzp];
+  const long long o = (long long)J * g.pf + m;
+  auto F = [&](int s) -> double2 & { return so.four[s * so.cfs + o]; };
+  const double2 U1 = u[0][1], U2 = u[1][1], U3 = u[2][1], U1m = u[0][0], U3m = u[2][0];
+  const double2 zero = make_double2(0.0, 0.0);
+  if (J <= g.ny) {
+    const double2 U2p = u[1][2];
+    const double dyf = g.dyf[J];
+    F(0) = cmul_ik(KX, U1);
+    F(1) = make_double2((U2p.x - U2.x) / dyf, (U2p.y - U2.y) / dyf);
+    F(2) = cmul_ik(KZ, U3);
+    const double2 t = cadd(cmul_ik(KZ, U1), cmul_ik(KX, U3));
+    F(4) = make_double2(0.5 * t.x, 0.5 * t.y);
+    if (so.mlp) F(6) = cmul_ik(KZ, U1);  // dU1/dz, for the rotation the closure reads
+  } else {
+    F(0) = F(1) = F(2) = F(4) = zero;
+    if (so.mlp) F(6) = zero;
+  }
+  const double dy = g.dy[J];
+  const double2 ik2x = cmul_ik(KX, U2), ik2z = cmul_ik(KZ, U2);
+  F(3) = make_double2(0.5 * ((U1.x - U1m.x) / dy + ik2x.x), 0.5 * ((U1.y - U1m.y) / dy + ik2x.y));
+  F(5) = make_double2(0.5 * (ik2z.x + (U3.x - U3m.x) / dy), 0.5 * (ik2z.y + (U3.y - U3m.y) / dy));
+  for (int n = 0; n < so.nth; ++n) {
+    const double2 th = so.cth[n * so.cfs + o];
+    F(kLesFour + 2 * n) = cmul_ik(KX, th);
+    F(kLesFour + 2 * n + 1) = cmul_ik(KZ, th);
+  }
+}
+
+// k_les_bc_four and the strain in one launch: mode m of plane j =
+// blockIdx.y (0..NY+1) takes U1..U3 at j-1, j, j+1; within three planes of
+// a wall it also loads the mode's six edge planes and applies les_chan's
+// BCs to them in registers (the same values k_les_bc_four stores), then
+// writes plane j of the copy and, with the spin-up gate open and j >= 1,
+// the strain at j.  No thread reads what another writes.
+__global__ void k_les_bc_strain(LesBcArgs la, LesStrainOut so) {
+  const BcArgs &a = la.b;
+  const Geo &g = a.g;
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= g.pf) return;
+  const long long pf = g.pf;
+  const int j = blockIdx.y, NY = g.ny;
+  const bool open = !(la.cnf && la.clk->time_step <= 100);
+  const double2 zero = make_double2(0.0, 0.0);
+  double2 u[3][3];
+#pragma unroll
+  for (int f = 0; f < 3; ++f)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const int p = j - 1 + d;
+      double2 v = zero;
+      if (p >= 0 && p <= NY + 1) v = la.cu[f][pf * p + m];
+      u[f][d].x = v.x;  // component-wise: a select of double2 values goes through scratch
+      u[f][d].y = v.y;
+    }
+  if (open && (j <= 3 || j >= NY - 2)) {
+    double2 r[3][6];
+#pragma unroll
+    for (int f = 0; f < 3; ++f)
+#pragma unroll
+      for (int e = 0; e < 6; ++e) r[f][e] = la.cu[f][pf * edge_plane(NY, e) + m];
+    auto at = [&](int f, int e) -> double2 & { return r[f][e]; };
+    bc_vel_edges(a, m == 0, at);
+    bc_les_edges(a, at);
+#pragma unroll
+    for (int e = 0; e < 6; ++e)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const bool hit = j - 1 + d == edge_plane(NY, e);
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+          u[f][d].x = hit ? r[f][e].x : u[f][d].x;
+          u[f][d].y = hit ? r[f][e].y : u[f][d].y;
+        }
+      }
+  }
+  double2 *dst[3] = {a.cu1, a.cu2, a.cu3};
+#pragma unroll
+  for (int f = 0; f < 3; ++f) dst[f][pf * j + m] = u[f][1];
+  if (open && j >= 1) les_strain_mode(g, so, m, j, u);
 }
 
 __global__ void k_fill_zero(double2 *p, long long n) {
@@ -321,6 +417,16 @@ hipError_t launch_les_bc_four(Ctx &c, hipStream_t st) {
   la.b.cu3 = c.cul + 2 * cfs;
   return klaunch(c, (const void *)k_les_bc_four, nullptr, dim3((c.g.pf + 255) / 256, c.g.ny + 2), dim3(256), 0, st,
                  la);
+}
+
+hipError_t launch_les_bc_strain(Ctx &c, hipStream_t st, const LesStrainOut &so) {
+  LesBcArgs la{bc_args(c), {c.cu[0], c.cu[1], c.cu[2]}, c.clock, c.p.create_new_flow};
+  const long long cfs = (long long)(c.g.ny + 2) * c.g.pf;
+  la.b.cu1 = c.cul;
+  la.b.cu2 = c.cul + cfs;
+  la.b.cu3 = c.cul + 2 * cfs;
+  hipLaunchKernelGGL(k_les_bc_strain, dim3((c.g.pf + 255) / 256, c.g.ny + 2), dim3(256), 0, st, la, so);
+  return hipGetLastError();
 }
 
 hipError_t launch_create_flow(Ctx &c, int ic_type, double ubulk0, unsigned long long seed, double amp, double ymask) {

@@ -303,6 +303,15 @@ hipError_t launch_les_bc_four(Ctx &c, hipStream_t st);
 hipError_t launch_les_model_rest(Ctx &c, hipStream_t st);  // NU_T, the products' transforms, the increments
 // k_les_strain, then the state's and the strain's inverse transforms in one launch
 hipError_t launch_les_strain_and_state(Ctx &c, hipStream_t st);
+// where k_les_bc_strain writes the strain and the scalars' gradients
+struct LesStrainOut {
+  double2 *four;        // slots [s][NY+2][pf], stride cfs
+  const double2 *cth;   // the scalars' Fourier fields
+  long long cfs;
+  int nth, mlp;
+};
+// les_chan's Fourier BCs onto the copy of CU and compute_strain_chan from it, one launch
+hipError_t launch_les_bc_strain(Ctx &c, hipStream_t st, const LesStrainOut &so);
 hipError_t launch_les_features(Ctx &c, hipStream_t st);  // the closure's inputs only (no model step)
 hipError_t launch_mlp_pack(const float *params, void *packed, hipStream_t st);
 hipError_t launch_solve_les_range(Ctx &c, int rk, int courant, int f0, int f1, hipStream_t st);

@@ -467,18 +467,18 @@ hipError_t launch_les_model_rest(Ctx &c, hipStream_t st) {
 }
 
 // The model's first part with the substep's inverse transform of the state:
-// the strain and the scalars' gradients in Fourier space (k_les_strain, from
-// the Fourier copy with les_chan's BCs), then one inverse transform launch
-// of the state fields (planes 0..NY+1, U from that copy), the strain's slots
-// (0..5; 6: dU1/dz with the closure) and the gradients (7 + 2n, 8 + 2n),
-// planes 1..NY+1 of the last two groups.
+// les_chan's Fourier BCs onto the copy of CU and, from that copy, the strain
+// and the scalars' gradients in Fourier space (k_les_bc_strain), then one
+// inverse transform launch of the state fields (planes 0..NY+1, U from the
+// copy), the strain's slots (0..5; 6: dU1/dz with the closure) and the
+// gradients (7 + 2n, 8 + 2n), planes 1..NY+1 of the last two groups.
 hipError_t launch_les_strain_and_state(Ctx &c, hipStream_t st) {
   const LesArgs a = les_args(c);
-  const int NY = c.g.ny, nth = c.p.n_th;
-  const unsigned mb = (unsigned)((c.g.pf + 255) / 256);
+  const int nth = c.p.n_th;
   hipError_t e;
-  if ((e = klaunch(c, (const void *)k_les_strain, nullptr, dim3(mb, NY + 1), dim3(256), 0, st, a)) != hipSuccess)
-    return e;
+  if (c.recording) return hipErrorNotSupported;  // LES contexts are never recorded
+  const LesStrainOut so{c.les_four, c.cth, a.cfs, nth, c.les_mlp ? 1 : 0};
+  if ((e = launch_les_bc_strain(c, st, so)) != hipSuccess) return e;
   const double2 *four[kLesFour + 2 * DH_MAX_TH];
   double *phys[kLesFour + 2 * DH_MAX_TH];
   int n = 0;

@@ -2378,8 +2378,7 @@ hipError_t launch_phase(Ctx &c, int rk, int phase) {
   switch (phase) {
     case 0: {
       if (!c.les) return launch_state_to_phys(c);
-      hipError_t e = launch_les_bc_four(c, c.stream);
-      if (e == hipSuccess) e = launch_les_strain_and_state(c, c.stream);
+      hipError_t e = launch_les_strain_and_state(c, c.stream);
       if (e == hipSuccess) e = launch_les_model_rest(c, c.stream);
       return e;
     }
@@ -2417,9 +2416,8 @@ hipError_t launch_substep(Ctx &c, int rk, int bnd_end, int bnd_begin) {
 #define SCHK(expr) \
   if ((e = (expr)) != hipSuccess) return e;
   if (c.les) {
-    // the model's strain, then its inverse transforms and U's in one
-    // launch (both read the BC'd Fourier copy)
-    SCHK(launch_les_bc_four(c, c.stream));
+    // les_chan's BCs and the model's strain in one launch, then its inverse
+    // transforms and U's in one launch (both read the BC'd Fourier copy)
     SCHK(launch_les_strain_and_state(c, c.stream));
     SCHK(launch_les_model_rest(c, c.stream));
   } else {
